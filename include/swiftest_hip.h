@@ -1,0 +1,153 @@
+/*
+ * swiftest_hip.h -- C ABI of the MI355X implementation of Swiftest's per-step
+ * hot path.
+ *
+ * Two layers:
+ *
+ *  1. Leaf operators: drop-in replacements for the numeric leaves that the
+ *     reference's step layer calls.  They take the reference's own host
+ *     arrays (Fortran (3,n) real(DP) == C double[n][3]; masks as 4-byte
+ *     logicals == int32, nonzero = .true.; pair lists 1-based int32) and
+ *     keep the reference's in/out contract.  Each call copies its inputs to
+ *     HBM, runs the kernel, and copies the outputs back.
+ *
+ *  2. Resident systems: the whole WHM / HELIO / SyMBA step (kicks, drifts,
+ *     encounter detection, SyMBA recursion) runs on the device with the state
+ *     kept in HBM across steps; the host only sees it at output time.  One
+ *     handle holds an ensemble of independent systems (a single system is an
+ *     ensemble of one).
+ *
+ * All functions return 0 on success or a negative SWG_ERR_* code; the message
+ * of the last error on the calling thread is available from swg_last_error().
+ * Nothing here throws.  A context owns one HIP stream; calls on one context
+ * are stream-ordered and must come from one host thread at a time.
+ */
+#ifndef SWIFTEST_HIP_H
+#define SWIFTEST_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWG_OK 0
+#define SWG_ERR_ARG (-1)       /* invalid argument */
+#define SWG_ERR_HIP (-2)       /* HIP runtime error (no device, out of memory, ...) */
+#define SWG_ERR_COLLISION (-3) /* a collision was detected; its resolution is host work */
+#define SWG_ERR_DRIFT (-4)     /* a massive body failed the Danby drift */
+#define SWG_ERR_DEPTH (-5)     /* recursion step below double precision (symba_step.f90:196-201) */
+
+#define SWG_WHM 0
+#define SWG_HELIO 1
+#define SWG_SYMBA 2
+
+typedef struct swg_ctx swg_ctx;
+typedef struct swg_system swg_system;
+
+/* Run configuration: the reference parameters that change the step
+ * (swiftest_io.f90:2285-2760).  short_circuit = 1 reproduces the
+ * reference's `pl check .or. tp check` as gfortran -O evaluates it: the pl-tp
+ * check is skipped in a system whose pl-pl check found an encounter
+ * (symba_step.f90:38, :210). */
+typedef struct swg_params {
+    double dt;
+    int32_t integrator;    /* SWG_WHM, SWG_HELIO or SWG_SYMBA */
+    int32_t lclose;        /* CHK_CLOSE */
+    int32_t short_circuit;
+    int32_t istep_out;     /* snapshot cadence in steps, 0 = none */
+    int32_t rhill_present; /* RHILL_PRESENT: keep the given rhill */
+    int32_t reserved[3];
+} swg_params;
+
+typedef struct swg_stats {
+    int64_t nplpl_enc;  /* level-0 pl-pl encounters in the last step (all systems) */
+    int64_t npltp_enc;  /* level-0 pl-tp encounters in the last step */
+    int32_t max_depth;  /* deepest recursion level entered in the last step, -1 = none */
+    int32_t reserved;
+    int64_t nsubsteps;  /* recursion substeps in the last step */
+    int64_t ncollision; /* collisions detected (cumulative) */
+    int64_t ndrifterr;  /* drift failures (cumulative) */
+} swg_stats;
+
+int swg_init(int device, swg_ctx **ctx);
+int swg_finalize(swg_ctx *ctx);
+const char *swg_last_error(void);
+
+/* ---- leaf operators ------------------------------------------------------ */
+
+/* swiftest_kick_getacch_int_all_tri_rad_pl (src/swiftest/swiftest_kick.f90:165)
+ * and, with radius == NULL, ..._tri_norad_pl (:274).  acc(3,npl) is accumulated
+ * into. */
+int swg_kick_getacch_int_all_tri_pl(swg_ctx *ctx, int32_t npl, int32_t nplm, const double *r,
+                                    const double *Gmass, const double *radius, double *acc);
+
+/* swiftest_kick_getacch_int_all_flat_rad_pl (swiftest_kick.f90:69) and, with
+ * radius == NULL, ..._flat_norad_pl (:118); k_plpl is the 1-based (2,nplpl)
+ * pair list. */
+int swg_kick_getacch_int_all_flat_pl(swg_ctx *ctx, int32_t npl, int64_t nplpl, const int32_t *k_plpl,
+                                     const double *r, const double *Gmass, const double *radius, double *acc);
+
+/* swiftest_kick_getacch_int_all_tp (swiftest_kick.f90:374). */
+int swg_kick_getacch_int_all_tp(swg_ctx *ctx, int32_t ntp, int32_t npl, const double *rtp, const double *rpl,
+                                const double *GMpl, const int32_t *lmask, double *acc);
+
+/* swiftest_drift_all (src/swiftest/swiftest_drift.f90:60), GR off. */
+int swg_drift_all(swg_ctx *ctx, const double *mu, double *x, double *v, int32_t n, double dt,
+                  const int32_t *lmask, int32_t *iflag);
+
+/* encounter_check_all_plpl / _pltp, TRIANGULAR (src/encounter/encounter_check.f90:14, :112).
+ * The list is computed on the device and kept there; *nenc receives its length.
+ * swg_encounter_list_copy() then copies index1, index2 (1-based) and lvdotr
+ * into caller arrays of length nenc (the caller allocates, as the Fortran
+ * allocatable-out wrapper does). */
+int swg_encounter_check_all_plpl(swg_ctx *ctx, int32_t npl, const double *r, const double *v,
+                                 const double *renc, double dt, int64_t *nenc);
+int swg_encounter_check_all_pltp(swg_ctx *ctx, int32_t npl, int32_t ntp, const double *rpl, const double *vpl,
+                                 const double *rtp, const double *vtp, const double *renc, double dt,
+                                 int64_t *nenc);
+int swg_encounter_list_copy(swg_ctx *ctx, int32_t *index1, int32_t *index2, int32_t *lvdotr);
+
+/* ---- resident systems ---------------------------------------------------- */
+
+/* Create nsys independent systems from initial conditions as the reference
+ * reads them (heliocentric rh, vh; Gmass; radius; optional rhill), then run
+ * the reference's initialize sequence (sort by mass or by 1/r^3, set_rhill,
+ * vh->vb, ...).  Bodies of all systems are concatenated system by system;
+ * npl[s] / ntp[s] give the counts. */
+int swg_system_create(swg_ctx *ctx, const swg_params *prm, int32_t nsys, const int32_t *npl, const int32_t *ntp,
+                      const double *GMcb, const double *pl_rh, const double *pl_vh, const double *Gmass,
+                      const double *radius, const double *rhill, const double *tp_rh, const double *tp_vh,
+                      swg_system **sys);
+/* Driver loop: nsteps steps with a snapshot every istep_out steps
+ * (swiftest_driver.f90:130-160). */
+int swg_system_run(swg_system *sys, int64_t nsteps);
+/* One step, no snapshot. */
+int swg_system_step(swg_system *sys);
+/* Copy the state to host arrays (any may be NULL).  pl_id gives, for each
+ * stored body, its 1-based position in the creation input. */
+int swg_system_get(swg_system *sys, double *pl_rh, double *pl_vh, double *pl_vb, double *tp_rh, double *tp_vh,
+                   double *tp_vb, int32_t *pl_status, int32_t *tp_status, int32_t *pl_id);
+int swg_system_get_misc(swg_system *sys, double *rhill, double *cb_vb, double *ptbeg, double *ptend,
+                        int32_t *lfirst);
+int swg_system_stats(swg_system *sys, swg_stats *st);
+/* Level-0 encounter list of the last step: which = 0 pl-pl, 1 pl-tp;
+ * 1-based body indices in storage order.  With index1 == NULL only *n is set. */
+int swg_system_enc_list(swg_system *sys, int32_t which, int64_t *n, int32_t *index1, int32_t *index2);
+/* Time the dominant kernels: launch the pl-tp and pl-pl acceleration kernels
+ * reps times on the current state (no state change) and report the average
+ * duration of one launch in ms, measured with HIP events on the system's
+ * stream. */
+int swg_system_time_kernels(swg_system *sys, int32_t reps, double *ms_kick_tp, double *ms_kick_pl);
+/* Live timing: while enabled, every launch of the pl-tp and pl-pl
+ * acceleration kernels during steps is bracketed by HIP events on the
+ * system's stream (enabling resets the record).  swg_system_kernel_times()
+ * returns the average launch duration in ms and the number of launches. */
+int swg_system_kernel_timing(swg_system *sys, int32_t enable);
+int swg_system_kernel_times(swg_system *sys, double *ms_kick_tp, int64_t *n_kick_tp, double *ms_kick_pl,
+                            int64_t *n_kick_pl);
+int swg_system_destroy(swg_system *sys);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

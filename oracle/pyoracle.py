@@ -1,0 +1,243 @@
+"""ctypes binding of the CPU restatement (libswiftest_oracle*.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBS: dict = {}
+
+D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+I32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class Params(C.Structure):
+    _fields_ = [("dt", C.c_double), ("lclose", C.c_int), ("short_circuit", C.c_int),
+                ("istep_out", C.c_int), ("rhill_present", C.c_int), ("nthreads", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("nplpl_enc", C.c_int64), ("npltp_enc", C.c_int64), ("max_depth", C.c_int32),
+                ("nsubsteps", C.c_int64), ("ncollision", C.c_int64), ("ndrifterr", C.c_int64)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib(omp: bool = False):
+    key = "omp" if omp else "strict"
+    if key not in _LIBS:
+        name = "libswiftest_oracle_omp.so" if omp else "libswiftest_oracle.so"
+        path = os.path.join(HERE, name)
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        vp = C.c_void_p
+        L.swo_kick_getacch_int_all_tri.argtypes = [C.c_int, C.c_int, D, D, vp, D]
+        L.swo_kick_getacch_int_all_flat.argtypes = [C.c_int, C.c_int64, I32, D, D, vp, D]
+        L.swo_kick_getacch_int_all_tp.argtypes = [C.c_int, C.c_int, D, D, D, I32, D]
+        L.swo_drift_all.argtypes = [D, D, D, C.c_int, C.c_double, I32, I32, C.c_int, C.c_double]
+        L.swo_drift_kepu_stumpff.argtypes = [C.c_double] + [C.POINTER(C.c_double)] * 4
+        L.swo_encounter_check_one.argtypes = [C.c_double] * 8 + [C.POINTER(C.c_int32)] * 2
+        pp = C.POINTER(C.POINTER(C.c_int32))
+        L.swo_encounter_check_all_plpl.argtypes = [C.c_int, D, D, D, C.c_double, pp, pp, pp]
+        L.swo_encounter_check_all_plpl.restype = C.c_int64
+        L.swo_encounter_check_all_pltp.argtypes = [C.c_int, C.c_int, D, D, D, D, D, C.c_double, pp, pp, pp]
+        L.swo_encounter_check_all_pltp.restype = C.c_int64
+        L.swo_free.argtypes = [vp]
+        L.swo_collision_check_one.argtypes = [C.c_double] * 9 + [C.c_int32] + [C.POINTER(C.c_int32)] * 2
+        L.swo_orbel_xv2aeq.argtypes = [C.c_double] * 7 + [C.POINTER(C.c_double)] * 3
+        L.swo_orbel_xv2a.argtypes = [C.c_double] * 7
+        L.swo_orbel_xv2a.restype = C.c_double
+        L.swo_pow_m1p5.argtypes = [C.c_double]
+        L.swo_pow_m1p5.restype = C.c_double
+        L.swo_powi.argtypes = [C.c_double, C.c_int]
+        L.swo_powi.restype = C.c_double
+        L.swo_sys_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, D, D, D, D, vp, D, D,
+                                  C.POINTER(Params)]
+        L.swo_sys_new.restype = vp
+        L.swo_sys_free.argtypes = [vp]
+        L.swo_sys_run.argtypes = [vp, C.c_int64]
+        L.swo_sys_step.argtypes = [vp]
+        L.swo_sys_get.argtypes = [vp] + [vp] * 8
+        L.swo_sys_get_misc.argtypes = [vp] * 6
+        L.swo_sys_get_order.argtypes = [vp, I32]
+        L.swo_sys_stats.argtypes = [vp, C.POINTER(Stats)]
+        L.swo_sys_npl.argtypes = [vp]
+        L.swo_sys_ntp.argtypes = [vp]
+        L.swo_sys_list.argtypes = [vp, C.c_int, vp, vp, vp]
+        L.swo_sys_list.restype = C.c_int64
+        _LIBS[key] = L
+    return _LIBS[key]
+
+
+def _c(a, dt=np.float64):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+# ---- leaf kernels -------------------------------------------------------
+
+def accel_plpl(r, Gm, radius=None, acc=None, nplm=None):
+    r = _c(r)
+    n = r.shape[0]
+    acc = np.zeros((n, 3)) if acc is None else _c(acc).copy()
+    rad = None if radius is None else _c(radius)
+    lib().swo_kick_getacch_int_all_tri(n, n if nplm is None else nplm, r, _c(Gm), _p(rad), acc)
+    return acc
+
+
+def accel_pairs(k_plpl, r, Gm, radius=None, acc=None):
+    r = _c(r)
+    n = r.shape[0]
+    k = _c(k_plpl, np.int32).reshape(-1, 2)
+    acc = np.zeros((n, 3)) if acc is None else _c(acc).copy()
+    rad = None if radius is None else _c(radius)
+    lib().swo_kick_getacch_int_all_flat(n, k.shape[0], k, r, _c(Gm), _p(rad), acc)
+    return acc
+
+
+def accel_pltp(rtp, rpl, GMpl, lmask=None, acc=None, omp=False):
+    rtp = _c(rtp)
+    ntp = rtp.shape[0]
+    rpl = _c(rpl)
+    lm = np.ones(ntp, np.int32) if lmask is None else _c(lmask, np.int32)
+    acc = np.zeros((ntp, 3)) if acc is None else _c(acc).copy()
+    lib(omp).swo_kick_getacch_int_all_tp(ntp, rpl.shape[0], rtp, rpl, _c(GMpl), lm, acc)
+    return acc
+
+
+def drift(mu, x, v, dt, lmask=None, omp=False):
+    x = _c(x).copy()
+    v = _c(v).copy()
+    n = x.shape[0]
+    mu = np.broadcast_to(np.asarray(mu, np.float64), (n,)).copy()
+    lm = np.ones(n, np.int32) if lmask is None else _c(lmask, np.int32)
+    fl = np.zeros(n, np.int32)
+    lib(omp).swo_drift_all(mu, x, v, n, dt, lm, fl, 0, 0.0)
+    return x, v, fl
+
+
+def stumpff(x):
+    c = [C.c_double() for _ in range(4)]
+    lib().swo_drift_kepu_stumpff(x, *[C.byref(q) for q in c])
+    return tuple(q.value for q in c)
+
+
+def encounter_one(xr, vr, renc, dt):
+    le, lv = C.c_int32(), C.c_int32()
+    lib().swo_encounter_check_one(*map(float, xr), *map(float, vr), float(renc), float(dt), C.byref(le), C.byref(lv))
+    return bool(le.value), bool(lv.value)
+
+
+def _take_list(n, p1, p2, p3):
+    L = lib()
+    out = []
+    for p in (p1, p2, p3):
+        arr = np.ctypeslib.as_array(p, shape=(max(n, 1),))[:n].copy() if n else np.zeros(0, np.int32)
+        L.swo_free(C.cast(p, C.c_void_p))
+        out.append(arr)
+    return out
+
+
+def encounters_plpl(r, v, renc, dt, omp=False):
+    p = [C.POINTER(C.c_int32)() for _ in range(3)]
+    n = lib(omp).swo_encounter_check_all_plpl(len(renc), _c(r), _c(v), _c(renc), dt, *[C.byref(q) for q in p])
+    return _take_list(n, *p)
+
+
+def encounters_pltp(rpl, vpl, rtp, vtp, renc, dt, omp=False):
+    p = [C.POINTER(C.c_int32)() for _ in range(3)]
+    n = lib(omp).swo_encounter_check_all_pltp(len(renc), _c(rtp).shape[0], _c(rpl), _c(vpl), _c(rtp), _c(vtp),
+                                              _c(renc), dt, *[C.byref(q) for q in p])
+    return _take_list(n, *p)
+
+
+def collision_one(xr, vr, Gmtot, rlim, dt, lvdotr):
+    lc, lcl = C.c_int32(), C.c_int32()
+    lib().swo_collision_check_one(*map(float, xr), *map(float, vr), Gmtot, rlim, dt, int(lvdotr),
+                                  C.byref(lc), C.byref(lcl))
+    return bool(lc.value), bool(lcl.value)
+
+
+def xv2aeq(mu, r, v):
+    a, e, q = C.c_double(), C.c_double(), C.c_double()
+    lib().swo_orbel_xv2aeq(mu, *map(float, r), *map(float, v), C.byref(a), C.byref(e), C.byref(q))
+    return a.value, e.value, q.value
+
+
+def pow_m1p5(x):
+    return lib().swo_pow_m1p5(float(x))
+
+
+def powi(x, n):
+    return lib().swo_powi(float(x), int(n))
+
+
+# ---- whole systems --------------------------------------------------------
+
+WHM, HELIO, SYMBA = 0, 1, 2
+
+
+class OracleSystem:
+    """A system integrated by the CPU restatement of the reference step layer."""
+
+    def __init__(self, integrator, sysic, dt, lclose=True, short_circuit=True, istep_out=0,
+                 rhill=None, nthreads=1, omp=False):
+        self.L = lib(omp)
+        prm = Params(dt, int(lclose), int(short_circuit), int(istep_out), int(rhill is not None), int(nthreads))
+        self.npl, self.ntp = sysic.npl, sysic.ntp
+        rh_in = None if rhill is None else _c(rhill)
+        self._keep = rh_in
+        self.h = self.L.swo_sys_new(integrator, self.npl, self.ntp, sysic.GMcb, sysic.Rcb, _c(sysic.pl_rh),
+                                    _c(sysic.pl_vh), _c(sysic.Gmass), _c(sysic.radius), _p(rh_in),
+                                    _c(sysic.tp_rh).reshape(-1, 3), _c(sysic.tp_vh).reshape(-1, 3), C.byref(prm))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.swo_sys_free(self.h)
+            self.h = None
+
+    def run(self, nsteps):
+        return self.L.swo_sys_run(self.h, int(nsteps))
+
+    def step(self):
+        return self.L.swo_sys_step(self.h)
+
+    def state(self):
+        npl, ntp = self.npl, self.ntp
+        out = dict(pl_rh=np.zeros((npl, 3)), pl_vh=np.zeros((npl, 3)), pl_vb=np.zeros((npl, 3)),
+                   tp_rh=np.zeros((ntp, 3)), tp_vh=np.zeros((ntp, 3)), tp_vb=np.zeros((ntp, 3)),
+                   pl_status=np.zeros(npl, np.int32), tp_status=np.zeros(ntp, np.int32))
+        self.L.swo_sys_get(self.h, *[_p(out[k]) for k in ("pl_rh", "pl_vh", "pl_vb", "tp_rh", "tp_vh", "tp_vb",
+                                                           "pl_status", "tp_status")])
+        rhill = np.zeros(npl)
+        cb_vb, ptb, pte = np.zeros(3), np.zeros(3), np.zeros(3)
+        lf = np.zeros(1, np.int32)
+        self.L.swo_sys_get_misc(self.h, _p(rhill), _p(cb_vb), _p(ptb), _p(pte), _p(lf))
+        ids = np.zeros(npl, np.int32)
+        self.L.swo_sys_get_order(self.h, ids)
+        out.update(rhill=rhill, cb_vb=cb_vb, ptbeg=ptb, ptend=pte, lfirst=int(lf[0]), pl_id=ids)
+        return out
+
+    def stats(self):
+        st = Stats()
+        self.L.swo_sys_stats(self.h, C.byref(st))
+        return {k: getattr(st, k) for k, _ in Stats._fields_}
+
+    def enc_list(self, which=0):
+        n = self.L.swo_sys_list(self.h, which, None, None, None)
+        a, b, c = (np.zeros(n, np.int32) for _ in range(3))
+        self.L.swo_sys_list(self.h, which, _p(a), _p(b), _p(c))
+        return a, b, c

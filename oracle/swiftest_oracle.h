@@ -1,0 +1,131 @@
+/*
+ * swiftest_oracle.h -- CPU restatement of Swiftest's per-step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (swiftest_amd/, include/)
+ * may link, load or call this code.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg use it, and only as the checker / CPU baseline.
+ *
+ * Every function restates the arithmetic of the Fortran reference
+ * (carlislewishard/swiftest v2023.10.2) in plain C, in the reference's
+ * operation order; each cites the reference file:line it follows.
+ *
+ * Floating-point rule (identical in the oracle and in the HIP kernels; this
+ * file is compiled with -ffp-contract=off so the compiler adds nothing):
+ *   (1) 3-vector squared norms and dot products are a chain of fused
+ *       multiply-adds:  d = fma(z1,z2, fma(y1,y2, x1*x2)).
+ *   (2) accumulations acc = acc +/- f*x inside pair sums (kick kernels,
+ *       SyMBA shell kicks, encounter-pair removal/add-back, WHM ah0) and the
+ *       velocity kick / linear drift updates (v += a*dt, r += p*dt) are one fma.
+ *   (3) every other operation is rounded separately, as written.
+ * The reference leaves contraction to its compiler (GCC -O3 -march=native
+ * contracts these same a*b+c shapes); the rule above fixes one choice.
+ * sqrt and division are IEEE (correctly rounded).  r2**(-1.5) in the SyMBA
+ * shell kick is evaluated correctly rounded by a double-double sequence
+ * (swo_pow_m1p5), identical on both sides; glibc's pow agrees with it to
+ * within 1 ulp (checked in tests).
+ *
+ * Arrays follow the reference layout: vectors are double[n][3] (the Fortran
+ * (3,n) column-major layout), masks are int32 (nonzero = .true.), pair index
+ * lists are 1-based int32.
+ */
+#ifndef SWIFTEST_ORACLE_H
+#define SWIFTEST_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (src/globals/globals_module.f90:86-103) ---- */
+#define SWO_ACTIVE 0
+#define SWO_INACTIVE 1
+#define SWO_DISCARDED_DRIFTERR (-7)
+#define SWO_COLLIDED (-14)
+
+/* ---- leaf kernels ---- */
+void swo_kick_getacch_int_all_tri(int npl, int nplm, const double *r, const double *Gmass,
+                                  const double *radius, double *acc);
+void swo_kick_getacch_int_all_flat(int npl, int64_t nplpl, const int32_t *k_plpl, const double *r,
+                                   const double *Gmass, const double *radius, double *acc);
+void swo_kick_getacch_int_all_tp(int ntp, int npl, const double *rtp, const double *rpl,
+                                 const double *GMpl, const int32_t *lmask, double *acc);
+
+void swo_drift_one(double mu, double *rx, double *ry, double *rz, double *vx, double *vy, double *vz,
+                   double dt, int32_t *iflag);
+void swo_drift_all(const double *mu, double *x, double *v, int n, double dt, const int32_t *lmask,
+                   int32_t *iflag, int lgr, double inv_c2);
+void swo_drift_kepu_stumpff(double x, double *c0, double *c1, double *c2, double *c3);
+
+void swo_encounter_check_one(double xr, double yr, double zr, double vxr, double vyr, double vzr,
+                             double renc, double dt, int32_t *lencounter, int32_t *lvdotr);
+/* Triangular pair lists.  Outputs are malloc'd (free with swo_free); indices 1-based. */
+int64_t swo_encounter_check_all_plpl(int npl, const double *r, const double *v, const double *renc,
+                                     double dt, int32_t **index1, int32_t **index2, int32_t **lvdotr);
+int64_t swo_encounter_check_all_pltp(int npl, int ntp, const double *rpl, const double *vpl,
+                                     const double *rtp, const double *vtp, const double *renc,
+                                     double dt, int32_t **index1, int32_t **index2, int32_t **lvdotr);
+void swo_free(void *p);
+
+void swo_collision_check_one(double xr, double yr, double zr, double vxr, double vyr, double vzr,
+                             double Gmtot, double rlim, double dt, int32_t lvdotr,
+                             int32_t *lcollision, int32_t *lclosest);
+void swo_orbel_xv2aeq(double mu, double rx, double ry, double rz, double vx, double vy, double vz,
+                      double *a, double *e, double *q);
+double swo_orbel_xv2a(double mu, double rx, double ry, double rz, double vx, double vy, double vz);
+double swo_pow_m1p5(double x);
+double swo_powi(double x, int n);
+
+/* ---- whole-system integrators (WHM / HELIO / SyMBA) ---- */
+#define SWO_WHM 0
+#define SWO_HELIO 1
+#define SWO_SYMBA 2
+
+typedef struct swo_sys swo_sys;
+
+typedef struct swo_params {
+    double dt;
+    int lclose;          /* CHK_CLOSE (radius overlap test in pl-pl gravity; collision checks) */
+    int short_circuit;   /* 1: `a .or. b` on encounter checks skips b when a is true (gfortran -O) */
+    int istep_out;       /* snapshot cadence; 0 = never */
+    int rhill_present;   /* RHILL_PRESENT: keep input rhill, never refresh */
+    int nthreads;        /* OpenMP threads for the pair loops (1 = serial) */
+} swo_params;
+
+typedef struct swo_stats {
+    int64_t nplpl_enc;     /* level-0 pl-pl encounters in the last step */
+    int64_t npltp_enc;     /* level-0 pl-tp encounters in the last step */
+    int32_t max_depth;     /* deepest recursion level entered in the last step */
+    int64_t nsubsteps;     /* recursion substeps in the last step */
+    int64_t ncollision;    /* collisions detected (cumulative) */
+    int64_t ndrifterr;     /* drift failures (cumulative) */
+} swo_stats;
+
+/* Create a system from initial conditions as read from the reference's
+ * input files (heliocentric rh, vh; Gmass; radius; optional rhill), then run
+ * the reference's initialize sequence for the chosen integrator. */
+swo_sys *swo_sys_new(int integrator, int npl, int ntp, double GMcb, double Rcb,
+                     const double *pl_rh, const double *pl_vh, const double *Gmass,
+                     const double *radius, const double *rhill_in,
+                     const double *tp_rh, const double *tp_vh, const swo_params *prm);
+void swo_sys_free(swo_sys *s);
+/* Run nsteps steps of the driver loop (step + output cadence).  Returns 0 or
+ * a negative code: -1 collision detected (merge not modelled), -2 fatal
+ * drift error (WHM pl), -3 recursion depth guard. */
+int swo_sys_run(swo_sys *s, int64_t nsteps);
+/* One integrator step only (no snapshot). */
+int swo_sys_step(swo_sys *s);
+void swo_sys_get(const swo_sys *s, double *pl_rh, double *pl_vh, double *pl_vb, double *tp_rh,
+                 double *tp_vh, double *tp_vb, int32_t *pl_status, int32_t *tp_status);
+void swo_sys_get_misc(const swo_sys *s, double *rhill, double *cb_vb, double *ptbeg, double *ptend,
+                      int32_t *lfirst);
+void swo_sys_get_order(const swo_sys *s, int32_t *pl_id);
+void swo_sys_stats(const swo_sys *s, swo_stats *st);
+int swo_sys_npl(const swo_sys *s);
+int swo_sys_ntp(const swo_sys *s);
+/* Encounter lists produced by the last step's level-0 checks (1-based). */
+int64_t swo_sys_list(const swo_sys *s, int which, int32_t *index1, int32_t *index2, int32_t *lvdotr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,258 @@
+"""Python host mirror of the reference's operator API over the C ABI.
+
+The functions keep the names and argument meaning of the reference's numeric
+leaves (src/swiftest/swiftest_kick.f90, swiftest_drift.f90,
+src/encounter/encounter_check.f90) and its in/out contract: accelerations are
+accumulated into the caller's array, drift updates positions and velocities
+in place and reports per-body flags, encounter checks return 1-based
+(index1, index2, lvdotr) lists ordered index1 then index2.
+
+Everything runs on the GPU through libswiftest_hip.so.  There is no CPU
+fallback: if the library is missing or no GPU is visible, the first call
+raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libswiftest_hip.so")
+
+WHM, HELIO, SYMBA = 0, 1, 2
+ERRORS = {-1: "argument", -2: "hip", -3: "collision", -4: "drift", -5: "depth"}
+
+
+class SwiftestHipError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"swiftest_hip error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [("dt", C.c_double), ("integrator", C.c_int32), ("lclose", C.c_int32),
+                ("short_circuit", C.c_int32), ("istep_out", C.c_int32), ("rhill_present", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("nplpl_enc", C.c_int64), ("npltp_enc", C.c_int64), ("max_depth", C.c_int32),
+                ("reserved", C.c_int32), ("nsubsteps", C.c_int64), ("ncollision", C.c_int64),
+                ("ndrifterr", C.c_int64)]
+
+
+_lib = None
+_ctx = None
+
+D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+I32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+vp = C.c_void_p
+
+
+def library():
+    """Load libswiftest_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise SwiftestHipError(-2, f"{LIBPATH} not built; run swiftest_amd/build.py")
+        L = C.CDLL(LIBPATH)
+        L.swg_last_error.restype = C.c_char_p
+        L.swg_init.argtypes = [C.c_int, C.POINTER(vp)]
+        L.swg_finalize.argtypes = [vp]
+        L.swg_kick_getacch_int_all_tri_pl.argtypes = [vp, C.c_int32, C.c_int32, D, D, vp, D]
+        L.swg_kick_getacch_int_all_flat_pl.argtypes = [vp, C.c_int32, C.c_int64, I32, D, D, vp, D]
+        L.swg_kick_getacch_int_all_tp.argtypes = [vp, C.c_int32, C.c_int32, D, D, D, I32, D]
+        L.swg_drift_all.argtypes = [vp, D, D, D, C.c_int32, C.c_double, I32, I32]
+        L.swg_encounter_check_all_plpl.argtypes = [vp, C.c_int32, D, D, D, C.c_double, C.POINTER(C.c_int64)]
+        L.swg_encounter_check_all_pltp.argtypes = [vp, C.c_int32, C.c_int32, D, D, D, D, D, C.c_double,
+                                                   C.POINTER(C.c_int64)]
+        L.swg_encounter_list_copy.argtypes = [vp, vp, vp, vp]
+        L.swg_system_create.argtypes = [vp, C.POINTER(Params), C.c_int32, I32, I32, D, D, D, D, D, vp, D, D,
+                                        C.POINTER(vp)]
+        L.swg_system_run.argtypes = [vp, C.c_int64]
+        L.swg_system_step.argtypes = [vp]
+        L.swg_system_get.argtypes = [vp] * 10
+        L.swg_system_get_misc.argtypes = [vp] * 6
+        L.swg_system_stats.argtypes = [vp, C.POINTER(Stats)]
+        L.swg_system_enc_list.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64), vp, vp]
+        L.swg_system_time_kernels.argtypes = [vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.swg_system_kernel_timing.argtypes = [vp, C.c_int32]
+        L.swg_system_kernel_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                              C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.swg_system_destroy.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise SwiftestHipError(rc, library().swg_last_error().decode())
+
+
+def context(device: int = 0):
+    """The process-wide library context on `device` (created on first use)."""
+    global _ctx
+    if _ctx is None:
+        L = library()
+        h = vp()
+        _check(L.swg_init(device, C.byref(h)))
+        _ctx = h
+    return _ctx
+
+
+def _c(a, dt=np.float64):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(vp)
+
+
+# ---- leaf operators (same names and contracts as the reference) ----------
+
+def swiftest_kick_getacch_int_all_tri(npl, nplm, r, Gmass, radius, acc):
+    """swiftest_kick_getacch_int_all_tri_rad_pl / _norad_pl (swiftest_kick.f90:165, :274).
+    acc is accumulated into and returned."""
+    acc = _c(acc).copy()
+    rad = None if radius is None else _c(radius)
+    _check(library().swg_kick_getacch_int_all_tri_pl(context(), int(npl), int(nplm), _c(r), _c(Gmass), _p(rad), acc))
+    return acc
+
+
+def swiftest_kick_getacch_int_all_flat(npl, k_plpl, r, Gmass, radius, acc):
+    """swiftest_kick_getacch_int_all_flat_rad_pl / _norad_pl (swiftest_kick.f90:69, :118)."""
+    acc = _c(acc).copy()
+    k = _c(k_plpl, np.int32).reshape(-1, 2)
+    rad = None if radius is None else _c(radius)
+    _check(library().swg_kick_getacch_int_all_flat_pl(context(), int(npl), k.shape[0], k, _c(r), _c(Gmass),
+                                                       _p(rad), acc))
+    return acc
+
+
+def swiftest_kick_getacch_int_all_tp(ntp, npl, rtp, rpl, GMpl, lmask, acc):
+    """swiftest_kick_getacch_int_all_tp (swiftest_kick.f90:374)."""
+    acc = _c(acc).copy()
+    _check(library().swg_kick_getacch_int_all_tp(context(), int(ntp), int(npl), _c(rtp), _c(rpl), _c(GMpl),
+                                                  _c(lmask, np.int32), acc))
+    return acc
+
+
+def swiftest_drift_all(mu, x, v, n, dt, lmask):
+    """swiftest_drift_all (swiftest_drift.f90:60), GR off.  Returns (x, v, iflag)."""
+    x = _c(x).copy()
+    v = _c(v).copy()
+    fl = np.zeros(int(n), np.int32)
+    mu = np.broadcast_to(np.asarray(mu, np.float64), (int(n),)).copy()
+    _check(library().swg_drift_all(context(), mu, x, v, int(n), float(dt), _c(lmask, np.int32), fl))
+    return x, v, fl
+
+
+def _take_list(n):
+    a, b, c = (np.zeros(max(n, 1), np.int32) for _ in range(3))
+    if n:
+        _check(library().swg_encounter_list_copy(context(), _p(a), _p(b), _p(c)))
+    return a[:n], b[:n], c[:n]
+
+
+def encounter_check_all_plpl(npl, r, v, renc, dt):
+    """encounter_check_all_plpl, TRIANGULAR (encounter_check.f90:14): (index1, index2, lvdotr)."""
+    n = C.c_int64()
+    _check(library().swg_encounter_check_all_plpl(context(), int(npl), _c(r), _c(v), _c(renc), float(dt),
+                                                   C.byref(n)))
+    return _take_list(n.value)
+
+
+def encounter_check_all_pltp(npl, ntp, rpl, vpl, rtp, vtp, renc, dt):
+    """encounter_check_all_pltp, TRIANGULAR (encounter_check.f90:112): (index1, index2, lvdotr)."""
+    n = C.c_int64()
+    _check(library().swg_encounter_check_all_pltp(context(), int(npl), int(ntp), _c(rpl), _c(vpl), _c(rtp),
+                                                   _c(vtp), _c(renc), float(dt), C.byref(n)))
+    return _take_list(n.value)
+
+
+# ---- resident systems -----------------------------------------------------
+
+class GpuSystem:
+    """An ensemble of independent systems integrated on the GPU.
+
+    `systems` is one swiftest_amd.ics.System or a list of them; all share the
+    integrator and dt.  Body arrays of the systems are concatenated in order.
+    """
+
+    def __init__(self, integrator, systems, dt, lclose=True, short_circuit=True, istep_out=0, rhill=None):
+        if not isinstance(systems, (list, tuple)):
+            systems = [systems]
+        self.L = library()
+        self.nsys = len(systems)
+        self.npl_s = np.array([s.npl for s in systems], np.int32)
+        self.ntp_s = np.array([s.ntp for s in systems], np.int32)
+        self.npl, self.ntp = int(self.npl_s.sum()), int(self.ntp_s.sum())
+        cat = lambda key, w: _c(np.concatenate([getattr(s, key).reshape(-1, w) for s in systems])) \
+            if systems else np.zeros((0, w))
+        prm = Params(float(dt), int(integrator), int(lclose), int(short_circuit), int(istep_out),
+                     int(rhill is not None))
+        rh = None if rhill is None else _c(rhill)
+        h = vp()
+        _check(self.L.swg_system_create(context(), C.byref(prm), self.nsys, self.npl_s, self.ntp_s,
+                                        _c([s.GMcb for s in systems]), cat("pl_rh", 3), cat("pl_vh", 3),
+                                        _c(np.concatenate([s.Gmass for s in systems])),
+                                        _c(np.concatenate([s.radius for s in systems])), _p(rh),
+                                        cat("tp_rh", 3), cat("tp_vh", 3), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.swg_system_destroy(self.h)
+            self.h = None
+
+    def run(self, nsteps):
+        _check(self.L.swg_system_run(self.h, int(nsteps)))
+
+    def step(self):
+        _check(self.L.swg_system_step(self.h))
+
+    def state(self):
+        npl, ntp = self.npl, self.ntp
+        out = dict(pl_rh=np.zeros((npl, 3)), pl_vh=np.zeros((npl, 3)), pl_vb=np.zeros((npl, 3)),
+                   tp_rh=np.zeros((ntp, 3)), tp_vh=np.zeros((ntp, 3)), tp_vb=np.zeros((ntp, 3)),
+                   pl_status=np.zeros(npl, np.int32), tp_status=np.zeros(ntp, np.int32),
+                   pl_id=np.zeros(npl, np.int32))
+        _check(self.L.swg_system_get(self.h, *[_p(out[k]) for k in (
+            "pl_rh", "pl_vh", "pl_vb", "tp_rh", "tp_vh", "tp_vb", "pl_status", "tp_status", "pl_id")]))
+        rhill = np.zeros(npl)
+        cb_vb, ptb, pte = (np.zeros((self.nsys, 3)) for _ in range(3))
+        lf = np.zeros(self.nsys, np.int32)
+        _check(self.L.swg_system_get_misc(self.h, _p(rhill), _p(cb_vb), _p(ptb), _p(pte), _p(lf)))
+        out.update(rhill=rhill, cb_vb=cb_vb, ptbeg=ptb, ptend=pte, lfirst=lf)
+        return out
+
+    def stats(self):
+        st = Stats()
+        _check(self.L.swg_system_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in Stats._fields_ if k != "reserved"}
+
+    def enc_list(self, which=0):
+        n = C.c_int64()
+        _check(self.L.swg_system_enc_list(self.h, which, C.byref(n), None, None))
+        a, b = np.zeros(max(n.value, 1), np.int32), np.zeros(max(n.value, 1), np.int32)
+        _check(self.L.swg_system_enc_list(self.h, which, C.byref(n), _p(a), _p(b)))
+        return a[:n.value], b[:n.value]
+
+    def kernel_timing(self, enable=True):
+        """Record HIP events around every acceleration-kernel launch of the
+        following steps (enabling resets the record)."""
+        _check(self.L.swg_system_kernel_timing(self.h, int(enable)))
+
+    def kernel_times(self):
+        """(avg ms of the pl-tp kernel, launches, avg ms of the pl-pl kernel, launches)."""
+        a, b = C.c_double(), C.c_double()
+        na, nb = C.c_int64(), C.c_int64()
+        _check(self.L.swg_system_kernel_times(self.h, C.byref(a), C.byref(na), C.byref(b), C.byref(nb)))
+        return a.value, na.value, b.value, nb.value
+
+    def time_kernels(self, reps=5):
+        tp_ms, pl_ms = C.c_double(), C.c_double()
+        _check(self.L.swg_system_time_kernels(self.h, int(reps), C.byref(tp_ms), C.byref(pl_ms)))
+        return tp_ms.value, pl_ms.value

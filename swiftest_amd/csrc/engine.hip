@@ -1,0 +1,945 @@
+// engine.hip -- host side of the resident-system path: device allocation,
+// the reference's initialize sequence, the step sequence of SyMBA / HELIO,
+// the recursion driver, snapshots, and the C ABI of include/swiftest_hip.h.
+//
+// Step sequence (symba_step.f90:14-115 with the HELIO branch of
+// helio_step.f90:37-119 folded in): the HELIO branch is the encounter branch
+// with empty lists, except that it clears lfirst; each system keeps its own
+// lfirst.  All systems of an ensemble descend the recursion together; a level
+// on which a system has no listed pair is a no-op for that system, so the
+// per-system result is the reference's.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <chrono>
+#include <cstdlib>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/swiftest_hip.h"
+#include "swg_engine.h"
+#include "swg_kernels.h"
+#include "swg_math.h"
+
+namespace swg {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int hipfail(hipError_t e, const char *what)
+{
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return SWG_ERR_HIP;
+}
+
+#define HIPCHK(x)                                    \
+    do {                                             \
+        hipError_t e_ = (x);                         \
+        if (e_ != hipSuccess) return hipfail(e_, #x); \
+    } while (0)
+
+template <class T>
+static hipError_t dalloc(T **p, size_t n)
+{
+    return hipMalloc((void **)p, sizeof(T) * (n > 0 ? n : 1));
+}
+
+// ---------------------------------------------------------------------------
+// host arithmetic of the initialize / snapshot sequence
+// ---------------------------------------------------------------------------
+
+// semi-major axis of swiftest_orbel_xv2el (swiftest_orbel.f90:913-1072)
+static double xv2a(double mu, const double *r, const double *v)
+{
+    double hx = r[1] * v[2] - r[2] * v[1], hy = r[2] * v[0] - r[0] * v[2], hz = r[0] * v[1] - r[1] * v[0];
+    double h2 = dot3(hx, hy, hz, hx, hy, hz);
+    if (h2 <= 10.0 * DBL_MIN) return 0.0;
+    double rr = std::sqrt(dot3(r[0], r[1], r[2], r[0], r[1], r[2]));
+    double v2 = dot3(v[0], v[1], v[2], v[0], v[1], v[2]);
+    double energy = 0.5 * v2 - mu / rr;
+    if (std::fabs((energy * rr) / mu) < std::sqrt(TINYVALUE)) return (0.5 * h2) / mu;
+    double a = (-0.5 * mu) / energy;
+    if (a < 0.0) {
+        double fac = -h2 / (mu * a);
+        if (!(fac > TINYVALUE)) return (0.5 * h2) / mu;
+    }
+    return a;
+}
+
+// swiftest_util_set_rhill (swiftest_util.f90:2260-2280): mu = GM_cb + Gm
+void host_set_rhill(double GMcb, int n, const double *rh, const double *vh, const double *Gm, double *rhill)
+{
+    if (GMcb <= DBL_MIN) return;
+    for (int i = 0; i < n; i++) {
+        double a = xv2a(GMcb + Gm[i], &rh[3 * i], &vh[3 * i]);
+        double fac = std::pow((Gm[i] / GMcb) / 3.0, THIRD);
+        if (a > 0.0)
+            rhill[i] = a * fac;
+        else
+            rhill[i] = std::sqrt(dot3(rh[3 * i], rh[3 * i + 1], rh[3 * i + 2], rh[3 * i], rh[3 * i + 1], rh[3 * i + 2])) * fac;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble
+// ---------------------------------------------------------------------------
+
+static void free_list(ListDev &L)
+{
+    void *p[] = {L.words, L.wpre, L.woff, L.nb, L.rowcnt, L.rowstart, L.i1, L.i2, L.lvdotr, L.status, L.level,
+                 L.colcnt, L.colstart, L.col};
+    for (void *q : p)
+        if (q) (void)hipFree(q);
+    L = ListDev{};
+}
+
+Ensemble::~Ensemble()
+{
+    for (void *p : owned)
+        if (p) (void)hipFree(p);
+    free_list(pp);
+    free_list(pt);
+    if (h_small) (void)hipHostFree(h_small);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    for (hipEvent_t e : ev_tp) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ev_pl) (void)hipEventDestroy(e);
+}
+
+template <class T>
+int Ensemble::own(T **p, size_t n)
+{
+    HIPCHK(dalloc(p, n));
+    owned.push_back((void *)*p);
+    return 0;
+}
+
+int Ensemble::alloc_vec(Vec3p &v, size_t n)
+{
+    if (own(&v.x, n) || own(&v.y, n) || own(&v.z, n)) return SWG_ERR_HIP;
+    return 0;
+}
+
+static int upload_aos(hipStream_t st, Vec3p v, const double *aos, size_t n, std::vector<double> &tmp)
+{
+    tmp.resize(3 * (n ? n : 1));
+    double *cx = tmp.data(), *cy = cx + n, *cz = cy + n;
+    for (size_t i = 0; i < n; i++) {
+        cx[i] = aos[3 * i];
+        cy[i] = aos[3 * i + 1];
+        cz[i] = aos[3 * i + 2];
+    }
+    HIPCHK(hipMemcpyAsync(v.x, cx, n * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(v.y, cy, n * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(v.z, cz, n * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+static int download_aos(hipStream_t st, Vec3p v, double *aos, size_t n)
+{
+    std::vector<double> tmp(3 * (n ? n : 1));
+    double *cx = tmp.data(), *cy = cx + n, *cz = cy + n;
+    HIPCHK(hipMemcpyAsync(cx, v.x, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(cy, v.y, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(cz, v.z, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (size_t i = 0; i < n; i++) {
+        aos[3 * i] = cx[i];
+        aos[3 * i + 1] = cy[i];
+        aos[3 * i + 2] = cz[i];
+    }
+    return 0;
+}
+
+int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const int32_t *ntp_s, const double *GMcb,
+                     const double *pl_rh, const double *pl_vh, const double *Gmass, const double *radius,
+                     const double *rhill_in, const double *tp_rh, const double *tp_vh)
+{
+    prm = p;
+    if (!(p.dt > 0.0)) return fail(SWG_ERR_ARG, "dt must be positive");
+    if (p.integrator != SWG_HELIO && p.integrator != SWG_SYMBA && p.integrator != SWG_WHM)
+        return fail(SWG_ERR_ARG, "unknown integrator");
+    if (ns <= 0) return fail(SWG_ERR_ARG, "nsys must be positive");
+    nsys = ns;
+    h_pl_off.assign(nsys + 1, 0);
+    h_tp_off.assign(nsys + 1, 0);
+    for (int s = 0; s < nsys; s++) {
+        if (npl_s[s] < 0 || ntp_s[s] < 0) return fail(SWG_ERR_ARG, "negative body count");
+        h_pl_off[s + 1] = h_pl_off[s] + npl_s[s];
+        h_tp_off[s + 1] = h_tp_off[s] + ntp_s[s];
+    }
+    npl = h_pl_off[nsys];
+    ntp = h_tp_off[nsys];
+    h_GMcb.assign(GMcb, GMcb + nsys);
+
+    // ---- the reference's initialize sequence on the host side ----
+    // set_rhill (unless given), then sort each system's massive bodies by mass,
+    // descending (helio_util.f90:26), carrying every per-body array along.
+    std::vector<double> rh(pl_rh, pl_rh + 3 * (size_t)npl), vh(pl_vh, pl_vh + 3 * (size_t)npl);
+    std::vector<double> gm(Gmass, Gmass + npl), rad(radius, radius + npl), rhill(npl);
+    for (int s = 0; s < nsys; s++) {
+        int a = h_pl_off[s], n = h_pl_off[s + 1] - a;
+        if (rhill_in && p.rhill_present)
+            std::copy(rhill_in + a, rhill_in + a + n, rhill.begin() + a);
+        else
+            host_set_rhill(GMcb[s], n, &rh[3 * (size_t)a], &vh[3 * (size_t)a], &gm[a], &rhill[a]);
+    }
+    h_pl_id.resize(npl);
+    {
+        std::vector<double> rh2(rh.size()), vh2(vh.size()), gm2(npl), rad2(npl), rhill2(npl);
+        for (int s = 0; s < nsys; s++) {
+            int a = h_pl_off[s], n = h_pl_off[s + 1] - a;
+            std::vector<int> idx(n);
+            std::iota(idx.begin(), idx.end(), 0);
+            // WHM sorts by 1/|rh|^3 (whm_util.f90:213-215), HELIO/SyMBA by mass
+            std::vector<double> key(n);
+            for (int q = 0; q < n; q++) {
+                if (p.integrator == SWG_WHM) {
+                    const double *x = &rh[3 * (size_t)(a + q)];
+                    double r2 = dot3(x[0], x[1], x[2], x[0], x[1], x[2]);
+                    key[q] = (1.0 / std::sqrt(r2)) / r2;
+                } else {
+                    key[q] = gm[a + q];
+                }
+            }
+            std::stable_sort(idx.begin(), idx.end(), [&](int u, int v) { return key[u] > key[v]; });
+            for (int q = 0; q < n; q++) {
+                int src = a + idx[q], dst = a + q;
+                for (int d = 0; d < 3; d++) {
+                    rh2[3 * (size_t)dst + d] = rh[3 * (size_t)src + d];
+                    vh2[3 * (size_t)dst + d] = vh[3 * (size_t)src + d];
+                }
+                gm2[dst] = gm[src];
+                rad2[dst] = rad[src];
+                rhill2[dst] = rhill[src];
+                h_pl_id[dst] = idx[q] + 1;
+            }
+        }
+        rh.swap(rh2);
+        vh.swap(vh2);
+        gm.swap(gm2);
+        rad.swap(rad2);
+        rhill.swap(rhill2);
+    }
+    h_Gm = gm;
+
+    // ---- device allocation ----
+    HIPCHK(hipHostMalloc((void **)&h_small, 4096 + 16 * sizeof(int32_t) * (size_t)nsys));
+    if (alloc_vec(pl.rh, npl) || alloc_vec(pl.vh, npl) || alloc_vec(pl.vb, npl) || alloc_vec(pl.rbeg, npl) ||
+        alloc_vec(pl.rend, npl) || own(&pl.Gm, npl) || own(&pl.R, npl) || own(&pl.rhill, npl) || own(&pl.renc, npl) ||
+        own(&pl.status, npl) || own(&pl.levelg, npl) || own(&pl.levelm, npl) || own(&pl.sys, npl))
+        return SWG_ERR_HIP;
+    if (alloc_vec(tp.rh, ntp) || alloc_vec(tp.vh, ntp) || alloc_vec(tp.vb, ntp) || own(&tp.status, ntp) ||
+        own(&tp.levelg, ntp) || own(&tp.levelm, ntp) || own(&tp.sys, ntp))
+        return SWG_ERR_HIP;
+    if (own(&sy.pl_off, nsys + 1) || own(&sy.tp_off, nsys + 1) || own(&sy.GMcb, nsys) || own(&sy.cbvb, 3 * nsys) ||
+        own(&sy.ptbeg, 3 * nsys) || own(&sy.ptend, 3 * nsys) || own(&sy.lfirst, nsys) || own(&sy.enc_pp, nsys) ||
+        own(&sy.enc_pt, nsys) || own(&sy.flag_pp, nsys) || own(&sy.flag_pt, nsys) || own(&sy.err, nsys) ||
+        own(&d_any, 4) || own(&d_total, 4) || own(&d_pt_run, nsys))
+        return SWG_ERR_HIP;
+    hipStream_t st = stream;
+    std::vector<double> tmp;
+    if (upload_aos(st, pl.rh, rh.data(), npl, tmp) || upload_aos(st, pl.vh, vh.data(), npl, tmp)) return SWG_ERR_HIP;
+    if (upload_aos(st, tp.rh, tp_rh, ntp, tmp) || upload_aos(st, tp.vh, tp_vh, ntp, tmp)) return SWG_ERR_HIP;
+    HIPCHK(hipMemcpy(pl.Gm, gm.data(), npl * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(pl.R, rad.data(), npl * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(pl.rhill, rhill.data(), npl * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<int32_t> zeros(std::max(std::max(npl, ntp), 3 * nsys) + 1, 0);
+    std::vector<int32_t> sys_pl(npl), sys_tp(ntp), ones(nsys, 1);
+    for (int s = 0; s < nsys; s++) {
+        for (int i = h_pl_off[s]; i < h_pl_off[s + 1]; i++) sys_pl[i] = s;
+        for (int i = h_tp_off[s]; i < h_tp_off[s + 1]; i++) sys_tp[i] = s;
+    }
+    HIPCHK(hipMemcpy(pl.status, zeros.data(), npl * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(tp.status, zeros.data(), ntp * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(pl.sys, sys_pl.data(), npl * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(tp.sys, sys_tp.data(), ntp * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(sy.pl_off, h_pl_off.data(), (nsys + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(sy.tp_off, h_tp_off.data(), (nsys + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(sy.GMcb, GMcb, nsys * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<double> dz(3 * (size_t)nsys, 0.0);
+    HIPCHK(hipMemcpy(sy.ptbeg, dz.data(), 3 * nsys * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(sy.ptend, dz.data(), 3 * nsys * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(sy.cbvb, dz.data(), 3 * nsys * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(sy.lfirst, ones.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice));
+    for (int32_t *q : {sy.enc_pp, sy.enc_pt, sy.flag_pp, sy.flag_pt, sy.err})
+        HIPCHK(hipMemcpy(q, zeros.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice));
+
+    // ---- tiles ----
+    build_tiles();
+
+    // ---- pl-pl bit matrix ----
+    {
+        std::vector<int64_t> woff(nsys);
+        std::vector<int32_t> nb(nsys);
+        int64_t w = 0;
+        for (int s = 0; s < nsys; s++) {
+            int n = h_pl_off[s + 1] - h_pl_off[s];
+            nb[s] = (n + 63) / 64;
+            woff[s] = w;
+            w += (int64_t)n * nb[s];
+        }
+        pp_words = w;
+        if (own2(&pp.words, w) || own2(&pp.wpre, w) || own2(&pp.woff, nsys) || own2(&pp.nb, nsys) ||
+            own2(&pp.rowcnt, npl) || own2(&pp.rowstart, npl) || own2(&pp.colcnt, npl) || own2(&pp.colstart, npl))
+            return SWG_ERR_HIP;
+        HIPCHK(hipMemcpy(pp.woff, woff.data(), nsys * sizeof(int64_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(pp.nb, nb.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    {
+        std::vector<int64_t> woff(nsys);
+        std::vector<int32_t> nb(nsys);
+        int64_t w = 0;
+        for (int s = 0; s < nsys; s++) {
+            int n = h_pl_off[s + 1] - h_pl_off[s], m = h_tp_off[s + 1] - h_tp_off[s];
+            nb[s] = (m + 63) / 64;
+            woff[s] = w;
+            w += (int64_t)n * nb[s];
+        }
+        pt_words = w;
+        if (own2(&pt.woff, nsys) || own2(&pt.nb, nsys) || own2(&pt.rowcnt, npl) || own2(&pt.rowstart, npl) ||
+            own2(&pt.colcnt, ntp) || own2(&pt.colstart, ntp))
+            return SWG_ERR_HIP;
+        HIPCHK(hipMemcpy(pt.woff, woff.data(), nsys * sizeof(int64_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(pt.nb, nb.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    int mx = std::max(npl, ntp);
+    HIPCHK(hipMalloc(&d_scan, scan_scratch_bytes(mx)));
+    owned.push_back(d_scan);
+    HIPCHK(hipEventCreate(&ev0));
+    HIPCHK(hipEventCreate(&ev1));
+
+    // ---- device part of the initialize sequence: vh->vb, tp vb = vh + cb%vb ----
+    if (prm.integrator == SWG_WHM) {
+        if (int rc = whm_setup()) return rc;
+    } else {
+        launch_snapshot(st, pl, tp, sy, nsys, npl, 0);
+        launch_init_tp_vb(st, tp, sy, ntp);
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipGetLastError());
+    st_.max_depth = -1;
+    return 0;
+}
+
+template <class T>
+int Ensemble::own2(T **p, size_t n)
+{
+    HIPCHK(dalloc(p, n));
+    return 0;
+}
+
+void Ensemble::build_tiles()
+{
+    // kick tiles: blocks of tuning.kick_*_block bodies of one system
+    std::vector<Tile> tpl, ttp;
+    for (int s = 0; s < nsys; s++) {
+        for (int b = h_pl_off[s]; b < h_pl_off[s + 1]; b += tune.kick_pl_block)
+            tpl.push_back(Tile{s, b, std::min(b + tune.kick_pl_block, h_pl_off[s + 1]), 0});
+        for (int b = h_tp_off[s]; b < h_tp_off[s + 1]; b += tune.kick_tp_block)
+            ttp.push_back(Tile{s, b, std::min(b + tune.kick_tp_block, h_tp_off[s + 1]), 0});
+    }
+    std::vector<int4> tpp;
+    for (int s = 0; s < nsys; s++) {
+        int n = h_pl_off[s + 1] - h_pl_off[s], nb = (n + 63) / 64;
+        for (int ib = 0; ib < nb; ib++)
+            for (int jb = ib; jb < nb; jb++) tpp.push_back(make_int4(s, ib, jb, 0));
+    }
+    auto put = [&](auto &vec, auto **dptr, int &cnt) {
+        using T = typename std::remove_reference<decltype(vec)>::type::value_type;
+        if (*dptr) (void)hipFree(*dptr);
+        *dptr = nullptr;
+        cnt = (int)vec.size();
+        if (hipMalloc((void **)dptr, sizeof(T) * (vec.size() ? vec.size() : 1)) == hipSuccess && cnt)
+            (void)hipMemcpy(*dptr, vec.data(), sizeof(T) * vec.size(), hipMemcpyHostToDevice);
+    };
+    put(tpl, &d_tiles_pl, ntiles_pl);
+    put(ttp, &d_tiles_tp, ntiles_tp);
+    put(tpp, &d_tiles_pp, ntiles_pp);
+    if (!d_tiles_pt) {
+        int total = 0;
+        for (int s = 0; s < nsys; s++) total += (h_tp_off[s + 1] - h_tp_off[s] + 63) / 64;
+        (void)hipMalloc((void **)&d_tiles_pt, sizeof(int4) * (total ? total : 1));
+    }
+}
+
+int Ensemble::ensure_list(ListDev &L, int64_t n, int64_t &cap)
+{
+    if (n <= cap) return 0;
+    int64_t c = std::max<int64_t>(n + n / 2, 1024);
+    for (void *q : {(void *)L.i1, (void *)L.i2, (void *)L.lvdotr, (void *)L.status, (void *)L.level, (void *)L.col})
+        if (q) (void)hipFree(q);
+    HIPCHK(dalloc(&L.i1, c));
+    HIPCHK(dalloc(&L.i2, c));
+    HIPCHK(dalloc(&L.lvdotr, c));
+    HIPCHK(dalloc(&L.status, c));
+    HIPCHK(dalloc(&L.level, c));
+    HIPCHK(dalloc(&L.col, c));
+    cap = c;
+    return 0;
+}
+
+// Level-0 pl-pl detection: bit matrix, row counts, scan, ordered fill.
+int Ensemble::detect_pp(double dt)
+{
+    hipStream_t st = stream;
+    launch_pp_tiles(st, pl, sy, pp, d_tiles_pp, ntiles_pp, dt);
+    launch_rows_count(st, pp, pl.sys, 1, sy.pl_off, nullptr, npl);
+    launch_scan_i32_i64(st, pp.rowcnt, pp.rowstart, npl, d_total, d_scan);
+    launch_sys_flags(st, pp, sy.pl_off, sy.enc_pp, nsys);
+    int64_t *ht = (int64_t *)h_small;
+    int32_t *hf = (int32_t *)(h_small + 64);
+    HIPCHK(hipMemcpyAsync(ht, d_total, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hf, sy.enc_pp, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hf + nsys, sy.err, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    npp = npl ? ht[0] : 0;
+    h_enc_pp.assign(hf, hf + nsys);
+    h_err.assign(hf + nsys, hf + 2 * nsys);
+    if (int rc = ensure_list(pp, npp, cap_pp)) return rc;
+    if (npp)
+        launch_rows_fill(st, pp, pl.sys, 1, sy.pl_off, sy.pl_off, npl, pl.levelg, pl.levelm, pl.levelg, pl.levelm);
+    return 0;
+}
+
+// Level-0 pl-tp detection for the systems in `run`.
+int Ensemble::detect_pt(double dt, const std::vector<int32_t> &run)
+{
+    hipStream_t st = stream;
+    npt = 0;
+    std::vector<int4> tiles;
+    for (int s = 0; s < nsys; s++) {
+        if (!run[s]) continue;
+        int m = h_tp_off[s + 1] - h_tp_off[s];
+        for (int tb = 0; tb < (m + 63) / 64; tb++) tiles.push_back(make_int4(s, tb, 0, 0));
+    }
+    if (tiles.empty()) {
+        HIPCHK(hipMemsetAsync(sy.enc_pt, 0, nsys * sizeof(int32_t), st));
+        return 0;
+    }
+    if (!pt.words) {
+        HIPCHK(dalloc(&pt.words, pt_words));
+        HIPCHK(dalloc(&pt.wpre, pt_words));
+    }
+    HIPCHK(hipMemcpyAsync(d_tiles_pt, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_pt_run, run.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    launch_pt_tiles(st, pl, tp, sy, pt, d_tiles_pt, (int)tiles.size(), dt);
+    launch_rows_count(st, pt, pl.sys, 0, sy.pl_off, d_pt_run, npl);
+    launch_scan_i32_i64(st, pt.rowcnt, pt.rowstart, npl, d_total, d_scan);
+    launch_sys_flags(st, pt, sy.pl_off, sy.enc_pt, nsys);
+    int64_t *ht = (int64_t *)h_small;
+    int32_t *hf = (int32_t *)(h_small + 64);
+    HIPCHK(hipMemcpyAsync(ht, d_total, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hf, sy.enc_pt, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    npt = npl ? ht[0] : 0;
+    h_enc_pt.assign(hf, hf + nsys);
+    if (int rc = ensure_list(pt, npt, cap_pt)) return rc;
+    if (npt)
+        launch_rows_fill(st, pt, pl.sys, 0, sy.pl_off, sy.tp_off, npl, pl.levelg, pl.levelm, tp.levelg, tp.levelm);
+    return 0;
+}
+
+int Ensemble::build_adjacency()
+{
+    hipStream_t st = stream;
+    if (npp) {
+        launch_col_count(st, pp, sy.pl_off, pl.sys, sy.pl_off, nullptr, npl, 1);
+        launch_scan_i32_i64(st, pp.colcnt, pp.colstart, npl, nullptr, d_scan);
+        launch_col_fill(st, pp, sy.pl_off, pl.sys, sy.pl_off, nullptr, npl, 1);
+    }
+    if (npt) {
+        launch_col_count(st, pt, sy.pl_off, tp.sys, sy.tp_off, d_pt_run, ntp, 0);
+        launch_scan_i32_i64(st, pt.colcnt, pt.colstart, ntp, nullptr, d_scan);
+        launch_col_fill(st, pt, sy.pl_off, tp.sys, sy.tp_off, d_pt_run, ntp, 0);
+    }
+    return 0;
+}
+
+int Ensemble::read_any(int32_t &v)
+{
+    int32_t *h = (int32_t *)(h_small + 32);
+    HIPCHK(hipMemcpyAsync(h, d_any, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    v = *h;
+    return 0;
+}
+
+// symba_step_recur_system (symba_step.f90:163-262)
+int Ensemble::recur(int ireci)
+{
+    hipStream_t st = stream;
+    const double dt = prm.dt;
+    double p3 = 1.0;
+    for (int q = 0; q < ireci; q++) p3 *= NTENC;
+    const double dtl = dt / p3, dth = 0.5 * dtl;
+    if (dtl / dt < DBL_EPSILON) return fail(SWG_ERR_DEPTH, "recursion step below double precision");
+    if (ireci > st_.max_depth) st_.max_depth = ireci;
+    const int irecp = ireci + 1;
+    const int nloops = ireci == 0 ? 1 : NTENC;
+    for (int j = 1; j <= nloops; j++) {
+        st_.nsubsteps++;
+        launch_clear_flags(st, sy, nsys, d_any);
+        launch_list_check_pp(st, pp, pl, sy, npp, irecp, dtl, d_any);
+        launch_list_check_pt(st, pt, pl, tp, sy, npt, irecp, dtl, prm.short_circuit, d_any);
+        int32_t lenc = 0;
+        if (int rc = read_any(lenc)) return rc;
+        // the adjacency arrays of a list exist only when the list is non-empty
+        auto kicks = [&]() {
+            if (npp) launch_kick_list_pl(st, pp, pl, npl, irecp, 1, dth);
+            if (npt) launch_kick_list_tp(st, pt, pl, tp, ntp, irecp, 1, dth);
+            if (ireci != 0) {
+                if (npp) launch_kick_list_pl(st, pp, pl, npl, irecp, -1, dth);
+                if (npt) launch_kick_list_tp(st, pt, pl, tp, ntp, irecp, -1, dth);
+            }
+        };
+        kicks();
+        launch_drift(st, pl.rh, pl.vb, pl.status, pl.levelg, pl.sys, sy.GMcb, sy.err, npl, dtl, ireci, 1, 0);
+        launch_drift(st, tp.rh, tp.vb, tp.status, tp.levelg, tp.sys, sy.GMcb, sy.err, ntp, dtl, ireci, 1, 1);
+        if (lenc) {
+            if (int rc = recur(irecp)) return rc;
+        }
+        kicks();
+        if (prm.lclose) launch_collision(st, pp, pt, pl, tp, sy, npp, npt, ireci, dtl, d_any);
+        launch_set_levels(st, pl, tp, pp, pt, npl, ntp, npp, npt, ireci);
+    }
+    return 0;
+}
+
+int Ensemble::check_errors()
+{
+    int32_t *hf = (int32_t *)(h_small + 64);
+    HIPCHK(hipMemcpyAsync(hf, sy.err, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    h_err.assign(hf, hf + nsys);
+    return err_code();
+}
+
+int Ensemble::err_code()
+{
+    for (int s = 0; s < nsys; s++) {
+        if (h_err[s] & 1) return fail(SWG_ERR_COLLISION, "collision detected in system " + std::to_string(s));
+        if (h_err[s] & 2) return fail(SWG_ERR_DRIFT, "massive body drift failure in system " + std::to_string(s));
+    }
+    return 0;
+}
+
+// Phase trace for diagnosis (SWG_TRACE=1 in the environment): synchronizes
+// the stream and prints the time since the previous mark to stderr.
+static bool trace_on()
+{
+    static int on = -1;
+    if (on < 0) {
+        const char *e = getenv("SWG_TRACE");
+        on = (e && e[0] == '1') ? 1 : 0;
+    }
+    return on == 1;
+}
+
+static void trace_mark(hipStream_t st, const char *what)
+{
+    static std::chrono::steady_clock::time_point last = std::chrono::steady_clock::now();
+    if (!trace_on()) return;
+    (void)hipStreamSynchronize(st);
+    auto now = std::chrono::steady_clock::now();
+    fprintf(stderr, "[swg] %-18s %10.3f ms\n", what, std::chrono::duration<double, std::milli>(now - last).count());
+    last = now;
+}
+
+// One SyMBA / HELIO step of every system.
+int Ensemble::step_helio_symba()
+{
+    hipStream_t st = stream;
+    trace_mark(st, "step start");
+    const double dt = prm.dt, dth = 0.5 * dt;
+    const bool symba = prm.integrator == SWG_SYMBA;
+    st_.nplpl_enc = 0;
+    st_.npltp_enc = 0;
+    st_.max_depth = -1;
+    st_.nsubsteps = 0;
+    launch_reset_levels(st, pl, tp, npl, ntp);
+    npp = npt = 0;
+    bool any = false;
+    if (symba) {
+        if (int rc = detect_pp(dt)) return rc;
+        trace_mark(st, "detect pl-pl");
+        if (int rc = err_code()) return rc;
+        std::vector<int32_t> run(nsys, 0);
+        bool anyrun = false;
+        for (int s = 0; s < nsys; s++) {
+            run[s] = (h_tp_off[s + 1] > h_tp_off[s]) && (h_pl_off[s + 1] > h_pl_off[s]) &&
+                     !(prm.short_circuit && h_enc_pp[s]);
+            anyrun |= run[s] != 0;
+        }
+        h_enc_pt.assign(nsys, 0);
+        if (anyrun) {
+            if (int rc = detect_pt(dt, run)) return rc;
+        } else {
+            HIPCHK(hipMemsetAsync(sy.enc_pt, 0, nsys * sizeof(int32_t), st));
+        }
+        for (int s = 0; s < nsys; s++) any |= h_enc_pp[s] || h_enc_pt[s];
+        st_.nplpl_enc = npp;
+        st_.npltp_enc = npt;
+        if (any) build_adjacency();
+        trace_mark(st, "detect pl-tp+adj");
+    } else {
+        HIPCHK(hipMemsetAsync(sy.enc_pp, 0, nsys * sizeof(int32_t), st));
+        HIPCHK(hipMemsetAsync(sy.enc_pt, 0, nsys * sizeof(int32_t), st));
+    }
+    const int use_radius = symba || prm.lclose;
+    auto kick_pl = [&](int lbeg) -> int {
+        if (ktiming && ntiles_pl)
+            if (int rc = record_kernel_event(ev_pl, nev_pl)) return rc;
+        launch_kick_pl(st, tune, pl, sy, pp, d_tiles_pl, ntiles_pl, dth, lbeg, use_radius, npp > 0);
+        if (ktiming && ntiles_pl)
+            if (int rc = record_kernel_event(ev_pl, nev_pl)) return rc;
+        return 0;
+    };
+    auto kick_tp = [&](int lbeg) -> int {
+        if (ktiming && ntiles_tp)
+            if (int rc = record_kernel_event(ev_tp, nev_tp)) return rc;
+        launch_kick_tp(st, tune, tp, pl, sy, pt, d_tiles_tp, ntiles_tp, dth, lbeg, npt > 0);
+        if (ktiming && ntiles_tp)
+            if (int rc = record_kernel_event(ev_tp, nev_tp)) return rc;
+        return 0;
+    };
+    // first half: pl then tp
+    launch_sums_begin(st, pl, sy, nsys, 0);
+    launch_pl_begin_apply(st, pl, sy, npl, dth, 0);
+    if (int rc = kick_pl(1)) return rc;
+    launch_drift(st, pl.rh, pl.vb, pl.status, pl.levelg, pl.sys, sy.GMcb, sy.err, npl, dt, -1, symba, 0);
+    launch_tp_pre(st, tp, sy, ntp, dth);
+    if (int rc = kick_tp(1)) return rc;
+    launch_drift(st, tp.rh, tp.vb, tp.status, tp.levelg, tp.sys, sy.GMcb, sy.err, ntp, dt, -1, symba, 1);
+    trace_mark(st, "first half");
+    if (any) {
+        if (int rc = recur(0)) return rc;
+        trace_mark(st, "recursion");
+    }
+    // second half
+    if (int rc = kick_pl(0)) return rc;
+    launch_sums_end(st, pl, sy, nsys);
+    launch_pl_end_apply(st, pl, sy, npl, dth);
+    if (int rc = kick_tp(0)) return rc;
+    launch_tp_post(st, tp, sy, ntp, dth);
+    launch_lfirst_update(st, sy, nsys);
+    trace_mark(st, "second half");
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int Ensemble::snapshot()
+{
+    hipStream_t st = stream;
+    if (prm.integrator == SWG_WHM) return whm_snapshot();
+    launch_snapshot(st, pl, tp, sy, nsys, npl, ntp);
+    if (!prm.rhill_present) {
+        std::vector<double> rh(3 * (size_t)npl), vh(3 * (size_t)npl), rhill(npl);
+        if (download_aos(st, pl.rh, rh.data(), npl) || download_aos(st, pl.vh, vh.data(), npl)) return SWG_ERR_HIP;
+        for (int s = 0; s < nsys; s++) {
+            int a = h_pl_off[s], n = h_pl_off[s + 1] - a;
+            host_set_rhill(h_GMcb[s], n, &rh[3 * (size_t)a], &vh[3 * (size_t)a], &h_Gm[a], &rhill[a]);
+        }
+        HIPCHK(hipMemcpyAsync(pl.rhill, rhill.data(), npl * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    return 0;
+}
+
+int Ensemble::step()
+{
+    int rc = prm.integrator == SWG_WHM ? step_whm() : step_helio_symba();
+    if (rc) return rc;
+    iloop++;
+    return 0;
+}
+
+int Ensemble::run(int64_t nsteps)
+{
+    if (iloop == 0 && prm.istep_out > 0 && !initial_snapshot_done) {
+        if (int rc = snapshot()) return rc;
+        initial_snapshot_done = true;
+    }
+    for (int64_t k = 0; k < nsteps; k++) {
+        if (int rc = step()) return rc;
+        if (prm.istep_out > 0) {
+            iout++;
+            if (iout == prm.istep_out) {
+                iout = 0;
+                if (int rc = snapshot()) return rc;
+            }
+        }
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    return check_errors();
+}
+
+int Ensemble::get(double *pl_rh, double *pl_vh, double *pl_vb, double *tp_rh, double *tp_vh, double *tp_vb,
+                  int32_t *pl_status, int32_t *tp_status, int32_t *pl_id)
+{
+    hipStream_t st = stream;
+    HIPCHK(hipStreamSynchronize(st));
+    if (pl_rh && download_aos(st, pl.rh, pl_rh, npl)) return SWG_ERR_HIP;
+    if (pl_vh && download_aos(st, pl.vh, pl_vh, npl)) return SWG_ERR_HIP;
+    if (pl_vb && download_aos(st, pl.vb, pl_vb, npl)) return SWG_ERR_HIP;
+    if (tp_rh && download_aos(st, tp.rh, tp_rh, ntp)) return SWG_ERR_HIP;
+    if (tp_vh && download_aos(st, tp.vh, tp_vh, ntp)) return SWG_ERR_HIP;
+    if (tp_vb && download_aos(st, tp.vb, tp_vb, ntp)) return SWG_ERR_HIP;
+    if (pl_status) HIPCHK(hipMemcpy(pl_status, pl.status, npl * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (tp_status) HIPCHK(hipMemcpy(tp_status, tp.status, ntp * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (pl_id) std::copy(h_pl_id.begin(), h_pl_id.end(), pl_id);
+    return 0;
+}
+
+int Ensemble::get_misc(double *rhill, double *cb_vb, double *ptbeg, double *ptend, int32_t *lfirst)
+{
+    HIPCHK(hipStreamSynchronize(stream));
+    if (rhill) HIPCHK(hipMemcpy(rhill, pl.rhill, npl * sizeof(double), hipMemcpyDeviceToHost));
+    if (cb_vb) HIPCHK(hipMemcpy(cb_vb, sy.cbvb, 3 * nsys * sizeof(double), hipMemcpyDeviceToHost));
+    if (ptbeg) HIPCHK(hipMemcpy(ptbeg, sy.ptbeg, 3 * nsys * sizeof(double), hipMemcpyDeviceToHost));
+    if (ptend) HIPCHK(hipMemcpy(ptend, sy.ptend, 3 * nsys * sizeof(double), hipMemcpyDeviceToHost));
+    if (lfirst) HIPCHK(hipMemcpy(lfirst, sy.lfirst, nsys * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int Ensemble::enc_list(int which, int64_t *n, int32_t *i1, int32_t *i2)
+{
+    ListDev &L = which == 0 ? pp : pt;
+    int64_t m = which == 0 ? npp : npt;
+    *n = m;
+    if (!i1 || !m) return 0;
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy(i1, L.i1, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(i2, L.i2, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < m; k++) {
+        i1[k] += 1;
+        i2[k] += 1;
+    }
+    return 0;
+}
+
+int Ensemble::record_kernel_event(std::vector<hipEvent_t> &pool, size_t &n)
+{
+    if (n == pool.size()) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        pool.push_back(e);
+    }
+    HIPCHK(hipEventRecord(pool[n], stream));
+    n++;
+    return 0;
+}
+
+// average duration of the recorded launches (start/stop event pairs)
+int Ensemble::kernel_times(double *ms_tp, int64_t *n_tp, double *ms_pl, int64_t *n_pl)
+{
+    HIPCHK(hipStreamSynchronize(stream));
+    auto avg = [&](std::vector<hipEvent_t> &pool, size_t n, double *ms, int64_t *cnt) -> int {
+        double tot = 0.0;
+        int64_t k = 0;
+        for (size_t q = 0; q + 1 < n; q += 2) {
+            float e = 0.f;
+            HIPCHK(hipEventElapsedTime(&e, pool[q], pool[q + 1]));
+            tot += e;
+            k++;
+        }
+        if (ms) *ms = k ? tot / k : 0.0;
+        if (cnt) *cnt = k;
+        return 0;
+    };
+    if (int rc = avg(ev_tp, nev_tp, ms_tp, n_tp)) return rc;
+    if (int rc = avg(ev_pl, nev_pl, ms_pl, n_pl)) return rc;
+    return 0;
+}
+
+int Ensemble::time_kernels(int reps, double *ms_tp, double *ms_pl)
+{
+    // Launch the two acceleration kernels on copies of the velocity state so
+    // that the timing runs leave the system unchanged.
+    hipStream_t st = stream;
+    Vec3p save_pl{}, save_tp{};
+    if (alloc_vec(save_pl, npl) || alloc_vec(save_tp, ntp)) return SWG_ERR_HIP;
+    for (int d = 0; d < 3; d++) {
+        double *src = d == 0 ? pl.vb.x : d == 1 ? pl.vb.y : pl.vb.z;
+        double *dst = d == 0 ? save_pl.x : d == 1 ? save_pl.y : save_pl.z;
+        HIPCHK(hipMemcpyAsync(dst, src, npl * sizeof(double), hipMemcpyDeviceToDevice, st));
+        src = d == 0 ? tp.vb.x : d == 1 ? tp.vb.y : tp.vb.z;
+        dst = d == 0 ? save_tp.x : d == 1 ? save_tp.y : save_tp.z;
+        HIPCHK(hipMemcpyAsync(dst, src, ntp * sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
+    const int use_radius = prm.integrator == SWG_SYMBA || prm.lclose;
+    float ms = 0.f;
+    if (ms_tp) {
+        HIPCHK(hipEventRecord(ev0, st));
+        for (int r = 0; r < reps; r++) launch_kick_tp(st, tune, tp, pl, sy, pt, d_tiles_tp, ntiles_tp, 0.0, 1, 0);
+        HIPCHK(hipEventRecord(ev1, st));
+        HIPCHK(hipEventSynchronize(ev1));
+        HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+        *ms_tp = ms / reps;
+    }
+    if (ms_pl) {
+        HIPCHK(hipEventRecord(ev0, st));
+        for (int r = 0; r < reps; r++)
+            launch_kick_pl(st, tune, pl, sy, pp, d_tiles_pl, ntiles_pl, 0.0, 1, use_radius, 0);
+        HIPCHK(hipEventRecord(ev1, st));
+        HIPCHK(hipEventSynchronize(ev1));
+        HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+        *ms_pl = ms / reps;
+    }
+    for (int d = 0; d < 3; d++) {
+        double *dst = d == 0 ? pl.vb.x : d == 1 ? pl.vb.y : pl.vb.z;
+        double *src = d == 0 ? save_pl.x : d == 1 ? save_pl.y : save_pl.z;
+        HIPCHK(hipMemcpyAsync(dst, src, npl * sizeof(double), hipMemcpyDeviceToDevice, st));
+        dst = d == 0 ? tp.vb.x : d == 1 ? tp.vb.y : tp.vb.z;
+        src = d == 0 ? save_tp.x : d == 1 ? save_tp.y : save_tp.z;
+        HIPCHK(hipMemcpyAsync(dst, src, ntp * sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace swg
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace swg;
+
+struct swg_system {
+    swg::Ensemble *e;
+};
+
+extern "C" {
+
+const char *swg_last_error(void) { return g_err.c_str(); }
+
+int swg_init(int device, swg_ctx **ctx)
+{
+    if (!ctx) return fail(SWG_ERR_ARG, "ctx is NULL");
+    *ctx = nullptr;
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(SWG_ERR_ARG, "no such device");
+    HIPCHK(hipSetDevice(device));
+    swg_ctx *c = new swg_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return hipfail(e, "hipStreamCreate");
+    }
+    c->leaf = new swg::LeafState();
+    *ctx = c;
+    return 0;
+}
+
+int swg_finalize(swg_ctx *ctx)
+{
+    if (!ctx) return 0;
+    (void)hipSetDevice(ctx->device);
+    delete ctx->leaf;
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return 0;
+}
+
+int swg_system_create(swg_ctx *ctx, const swg_params *prm, int32_t nsys, const int32_t *npl, const int32_t *ntp,
+                      const double *GMcb, const double *pl_rh, const double *pl_vh, const double *Gmass,
+                      const double *radius, const double *rhill, const double *tp_rh, const double *tp_vh,
+                      swg_system **sys)
+{
+    if (!ctx || !prm || !sys || !npl || !ntp || !GMcb) return fail(SWG_ERR_ARG, "NULL argument");
+    *sys = nullptr;
+    HIPCHK(hipSetDevice(ctx->device));
+    swg::Ensemble *e = new swg::Ensemble();
+    e->stream = ctx->stream;
+    int rc = e->create(*prm, nsys, npl, ntp, GMcb, pl_rh, pl_vh, Gmass, radius, rhill, tp_rh, tp_vh);
+    if (rc) {
+        delete e;
+        return rc;
+    }
+    *sys = new swg_system{e};
+    return 0;
+}
+
+int swg_system_run(swg_system *sys, int64_t nsteps)
+{
+    if (!sys || nsteps < 0) return fail(SWG_ERR_ARG, "bad argument");
+    return sys->e->run(nsteps);
+}
+
+int swg_system_step(swg_system *sys)
+{
+    if (!sys) return fail(SWG_ERR_ARG, "NULL system");
+    int rc = sys->e->step();
+    if (rc) return rc;
+    return sys->e->check_errors();
+}
+
+int swg_system_get(swg_system *sys, double *pl_rh, double *pl_vh, double *pl_vb, double *tp_rh, double *tp_vh,
+                   double *tp_vb, int32_t *pl_status, int32_t *tp_status, int32_t *pl_id)
+{
+    if (!sys) return fail(SWG_ERR_ARG, "NULL system");
+    return sys->e->get(pl_rh, pl_vh, pl_vb, tp_rh, tp_vh, tp_vb, pl_status, tp_status, pl_id);
+}
+
+int swg_system_get_misc(swg_system *sys, double *rhill, double *cb_vb, double *ptbeg, double *ptend, int32_t *lfirst)
+{
+    if (!sys) return fail(SWG_ERR_ARG, "NULL system");
+    return sys->e->get_misc(rhill, cb_vb, ptbeg, ptend, lfirst);
+}
+
+int swg_system_stats(swg_system *sys, swg_stats *st)
+{
+    if (!sys || !st) return fail(SWG_ERR_ARG, "NULL argument");
+    *st = sys->e->st_;
+    return 0;
+}
+
+int swg_system_enc_list(swg_system *sys, int32_t which, int64_t *n, int32_t *index1, int32_t *index2)
+{
+    if (!sys || !n || (which != 0 && which != 1)) return fail(SWG_ERR_ARG, "bad argument");
+    return sys->e->enc_list(which, n, index1, index2);
+}
+
+int swg_system_time_kernels(swg_system *sys, int32_t reps, double *ms_kick_tp, double *ms_kick_pl)
+{
+    if (!sys || reps <= 0) return fail(SWG_ERR_ARG, "bad argument");
+    return sys->e->time_kernels(reps, ms_kick_tp, ms_kick_pl);
+}
+
+int swg_system_kernel_timing(swg_system *sys, int32_t enable)
+{
+    if (!sys) return fail(SWG_ERR_ARG, "NULL system");
+    HIPCHK(hipStreamSynchronize(sys->e->stream));
+    sys->e->ktiming = enable != 0;
+    sys->e->nev_tp = sys->e->nev_pl = 0;
+    return 0;
+}
+
+int swg_system_kernel_times(swg_system *sys, double *ms_kick_tp, int64_t *n_kick_tp, double *ms_kick_pl,
+                            int64_t *n_kick_pl)
+{
+    if (!sys) return fail(SWG_ERR_ARG, "NULL system");
+    return sys->e->kernel_times(ms_kick_tp, n_kick_tp, ms_kick_pl, n_kick_pl);
+}
+
+int swg_system_destroy(swg_system *sys)
+{
+    if (!sys) return 0;
+    delete sys->e;
+    delete sys;
+    return 0;
+}
+
+}  // extern "C"

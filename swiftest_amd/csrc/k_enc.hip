@@ -1,0 +1,646 @@
+// k_enc.hip -- close-encounter detection and the SyMBA recursion kernels.
+//
+// Level-0 detection (encounter_check.f90:436-570) is a brute-force sweep of
+// every pair with the exact narrow-phase test (encounter_check_one :573-621).
+// A wavefront owns 64 candidate partners; for each body of the row block it
+// evaluates the 64 tests and __ballot() packs them into one 64-bit word of a
+// per-system bit matrix.  Rows are then counted (popcount + a wave scan that
+// also records each word's prefix), scanned, and filled, which yields the
+// reference's list order (index1 ascending, then index2) with no sort.  The
+// same bit matrix gives every body its index2-side adjacency in list order.
+//
+// Recursion kernels restate symba_encounter_check_list_* (symba_encounter_
+// check.f90:90-235), symba_kick_list_* (symba_kick.f90:126-340),
+// collision_check_one (collision_check.f90:16-50) and the level bookkeeping
+// of symba_step.f90:118-160.  The shell kicks are computed per body by
+// gathering that body's pairs in list order, which reproduces the reference's
+// serial accumulation exactly.
+#include "swg_kernels.h"
+#include "swg_math.h"
+
+namespace swg {
+
+static inline int cdiv(int64_t a, int b) { return (int)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// pl-pl pair tiles: tile (s, ib, jb), jb >= ib, one wavefront each
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_pp_tiles(PlDev pl, SysDev sy, ListDev pp, const int4 *tiles, double dt)
+{
+    __shared__ double si[7][64];
+    const int4 t = tiles[blockIdx.x];
+    const int s = t.x, ib = t.y, jb = t.z;
+    const int p0 = sy.pl_off[s], n = sy.pl_off[s + 1] - p0, nb = pp.nb[s];
+    const int lane = threadIdx.x;
+    const int il0 = ib * 64 + lane;
+    if (il0 < n) {
+        int g = p0 + il0;
+        si[0][lane] = pl.rh.x[g];
+        si[1][lane] = pl.rh.y[g];
+        si[2][lane] = pl.rh.z[g];
+        si[3][lane] = pl.vb.x[g];
+        si[4][lane] = pl.vb.y[g];
+        si[5][lane] = pl.vb.z[g];
+        si[6][lane] = pl.renc[g];
+    }
+    const int jl = jb * 64 + lane;
+    const bool vj = jl < n;
+    double xj = 0, yj = 0, zj = 0, vxj = 0, vyj = 0, vzj = 0, rj = 0;
+    if (vj) {
+        int g = p0 + jl;
+        xj = pl.rh.x[g];
+        yj = pl.rh.y[g];
+        zj = pl.rh.z[g];
+        vxj = pl.vb.x[g];
+        vyj = pl.vb.y[g];
+        vzj = pl.vb.z[g];
+        rj = pl.renc[g];
+    }
+    __syncthreads();
+    uint64_t *wrow = pp.words + pp.woff[s] + jb;
+    const int imax = min(64, n - ib * 64);
+    for (int q = 0; q < imax; q++) {
+        const int il = ib * 64 + q;
+        bool f = false;
+        if (vj && jl > il) {
+            bool lv;
+            f = encounter_one(xj - si[0][q], yj - si[1][q], zj - si[2][q], vxj - si[3][q], vyj - si[4][q],
+                              vzj - si[5][q], si[6][q] + rj, dt, lv);
+        }
+        uint64_t m = __ballot(f);
+        if (lane == 0) wrow[(int64_t)il * nb] = m;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pl-tp pair tiles: tile (s, tb), one wavefront = 64 particles, all pl rows
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_pt_tiles(PlDev pl, TpDev tp, SysDev sy, ListDev pt, const int4 *tiles,
+                                                 double dt)
+{
+    __shared__ double si[7][64];
+    const int4 t = tiles[blockIdx.x];
+    const int s = t.x, tb = t.y;
+    const int p0 = sy.pl_off[s], npl = sy.pl_off[s + 1] - p0;
+    const int t0 = sy.tp_off[s], ntp = sy.tp_off[s + 1] - t0;
+    const int nb = pt.nb[s];
+    const int lane = threadIdx.x;
+    const int jl = tb * 64 + lane;
+    bool vj = jl < ntp;
+    double xj = 0, yj = 0, zj = 0, vxj = 0, vyj = 0, vzj = 0;
+    if (vj) {
+        int g = t0 + jl;
+        vj = tp.status[g] == ACTIVE;
+        xj = tp.rh.x[g];
+        yj = tp.rh.y[g];
+        zj = tp.rh.z[g];
+        vxj = tp.vb.x[g];
+        vyj = tp.vb.y[g];
+        vzj = tp.vb.z[g];
+    }
+    uint64_t *wcol = pt.words + pt.woff[s] + tb;
+    for (int ib = 0; ib < npl; ib += 64) {
+        __syncthreads();
+        if (ib + lane < npl) {
+            int g = p0 + ib + lane;
+            si[0][lane] = pl.rh.x[g];
+            si[1][lane] = pl.rh.y[g];
+            si[2][lane] = pl.rh.z[g];
+            si[3][lane] = pl.vb.x[g];
+            si[4][lane] = pl.vb.y[g];
+            si[5][lane] = pl.vb.z[g];
+            si[6][lane] = pl.renc[g];
+        }
+        __syncthreads();
+        const int qn = min(64, npl - ib);
+        for (int q = 0; q < qn; q++) {
+            bool f = false;
+            if (vj) {
+                bool lv;
+                f = encounter_one(xj - si[0][q], yj - si[1][q], zj - si[2][q], vxj - si[3][q], vyj - si[4][q],
+                                  vzj - si[5][q], si[6][q] + 0.0, dt, lv);
+            }
+            uint64_t m = __ballot(f);
+            if (lane == 0) wcol[(int64_t)(ib + q) * nb] = m;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// row counts: one wavefront per row; records each word's exclusive prefix
+// ---------------------------------------------------------------------------
+__device__ inline int wave_incl_scan(int v)
+{
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        int u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(64) void k_rows_count(ListDev L, const int32_t *row_sys, int upper,
+                                                   const int32_t *pl_off, const int32_t *run, int nrows)
+{
+    const int row = blockIdx.x;
+    if (row >= nrows) return;
+    const int s = row_sys[row];
+    const int lane = threadIdx.x;
+    if (run && !run[s]) {
+        if (lane == 0) L.rowcnt[row] = 0;
+        return;
+    }
+    const int il = row - pl_off[s];
+    const int nb = L.nb[s];
+    const int b0 = upper ? il / 64 : 0;
+    const int64_t base = L.woff[s] + (int64_t)il * nb;
+    int running = 0;
+    for (int c = b0; c < nb; c += 64) {
+        int b = c + lane;
+        int pc = 0;
+        if (b < nb) pc = __popcll(L.words[base + b]);
+        int incl = wave_incl_scan(pc);
+        if (b < nb) L.wpre[base + b] = running + incl - pc;
+        running += __shfl(incl, 63, 64);
+    }
+    if (lane == 0) L.rowcnt[row] = running;
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan int32 -> int64 (reduce, scan partials, apply)
+// ---------------------------------------------------------------------------
+constexpr int SCAN_T = 256, SCAN_ITEMS = 4, SCAN_CHUNK = SCAN_T * SCAN_ITEMS;
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const int32_t *in, int n, int64_t *part)
+{
+    __shared__ int64_t sh[SCAN_T];
+    int64_t v = 0;
+    int base = blockIdx.x * SCAN_CHUNK;
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        int i = base + k * SCAN_T + threadIdx.x;
+        if (i < n) v += in[i];
+    }
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = SCAN_T / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_partials(int64_t *part, int np, int64_t *total)
+{
+    __shared__ int64_t sh[SCAN_T];
+    int64_t carry = 0;
+    for (int c = 0; c < np; c += SCAN_T) {
+        int i = c + threadIdx.x;
+        int64_t v = i < np ? part[i] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < SCAN_T; o <<= 1) {
+            int64_t u = (int)threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += u;
+            __syncthreads();
+        }
+        if (i < np) part[i] = carry + sh[threadIdx.x] - v;
+        carry += sh[SCAN_T - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_apply(const int32_t *in, int n, const int64_t *part, int64_t *out)
+{
+    __shared__ int64_t sh[SCAN_T];
+    int base = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
+    int64_t loc[SCAN_ITEMS];
+    int64_t sum = 0;
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        int i = base + k;
+        loc[k] = sum;
+        if (i < n) sum += in[i];
+    }
+    sh[threadIdx.x] = sum;
+    __syncthreads();
+    for (int o = 1; o < SCAN_T; o <<= 1) {
+        int64_t u = (int)threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += u;
+        __syncthreads();
+    }
+    int64_t off = part[blockIdx.x] + sh[threadIdx.x] - sum;
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        int i = base + k;
+        if (i < n) out[i] = off + loc[k];
+    }
+}
+
+size_t scan_scratch_bytes(int n) { return sizeof(int64_t) * (size_t)(cdiv(n > 0 ? n : 1, SCAN_CHUNK) + 1); }
+
+void launch_scan_i32_i64(hipStream_t st, const int32_t *in, int64_t *out, int n, int64_t *total, void *scratch)
+{
+    int64_t *part = (int64_t *)scratch;
+    int nb = cdiv(n > 0 ? n : 1, SCAN_CHUNK);
+    k_scan_reduce<<<nb, SCAN_T, 0, st>>>(in, n, part);
+    k_scan_partials<<<1, SCAN_T, 0, st>>>(part, nb, total);
+    if (n) k_scan_apply<<<nb, SCAN_T, 0, st>>>(in, n, part, out);
+}
+
+// ---------------------------------------------------------------------------
+// ordered fill: one wavefront per row, each lane emits the bits of one word
+// ---------------------------------------------------------------------------
+// Also marks both bodies of every listed pair as being at recursion level 0
+// (symba_encounter_check.f90:74-79, :286-290).
+__global__ __launch_bounds__(64) void k_rows_fill(ListDev L, const int32_t *row_sys, int upper,
+                                                  const int32_t *pl_off, const int32_t *col_off, int nrows,
+                                                  int32_t *lev1g, int32_t *lev1m, int32_t *lev2g, int32_t *lev2m)
+{
+    const int row = blockIdx.x;
+    if (row >= nrows) return;
+    const int cnt = L.rowcnt[row];
+    if (cnt == 0) return;
+    const int s = row_sys[row];
+    const int il = row - pl_off[s];
+    const int nb = L.nb[s];
+    const int b0 = upper ? il / 64 : 0;
+    const int64_t base = L.woff[s] + (int64_t)il * nb;
+    const int64_t start = L.rowstart[row];
+    const int c0 = col_off[s];
+    for (int b = b0 + (int)threadIdx.x; b < nb; b += 64) {
+        uint64_t w = L.words[base + b];
+        int64_t pos = start + L.wpre[base + b];
+        while (w) {
+            int bit = __ffsll((unsigned long long)w) - 1;
+            w &= w - 1;
+            const int col = c0 + b * 64 + bit;
+            L.i1[pos] = row;
+            L.i2[pos] = col;
+            if (lev2g) {
+                lev2g[col] = 0;
+                lev2m[col] = 0;
+            }
+            L.lvdotr[pos] = 1;
+            L.status[pos] = (int8_t)ACTIVE;
+            L.level[pos] = 0;
+            pos++;
+        }
+    }
+    if (lev1g && threadIdx.x == 0) {
+        lev1g[row] = 0;
+        lev1m[row] = 0;
+    }
+}
+
+// per-system "any pair listed" flags
+__global__ void k_sys_flags(ListDev L, const int32_t *pl_off, int32_t *flag, int nsys)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    int p0 = pl_off[s], p1 = pl_off[s + 1];
+    int64_t n = 0;
+    if (p1 > p0) n = L.rowstart[p1 - 1] + L.rowcnt[p1 - 1] - L.rowstart[p0];
+    flag[s] = n > 0;
+}
+
+// index2-side adjacency: for column c (a pl body for pl-pl, a particle for
+// pl-tp), the pairs (p, c) in list order, i.e. rows p ascending.
+__global__ void k_col_count(ListDev L, const int32_t *row_off, const int32_t *col_sys, const int32_t *col_off,
+                            const int32_t *run, int ncols, int upper)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    int s = col_sys[c];
+    if (run && !run[s]) {
+        L.colcnt[c] = 0;
+        return;
+    }
+    int cl = c - col_off[s];
+    int nrow = upper ? cl : row_off[s + 1] - row_off[s];
+    int nb = L.nb[s];
+    const uint64_t *w = L.words + L.woff[s] + cl / 64;
+    int bit = cl & 63;
+    int cnt = 0;
+    for (int p = 0; p < nrow; p++) cnt += (int)((w[(int64_t)p * nb] >> bit) & 1ull);
+    L.colcnt[c] = cnt;
+}
+
+__global__ void k_col_fill(ListDev L, const int32_t *row_off, const int32_t *col_sys, const int32_t *col_off,
+                           const int32_t *run, int ncols, int upper)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    if (L.colcnt[c] == 0) return;
+    int s = col_sys[c];
+    if (run && !run[s]) return;
+    int cl = c - col_off[s];
+    int r0 = row_off[s];
+    int nrow = upper ? cl : row_off[s + 1] - r0;
+    int nb = L.nb[s];
+    int64_t wbase = L.woff[s] + cl / 64;
+    int bit = cl & 63;
+    uint64_t lowm = (bit == 0) ? 0ull : ((~0ull) >> (64 - bit));
+    int64_t o = L.colstart[c];
+    for (int p = 0; p < nrow; p++) {
+        uint64_t w = L.words[wbase + (int64_t)p * nb];
+        if ((w >> bit) & 1ull) {
+            int64_t k = L.rowstart[r0 + p] + L.wpre[wbase + (int64_t)p * nb] + __popcll(w & lowm);
+            L.col[o++] = (int32_t)k;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SyMBA recursion
+// ---------------------------------------------------------------------------
+
+// symba_encounter_check_list_plpl (symba_encounter_check.f90:90-160)
+__global__ void k_list_check_pp(ListDev pp, PlDev pl, SysDev sy, int64_t n, int irec, double dt, int32_t *any)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    if (pp.status[k] != ACTIVE || pp.level[k] != irec - 1) return;
+    const int i = pp.i1[k], j = pp.i2[k];
+    double xr = pl.rh.x[j] - pl.rh.x[i], yr = pl.rh.y[j] - pl.rh.y[i], zr = pl.rh.z[j] - pl.rh.z[i];
+    double vxr = pl.vb.x[j] - pl.vb.x[i], vyr = pl.vb.y[j] - pl.vb.y[i], vzr = pl.vb.z[j] - pl.vb.z[i];
+    const double rsh = rshell_pow(irec);
+    double rci = (pl.rhill[i] * RHSCALE) * rsh, rcj = (pl.rhill[j] * RHSCALE) * rsh;
+    bool lv;
+    bool le = encounter_one(xr, yr, zr, vxr, vyr, vzr, rci + rcj, dt, lv);
+    pp.lvdotr[k] = lv;
+    if (le) {
+        double rji2 = dot3(xr, yr, zr, xr, yr, zr);
+        double rs = pl.R[i] + pl.R[j];
+        le = rji2 > rs * rs;
+    }
+    if (le) {
+        pp.level[k] = (int8_t)irec;
+        pl.levelg[i] = irec;
+        pl.levelg[j] = irec;
+        atomicMax(&pl.levelm[i], irec);
+        atomicMax(&pl.levelm[j], irec);
+        sy.flag_pp[pl.sys[i]] = 1;
+        *any = 1;
+    }
+}
+
+// symba_encounter_check_list_pltp (symba_encounter_check.f90:163-235); with
+// short_circuit the check is skipped in systems whose pl-pl check of this
+// substep found an encounter (gfortran evaluates `a .or. b` left to right and
+// stops at the first .true.).
+__global__ void k_list_check_pt(ListDev pt, PlDev pl, TpDev tp, SysDev sy, int64_t n, int irec, double dt,
+                                int short_circuit, int32_t *any)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    if (pt.status[k] != ACTIVE || pt.level[k] != irec - 1) return;
+    const int i = pt.i1[k], j = pt.i2[k];
+    const int s = pl.sys[i];
+    if (short_circuit && sy.flag_pp[s]) return;
+    double xr = tp.rh.x[j] - pl.rh.x[i], yr = tp.rh.y[j] - pl.rh.y[i], zr = tp.rh.z[j] - pl.rh.z[i];
+    double vxr = tp.vb.x[j] - pl.vb.x[i], vyr = tp.vb.y[j] - pl.vb.y[i], vzr = tp.vb.z[j] - pl.vb.z[i];
+    double rci = (pl.rhill[i] * RHSCALE) * rshell_pow(irec);
+    bool lv;
+    bool le = encounter_one(xr, yr, zr, vxr, vyr, vzr, rci, dt, lv);
+    pt.lvdotr[k] = lv;
+    if (le) {
+        double rji2 = dot3(xr, yr, zr, xr, yr, zr);
+        le = rji2 > pl.R[i] * pl.R[i];
+    }
+    if (le) {
+        pt.level[k] = (int8_t)irec;
+        pl.levelg[i] = irec;
+        atomicMax(&pl.levelm[i], irec);
+        tp.levelg[j] = irec;
+        atomicMax(&tp.levelm[j], irec);
+        sy.flag_pt[s] = 1;
+        *any = 1;
+    }
+}
+
+// symba_kick_list_plpl (symba_kick.f90:126-232), gathered per body: pairs
+// with index2 == b (partners p < b) then index1 == b (partners j > b), i.e.
+// the body's pairs in list order.
+__global__ void k_kick_list_pl(ListDev pp, PlDev pl, int npl, int irec, int sgn, double dt)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= npl) return;
+    const int irm1 = irec - 1;
+    if (pl.levelg[b] < irm1) return;
+    const int irecl = sgn < 0 ? irec - 1 : irec;
+    const double shellpow = powi(RSHELL, 2 * irecl);
+    const double xb = pl.rh.x[b], yb = pl.rh.y[b], zb = pl.rh.z[b], hb = pl.rhill[b];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    bool any = false;
+    const int64_t c0 = pp.colstart[b], c1 = c0 + pp.colcnt[b];
+    for (int64_t c = c0; c < c1; c++) {
+        int k = pp.col[c];
+        int p = pp.i1[k];
+        if (pp.status[k] != ACTIVE || pl.levelg[p] < irm1) continue;
+        double sh = pl.rhill[p] + hb;
+        double ri = ((sh * sh) * (RHSCALE * RHSCALE)) * shellpow;
+        double dx = xb - pl.rh.x[p], dy = yb - pl.rh.y[p], dz = zb - pl.rh.z[p];
+        double r2 = dot3(dx, dy, dz, dx, dy, dz);
+        double fac;
+        if (!shell_fac(ri, r2, fac)) continue;
+        any = true;
+        double faci = fac * pl.Gm[p];
+        ax = fma(-faci, dx, ax);
+        ay = fma(-faci, dy, ay);
+        az = fma(-faci, dz, az);
+    }
+    const int64_t r0 = pp.rowstart[b], r1 = r0 + pp.rowcnt[b];
+    for (int64_t k = r0; k < r1; k++) {
+        int j = pp.i2[k];
+        if (pp.status[k] != ACTIVE || pl.levelg[j] < irm1) continue;
+        double sh = hb + pl.rhill[j];
+        double ri = ((sh * sh) * (RHSCALE * RHSCALE)) * shellpow;
+        double dx = pl.rh.x[j] - xb, dy = pl.rh.y[j] - yb, dz = pl.rh.z[j] - zb;
+        double r2 = dot3(dx, dy, dz, dx, dy, dz);
+        double fac;
+        if (!shell_fac(ri, r2, fac)) continue;
+        any = true;
+        double facj = fac * pl.Gm[j];
+        ax = fma(facj, dx, ax);
+        ay = fma(facj, dy, ay);
+        az = fma(facj, dz, az);
+    }
+    if (any) {
+        const double sdt = (double)sgn * dt;
+        pl.vb.x[b] = fma(sdt, ax, pl.vb.x[b]);
+        pl.vb.y[b] = fma(sdt, ay, pl.vb.y[b]);
+        pl.vb.z[b] = fma(sdt, az, pl.vb.z[b]);
+    }
+}
+
+// symba_kick_list_pltp (symba_kick.f90:235-340), gathered per particle
+__global__ void k_kick_list_tp(ListDev pt, PlDev pl, TpDev tp, int ntp, int irec, int sgn, double dt)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ntp) return;
+    const int irm1 = irec - 1;
+    if (tp.levelg[j] < irm1) return;
+    const int64_t c0 = pt.colstart[j], c1 = c0 + pt.colcnt[j];
+    if (c1 == c0) return;
+    const int irecl = sgn < 0 ? irec - 1 : irec;
+    const double shellpow = powi(RSHELL, 2 * irecl);
+    const double xt = tp.rh.x[j], yt = tp.rh.y[j], zt = tp.rh.z[j];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    bool any = false;
+    for (int64_t c = c0; c < c1; c++) {
+        int k = pt.col[c];
+        int i = pt.i1[k];
+        if (pt.status[k] != ACTIVE || pl.levelg[i] < irm1) continue;
+        double hi = pl.rhill[i];
+        double ri = ((hi * hi) * (RHSCALE * RHSCALE)) * shellpow;
+        double dx = xt - pl.rh.x[i], dy = yt - pl.rh.y[i], dz = zt - pl.rh.z[i];
+        double r2 = dot3(dx, dy, dz, dx, dy, dz);
+        double fac;
+        if (!shell_fac(ri, r2, fac)) continue;
+        any = true;
+        double faci = fac * pl.Gm[i];
+        ax = fma(-faci, dx, ax);
+        ay = fma(-faci, dy, ay);
+        az = fma(-faci, dz, az);
+    }
+    if (any) {
+        const double sdt = (double)sgn * dt;
+        tp.vb.x[j] = fma(sdt, ax, tp.vb.x[j]);
+        tp.vb.y[j] = fma(sdt, ay, tp.vb.y[j]);
+        tp.vb.z[j] = fma(sdt, az, tp.vb.z[j]);
+    }
+}
+
+// collision_check_plpl / _pltp detection (collision_check.f90:53-230)
+__global__ void k_collision(ListDev pp, ListDev pt, PlDev pl, TpDev tp, SysDev sy, int64_t npp, int64_t npt,
+                            int irec, double dt, int32_t *any)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < npp) {
+        if (pp.status[k] != ACTIVE) return;
+        int i = pp.i1[k], j = pp.i2[k];
+        if (pl.levelg[i] < irec) return;
+        bool c = collision_one(pl.rh.x[i] - pl.rh.x[j], pl.rh.y[i] - pl.rh.y[j], pl.rh.z[i] - pl.rh.z[j],
+                               pl.vb.x[i] - pl.vb.x[j], pl.vb.y[i] - pl.vb.y[j], pl.vb.z[i] - pl.vb.z[j],
+                               pl.Gm[i] + pl.Gm[j], pl.R[i] + pl.R[j], dt, pp.lvdotr[k] != 0);
+        if (c) {
+            pp.status[k] = (int8_t)COLLIDED;
+            atomicOr(&sy.err[pl.sys[i]], 1);
+            *any = 1;
+        }
+        return;
+    }
+    k -= npp;
+    if (k >= npt) return;
+    if (pt.status[k] != ACTIVE) return;
+    int i = pt.i1[k], j = pt.i2[k];
+    if (tp.levelg[j] < irec) return;
+    bool c = collision_one(pl.rh.x[i] - tp.rh.x[j], pl.rh.y[i] - tp.rh.y[j], pl.rh.z[i] - tp.rh.z[j],
+                           pl.vb.x[i] - tp.vb.x[j], pl.vb.y[i] - tp.vb.y[j], pl.vb.z[i] - tp.vb.z[j], pl.Gm[i],
+                           pl.R[i], dt, pt.lvdotr[k] != 0);
+    if (c) {
+        pt.status[k] = (int8_t)COLLIDED;
+        atomicOr(&sy.err[pl.sys[i]], 1);
+        *any = 1;
+    }
+}
+
+// symba_step_set_recur_levels_system (symba_step.f90:118-160)
+__global__ void k_set_levels(PlDev pl, TpDev tp, ListDev pp, ListDev pt, int npl, int ntp, int64_t npp, int64_t npt,
+                             int ireci)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int irecp = ireci + 1;
+    if (k < npl && pl.levelg[k] == irecp) pl.levelg[k] = ireci;
+    if (k < ntp && tp.levelg[k] == irecp) tp.levelg[k] = ireci;
+    if (k < npp && pp.level[k] == irecp) pp.level[k] = (int8_t)ireci;
+    if (k < npt && pt.level[k] == irecp) pt.level[k] = (int8_t)ireci;
+}
+
+__global__ void k_clear_flags(SysDev sy, int nsys, int32_t *any)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsys) {
+        sy.flag_pp[s] = 0;
+        sy.flag_pt[s] = 0;
+    }
+    if (s == 0 && any) *any = 0;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void launch_pp_tiles(hipStream_t st, PlDev pl, SysDev sy, ListDev pp, const int4 *tiles, int ntiles, double dt)
+{
+    if (ntiles) k_pp_tiles<<<ntiles, 64, 0, st>>>(pl, sy, pp, tiles, dt);
+}
+void launch_pt_tiles(hipStream_t st, PlDev pl, TpDev tp, SysDev sy, ListDev pt, const int4 *tiles, int ntiles,
+                     double dt)
+{
+    if (ntiles) k_pt_tiles<<<ntiles, 64, 0, st>>>(pl, tp, sy, pt, tiles, dt);
+}
+void launch_rows_count(hipStream_t st, ListDev L, const int32_t *row_sys, int upper,
+                       const int32_t *pl_off, const int32_t *run, int nrows)
+{
+    if (nrows) k_rows_count<<<nrows, 64, 0, st>>>(L, row_sys, upper, pl_off, run, nrows);
+}
+void launch_rows_fill(hipStream_t st, ListDev L, const int32_t *row_sys, int upper, const int32_t *pl_off,
+                      const int32_t *col_off, int nrows, int32_t *lev1g, int32_t *lev1m, int32_t *lev2g,
+                      int32_t *lev2m)
+{
+    if (nrows)
+        k_rows_fill<<<nrows, 64, 0, st>>>(L, row_sys, upper, pl_off, col_off, nrows, lev1g, lev1m, lev2g, lev2m);
+}
+void launch_sys_flags(hipStream_t st, ListDev L, const int32_t *pl_off, int32_t *flag, int nsys)
+{
+    k_sys_flags<<<cdiv(nsys, 256), 256, 0, st>>>(L, pl_off, flag, nsys);
+}
+void launch_col_count(hipStream_t st, ListDev L, const int32_t *row_off, const int32_t *col_sys,
+                      const int32_t *col_off, const int32_t *run, int ncols, int upper)
+{
+    if (ncols) k_col_count<<<cdiv(ncols, 64), 64, 0, st>>>(L, row_off, col_sys, col_off, run, ncols, upper);
+}
+void launch_col_fill(hipStream_t st, ListDev L, const int32_t *row_off, const int32_t *col_sys,
+                     const int32_t *col_off, const int32_t *run, int ncols, int upper)
+{
+    if (ncols) k_col_fill<<<cdiv(ncols, 64), 64, 0, st>>>(L, row_off, col_sys, col_off, run, ncols, upper);
+}
+void launch_list_check_pp(hipStream_t st, ListDev pp, PlDev pl, SysDev sy, int64_t n, int irec, double dt,
+                          int32_t *any)
+{
+    if (n) k_list_check_pp<<<cdiv(n, 256), 256, 0, st>>>(pp, pl, sy, n, irec, dt, any);
+}
+void launch_list_check_pt(hipStream_t st, ListDev pt, PlDev pl, TpDev tp, SysDev sy, int64_t n, int irec, double dt,
+                          int short_circuit, int32_t *any)
+{
+    if (n) k_list_check_pt<<<cdiv(n, 256), 256, 0, st>>>(pt, pl, tp, sy, n, irec, dt, short_circuit, any);
+}
+void launch_kick_list_pl(hipStream_t st, ListDev pp, PlDev pl, int npl, int irec, int sgn, double dt)
+{
+    if (npl) k_kick_list_pl<<<cdiv(npl, 128), 128, 0, st>>>(pp, pl, npl, irec, sgn, dt);
+}
+void launch_kick_list_tp(hipStream_t st, ListDev pt, PlDev pl, TpDev tp, int ntp, int irec, int sgn, double dt)
+{
+    if (ntp) k_kick_list_tp<<<cdiv(ntp, 128), 128, 0, st>>>(pt, pl, tp, ntp, irec, sgn, dt);
+}
+void launch_collision(hipStream_t st, ListDev pp, ListDev pt, PlDev pl, TpDev tp, SysDev sy, int64_t npp,
+                      int64_t npt, int irec, double dt, int32_t *any)
+{
+    if (npp + npt) k_collision<<<cdiv(npp + npt, 256), 256, 0, st>>>(pp, pt, pl, tp, sy, npp, npt, irec, dt, any);
+}
+void launch_set_levels(hipStream_t st, PlDev pl, TpDev tp, ListDev pp, ListDev pt, int npl, int ntp, int64_t npp,
+                       int64_t npt, int ireci)
+{
+    int64_t n = npl;
+    if (ntp > n) n = ntp;
+    if (npp > n) n = npp;
+    if (npt > n) n = npt;
+    if (n) k_set_levels<<<cdiv(n, 256), 256, 0, st>>>(pl, tp, pp, pt, npl, ntp, npp, npt, ireci);
+}
+void launch_clear_flags(hipStream_t st, SysDev sy, int nsys, int32_t *any)
+{
+    k_clear_flags<<<cdiv(nsys, 256), 256, 0, st>>>(sy, nsys, any);
+}
+
+}  // namespace swg

@@ -1,0 +1,943 @@
+// k_leaf.hip -- leaf-operator entry points (the reference's numeric leaves
+// with their host-array contract) and the WHM integrator.
+//
+// Leaf operators: swiftest_kick.f90:69-415, swiftest_drift.f90:60-138,
+// encounter_check.f90:14-140.  WHM: whm_step.f90:14-104, whm_kick.f90:14-318,
+// whm_coord.f90:14-124, whm_util.f90:117-228.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "swg_engine.h"
+#include "swg_math.h"
+
+namespace swg {
+
+static inline int cdiv(int64_t a, int b) { return (int)((a + b - 1) / b); }
+
+#define HIPCHK(x)                                    \
+    do {                                             \
+        hipError_t e_ = (x);                         \
+        if (e_ != hipSuccess) return hipfail(e_, #x); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// leaf kernels (AoS in HBM, as the reference lays its (3,n) arrays out)
+// ---------------------------------------------------------------------------
+
+// int_one_pl contribution of pair (i, j) to body b's ahi (b == i) or ahj (b == j)
+__device__ inline void one_pl_half(double rji2, double xr, double yr, double zr, double Gm_other, bool b_is_i,
+                                   double &ax, double &ay, double &az)
+{
+    double irij3 = 1.0 / (rji2 * sqrt(rji2));
+    double fac = Gm_other * irij3;
+    if (b_is_i) {
+        ax = fma(fac, xr, ax);
+        ay = fma(fac, yr, ay);
+        az = fma(fac, zr, az);
+    } else {
+        ax = fma(-fac, xr, ax);
+        ay = fma(-fac, yr, ay);
+        az = fma(-fac, zr, az);
+    }
+}
+
+// swiftest_kick_getacch_int_all_tri_{rad,norad}_pl (swiftest_kick.f90:165-371)
+__global__ void k_leaf_tri(int npl, int nplm, const double *r, const double *Gm, const double *R, double *acc)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= npl) return;
+    const int nplt = npl - nplm;
+    const double xb = r[3 * b], yb = r[3 * b + 1], zb = r[3 * b + 2];
+    if (nplt > nplm) {
+        // triangular loops with the (ahi, ahj) reduction, one-thread order
+        double hix = 0, hiy = 0, hiz = 0, hjx = 0, hjy = 0, hjz = 0;
+        for (int i = 0; i < min(b, nplm); i++) {  // pairs (i, b): b plays j
+            double xr = xb - r[3 * i], yr = yb - r[3 * i + 1], zr = zb - r[3 * i + 2];
+            double rji2 = dot3(xr, yr, zr, xr, yr, zr);
+            if (R) {
+                double s = R[i] + R[b];
+                if (!(rji2 > s * s)) continue;
+            }
+            one_pl_half(rji2, xr, yr, zr, Gm[i], false, hjx, hjy, hjz);
+        }
+        if (b < nplm) {
+            for (int j = b + 1; j < npl; j++) {  // pairs (b, j)
+                double xr = r[3 * j] - xb, yr = r[3 * j + 1] - yb, zr = r[3 * j + 2] - zb;
+                double rji2 = dot3(xr, yr, zr, xr, yr, zr);
+                if (R) {
+                    double s = R[b] + R[j];
+                    if (!(rji2 > s * s)) continue;
+                }
+                one_pl_half(rji2, xr, yr, zr, Gm[j], true, hix, hiy, hiz);
+            }
+        }
+        acc[3 * b] = (acc[3 * b] + hix) + hjx;
+        acc[3 * b + 1] = (acc[3 * b + 1] + hiy) + hjy;
+        acc[3 * b + 2] = (acc[3 * b + 2] + hiz) + hjz;
+        return;
+    }
+    double ax = acc[3 * b], ay = acc[3 * b + 1], az = acc[3 * b + 2];
+    const int jend = b < nplm ? npl : nplm;
+    for (int j = 0; j < jend; j++) {
+        if (j == b) continue;
+        double rx = r[3 * j] - xb, ry = r[3 * j + 1] - yb, rz = r[3 * j + 2] - zb;
+        double rji2 = dot3(rx, ry, rz, rx, ry, rz);
+        if (R) {
+            double s = R[b] + R[j];
+            if (!(rji2 > s * s)) continue;
+        }
+        double fac = Gm[j] / (rji2 * sqrt(rji2));
+        ax = fma(fac, rx, ax);
+        ay = fma(fac, ry, ay);
+        az = fma(fac, rz, az);
+    }
+    acc[3 * b] = ax;
+    acc[3 * b + 1] = ay;
+    acc[3 * b + 2] = az;
+}
+
+// swiftest_kick_getacch_int_all_flat_{rad,norad}_pl (swiftest_kick.f90:69-162)
+// gathered per body: rows = pair ids with index1 == b, cols = index2 == b,
+// each in list order (built on the host).
+__global__ void k_leaf_flat(int npl, const int32_t *k, const double *r, const double *Gm, const double *R,
+                            const int64_t *rs, const int64_t *cs, const int64_t *rowl, const int64_t *coll,
+                            double *acc)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= npl) return;
+    double hix = 0, hiy = 0, hiz = 0, hjx = 0, hjy = 0, hjz = 0;
+    for (int64_t q = rs[b]; q < rs[b + 1]; q++) {
+        int64_t p = rowl[q];
+        int i = k[2 * p] - 1, j = k[2 * p + 1] - 1;
+        double xr = r[3 * j] - r[3 * i], yr = r[3 * j + 1] - r[3 * i + 1], zr = r[3 * j + 2] - r[3 * i + 2];
+        double rji2 = dot3(xr, yr, zr, xr, yr, zr);
+        if (R) {
+            double s = R[i] + R[j];
+            if (!(rji2 > s * s)) continue;
+        }
+        one_pl_half(rji2, xr, yr, zr, Gm[j], true, hix, hiy, hiz);
+    }
+    for (int64_t q = cs[b]; q < cs[b + 1]; q++) {
+        int64_t p = coll[q];
+        int i = k[2 * p] - 1, j = k[2 * p + 1] - 1;
+        double xr = r[3 * j] - r[3 * i], yr = r[3 * j + 1] - r[3 * i + 1], zr = r[3 * j + 2] - r[3 * i + 2];
+        double rji2 = dot3(xr, yr, zr, xr, yr, zr);
+        if (R) {
+            double s = R[i] + R[j];
+            if (!(rji2 > s * s)) continue;
+        }
+        one_pl_half(rji2, xr, yr, zr, Gm[i], false, hjx, hjy, hjz);
+    }
+    acc[3 * b] = (acc[3 * b] + hix) + hjx;
+    acc[3 * b + 1] = (acc[3 * b + 1] + hiy) + hjy;
+    acc[3 * b + 2] = (acc[3 * b + 2] + hiz) + hjz;
+}
+
+// swiftest_kick_getacch_int_all_tp (swiftest_kick.f90:374-415): particle sum
+// from zero over pl ascending, added to acc once (OpenMP reduction).
+template <int BLK>
+__global__ __launch_bounds__(BLK) void k_leaf_tp(int ntp, int npl, const double *rtp, const double *rpl,
+                                                 const double *Gm, const int32_t *lmask, double *acc)
+{
+    __shared__ double sx[BLK], sy[BLK], sz[BLK], sg[BLK];
+    int t = blockIdx.x * BLK + threadIdx.x;
+    bool valid = t < ntp && lmask[t] != 0;
+    double xt = 0, yt = 0, zt = 0;
+    if (valid) {
+        xt = rtp[3 * t];
+        yt = rtp[3 * t + 1];
+        zt = rtp[3 * t + 2];
+    }
+    double px = 0.0, py = 0.0, pz = 0.0;
+    for (int j0 = 0; j0 < npl; j0 += BLK) {
+        int j = j0 + threadIdx.x;
+        if (j < npl) {
+            sx[threadIdx.x] = rpl[3 * j];
+            sy[threadIdx.x] = rpl[3 * j + 1];
+            sz[threadIdx.x] = rpl[3 * j + 2];
+            sg[threadIdx.x] = Gm[j];
+        }
+        __syncthreads();
+        int jn = min(BLK, npl - j0);
+        if (valid) {
+            for (int q = 0; q < jn; q++) {
+                double rx = xt - sx[q], ry = yt - sy[q], rz = zt - sz[q];
+                double rji2 = dot3(rx, ry, rz, rx, ry, rz);
+                double fac = sg[q] / (rji2 * sqrt(rji2));
+                px = fma(-fac, rx, px);
+                py = fma(-fac, ry, py);
+                pz = fma(-fac, rz, pz);
+            }
+        }
+        __syncthreads();
+    }
+    if (!valid) return;
+    acc[3 * t] = acc[3 * t] + px;
+    acc[3 * t + 1] = acc[3 * t + 1] + py;
+    acc[3 * t + 2] = acc[3 * t + 2] + pz;
+}
+
+// swiftest_drift_all (swiftest_drift.f90:60-108), GR off
+__global__ void k_leaf_drift(const double *mu, double *x, double *v, int n, double dt, const int32_t *lmask,
+                             int32_t *iflag)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !lmask[i]) return;
+    double rx = x[3 * i], ry = x[3 * i + 1], rz = x[3 * i + 2], vx = v[3 * i], vy = v[3 * i + 1], vz = v[3 * i + 2];
+    iflag[i] = drift_one(mu[i], rx, ry, rz, vx, vy, vz, dt);
+    x[3 * i] = rx;
+    x[3 * i + 1] = ry;
+    x[3 * i + 2] = rz;
+    v[3 * i] = vx;
+    v[3 * i + 1] = vy;
+    v[3 * i + 2] = vz;
+}
+
+__global__ void k_aos_to_soa(const double *a, Vec3p s, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    s.x[i] = a[3 * i];
+    s.y[i] = a[3 * i + 1];
+    s.z[i] = a[3 * i + 2];
+}
+
+__global__ void k_fill_i32(int32_t *p, int n, int32_t v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_copy_i32_plus1(const int32_t *a, int32_t *b, const int8_t *c, int32_t *cc, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    b[i] = a[i] + 1;
+    if (c) cc[i] = c[i];
+}
+
+// ---------------------------------------------------------------------------
+// WHM (Jacobi) integrator: massive bodies are advanced by one thread per
+// system (the reference's coordinate maps and ah2 are serial recurrences);
+// test particles in parallel.
+// ---------------------------------------------------------------------------
+struct WhmArgs {
+    PlDev pl;
+    SysDev sy;
+    WhmDev w;
+    int32_t *tlfirst;
+    double *ah0;  // 3*nsys
+    int lclose;
+};
+
+// whm_coord_h2j_pl (whm_coord.f90:14-51)
+__device__ void whm_h2j(const WhmArgs &a, int p0, int n)
+{
+    const PlDev &pl = a.pl;
+    const WhmDev &w = a.w;
+    w.xj.x[p0] = pl.rh.x[p0];
+    w.xj.y[p0] = pl.rh.y[p0];
+    w.xj.z[p0] = pl.rh.z[p0];
+    w.vj.x[p0] = pl.vh.x[p0];
+    w.vj.y[p0] = pl.vh.y[p0];
+    w.vj.z[p0] = pl.vh.z[p0];
+    double sx[3] = {0, 0, 0}, sv[3] = {0, 0, 0};
+    for (int q = 1; q < n; q++) {
+        int i = p0 + q;
+        const double *rh[3] = {pl.rh.x, pl.rh.y, pl.rh.z}, *vh[3] = {pl.vh.x, pl.vh.y, pl.vh.z};
+        double *xj[3] = {w.xj.x, w.xj.y, w.xj.z}, *vj[3] = {w.vj.x, w.vj.y, w.vj.z};
+        for (int d = 0; d < 3; d++) {
+            sx[d] = sx[d] + pl.Gm[i - 1] * rh[d][i - 1];
+            sv[d] = sv[d] + pl.Gm[i - 1] * vh[d][i - 1];
+            double cap = sx[d] / w.eta[i - 1], capv = sv[d] / w.eta[i - 1];
+            xj[d][i] = rh[d][i] - cap;
+            vj[d][i] = vh[d][i] - capv;
+        }
+    }
+}
+
+// whm_coord_j2h_pl (whm_coord.f90:54-90)
+__device__ void whm_j2h(const WhmArgs &a, int p0, int n)
+{
+    const PlDev &pl = a.pl;
+    const WhmDev &w = a.w;
+    pl.rh.x[p0] = w.xj.x[p0];
+    pl.rh.y[p0] = w.xj.y[p0];
+    pl.rh.z[p0] = w.xj.z[p0];
+    pl.vh.x[p0] = w.vj.x[p0];
+    pl.vh.y[p0] = w.vj.y[p0];
+    pl.vh.z[p0] = w.vj.z[p0];
+    double sx[3] = {0, 0, 0}, sv[3] = {0, 0, 0};
+    for (int q = 1; q < n; q++) {
+        int i = p0 + q;
+        double *rh[3] = {pl.rh.x, pl.rh.y, pl.rh.z}, *vh[3] = {pl.vh.x, pl.vh.y, pl.vh.z};
+        const double *xj[3] = {w.xj.x, w.xj.y, w.xj.z}, *vj[3] = {w.vj.x, w.vj.y, w.vj.z};
+        for (int d = 0; d < 3; d++) {
+            sx[d] = sx[d] + (pl.Gm[i - 1] * xj[d][i - 1]) / w.eta[i - 1];
+            sv[d] = sv[d] + (pl.Gm[i - 1] * vj[d][i - 1]) / w.eta[i - 1];
+            rh[d][i] = xj[d][i] + sx[d];
+            vh[d][i] = vj[d][i] + sv[d];
+        }
+    }
+}
+
+// whm_kick_getacch_pl (whm_kick.f90:14-67): ir3h/ir3j, ah0, ah1, ah2, then the
+// direct sum (swiftest_kick_getacch_int_pl, swiftest_kick.f90:12-43)
+__device__ void whm_getacch_pl(const WhmArgs &a, int s, int p0, int n)
+{
+    const PlDev &pl = a.pl;
+    const WhmDev &w = a.w;
+    const double GMcb = a.sy.GMcb[s];
+    for (int q = 0; q < n; q++) {
+        int i = p0 + q;
+        double r2 = dot3(pl.rh.x[i], pl.rh.y[i], pl.rh.z[i], pl.rh.x[i], pl.rh.y[i], pl.rh.z[i]);
+        double ir = 1.0 / sqrt(r2);
+        w.ir3h[i] = ir / r2;
+        r2 = dot3(w.xj.x[i], w.xj.y[i], w.xj.z[i], w.xj.x[i], w.xj.y[i], w.xj.z[i]);
+        ir = 1.0 / sqrt(r2);
+        w.ir3j[i] = ir / r2;
+    }
+    double h0[3] = {0, 0, 0};
+    for (int q = 1; q < n; q++) {
+        int i = p0 + q;
+        double x = pl.rh.x[i], y = pl.rh.y[i], z = pl.rh.z[i];
+        double r2 = dot3(x, y, z, x, y, z);
+        double fac = pl.Gm[i] * (1.0 / (r2 * sqrt(r2)));
+        h0[0] = fma(-fac, x, h0[0]);
+        h0[1] = fma(-fac, y, h0[1]);
+        h0[2] = fma(-fac, z, h0[2]);
+    }
+    double *ah[3] = {w.ah.x, w.ah.y, w.ah.z};
+    const double *xj[3] = {w.xj.x, w.xj.y, w.xj.z}, *rh[3] = {pl.rh.x, pl.rh.y, pl.rh.z};
+    for (int q = 0; q < n; q++)
+        for (int d = 0; d < 3; d++) ah[d][p0 + q] = ah[d][p0 + q] + h0[d];
+    for (int q = 1; q < n; q++) {
+        int i = p0 + q;
+        if (pl.status[i] == INACTIVE) continue;
+        for (int d = 0; d < 3; d++) {
+            double a1j = xj[d][i] * w.ir3j[i];
+            double a1h = rh[d][i] * w.ir3h[i];
+            ah[d][i] = ah[d][i] + GMcb * (a1j - a1h);
+        }
+    }
+    double etaj = GMcb, ah2o[3] = {0, 0, 0};
+    for (int q = 1; q < n; q++) {
+        int i = p0 + q;
+        if (pl.status[i] == INACTIVE) continue;
+        etaj = etaj + pl.Gm[i - 1];
+        double fac = ((pl.Gm[i] * GMcb) * w.ir3j[i]) / etaj;
+        for (int d = 0; d < 3; d++) {
+            double ah2 = ah2o[d] + fac * xj[d][i];
+            ah[d][i] = ah[d][i] + ah2;
+            ah2o[d] = ah2;
+        }
+    }
+    for (int q = 0; q < n; q++) {
+        int i = p0 + q;
+        double ax = ah[0][i], ay = ah[1][i], az = ah[2][i];
+        for (int u = 0; u < n; u++) {
+            int j = p0 + u;
+            if (j == i) continue;
+            double rx = rh[0][j] - rh[0][i], ry = rh[1][j] - rh[1][i], rz = rh[2][j] - rh[2][i];
+            double rji2 = dot3(rx, ry, rz, rx, ry, rz);
+            if (a.lclose) {
+                double rs = pl.R[i] + pl.R[j];
+                if (!(rji2 > rs * rs)) continue;
+            }
+            double fac = pl.Gm[j] / (rji2 * sqrt(rji2));
+            ax = fma(fac, rx, ax);
+            ay = fma(fac, ry, ay);
+            az = fma(fac, rz, az);
+        }
+        ah[0][i] = ax;
+        ah[1][i] = ay;
+        ah[2][i] = az;
+    }
+}
+
+// whm_step_pl with whm_kick_vh_pl and whm_drift_pl (whm_step.f90:38-69,
+// whm_kick.f90:218-262, whm_drift.f90:14-58); one thread per system.
+__global__ void k_whm_pl(WhmArgs a, int nsys, double dt)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    const PlDev &pl = a.pl;
+    const WhmDev &w = a.w;
+    int p0 = a.sy.pl_off[s], n = a.sy.pl_off[s + 1] - p0;
+    a.tlfirst[s] = a.sy.lfirst[s];
+    if (n == 0) return;
+    const double dth = 0.5 * dt;
+    if (a.sy.lfirst[s]) {
+        whm_h2j(a, p0, n);
+        for (int q = 0; q < n; q++) w.ah.x[p0 + q] = w.ah.y[p0 + q] = w.ah.z[p0 + q] = 0.0;
+        whm_getacch_pl(a, s, p0, n);
+        a.sy.lfirst[s] = 0;
+    }
+    for (int q = 0; q < n; q++) {
+        int i = p0 + q;
+        pl.rbeg.x[i] = pl.rh.x[i];
+        pl.rbeg.y[i] = pl.rh.y[i];
+        pl.rbeg.z[i] = pl.rh.z[i];
+        if (pl.status[i] != INACTIVE) {
+            pl.vh.x[i] = fma(w.ah.x[i], dth, pl.vh.x[i]);
+            pl.vh.y[i] = fma(w.ah.y[i], dth, pl.vh.y[i]);
+            pl.vh.z[i] = fma(w.ah.z[i], dth, pl.vh.z[i]);
+        }
+    }
+    // whm_coord_vh2vj_pl (whm_coord.f90:93-124)
+    w.vj.x[p0] = pl.vh.x[p0];
+    w.vj.y[p0] = pl.vh.y[p0];
+    w.vj.z[p0] = pl.vh.z[p0];
+    {
+        double sv[3] = {0, 0, 0};
+        const double *vh[3] = {pl.vh.x, pl.vh.y, pl.vh.z};
+        double *vj[3] = {w.vj.x, w.vj.y, w.vj.z};
+        for (int q = 1; q < n; q++) {
+            int i = p0 + q;
+            for (int d = 0; d < 3; d++) {
+                sv[d] = sv[d] + pl.Gm[i - 1] * vh[d][i - 1];
+                vj[d][i] = vh[d][i] - sv[d] / w.eta[i - 1];
+            }
+        }
+    }
+    for (int q = 0; q < n; q++) {
+        int i = p0 + q;
+        if (pl.status[i] == INACTIVE) continue;
+        double rx = w.xj.x[i], ry = w.xj.y[i], rz = w.xj.z[i], vx = w.vj.x[i], vy = w.vj.y[i], vz = w.vj.z[i];
+        int fl = drift_one(w.muj[i], rx, ry, rz, vx, vy, vz, dt);
+        w.xj.x[i] = rx;
+        w.xj.y[i] = ry;
+        w.xj.z[i] = rz;
+        w.vj.x[i] = vx;
+        w.vj.y[i] = vy;
+        w.vj.z[i] = vz;
+        if (fl) {
+            pl.status[i] = DISCARDED_DRIFTERR;
+            a.sy.err[s] |= 2;
+            return;
+        }
+    }
+    whm_j2h(a, p0, n);
+    for (int q = 0; q < n; q++) w.ah.x[p0 + q] = w.ah.y[p0 + q] = w.ah.z[p0 + q] = 0.0;
+    whm_getacch_pl(a, s, p0, n);
+    for (int q = 0; q < n; q++) {
+        int i = p0 + q;
+        pl.rend.x[i] = pl.rh.x[i];
+        pl.rend.y[i] = pl.rh.y[i];
+        pl.rend.z[i] = pl.rh.z[i];
+        if (pl.status[i] != INACTIVE) {
+            pl.vh.x[i] = fma(w.ah.x[i], dth, pl.vh.x[i]);
+            pl.vh.y[i] = fma(w.ah.y[i], dth, pl.vh.y[i]);
+            pl.vh.z[i] = fma(w.ah.z[i], dth, pl.vh.z[i]);
+        }
+    }
+}
+
+// ah0 of the test particles (whm_kick.f90:82-104): -sum Gm r/|r|^3 over all pl
+// at rbeg or rend, one thread per system
+__global__ void k_whm_tp_ah0(WhmArgs a, int nsys, int lbeg)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    int p0 = a.sy.pl_off[s], n = a.sy.pl_off[s + 1] - p0;
+    Vec3p rp = lbeg ? a.pl.rbeg : a.pl.rend;
+    double h0[3] = {0, 0, 0};
+    for (int q = 0; q < n; q++) {
+        int i = p0 + q;
+        double x = rp.x[i], y = rp.y[i], z = rp.z[i];
+        double r2 = dot3(x, y, z, x, y, z);
+        double fac = a.pl.Gm[i] * (1.0 / (r2 * sqrt(r2)));
+        h0[0] = fma(-fac, x, h0[0]);
+        h0[1] = fma(-fac, y, h0[1]);
+        h0[2] = fma(-fac, z, h0[2]);
+    }
+    for (int d = 0; d < 3; d++) a.ah0[3 * s + d] = h0[d];
+}
+
+// whm_kick_vh_tp (whm_kick.f90:265-318): at the start of a step the
+// accelerations of the previous step's end are reused, except on the first
+// step; ah = (0 + ah0) + sum over pl; vh += ah*dt.
+template <int BLK>
+__global__ __launch_bounds__(BLK) void k_whm_tp_kick(WhmArgs a, TpDev tp, const Tile *tiles, double dt, int lbeg)
+{
+    __shared__ double sx[BLK], sy[BLK], sz[BLK], sg[BLK];
+    const Tile tl = tiles[blockIdx.x];
+    const int s = tl.s;
+    const int p0 = a.sy.pl_off[s], p1 = a.sy.pl_off[s + 1];
+    const int j = tl.b0 + (int)threadIdx.x;
+    const bool valid = j < tl.b1 && tp.status[j] == ACTIVE;
+    const bool recompute = !lbeg || a.tlfirst[s];
+    if (!recompute) {
+        if (valid) {
+            tp.vh.x[j] = fma(a.w.tah.x[j], dt, tp.vh.x[j]);
+            tp.vh.y[j] = fma(a.w.tah.y[j], dt, tp.vh.y[j]);
+            tp.vh.z[j] = fma(a.w.tah.z[j], dt, tp.vh.z[j]);
+        }
+        return;
+    }
+    Vec3p rp = lbeg ? a.pl.rbeg : a.pl.rend;
+    double xt = 0, yt = 0, zt = 0;
+    if (valid) {
+        xt = tp.rh.x[j];
+        yt = tp.rh.y[j];
+        zt = tp.rh.z[j];
+    }
+    double px = 0.0, py = 0.0, pz = 0.0;
+    for (int it = p0; it < p1; it += BLK) {
+        int q0 = it + (int)threadIdx.x;
+        if (q0 < p1) {
+            sx[threadIdx.x] = rp.x[q0];
+            sy[threadIdx.x] = rp.y[q0];
+            sz[threadIdx.x] = rp.z[q0];
+            sg[threadIdx.x] = a.pl.Gm[q0];
+        }
+        __syncthreads();
+        const int qn = min(BLK, p1 - it);
+        if (valid) {
+            for (int q = 0; q < qn; q++) {
+                double rx = xt - sx[q], ry = yt - sy[q], rz = zt - sz[q];
+                double rji2 = dot3(rx, ry, rz, rx, ry, rz);
+                double fac = sg[q] / (rji2 * sqrt(rji2));
+                px = fma(-fac, rx, px);
+                py = fma(-fac, ry, py);
+                pz = fma(-fac, rz, pz);
+            }
+        }
+        __syncthreads();
+    }
+    if (!valid) return;
+    double ax = (0.0 + a.ah0[3 * s]) + px, ay = (0.0 + a.ah0[3 * s + 1]) + py, az = (0.0 + a.ah0[3 * s + 2]) + pz;
+    a.w.tah.x[j] = ax;
+    a.w.tah.y[j] = ay;
+    a.w.tah.z[j] = az;
+    tp.vh.x[j] = fma(ax, dt, tp.vh.x[j]);
+    tp.vh.y[j] = fma(ay, dt, tp.vh.y[j]);
+    tp.vh.z[j] = fma(az, dt, tp.vh.z[j]);
+}
+
+__global__ void k_whm_h2j_all(WhmArgs a, int nsys)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    int p0 = a.sy.pl_off[s], n = a.sy.pl_off[s + 1] - p0;
+    if (n) whm_h2j(a, p0, n);
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble: WHM set-up, step and snapshot
+// ---------------------------------------------------------------------------
+static WhmArgs whm_args(Ensemble &e)
+{
+    WhmArgs a;
+    a.pl = e.pl;
+    a.sy = e.sy;
+    a.w = e.whm;
+    a.tlfirst = e.sy.flag_pt;  // per-system scratch flag
+    a.ah0 = e.sy.ptend;        // per-system 3-vector scratch
+    a.lclose = e.prm.lclose;
+    return a;
+}
+
+int Ensemble::whm_setup()
+{
+    if (alloc_vec(whm.xj, npl) || alloc_vec(whm.vj, npl) || alloc_vec(whm.ah, npl) || alloc_vec(whm.tah, ntp) ||
+        own(&whm.eta, npl) || own(&whm.muj, npl) || own(&whm.ir3h, npl) || own(&whm.ir3j, npl))
+        return SWG_ERR_HIP;
+    // whm_util_set_mu_eta_pl (whm_util.f90:175-198), per system
+    std::vector<double> eta(npl), muj(npl);
+    for (int s = 0; s < nsys; s++) {
+        int a = h_pl_off[s], n = h_pl_off[s + 1] - a;
+        if (!n) continue;
+        eta[a] = h_GMcb[s] + h_Gm[a];
+        muj[a] = eta[a];
+        for (int q = 1; q < n; q++) {
+            eta[a + q] = eta[a + q - 1] + h_Gm[a + q];
+            muj[a + q] = (h_GMcb[s] * eta[a + q]) / eta[a + q - 1];
+        }
+    }
+    HIPCHK(hipMemcpy(whm.eta, eta.data(), npl * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(whm.muj, muj.data(), npl * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<double> z(3 * (size_t)std::max(npl, ntp) + 3, 0.0);
+    for (double *p : {whm.ah.x, whm.ah.y, whm.ah.z, whm.xj.x, whm.xj.y, whm.xj.z, whm.vj.x, whm.vj.y, whm.vj.z})
+        HIPCHK(hipMemcpy(p, z.data(), npl * sizeof(double), hipMemcpyHostToDevice));
+    for (double *p : {whm.tah.x, whm.tah.y, whm.tah.z})
+        HIPCHK(hipMemcpy(p, z.data(), ntp * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int Ensemble::step_whm()
+{
+    hipStream_t st = stream;
+    const double dt = prm.dt, dth = 0.5 * dt;
+    WhmArgs a = whm_args(*this);
+    k_whm_pl<<<cdiv(nsys, 64), 64, 0, st>>>(a, nsys, dt);
+    k_whm_tp_ah0<<<cdiv(nsys, 64), 64, 0, st>>>(a, nsys, 1);
+    if (ntiles_tp) k_whm_tp_kick<256><<<ntiles_tp, 256, 0, st>>>(a, tp, d_tiles_tp, dth, 1);
+    launch_drift(st, tp.rh, tp.vh, tp.status, tp.levelg, tp.sys, sy.GMcb, sy.err, ntp, dt, -1, 0, 1);
+    k_whm_tp_ah0<<<cdiv(nsys, 64), 64, 0, st>>>(a, nsys, 0);
+    if (ntiles_tp) k_whm_tp_kick<256><<<ntiles_tp, 256, 0, st>>>(a, tp, d_tiles_tp, dth, 0);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int Ensemble::whm_snapshot()
+{
+    WhmArgs a = whm_args(*this);
+    k_whm_h2j_all<<<cdiv(nsys, 64), 64, 0, stream>>>(a, nsys);
+    HIPCHK(hipGetLastError());
+    if (!prm.rhill_present) {
+        std::vector<double> rh(3 * (size_t)npl), vh(3 * (size_t)npl), rhill(npl);
+        HIPCHK(hipStreamSynchronize(stream));
+        std::vector<double> cx(npl), cy(npl), cz(npl);
+        for (int which = 0; which < 2; which++) {
+            Vec3p v = which == 0 ? pl.rh : pl.vh;
+            std::vector<double> &o = which == 0 ? rh : vh;
+            HIPCHK(hipMemcpy(cx.data(), v.x, npl * sizeof(double), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(cy.data(), v.y, npl * sizeof(double), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(cz.data(), v.z, npl * sizeof(double), hipMemcpyDeviceToHost));
+            for (int i = 0; i < npl; i++) {
+                o[3 * i] = cx[i];
+                o[3 * i + 1] = cy[i];
+                o[3 * i + 2] = cz[i];
+            }
+        }
+        for (int s = 0; s < nsys; s++) {
+            int b = h_pl_off[s], n = h_pl_off[s + 1] - b;
+            host_set_rhill(h_GMcb[s], n, &rh[3 * (size_t)b], &vh[3 * (size_t)b], &h_Gm[b], &rhill[b]);
+        }
+        HIPCHK(hipMemcpy(pl.rhill, rhill.data(), npl * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// leaf scratch
+// ---------------------------------------------------------------------------
+LeafState::~LeafState()
+{
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    void *q[] = {L.words, L.wpre, L.woff, L.nb, L.rowcnt, L.rowstart, L.i1, L.i2, L.lvdotr, L.status, L.level};
+    for (void *p : q)
+        if (p) (void)hipFree(p);
+}
+
+void *LeafState::get(int slot, size_t bytes, int *rc)
+{
+    if ((int)bufs.size() <= slot) {
+        bufs.resize(slot + 1, nullptr);
+        sizes.resize(slot + 1, 0);
+    }
+    if (sizes[slot] < bytes) {
+        if (bufs[slot]) (void)hipFree(bufs[slot]);
+        bufs[slot] = nullptr;
+        sizes[slot] = 0;
+        hipError_t e = hipMalloc(&bufs[slot], bytes ? bytes : 8);
+        if (e != hipSuccess) {
+            *rc = hipfail(e, "hipMalloc (leaf scratch)");
+            return nullptr;
+        }
+        sizes[slot] = bytes;
+    }
+    return bufs[slot];
+}
+
+}  // namespace swg
+
+// ===========================================================================
+// C ABI: leaf operators
+// ===========================================================================
+using namespace swg;
+
+#define GETBUF(var, T, slot, count)                                                  \
+    T *var = (T *)ctx->leaf->get(slot, sizeof(T) * (size_t)(count), &rc);            \
+    if (!var) return rc;
+
+extern "C" {
+
+int swg_kick_getacch_int_all_tri_pl(swg_ctx *ctx, int32_t npl, int32_t nplm, const double *r, const double *Gmass,
+                                    const double *radius, double *acc)
+{
+    if (!ctx || npl < 0 || nplm < 0 || nplm > npl) return fail(SWG_ERR_ARG, "bad argument");
+    if (npl == 0) return 0;
+    if (!r || !Gmass || !acc) return fail(SWG_ERR_ARG, "NULL array");
+    int rc = 0;
+    hipStream_t st = ctx->stream;
+    GETBUF(dr, double, 0, 3 * npl);
+    GETBUF(dg, double, 1, npl);
+    GETBUF(da, double, 2, 3 * npl);
+    double *dR = nullptr;
+    if (radius) {
+        GETBUF(t, double, 3, npl);
+        dR = t;
+        HIPCHK(hipMemcpyAsync(dR, radius, npl * sizeof(double), hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipMemcpyAsync(dr, r, 3 * npl * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dg, Gmass, npl * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(da, acc, 3 * npl * sizeof(double), hipMemcpyHostToDevice, st));
+    k_leaf_tri<<<cdiv(npl, 128), 128, 0, st>>>(npl, nplm, dr, dg, dR, da);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(acc, da, 3 * npl * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int swg_kick_getacch_int_all_flat_pl(swg_ctx *ctx, int32_t npl, int64_t nplpl, const int32_t *k_plpl,
+                                     const double *r, const double *Gmass, const double *radius, double *acc)
+{
+    if (!ctx || npl < 0 || nplpl < 0) return fail(SWG_ERR_ARG, "bad argument");
+    if (npl == 0) return 0;
+    if (!r || !Gmass || !acc || (nplpl && !k_plpl)) return fail(SWG_ERR_ARG, "NULL array");
+    // per-body pair lists in list order (host counting sort)
+    std::vector<int64_t> rs(npl + 1, 0), cs(npl + 1, 0), rowl(nplpl ? nplpl : 1), coll(nplpl ? nplpl : 1);
+    for (int64_t p = 0; p < nplpl; p++) {
+        int i = k_plpl[2 * p] - 1, j = k_plpl[2 * p + 1] - 1;
+        if (i < 0 || i >= npl || j < 0 || j >= npl) return fail(SWG_ERR_ARG, "pair index out of range");
+        rs[i + 1]++;
+        cs[j + 1]++;
+    }
+    for (int b = 0; b < npl; b++) {
+        rs[b + 1] += rs[b];
+        cs[b + 1] += cs[b];
+    }
+    {
+        std::vector<int64_t> fr(rs.begin(), rs.end() - 1), fc(cs.begin(), cs.end() - 1);
+        for (int64_t p = 0; p < nplpl; p++) {
+            rowl[fr[k_plpl[2 * p] - 1]++] = p;
+            coll[fc[k_plpl[2 * p + 1] - 1]++] = p;
+        }
+    }
+    int rc = 0;
+    hipStream_t st = ctx->stream;
+    GETBUF(dr, double, 0, 3 * npl);
+    GETBUF(dg, double, 1, npl);
+    GETBUF(da, double, 2, 3 * npl);
+    GETBUF(dk, int32_t, 4, 2 * (nplpl ? nplpl : 1));
+    GETBUF(drs, int64_t, 5, npl + 1);
+    GETBUF(dcs, int64_t, 6, npl + 1);
+    GETBUF(drl, int64_t, 7, rowl.size());
+    GETBUF(dcl, int64_t, 8, coll.size());
+    double *dR = nullptr;
+    if (radius) {
+        GETBUF(t, double, 3, npl);
+        dR = t;
+        HIPCHK(hipMemcpyAsync(dR, radius, npl * sizeof(double), hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipMemcpyAsync(dr, r, 3 * npl * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dg, Gmass, npl * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(da, acc, 3 * npl * sizeof(double), hipMemcpyHostToDevice, st));
+    if (nplpl) HIPCHK(hipMemcpyAsync(dk, k_plpl, 2 * nplpl * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(drs, rs.data(), (npl + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dcs, cs.data(), (npl + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(drl, rowl.data(), rowl.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dcl, coll.data(), coll.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    k_leaf_flat<<<cdiv(npl, 128), 128, 0, st>>>(npl, dk, dr, dg, dR, drs, dcs, drl, dcl, da);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(acc, da, 3 * npl * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int swg_kick_getacch_int_all_tp(swg_ctx *ctx, int32_t ntp, int32_t npl, const double *rtp, const double *rpl,
+                                const double *GMpl, const int32_t *lmask, double *acc)
+{
+    if (!ctx || ntp < 0 || npl < 0) return fail(SWG_ERR_ARG, "bad argument");
+    if (ntp == 0 || npl == 0) return 0;  // swiftest_kick.f90:58
+    if (!rtp || !rpl || !GMpl || !lmask || !acc) return fail(SWG_ERR_ARG, "NULL array");
+    int rc = 0;
+    hipStream_t st = ctx->stream;
+    GETBUF(dt_, double, 0, 3 * ntp);
+    GETBUF(dp, double, 1, 3 * npl);
+    GETBUF(dg, double, 2, npl);
+    GETBUF(dm, int32_t, 3, ntp);
+    GETBUF(da, double, 4, 3 * ntp);
+    HIPCHK(hipMemcpyAsync(dt_, rtp, 3 * ntp * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dp, rpl, 3 * npl * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dg, GMpl, npl * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dm, lmask, ntp * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(da, acc, 3 * ntp * sizeof(double), hipMemcpyHostToDevice, st));
+    k_leaf_tp<256><<<cdiv(ntp, 256), 256, 0, st>>>(ntp, npl, dt_, dp, dg, dm, da);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(acc, da, 3 * ntp * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int swg_drift_all(swg_ctx *ctx, const double *mu, double *x, double *v, int32_t n, double dt, const int32_t *lmask,
+                  int32_t *iflag)
+{
+    if (!ctx || n < 0) return fail(SWG_ERR_ARG, "bad argument");
+    if (n == 0) return 0;
+    if (!mu || !x || !v || !lmask || !iflag) return fail(SWG_ERR_ARG, "NULL array");
+    int rc = 0;
+    hipStream_t st = ctx->stream;
+    GETBUF(dmu, double, 0, n);
+    GETBUF(dx, double, 1, 3 * n);
+    GETBUF(dv, double, 2, 3 * n);
+    GETBUF(dm, int32_t, 3, n);
+    GETBUF(df, int32_t, 4, n);
+    HIPCHK(hipMemcpyAsync(dmu, mu, n * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dx, x, 3 * n * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dv, v, 3 * n * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dm, lmask, n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(df, iflag, n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    k_leaf_drift<<<cdiv(n, 128), 128, 0, st>>>(dmu, dx, dv, n, dt, dm, df);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(x, dx, 3 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(v, dv, 3 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(iflag, df, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// Level-0 triangular encounter check of one body set against itself
+// (which = 0) or of npl bodies against ntp particles (which = 1), using the
+// resident-path kernels on a one-system layout.
+static int leaf_encounter(swg_ctx *ctx, int which, int32_t npl, int32_t ntp, const double *rpl, const double *vpl,
+                          const double *rtp, const double *vtp, const double *renc, double dt, int64_t *nenc)
+{
+    int rc = 0;
+    hipStream_t st = ctx->stream;
+    LeafState &ls = *ctx->leaf;
+    *nenc = 0;
+    ls.nenc = 0;
+    if (npl == 0 || (which == 1 && ntp == 0)) return 0;
+    PlDev pl{};
+    TpDev tp{};
+    SysDev sy{};
+    GETBUF(prx, double, 10, 3 * npl);
+    GETBUF(pvx, double, 11, 3 * npl);
+    GETBUF(pre, double, 12, npl);
+    GETBUF(psys, int32_t, 13, std::max(npl, ntp));
+    GETBUF(offs, int32_t, 14, 4);
+    GETBUF(woff, int64_t, 15, 1);
+    GETBUF(nbp, int32_t, 16, 1);
+    GETBUF(tot, int64_t, 17, 1);
+    int ncol = which == 0 ? npl : ntp;
+    GETBUF(tiles, int4, 18, which == 0 ? ((npl + 63) / 64) * ((npl + 63) / 64 + 1) / 2 : (ntp + 63) / 64);
+    GETBUF(scan, char, 19, scan_scratch_bytes(std::max(npl, ntp)));
+    pl.rh = Vec3p{prx, prx + npl, prx + 2 * npl};
+    pl.vb = Vec3p{pvx, pvx + npl, pvx + 2 * npl};
+    pl.renc = pre;
+    pl.sys = psys;
+    std::vector<double> tmp(3 * (size_t)std::max(npl, ntp));
+    auto soa = [&](const double *aos, double *dst, int n) -> int {
+        for (int i = 0; i < n; i++) {
+            tmp[i] = aos[3 * i];
+            tmp[n + i] = aos[3 * i + 1];
+            tmp[2 * n + i] = aos[3 * i + 2];
+        }
+        HIPCHK(hipMemcpy(dst, tmp.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice));
+        return 0;
+    };
+    if ((rc = soa(rpl, prx, npl)) || (rc = soa(vpl, pvx, npl))) return rc;
+    HIPCHK(hipMemcpy(pre, renc, npl * sizeof(double), hipMemcpyHostToDevice));
+    k_fill_i32<<<cdiv(std::max(npl, ntp), 256), 256, 0, st>>>(psys, std::max(npl, ntp), 0);
+    int32_t ho[4] = {0, npl, 0, ntp};
+    HIPCHK(hipMemcpyAsync(offs, ho, sizeof ho, hipMemcpyHostToDevice, st));
+    sy.pl_off = offs;
+    sy.tp_off = offs + 2;
+    if (which == 1) {
+        GETBUF(trx, double, 20, 3 * ntp);
+        GETBUF(tvx, double, 21, 3 * ntp);
+        GETBUF(tst, int32_t, 22, ntp);
+        tp.rh = Vec3p{trx, trx + ntp, trx + 2 * ntp};
+        tp.vb = Vec3p{tvx, tvx + ntp, tvx + 2 * ntp};
+        tp.status = tst;
+        tp.sys = psys;
+        if ((rc = soa(rtp, trx, ntp)) || (rc = soa(vtp, tvx, ntp))) return rc;
+        k_fill_i32<<<cdiv(ntp, 256), 256, 0, st>>>(tst, ntp, ACTIVE);
+    }
+    ListDev &L = ls.L;
+    int nb = (ncol + 63) / 64;
+    int64_t words = (int64_t)npl * nb;
+    GETBUF(w, uint64_t, 23, words);
+    GETBUF(wp, int32_t, 24, words);
+    GETBUF(rcnt, int32_t, 25, npl);
+    GETBUF(rst, int64_t, 26, npl);
+    L.words = w;
+    L.wpre = wp;
+    L.rowcnt = rcnt;
+    L.rowstart = rst;
+    L.woff = woff;
+    L.nb = nbp;
+    int64_t z = 0;
+    HIPCHK(hipMemcpyAsync(woff, &z, sizeof z, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(nbp, &nb, sizeof nb, hipMemcpyHostToDevice, st));
+    std::vector<int4> ht;
+    if (which == 0) {
+        for (int ib = 0; ib < nb; ib++)
+            for (int jb = ib; jb < nb; jb++) ht.push_back(make_int4(0, ib, jb, 0));
+    } else {
+        for (int tb = 0; tb < nb; tb++) ht.push_back(make_int4(0, tb, 0, 0));
+    }
+    HIPCHK(hipMemcpyAsync(tiles, ht.data(), ht.size() * sizeof(int4), hipMemcpyHostToDevice, st));
+    if (which == 0)
+        launch_pp_tiles(st, pl, sy, L, tiles, (int)ht.size(), dt);
+    else
+        launch_pt_tiles(st, pl, tp, sy, L, tiles, (int)ht.size(), dt);
+    launch_rows_count(st, L, psys, which == 0, offs, nullptr, npl);
+    launch_scan_i32_i64(st, rcnt, rst, npl, tot, scan);
+    int64_t n = 0;
+    HIPCHK(hipMemcpyAsync(&n, tot, sizeof n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (n > ls.cap) {
+        for (void *q : {(void *)L.i1, (void *)L.i2, (void *)L.lvdotr, (void *)L.status, (void *)L.level})
+            if (q) (void)hipFree(q);
+        int64_t c = n + n / 2 + 64;
+        HIPCHK(hipMalloc((void **)&L.i1, c * sizeof(int32_t)));
+        HIPCHK(hipMalloc((void **)&L.i2, c * sizeof(int32_t)));
+        HIPCHK(hipMalloc((void **)&L.lvdotr, c));
+        HIPCHK(hipMalloc((void **)&L.status, c));
+        HIPCHK(hipMalloc((void **)&L.level, c));
+        ls.cap = c;
+    }
+    if (n) launch_rows_fill(st, L, psys, which == 0, offs, offs + (which == 0 ? 0 : 2), npl);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    ls.nenc = n;
+    *nenc = n;
+    return 0;
+}
+
+int swg_encounter_check_all_plpl(swg_ctx *ctx, int32_t npl, const double *r, const double *v, const double *renc,
+                                 double dt, int64_t *nenc)
+{
+    if (!ctx || !nenc || npl < 0) return fail(SWG_ERR_ARG, "bad argument");
+    if (npl && (!r || !v || !renc)) return fail(SWG_ERR_ARG, "NULL array");
+    return leaf_encounter(ctx, 0, npl, 0, r, v, nullptr, nullptr, renc, dt, nenc);
+}
+
+int swg_encounter_check_all_pltp(swg_ctx *ctx, int32_t npl, int32_t ntp, const double *rpl, const double *vpl,
+                                 const double *rtp, const double *vtp, const double *renc, double dt, int64_t *nenc)
+{
+    if (!ctx || !nenc || npl < 0 || ntp < 0) return fail(SWG_ERR_ARG, "bad argument");
+    if (npl && ntp && (!rpl || !vpl || !rtp || !vtp || !renc)) return fail(SWG_ERR_ARG, "NULL array");
+    return leaf_encounter(ctx, 1, npl, ntp, rpl, vpl, rtp, vtp, renc, dt, nenc);
+}
+
+int swg_encounter_list_copy(swg_ctx *ctx, int32_t *index1, int32_t *index2, int32_t *lvdotr)
+{
+    if (!ctx) return fail(SWG_ERR_ARG, "NULL context");
+    LeafState &ls = *ctx->leaf;
+    int64_t n = ls.nenc;
+    if (!n) return 0;
+    int rc = 0;
+    hipStream_t st = ctx->stream;
+    GETBUF(a, int32_t, 30, n);
+    GETBUF(b, int32_t, 31, n);
+    GETBUF(c, int32_t, 32, n);
+    k_copy_i32_plus1<<<cdiv(n, 256), 256, 0, st>>>(ls.L.i1, a, ls.L.lvdotr, c, n);
+    k_copy_i32_plus1<<<cdiv(n, 256), 256, 0, st>>>(ls.L.i2, b, nullptr, nullptr, n);
+    HIPCHK(hipGetLastError());
+    if (index1) HIPCHK(hipMemcpyAsync(index1, a, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (index2) HIPCHK(hipMemcpyAsync(index2, b, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (lvdotr) HIPCHK(hipMemcpyAsync(lvdotr, c, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+}  // extern "C"

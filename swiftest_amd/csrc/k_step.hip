@@ -1,0 +1,478 @@
+// k_step.hip -- kernels of the HELIO / SyMBA step outside the recursion:
+// frame changes, linear drift, velocity kicks with the direct-sum
+// accelerations, and the Danby drift.
+//
+// Reference behaviour: helio_step.f90:37-119, symba_step.f90:55-115,
+// helio_kick.f90:91-169, helio_drift.f90:14-200, swiftest_util.f90:363-485,
+// swiftest_kick.f90:69-415, symba_kick.f90:36-123, swiftest_drift.f90:60-138.
+#include "swg_kernels.h"
+#include "swg_math.h"
+
+namespace swg {
+
+static inline int cdiv(int64_t a, int b) { return (int)((a + b - 1) / b); }
+
+__device__ inline bool pl_active(const PlDev &pl, int i) { return pl.status[i] != INACTIVE; }
+__device__ inline bool tp_active(const TpDev &tp, int j) { return tp.status[j] == ACTIVE; }
+
+// symba_step_reset_system (symba_step.f90:265-331): recursion levels to -1
+__global__ void k_reset_levels(PlDev pl, TpDev tp, int npl, int ntp)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < npl) {
+        pl.levelg[i] = -1;
+        pl.levelm[i] = -1;
+        pl.renc[i] = (pl.rhill[i] * RHSCALE) * 1.0;  // set_renc(0), symba_util.f90:245-267
+    }
+    if (i < ntp) {
+        tp.levelg[i] = -1;
+        tp.levelm[i] = -1;
+    }
+}
+
+// Per-system sequential sums at the start of a step, one thread per
+// component, in the reference's summation order:
+//   vh->vb (swiftest_util.f90:424-459) when lfirst: cb%vb = -sum(Gm vh) / (GM_cb + sum(Gm))
+//   lindrift pt (helio_drift.f90:129-163): pt = sum_masked(Gm vb) / GM_cb
+__global__ void k_sums_begin(PlDev pl, SysDev sy, int nsys, int force)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * nsys) return;
+    int s = t / 3, d = t % 3;
+    int p0 = sy.pl_off[s], p1 = sy.pl_off[s + 1];
+    if (p1 == p0) return;
+    const double *vh = d == 0 ? pl.vh.x : d == 1 ? pl.vh.y : pl.vh.z;
+    const double *vb = d == 0 ? pl.vb.x : d == 1 ? pl.vb.y : pl.vb.z;
+    bool first = force || sy.lfirst[s];
+    double cb = 0.0;
+    if (first) {
+        double sm = 0.0;
+        for (int i = p0; i < p1; i++) sm = sm + pl.Gm[i];
+        double Gmtot = sy.GMcb[s] + sm;
+        double c = 0.0;
+        for (int i = p0; i < p1; i++) c = c - pl.Gm[i] * vh[i];
+        cb = c / Gmtot;
+        sy.cbvb[3 * s + d] = cb;
+    }
+    double pt = 0.0;
+    for (int i = p0; i < p1; i++) {
+        if (!pl_active(pl, i)) continue;
+        double v = first ? vh[i] + cb : vb[i];
+        pt = pt + pl.Gm[i] * v;
+    }
+    sy.ptbeg[3 * s + d] = pt / sy.GMcb[s];
+}
+
+// vb = vh + cb%vb when lfirst; then the linear drift rh += pt*dt
+__global__ void k_pl_begin_apply(PlDev pl, SysDev sy, int npl, double dth, int force)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npl) return;
+    int s = pl.sys[i];
+    if (force || sy.lfirst[s]) {
+        pl.vb.x[i] = pl.vh.x[i] + sy.cbvb[3 * s];
+        pl.vb.y[i] = pl.vh.y[i] + sy.cbvb[3 * s + 1];
+        pl.vb.z[i] = pl.vh.z[i] + sy.cbvb[3 * s + 2];
+    }
+    if (pl_active(pl, i)) {
+        pl.rh.x[i] = fma(sy.ptbeg[3 * s], dth, pl.rh.x[i]);
+        pl.rh.y[i] = fma(sy.ptbeg[3 * s + 1], dth, pl.rh.y[i]);
+        pl.rh.z[i] = fma(sy.ptbeg[3 * s + 2], dth, pl.rh.z[i]);
+    }
+}
+
+// Massive-body half kick (helio_kick_vb_pl, helio_kick.f90:91-132):
+//   ah = full-row direct sum over j ascending, j /= i, with the radius overlap
+//        test (swiftest_kick.f90:219-240, symba_kick.f90:14-33);
+//   ah -= (0 + ahi) + ahj over the pl-pl encounter list (symba_kick.f90:61-70,
+//        swiftest_kick.f90:69-115): ahi over pairs where i is index1, ahj over
+//        pairs where i is index2, each in list order;
+//   rbeg/rend = rh;  vb += ah*dt.
+// One lane owns body i and walks j in order, so each sum keeps the
+// reference's sequence; the j bodies are staged through LDS in blocks of BLK.
+template <int BLK>
+__global__ __launch_bounds__(BLK) void k_kick_pl(PlDev pl, SysDev sy, ListDev pp, const Tile *tiles, double dt,
+                                                 int lbeg, int use_radius, int do_pairs)
+{
+    __shared__ double sx[BLK], syy[BLK], sz[BLK], sgm[BLK], sr[BLK];
+    const Tile tl = tiles[blockIdx.x];
+    const int p0 = sy.pl_off[tl.s], p1 = sy.pl_off[tl.s + 1];
+    const int i = tl.b0 + (int)threadIdx.x;
+    const bool valid = i < tl.b1;
+    double xi = 0.0, yi = 0.0, zi = 0.0, Ri = 0.0;
+    if (valid) {
+        xi = pl.rh.x[i];
+        yi = pl.rh.y[i];
+        zi = pl.rh.z[i];
+        Ri = pl.R[i];
+    }
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int jt = p0; jt < p1; jt += BLK) {
+        int j = jt + (int)threadIdx.x;
+        if (j < p1) {
+            sx[threadIdx.x] = pl.rh.x[j];
+            syy[threadIdx.x] = pl.rh.y[j];
+            sz[threadIdx.x] = pl.rh.z[j];
+            sgm[threadIdx.x] = pl.Gm[j];
+            sr[threadIdx.x] = pl.R[j];
+        }
+        __syncthreads();
+        const int jn = min(BLK, p1 - jt);
+        if (valid) {
+            for (int q = 0; q < jn; q++) {
+                if (jt + q == i) continue;
+                double rx = sx[q] - xi, ry = syy[q] - yi, rz = sz[q] - zi;
+                double rji2 = dot3(rx, ry, rz, rx, ry, rz);
+                if (use_radius) {
+                    double rs = Ri + sr[q];
+                    if (!(rji2 > rs * rs)) continue;
+                }
+                double fac = sgm[q] / (rji2 * sqrt(rji2));
+                ax = fma(fac, rx, ax);
+                ay = fma(fac, ry, ay);
+                az = fma(fac, rz, az);
+            }
+        }
+        __syncthreads();
+    }
+    if (!valid) return;
+    if (do_pairs) {
+        double hix = 0.0, hiy = 0.0, hiz = 0.0, hjx = 0.0, hjy = 0.0, hjz = 0.0;
+        const int64_t c0 = pp.colstart[i], c1 = c0 + pp.colcnt[i];
+        for (int64_t c = c0; c < c1; c++) {  // pairs (p, i), p < i
+            int p = pp.i1[pp.col[c]];
+            double rx = xi - pl.rh.x[p], ry = yi - pl.rh.y[p], rz = zi - pl.rh.z[p];
+            double rji2 = dot3(rx, ry, rz, rx, ry, rz);
+            double rs = pl.R[p] + Ri;
+            if (!(rji2 > rs * rs)) continue;
+            double irij3 = 1.0 / (rji2 * sqrt(rji2));
+            double faci = pl.Gm[p] * irij3;
+            hjx = fma(-faci, rx, hjx);
+            hjy = fma(-faci, ry, hjy);
+            hjz = fma(-faci, rz, hjz);
+        }
+        const int64_t r0 = pp.rowstart[i], r1 = r0 + pp.rowcnt[i];
+        for (int64_t k = r0; k < r1; k++) {  // pairs (i, j), j > i
+            int j = pp.i2[k];
+            double rx = pl.rh.x[j] - xi, ry = pl.rh.y[j] - yi, rz = pl.rh.z[j] - zi;
+            double rji2 = dot3(rx, ry, rz, rx, ry, rz);
+            double rs = Ri + pl.R[j];
+            if (!(rji2 > rs * rs)) continue;
+            double irij3 = 1.0 / (rji2 * sqrt(rji2));
+            double facj = pl.Gm[j] * irij3;
+            hix = fma(facj, rx, hix);
+            hiy = fma(facj, ry, hiy);
+            hiz = fma(facj, rz, hiz);
+        }
+        ax = ax - ((0.0 + hix) + hjx);
+        ay = ay - ((0.0 + hiy) + hjy);
+        az = az - ((0.0 + hiz) + hjz);
+    }
+    Vec3p rb = lbeg ? pl.rbeg : pl.rend;
+    rb.x[i] = xi;
+    rb.y[i] = yi;
+    rb.z[i] = zi;
+    if (pl_active(pl, i)) {
+        pl.vb.x[i] = fma(ax, dt, pl.vb.x[i]);
+        pl.vb.y[i] = fma(ay, dt, pl.vb.y[i]);
+        pl.vb.z[i] = fma(az, dt, pl.vb.z[i]);
+    }
+}
+
+// Danby drift of the masked bodies (helio_drift_body, helio_drift.f90:14-54,
+// with the SyMBA level mask of symba_drift.f90:14-61): mu = GM_cb, (rh, vb).
+__global__ void k_drift(Vec3p r, Vec3p v, int32_t *status, const int32_t *levelg, const int32_t *sys,
+                        const double *GMcb, int32_t *err, int n, double dt, int irec, int use_level, int is_tp)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int st = status[i];
+    bool m = is_tp ? (st == ACTIVE) : (st != INACTIVE);
+    if (use_level) m = m && (levelg[i] == irec);
+    if (!m) return;
+    int s = sys[i];
+    double rx = r.x[i], ry = r.y[i], rz = r.z[i], vx = v.x[i], vy = v.y[i], vz = v.z[i];
+    int fl = drift_one(GMcb[s], rx, ry, rz, vx, vy, vz, dt);
+    r.x[i] = rx;
+    r.y[i] = ry;
+    r.z[i] = rz;
+    v.x[i] = vx;
+    v.y[i] = vy;
+    v.z[i] = vz;
+    if (fl != 0) {
+        status[i] = DISCARDED_DRIFTERR;
+        if (!is_tp) atomicOr(&err[s], 2);
+    }
+}
+
+// Test particles at the start of a step: vb = vh - ptbeg when lfirst
+// (swiftest_util.f90:398-421), then rh += ptbeg*dt (helio_drift.f90:166-200).
+__global__ void k_tp_pre(TpDev tp, SysDev sy, int ntp, double dth)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ntp || !tp_active(tp, j)) return;
+    int s = tp.sys[j];
+    double px = sy.ptbeg[3 * s], py = sy.ptbeg[3 * s + 1], pz = sy.ptbeg[3 * s + 2];
+    if (sy.lfirst[s]) {
+        tp.vb.x[j] = tp.vh.x[j] + (-px);
+        tp.vb.y[j] = tp.vh.y[j] + (-py);
+        tp.vb.z[j] = tp.vh.z[j] + (-pz);
+    }
+    tp.rh.x[j] = fma(px, dth, tp.rh.x[j]);
+    tp.rh.y[j] = fma(py, dth, tp.rh.y[j]);
+    tp.rh.z[j] = fma(pz, dth, tp.rh.z[j]);
+}
+
+// Test-particle half kick (helio_kick_vb_tp, helio_kick.f90:135-169):
+//   p = sum over pl j ascending of -Gm_j (r_t - r_j)/|.|^3 at pl rbeg/rend,
+//       started from zero and added to ah = 0 once (the OpenMP reduction of
+//       swiftest_kick.f90:394-411);
+//   ah += Gm_i dx/|dx|^3 for the particle's listed encounter pairs in list
+//       order (symba_kick.f90:106-119);  vb += ah*dt.
+template <int BLK>
+__global__ __launch_bounds__(BLK) void k_kick_tp(TpDev tp, PlDev pl, SysDev sy, ListDev pt, const Tile *tiles,
+                                                 double dt, int lbeg, int do_pairs)
+{
+    __shared__ double sx[BLK], syy[BLK], sz[BLK], sgm[BLK];
+    const Tile tl = tiles[blockIdx.x];
+    const int p0 = sy.pl_off[tl.s], p1 = sy.pl_off[tl.s + 1];
+    const int j = tl.b0 + (int)threadIdx.x;
+    const bool valid = (j < tl.b1) && tp_active(tp, j);
+    const Vec3p rp = lbeg ? pl.rbeg : pl.rend;
+    double xt = 0.0, yt = 0.0, zt = 0.0;
+    if (valid) {
+        xt = tp.rh.x[j];
+        yt = tp.rh.y[j];
+        zt = tp.rh.z[j];
+    }
+    double px = 0.0, py = 0.0, pz = 0.0;
+    for (int it = p0; it < p1; it += BLK) {
+        int q0 = it + (int)threadIdx.x;
+        if (q0 < p1) {
+            sx[threadIdx.x] = rp.x[q0];
+            syy[threadIdx.x] = rp.y[q0];
+            sz[threadIdx.x] = rp.z[q0];
+            sgm[threadIdx.x] = pl.Gm[q0];
+        }
+        __syncthreads();
+        const int qn = min(BLK, p1 - it);
+        if (valid) {
+            for (int q = 0; q < qn; q++) {
+                double rx = xt - sx[q], ry = yt - syy[q], rz = zt - sz[q];
+                double rji2 = dot3(rx, ry, rz, rx, ry, rz);
+                double fac = sgm[q] / (rji2 * sqrt(rji2));
+                px = fma(-fac, rx, px);
+                py = fma(-fac, ry, py);
+                pz = fma(-fac, rz, pz);
+            }
+        }
+        __syncthreads();
+    }
+    if (!valid) return;
+    double ax = 0.0 + px, ay = 0.0 + py, az = 0.0 + pz;
+    if (do_pairs) {
+        const int64_t c0 = pt.colstart[j], c1 = c0 + pt.colcnt[j];
+        for (int64_t c = c0; c < c1; c++) {
+            int i = pt.i1[pt.col[c]];
+            double dx = xt - rp.x[i], dy = yt - rp.y[i], dz = zt - rp.z[i];
+            double rjj = dot3(dx, dy, dz, dx, dy, dz);
+            double fac = pl.Gm[i] / (rjj * sqrt(rjj));
+            ax = fma(fac, dx, ax);
+            ay = fma(fac, dy, ay);
+            az = fma(fac, dz, az);
+        }
+    }
+    tp.vb.x[j] = fma(ax, dt, tp.vb.x[j]);
+    tp.vb.y[j] = fma(ay, dt, tp.vb.y[j]);
+    tp.vb.z[j] = fma(az, dt, tp.vb.z[j]);
+}
+
+// Per-system sums at the end of a step (6 threads per system):
+//   threads 0-2: lindrift pt = sum_masked(Gm vb) / GM_cb  -> ptend
+//   threads 3-5: vb->vh (swiftest_util.f90:363-395): cb%vb -= Gm vb / GM_cb
+//                over i = npl..1 (active only)
+__global__ void k_sums_end(PlDev pl, SysDev sy, int nsys)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 6 * nsys) return;
+    int s = t / 6, w = t % 6, d = w % 3;
+    int p0 = sy.pl_off[s], p1 = sy.pl_off[s + 1];
+    if (p1 == p0) return;
+    const double *vb = d == 0 ? pl.vb.x : d == 1 ? pl.vb.y : pl.vb.z;
+    const double gm = sy.GMcb[s];
+    if (w < 3) {
+        double pt = 0.0;
+        for (int i = p0; i < p1; i++)
+            if (pl_active(pl, i)) pt = pt + pl.Gm[i] * vb[i];
+        sy.ptend[3 * s + d] = pt / gm;
+    } else {
+        double c = 0.0;
+        for (int i = p1 - 1; i >= p0; i--)
+            if (pl_active(pl, i)) c = c - (pl.Gm[i] * vb[i]) / gm;
+        sy.cbvb[3 * s + d] = c;
+    }
+}
+
+// rh += ptend*dt (masked); vh = vb - cb%vb (all bodies)
+__global__ void k_pl_end_apply(PlDev pl, SysDev sy, int npl, double dth)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npl) return;
+    int s = pl.sys[i];
+    if (pl_active(pl, i)) {
+        pl.rh.x[i] = fma(sy.ptend[3 * s], dth, pl.rh.x[i]);
+        pl.rh.y[i] = fma(sy.ptend[3 * s + 1], dth, pl.rh.y[i]);
+        pl.rh.z[i] = fma(sy.ptend[3 * s + 2], dth, pl.rh.z[i]);
+    }
+    pl.vh.x[i] = pl.vb.x[i] - sy.cbvb[3 * s];
+    pl.vh.y[i] = pl.vb.y[i] - sy.cbvb[3 * s + 1];
+    pl.vh.z[i] = pl.vb.z[i] - sy.cbvb[3 * s + 2];
+}
+
+// Test particles at the end of a step: rh += ptend*dt, vh = vb + ptend
+// (vb2vh with vbcb = -ptend)
+__global__ void k_tp_post(TpDev tp, SysDev sy, int ntp, double dth)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ntp || !tp_active(tp, j)) return;
+    int s = tp.sys[j];
+    double px = sy.ptend[3 * s], py = sy.ptend[3 * s + 1], pz = sy.ptend[3 * s + 2];
+    tp.rh.x[j] = fma(px, dth, tp.rh.x[j]);
+    tp.rh.y[j] = fma(py, dth, tp.rh.y[j]);
+    tp.rh.z[j] = fma(pz, dth, tp.rh.z[j]);
+    tp.vh.x[j] = tp.vb.x[j] - (-px);
+    tp.vh.y[j] = tp.vb.y[j] - (-py);
+    tp.vh.z[j] = tp.vb.z[j] - (-pz);
+}
+
+// The HELIO branch clears lfirst after its vh->vb; the encounter branch keeps
+// it (helio_step.f90:58-61 vs symba_step.f90:81, then symba_step.f90:47).
+__global__ void k_lfirst_update(SysDev sy, int nsys)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    if (sy.lfirst[s] && !(sy.enc_pp[s] || sy.enc_pt[s])) sy.lfirst[s] = 0;
+}
+
+// Snapshot coordinate refresh (swiftest_util.f90:2663-2682): pl vh->vb, then
+// tp vb = vh - ptbeg for active particles.
+__global__ void k_snapshot_tp(TpDev tp, SysDev sy, int ntp)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ntp || !tp_active(tp, j)) return;
+    int s = tp.sys[j];
+    tp.vb.x[j] = tp.vh.x[j] + (-sy.ptbeg[3 * s]);
+    tp.vb.y[j] = tp.vh.y[j] + (-sy.ptbeg[3 * s + 1]);
+    tp.vb.z[j] = tp.vh.z[j] + (-sy.ptbeg[3 * s + 2]);
+}
+
+__global__ void k_vh2vb_sum(PlDev pl, SysDev sy, int nsys)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * nsys) return;
+    int s = t / 3, d = t % 3;
+    int p0 = sy.pl_off[s], p1 = sy.pl_off[s + 1];
+    if (p1 == p0) return;
+    const double *vh = d == 0 ? pl.vh.x : d == 1 ? pl.vh.y : pl.vh.z;
+    double sm = 0.0;
+    for (int i = p0; i < p1; i++) sm = sm + pl.Gm[i];
+    double Gmtot = sy.GMcb[s] + sm;
+    double c = 0.0;
+    for (int i = p0; i < p1; i++) c = c - pl.Gm[i] * vh[i];
+    sy.cbvb[3 * s + d] = c / Gmtot;
+}
+
+__global__ void k_vh2vb_apply(PlDev pl, SysDev sy, int npl)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npl) return;
+    int s = pl.sys[i];
+    pl.vb.x[i] = pl.vh.x[i] + sy.cbvb[3 * s];
+    pl.vb.y[i] = pl.vh.y[i] + sy.cbvb[3 * s + 1];
+    pl.vb.z[i] = pl.vh.z[i] + sy.cbvb[3 * s + 2];
+}
+
+// Initialization (helio_util.f90:26-28): tp vb = vh + cb%vb
+__global__ void k_init_tp_vb(TpDev tp, SysDev sy, int ntp)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ntp) return;
+    int s = tp.sys[j];
+    tp.vb.x[j] = tp.vh.x[j] + sy.cbvb[3 * s];
+    tp.vb.y[j] = tp.vh.y[j] + sy.cbvb[3 * s + 1];
+    tp.vb.z[j] = tp.vh.z[j] + sy.cbvb[3 * s + 2];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void launch_reset_levels(hipStream_t st, PlDev pl, TpDev tp, int npl, int ntp)
+{
+    int n = npl > ntp ? npl : ntp;
+    if (n) k_reset_levels<<<cdiv(n, 256), 256, 0, st>>>(pl, tp, npl, ntp);
+}
+void launch_sums_begin(hipStream_t st, PlDev pl, SysDev sy, int nsys, int force)
+{
+    k_sums_begin<<<cdiv(3 * nsys, 64), 64, 0, st>>>(pl, sy, nsys, force);
+}
+void launch_pl_begin_apply(hipStream_t st, PlDev pl, SysDev sy, int npl, double dth, int force)
+{
+    if (npl) k_pl_begin_apply<<<cdiv(npl, 256), 256, 0, st>>>(pl, sy, npl, dth, force);
+}
+void launch_kick_pl(hipStream_t st, const Tuning &tu, PlDev pl, SysDev sy, ListDev pp, const Tile *tiles,
+                    int ntiles, double dt, int lbeg, int use_radius, int do_pairs)
+{
+    if (!ntiles) return;
+    switch (tu.kick_pl_block) {
+    case 64: k_kick_pl<64><<<ntiles, 64, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
+    case 128: k_kick_pl<128><<<ntiles, 128, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
+    default: k_kick_pl<256><<<ntiles, 256, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
+    }
+}
+void launch_drift(hipStream_t st, Vec3p r, Vec3p v, int32_t *status, int32_t *levelg, const int32_t *sys,
+                  const double *GMcb, int32_t *err, int n, double dt, int irec, int use_level, int is_tp)
+{
+    if (n) k_drift<<<cdiv(n, 128), 128, 0, st>>>(r, v, status, levelg, sys, GMcb, err, n, dt, irec, use_level, is_tp);
+}
+void launch_tp_pre(hipStream_t st, TpDev tp, SysDev sy, int ntp, double dth)
+{
+    if (ntp) k_tp_pre<<<cdiv(ntp, 256), 256, 0, st>>>(tp, sy, ntp, dth);
+}
+void launch_kick_tp(hipStream_t st, const Tuning &tu, TpDev tp, PlDev pl, SysDev sy, ListDev pt,
+                    const Tile *tiles, int ntiles, double dt, int lbeg, int do_pairs)
+{
+    if (!ntiles) return;
+    switch (tu.kick_tp_block) {
+    case 64: k_kick_tp<64><<<ntiles, 64, 0, st>>>(tp, pl, sy, pt, tiles, dt, lbeg, do_pairs); break;
+    case 128: k_kick_tp<128><<<ntiles, 128, 0, st>>>(tp, pl, sy, pt, tiles, dt, lbeg, do_pairs); break;
+    default: k_kick_tp<256><<<ntiles, 256, 0, st>>>(tp, pl, sy, pt, tiles, dt, lbeg, do_pairs); break;
+    }
+}
+void launch_sums_end(hipStream_t st, PlDev pl, SysDev sy, int nsys)
+{
+    k_sums_end<<<cdiv(6 * nsys, 64), 64, 0, st>>>(pl, sy, nsys);
+}
+void launch_pl_end_apply(hipStream_t st, PlDev pl, SysDev sy, int npl, double dth)
+{
+    if (npl) k_pl_end_apply<<<cdiv(npl, 256), 256, 0, st>>>(pl, sy, npl, dth);
+}
+void launch_tp_post(hipStream_t st, TpDev tp, SysDev sy, int ntp, double dth)
+{
+    if (ntp) k_tp_post<<<cdiv(ntp, 256), 256, 0, st>>>(tp, sy, ntp, dth);
+}
+void launch_lfirst_update(hipStream_t st, SysDev sy, int nsys)
+{
+    k_lfirst_update<<<cdiv(nsys, 256), 256, 0, st>>>(sy, nsys);
+}
+void launch_snapshot(hipStream_t st, PlDev pl, TpDev tp, SysDev sy, int nsys, int npl, int ntp)
+{
+    k_vh2vb_sum<<<cdiv(3 * nsys, 64), 64, 0, st>>>(pl, sy, nsys);
+    if (npl) k_vh2vb_apply<<<cdiv(npl, 256), 256, 0, st>>>(pl, sy, npl);
+    if (ntp) k_snapshot_tp<<<cdiv(ntp, 256), 256, 0, st>>>(tp, sy, ntp);
+}
+void launch_init_tp_vb(hipStream_t st, TpDev tp, SysDev sy, int ntp)
+{
+    if (ntp) k_init_tp_vb<<<cdiv(ntp, 256), 256, 0, st>>>(tp, sy, ntp);
+}
+
+}  // namespace swg

@@ -1,0 +1,110 @@
+// swg_engine.h -- host-side objects of the library.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "../../include/swiftest_hip.h"
+#include "swg_kernels.h"
+
+namespace swg {
+
+extern thread_local std::string g_err;
+int fail(int code, const std::string &msg);
+int hipfail(hipError_t e, const char *what);
+void host_set_rhill(double GMcb, int n, const double *rh, const double *vh, const double *Gm, double *rhill);
+
+// Device arrays of the WHM integrator (Jacobi coordinates, whm_module.f90).
+struct WhmDev {
+    Vec3p xj, vj, ah, tah;
+    double *eta, *muj, *ir3h, *ir3j;
+    int32_t *iflag;
+};
+
+// A resident ensemble of independent systems.
+struct Ensemble {
+    hipStream_t stream = nullptr;
+    swg_params prm{};
+    Tuning tune{};
+    int nsys = 0, npl = 0, ntp = 0;
+    std::vector<int32_t> h_pl_off, h_tp_off, h_pl_id, h_enc_pp, h_enc_pt, h_err;
+    std::vector<double> h_GMcb, h_Gm;
+
+    PlDev pl{};
+    TpDev tp{};
+    SysDev sy{};
+    ListDev pp{}, pt{};
+    WhmDev whm{};
+    int64_t pp_words = 0, pt_words = 0, npp = 0, npt = 0, cap_pp = 0, cap_pt = 0;
+
+    Tile *d_tiles_pl = nullptr, *d_tiles_tp = nullptr;
+    int4 *d_tiles_pp = nullptr, *d_tiles_pt = nullptr;
+    int ntiles_pl = 0, ntiles_tp = 0, ntiles_pp = 0;
+    int32_t *d_any = nullptr, *d_pt_run = nullptr;
+    int64_t *d_total = nullptr;
+    void *d_scan = nullptr;
+    char *h_small = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<void *> owned;
+
+    swg_stats st_{};
+    int64_t iloop = 0, iout = 0;
+    bool initial_snapshot_done = false;
+
+    // live timing of the two acceleration kernels: HIP events recorded on the
+    // launch stream around every launch while enabled
+    bool ktiming = false;
+    std::vector<hipEvent_t> ev_tp, ev_pl;
+    size_t nev_tp = 0, nev_pl = 0;
+    int record_kernel_event(std::vector<hipEvent_t> &pool, size_t &n);
+    int kernel_times(double *ms_tp, int64_t *n_tp, double *ms_pl, int64_t *n_pl);
+
+    ~Ensemble();
+    int create(const swg_params &p, int nsys, const int32_t *npl, const int32_t *ntp, const double *GMcb,
+               const double *pl_rh, const double *pl_vh, const double *Gmass, const double *radius,
+               const double *rhill, const double *tp_rh, const double *tp_vh);
+    int run(int64_t nsteps);
+    int step();
+    int snapshot();
+    int get(double *pl_rh, double *pl_vh, double *pl_vb, double *tp_rh, double *tp_vh, double *tp_vb,
+            int32_t *pl_status, int32_t *tp_status, int32_t *pl_id);
+    int get_misc(double *rhill, double *cb_vb, double *ptbeg, double *ptend, int32_t *lfirst);
+    int enc_list(int which, int64_t *n, int32_t *i1, int32_t *i2);
+    int time_kernels(int reps, double *ms_tp, double *ms_pl);
+    int check_errors();
+    int err_code();
+
+    // internals
+    template <class T> int own(T **p, size_t n);
+    template <class T> int own2(T **p, size_t n);
+    int alloc_vec(Vec3p &v, size_t n);
+    void build_tiles();
+    int ensure_list(ListDev &L, int64_t n, int64_t &cap);
+    int detect_pp(double dt);
+    int detect_pt(double dt, const std::vector<int32_t> &run);
+    int build_adjacency();
+    int read_any(int32_t &v);
+    int recur(int ireci);
+    int step_helio_symba();
+    int whm_setup();
+    int step_whm();
+    int whm_snapshot();
+};
+
+// Scratch of the leaf-operator entry points.
+struct LeafState {
+    std::vector<void *> bufs;
+    std::vector<size_t> sizes;
+    ListDev L{};
+    int64_t nenc = 0, cap = 0;
+    ~LeafState();
+    void *get(int slot, size_t bytes, int *rc);
+};
+
+}  // namespace swg
+
+// A library context: one device, one stream, leaf-operator scratch.
+struct swg_ctx {
+    int device;
+    hipStream_t stream;
+    swg::LeafState *leaf;
+};

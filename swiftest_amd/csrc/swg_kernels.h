@@ -1,0 +1,72 @@
+// swg_kernels.h -- host-side launchers of the device kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "swg_state.h"
+
+namespace swg {
+
+// A block of consecutive bodies of one system: bodies [b0, b1) of system s.
+struct Tile {
+    int32_t s, b0, b1, pad;
+};
+
+// Kernel-tuning knobs chosen per launch shape by the autotuner.
+struct Tuning {
+    int kick_pl_block = 256;   // 64, 128 or 256 threads per workgroup
+    int kick_tp_block = 256;
+};
+
+// ---- step kernels (k_step.hip) ----
+void launch_reset_levels(hipStream_t st, PlDev pl, TpDev tp, int npl, int ntp);
+void launch_sums_begin(hipStream_t st, PlDev pl, SysDev sy, int nsys, int force_vh2vb);
+void launch_pl_begin_apply(hipStream_t st, PlDev pl, SysDev sy, int npl, double dth, int force_vh2vb);
+void launch_kick_pl(hipStream_t st, const Tuning &tu, PlDev pl, SysDev sy, ListDev pp, const Tile *tiles,
+                    int ntiles, double dt, int lbeg, int use_radius, int do_pairs);
+void launch_drift(hipStream_t st, Vec3p r, Vec3p v, int32_t *status, int32_t *levelg, const int32_t *sys,
+                  const double *GMcb, int32_t *err, int n, double dt, int irec, int use_level, int is_tp);
+void launch_tp_pre(hipStream_t st, TpDev tp, SysDev sy, int ntp, double dth);
+void launch_kick_tp(hipStream_t st, const Tuning &tu, TpDev tp, PlDev pl, SysDev sy, ListDev pt,
+                    const Tile *tiles, int ntiles, double dt, int lbeg, int do_pairs);
+void launch_sums_end(hipStream_t st, PlDev pl, SysDev sy, int nsys);
+void launch_pl_end_apply(hipStream_t st, PlDev pl, SysDev sy, int npl, double dth);
+void launch_tp_post(hipStream_t st, TpDev tp, SysDev sy, int ntp, double dth);
+void launch_lfirst_update(hipStream_t st, SysDev sy, int nsys);
+void launch_snapshot(hipStream_t st, PlDev pl, TpDev tp, SysDev sy, int nsys, int npl, int ntp);
+void launch_init_tp_vb(hipStream_t st, TpDev tp, SysDev sy, int ntp);
+
+// ---- encounter detection (k_enc.hip) ----
+void launch_pp_tiles(hipStream_t st, PlDev pl, SysDev sy, ListDev pp, const int4 *tiles, int ntiles,
+                     double dt);
+void launch_pt_tiles(hipStream_t st, PlDev pl, TpDev tp, SysDev sy, ListDev pt, const int4 *tiles,
+                     int ntiles, double dt);
+void launch_rows_count(hipStream_t st, ListDev L, const int32_t *row_sys, int upper, const int32_t *pl_off,
+                       const int32_t *run, int nrows);
+void launch_scan_i32_i64(hipStream_t st, const int32_t *in, int64_t *out, int n, int64_t *total,
+                         void *scratch);
+size_t scan_scratch_bytes(int n);
+void launch_rows_fill(hipStream_t st, ListDev L, const int32_t *row_sys, int upper, const int32_t *pl_off,
+                      const int32_t *col_off, int nrows, int32_t *lev1g = nullptr, int32_t *lev1m = nullptr,
+                      int32_t *lev2g = nullptr, int32_t *lev2m = nullptr);
+void launch_sys_flags(hipStream_t st, ListDev L, const int32_t *pl_off, int32_t *flag, int nsys);
+void launch_col_count(hipStream_t st, ListDev L, const int32_t *row_off, const int32_t *col_sys,
+                      const int32_t *col_off, const int32_t *run, int ncols, int upper);
+void launch_col_fill(hipStream_t st, ListDev L, const int32_t *row_off, const int32_t *col_sys,
+                     const int32_t *col_off, const int32_t *run, int ncols, int upper);
+
+// ---- SyMBA recursion (k_enc.hip) ----
+void launch_list_check_pp(hipStream_t st, ListDev pp, PlDev pl, SysDev sy, int64_t n, int irec, double dt,
+                          int32_t *any);
+void launch_list_check_pt(hipStream_t st, ListDev pt, PlDev pl, TpDev tp, SysDev sy, int64_t n, int irec,
+                          double dt, int short_circuit, int32_t *any);
+void launch_kick_list_pl(hipStream_t st, ListDev pp, PlDev pl, int npl, int irec, int sgn, double dt);
+void launch_kick_list_tp(hipStream_t st, ListDev pt, PlDev pl, TpDev tp, int ntp, int irec, int sgn,
+                         double dt);
+void launch_collision(hipStream_t st, ListDev pp, ListDev pt, PlDev pl, TpDev tp, SysDev sy, int64_t npp,
+                      int64_t npt, int irec, double dt, int32_t *any);
+void launch_set_levels(hipStream_t st, PlDev pl, TpDev tp, ListDev pp, ListDev pt, int npl, int ntp,
+                       int64_t npp, int64_t npt, int ireci);
+void launch_clear_flags(hipStream_t st, SysDev sy, int nsys, int32_t *any);
+
+}  // namespace swg

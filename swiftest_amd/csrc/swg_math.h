@@ -1,0 +1,429 @@
+// swg_math.h -- per-body / per-pair arithmetic of the hot path, device side.
+//
+// Floating-point rule (the library is compiled with -ffp-contract=off, so the
+// only fused operations are the fma() calls written here):
+//   (1) 3-vector squared norms and dot products: fma(z1,z2, fma(y1,y2, x1*x2));
+//   (2) pair-sum accumulations acc +/- f*x, velocity kicks v += a*dt and
+//       linear drifts r += p*dt: one fma;
+//   (3) everything else rounded separately, in the reference's order.
+// sqrt and '/' are the IEEE operations.  With this rule every kernel is
+// bit-reproducible against a strict CPU evaluation of the same expressions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace swg {
+
+// src/globals/globals_module.f90:33-38
+constexpr double PIBY2 = 1.570796326794896619231321691639751442099;
+constexpr double PI3BY2 = 4.712388980384689857693965074919254326296;
+constexpr double TWOPI = 6.283185307179586476925286766559005768394;
+constexpr double THIRD = 0.333333333333333333333333333333333333333;
+constexpr double SIXTH = 0.166666666666666666666666666666666666667;
+// src/swiftest/swiftest_drift.f90:11-16
+constexpr double E2MAX = 0.36, DM2MAX = 0.16, E2DM2MAX = 0.0016, DANBYB = 1.0E-13;
+constexpr int NLAG1 = 50, NLAG2 = 40;
+// src/symba/symba_module.f90:20-23
+constexpr int NTENC = 3;
+constexpr double RHSCALE = 6.5, RSHELL = 0.48075;
+// src/swiftest/swiftest_orbel.f90:11
+constexpr double TINYVALUE = 4.0e-15;
+constexpr double DBL_TINY = 2.2250738585072014e-308;   // tiny(1.0_DP)
+constexpr double VSMALL = 1.4916681462400413e-154;     // sqrt(tiny(1.0_DP)), globals_module.f90:135
+
+// status codes, globals_module.f90:86-103
+constexpr int ACTIVE = 0, INACTIVE = 1, DISCARDED_DRIFTERR = -7, COLLIDED = -14;
+
+__host__ __device__ inline double dot3(double x1, double y1, double z1, double x2, double y2, double z2)
+{
+    return fma(z1, z2, fma(y1, y2, x1 * x2));
+}
+
+// ---------------------------------------------------------------------------
+// Danby / universal-variable Kepler drift (swiftest_drift.f90:111-580).
+// ---------------------------------------------------------------------------
+
+// swiftest_orbel_scget (swiftest_orbel.f90:147-172)
+__device__ inline void scget(double angle, double &sx, double &cx)
+{
+    int nper = (int)(angle / TWOPI);
+    double x = angle - nper * TWOPI;
+    if (x < 0.0) x = x + TWOPI;
+    sx = sin(x);
+    cx = sqrt(1.0 - sx * sx);
+    if ((x > PIBY2) && (x < PI3BY2)) cx = -cx;
+}
+
+// Stumpff c0..c3 with /4 argument reduction (:536-580)
+__host__ __device__ inline void stumpff(double x, double &c0, double &c1, double &c2, double &c3)
+{
+    int n = 0;
+    while (fabs(x) >= 0.1) {
+        n += 1;
+        x = x / 4.0;
+    }
+    double t = 1.0 - x / 182.0;
+    t = 1.0 - (x * t) / 132.0;
+    t = 1.0 - (x * t) / 90.0;
+    t = 1.0 - (x * t) / 56.0;
+    t = 1.0 - (x * t) / 30.0;
+    c2 = (1.0 - (x * t) / 12.0) / 2.0;
+    t = 1.0 - x / 210.0;
+    t = 1.0 - (x * t) / 156.0;
+    t = 1.0 - (x * t) / 110.0;
+    t = 1.0 - (x * t) / 72.0;
+    t = 1.0 - (x * t) / 42.0;
+    c3 = (1.0 - (x * t) / 20.0) / 6.0;
+    c1 = 1.0 - x * c3;
+    c0 = 1.0 - x * c2;
+    for (int i = n; i >= 1; i--) {
+        c3 = (c2 + c0 * c3) / 4.0;
+        c2 = (c1 * c1) / 2.0;
+        c1 = c0 * c1;
+        c0 = (2.0 * c0) * c0 - 1.0;
+    }
+}
+
+// Kepler's equation in difference form for small dm and e (:234-276)
+__device__ inline void kepmd(double dm, double es, double ec, double &x, double &s, double &c)
+{
+    const double a0 = 39916800.0, a1 = 6652800.0, a2 = 332640.0, a3 = 7920.0, a4 = 110.0;
+    double fac1 = 1.0 / (1.0 - ec);
+    double q = fac1 * dm;
+    double fac2 = (es * es) * fac1 - ec / 3.0;
+    x = q * (1.0 - ((0.5 * fac1) * q) * (es - q * fac2));
+    double y = x * x;
+    s = (x * (a0 - y * (a1 - y * (a2 - y * (a3 - y * (a4 - y)))))) / a0;
+    c = sqrt(1.0 - s * s);
+    double f = ((x - ec * s) + es * (1.0 - c)) - dm;
+    double fp = (1.0 - ec * c) + es * s;
+    double fpp = ec * s + es * c;
+    double fppp = ec * c - es * s;
+    double dx = -f / fp;
+    dx = -f / (fp + (dx * fpp) / 2.0);
+    dx = -f / ((fp + (dx * fpp) / 2.0) + ((dx * dx) * fppp) * SIXTH);
+    x = x + dx;
+    y = x * x;
+    s = (x * (a0 - y * (a1 - y * (a2 - y * (a3 - y * (a4 - y)))))) / a0;
+    c = sqrt(1.0 - s * s);
+}
+
+// f(s) of Kepler's equation in universal variables (:307-334)
+__device__ inline double kepu_fchk(double dt, double r0, double mu, double alpha, double u, double s)
+{
+    double c0, c1, c2, c3;
+    stumpff((s * s) * alpha, c0, c1, c2, c3);
+    c1 = c1 * s;
+    c2 = c2 * (s * s);
+    c3 = c3 * ((s * s) * s);
+    return ((r0 * c1 + u * c2) + mu * c3) - dt;
+}
+
+// real root of the cubic for the hyperbolic first guess (:486-533)
+__device__ inline int kepu_p3solve(double dt, double r0, double mu, double alpha, double u, double &s)
+{
+    double denom = (mu - alpha * r0) * SIXTH;
+    double a2 = (0.5 * u) / denom;
+    double a1 = r0 / denom;
+    double a0 = -dt / denom;
+    double q = (a1 - (a2 * a2) * THIRD) * THIRD;
+    double r = ((a1 * a2 - 3.0 * a0) * SIXTH) - ((a2 * a2) * a2) / 27.0;
+    double sq2 = (q * q) * q + r * r;
+    if (sq2 >= 0.0) {
+        double sq = sqrt(sq2);
+        double p1 = ((r + sq) <= 0.0) ? -pow(-(r + sq), THIRD) : pow(r + sq, THIRD);
+        double p2 = ((r - sq) <= 0.0) ? -pow(-(r - sq), THIRD) : pow(r - sq, THIRD);
+        s = (p1 + p2) - a2 * THIRD;
+        return 0;
+    }
+    s = 0.0;
+    return 1;
+}
+
+// initial guess (:337-378)
+__device__ inline double kepu_guess(double dt, double r0, double mu, double alpha, double u)
+{
+    double s;
+    if (alpha > 0.0) {
+        if (dt / r0 <= 0.4) {
+            s = dt / r0 - ((dt * dt) * u) / (((2.0 * r0) * r0) * r0);
+        } else {
+            double a = mu / alpha;
+            double en = sqrt(mu / ((a * a) * a));
+            double ec = 1.0 - r0 / a;
+            double es = u / ((en * a) * a);
+            double e = sqrt(ec * ec + es * es);
+            double y = en * dt - es;
+            double sy, cy;
+            scget(y, sy, cy);
+            double sigma = copysign(1.0, es * cy + ec * sy);
+            double x = y + (sigma * 0.85) * e;
+            s = x / sqrt(alpha);
+        }
+    } else {
+        if (kepu_p3solve(dt, r0, mu, alpha, u, s) != 0) s = dt / r0;
+    }
+    return s;
+}
+
+// Newton with Halley/third-order corrections, at most 7 iterations (:435-483)
+__device__ inline int kepu_new(double &s, double dt, double r0, double mu, double alpha, double u, double &fp,
+                               double &c1, double &c2, double &c3)
+{
+    for (int nc = 0; nc <= 6; nc++) {
+        double c0;
+        stumpff((s * s) * alpha, c0, c1, c2, c3);
+        c1 = c1 * s;
+        c2 = (c2 * s) * s;
+        c3 = ((c3 * s) * s) * s;
+        double f = ((r0 * c1 + u * c2) + mu * c3) - dt;
+        fp = (r0 * c0 + u * c1) + mu * c2;
+        double fpp = (-(r0 * alpha) + mu) * c1 + u * c0;
+        double fppp = (-(r0 * alpha) + mu) * c0 - (u * alpha) * c1;
+        double ds = -f / fp;
+        ds = -f / (fp + (ds * fpp) / 2.0);
+        ds = -f / ((fp + (ds * fpp) / 2.0) + ((ds * ds) * fppp) / 6.0);
+        s = s + ds;
+        double fdt = f / dt;
+        if (fdt * fdt < DANBYB * DANBYB) return 0;
+    }
+    return 1;
+}
+
+// Laguerre fallback, Danby (1988) (:381-432)
+__device__ inline int kepu_lag(double &s, double dt, double r0, double mu, double alpha, double u, double &fp,
+                               double &c1, double &c2, double &c3)
+{
+    const double ln = 5.0;
+    int ncmax = (alpha < 0.0) ? NLAG2 : NLAG1;
+    for (int nc = 0; nc <= ncmax; nc++) {
+        double c0;
+        stumpff((s * s) * alpha, c0, c1, c2, c3);
+        c1 = c1 * s;
+        c2 = (c2 * s) * s;
+        c3 = ((c3 * s) * s) * s;
+        double f = ((r0 * c1 + u * c2) + mu * c3) - dt;
+        fp = (r0 * c0 + u * c1) + mu * c2;
+        double fpp = (-(r0 * alpha) + mu) * c1 + u * c0;
+        double disc = fabs((((ln - 1.0) * (ln - 1.0)) * fp) * fp - (((ln - 1.0) * ln) * f) * fpp);
+        double ds = -((ln * f) / (fp + copysign(1.0, fp) * sqrt(disc)));
+        s = s + ds;
+        double fdt = f / dt;
+        if (fdt * fdt < DANBYB * DANBYB) return 0;
+    }
+    return 2;
+}
+
+// one Danby drift of (r, v) over dt0 (:141-231); returns iflag
+__device__ inline int drift_dan(double mu, double &rx0, double &ry0, double &rz0, double &vx0, double &vy0,
+                                double &vz0, double dt0)
+{
+    double dt = dt0;
+    double r0 = sqrt(dot3(rx0, ry0, rz0, rx0, ry0, rz0));
+    double v0s = dot3(vx0, vy0, vz0, vx0, vy0, vz0);
+    double u = dot3(rx0, ry0, rz0, vx0, vy0, vz0);
+    double alpha = (2.0 * mu) / r0 - v0s;
+    double f, g, fdot, gdot;
+    if (alpha > 0.0) {
+        double a = mu / alpha;
+        double asq = a * a;
+        double en = sqrt(mu / (a * asq));
+        double ec = 1.0 - r0 / a;
+        double es = u / (en * asq);
+        double esq = ec * ec + es * es;
+        double dm = dt * en - (double)((int)((dt * en) / TWOPI)) * TWOPI;
+        dt = dm / en;
+        if ((esq < E2MAX) && (dm * dm < DM2MAX) && (esq * (dm * dm) < E2DM2MAX)) {
+            double xkep, s, c;
+            kepmd(dm, es, ec, xkep, s, c);
+            double fchk = ((xkep - ec * s) + es * (1.0 - c)) - dm;
+            if (fchk * fchk > DANBYB * DANBYB) return 1;
+            double fp = (1.0 - ec * c) + es * s;
+            f = (a / r0) * (c - 1.0) + 1.0;
+            g = dt + (s - xkep) / en;
+            fdot = ((-(a / (r0 * fp))) * en) * s;
+            gdot = (c - 1.0) / fp + 1.0;
+            double rx = rx0 * f + vx0 * g, ry = ry0 * f + vy0 * g, rz = rz0 * f + vz0 * g;
+            double vx = rx0 * fdot + vx0 * gdot, vy = ry0 * fdot + vy0 * gdot, vz = rz0 * fdot + vz0 * gdot;
+            rx0 = rx; ry0 = ry; rz0 = rz;
+            vx0 = vx; vy0 = vy; vz0 = vz;
+            return 0;
+        }
+    }
+    double s = kepu_guess(dt, r0, mu, alpha, u);
+    double st = s, fp, c1, c2, c3;
+    int iflag = kepu_new(s, dt, r0, mu, alpha, u, fp, c1, c2, c3);
+    if (iflag != 0) {
+        double fo = kepu_fchk(dt, r0, mu, alpha, u, st);
+        double fn = kepu_fchk(dt, r0, mu, alpha, u, s);
+        if (fabs(fo) < fabs(fn)) s = st;
+        iflag = kepu_lag(s, dt, r0, mu, alpha, u, fp, c1, c2, c3);
+    }
+    if (iflag == 0) {
+        f = 1.0 - (mu / r0) * c2;
+        g = dt - mu * c3;
+        fdot = -((mu / (fp * r0)) * c1);
+        gdot = 1.0 - (mu / fp) * c2;
+        double rx = rx0 * f + vx0 * g, ry = ry0 * f + vy0 * g, rz = rz0 * f + vz0 * g;
+        double vx = rx0 * fdot + vx0 * gdot, vy = ry0 * fdot + vy0 * gdot, vz = rz0 * fdot + vz0 * gdot;
+        rx0 = rx; ry0 = ry; rz0 = rz;
+        vx0 = vx; vy0 = vy; vz0 = vz;
+    }
+    return iflag;
+}
+
+// swiftest_drift_one (:111-138): on failure retry as ten dt/10 substeps,
+// stopping at the first substep that fails.
+__device__ inline int drift_one(double mu, double &rx, double &ry, double &rz, double &vx, double &vy, double &vz,
+                                double dt)
+{
+    int fl = drift_dan(mu, rx, ry, rz, vx, vy, vz, dt);
+    if (fl != 0) {
+        double dttmp = 0.1 * dt;
+        for (int i = 0; i < 10; i++) {
+            fl = drift_dan(mu, rx, ry, rz, vx, vy, vz, dttmp);
+            if (fl != 0) break;
+        }
+    }
+    return fl;
+}
+
+// ---------------------------------------------------------------------------
+// Encounter / collision tests
+// ---------------------------------------------------------------------------
+
+// encounter_check_one (encounter_check.f90:573-621); returns lencounter and
+// writes lvdotr.
+__host__ __device__ inline bool encounter_one(double xr, double yr, double zr, double vxr, double vyr, double vzr,
+                                              double renc, double dt, bool &lvdotr)
+{
+    double r2 = dot3(xr, yr, zr, xr, yr, zr);
+    double r2crit = renc * renc;
+    double vdotr, r2min;
+    if (r2 > r2crit) {
+        vdotr = dot3(vxr, vyr, vzr, xr, yr, zr);
+        if (vdotr > 0.0) {
+            r2min = r2;
+        } else {
+            double v2 = dot3(vxr, vyr, vzr, vxr, vyr, vzr);
+            if (v2 <= VSMALL) {
+                r2min = r2;
+            } else {
+                double tmin = -vdotr / v2;
+                if (tmin < dt)
+                    r2min = r2 - (vdotr * vdotr) / v2;
+                else
+                    r2min = (r2 + (2.0 * vdotr) * dt) + v2 * (dt * dt);
+            }
+        }
+    } else {
+        vdotr = -1.0;
+        r2min = r2;
+    }
+    lvdotr = (vdotr < 0.0);
+    return lvdotr && (r2min <= r2crit);
+}
+
+// swiftest_orbel_xv2aeq (swiftest_orbel.f90:700-764), pericentre only
+__device__ inline double xv2q(double mu, double rx, double ry, double rz, double vx, double vy, double vz)
+{
+    double r = sqrt(dot3(rx, ry, rz, rx, ry, rz));
+    double v2 = dot3(vx, vy, vz, vx, vy, vz);
+    double hx = ry * vz - rz * vy, hy = rz * vx - rx * vz, hz = rx * vy - ry * vx;
+    double h2 = dot3(hx, hy, hz, hx, hy, hz);
+    if (h2 < DBL_TINY) return 0.0;
+    double energy = 0.5 * v2 - mu / r;
+    double a = 0.0, e = 0.0, fac = 0.0;
+    int type;
+    if (fabs((energy * r) / mu) < sqrt(TINYVALUE)) {
+        type = 0;
+    } else {
+        a = (-0.5 * mu) / energy;
+        if (a < 0.0) {
+            fac = -h2 / (mu * a);
+            type = (fac > TINYVALUE) ? 1 : 0;
+        } else {
+            type = -1;
+        }
+    }
+    if (type == -1) {
+        fac = 1.0 - h2 / (mu * a);
+        if (fac > TINYVALUE) e = sqrt(fac);
+        return a * (1.0 - e);
+    }
+    if (type == 0) return (0.5 * h2) / mu;
+    e = sqrt(1.0 + fac);
+    return a * (1.0 - e);
+}
+
+// collision_check_one (collision_check.f90:16-50)
+__device__ inline bool collision_one(double xr, double yr, double zr, double vxr, double vyr, double vzr,
+                                     double Gmtot, double rlim, double dt, bool lvdotr)
+{
+    double r2 = dot3(xr, yr, zr, xr, yr, zr);
+    if (r2 <= rlim * rlim) return true;
+    double vdotr = dot3(xr, yr, zr, vxr, vyr, vzr);
+    if (lvdotr && (vdotr > 0.0)) {
+        double tcr2 = r2 / dot3(vxr, vyr, vzr, vxr, vyr, vzr);
+        if (tcr2 <= dt * dt) return xv2q(Gmtot, xr, yr, zr, vxr, vyr, vzr) < rlim;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// SyMBA shell kick factor (symba_kick.f90:187-208)
+// ---------------------------------------------------------------------------
+
+// x**n for a run-time integer n by binary powering (what RSHELL**(2*irecl)
+// compiles to)
+__host__ __device__ inline double powi(double x, int m)
+{
+    unsigned n = m < 0 ? -(unsigned)m : (unsigned)m;
+    double y = (n % 2) ? x : 1.0;
+    while (n >>= 1) {
+        x = x * x;
+        if (n % 2) y = y * x;
+    }
+    return m < 0 ? 1.0 / y : y;
+}
+
+// r2**(-1.5), correctly rounded via a double-double x^(3/2) and a corrected
+// reciprocal
+__host__ __device__ inline double pow_m1p5(double x)
+{
+    double s = sqrt(x);
+    double sl = fma(-s, s, x) / (2.0 * s);
+    double ph = x * s;
+    double pl = fma(x, s, -ph) + x * sl;
+    double yh = 1.0 / ph;
+    double e = fma(-yh, ph, 1.0) - yh * pl;
+    return fma(yh, e, yh);
+}
+
+// fac for one listed pair at shell radius^2 ri; false when r2 < rim1 (the
+// pair is dropped for this kick)
+__device__ inline bool shell_fac(double ri, double r2, double &fac)
+{
+    double rim1 = ri * (RSHELL * RSHELL);
+    if (r2 < rim1) return false;
+    if (r2 < ri) {
+        double ris = sqrt(ri);
+        double r = sqrt(r2);
+        double rr = (ris - r) / (ris * (1.0 - RSHELL));
+        fac = pow_m1p5(r2) * ((1.0 - 3.0 * (rr * rr)) + 2.0 * ((rr * rr) * rr));
+    } else {
+        fac = 1.0 / (r2 * sqrt(r2));
+    }
+    return true;
+}
+
+// renc at recursion depth irec: rhill * RHSCALE * RSHELL**irec with the
+// power built by repeated multiplication (symba_util.f90:245-267)
+__host__ __device__ inline double rshell_pow(int irec)
+{
+    double r = 1.0;
+    for (int i = 1; i <= irec; i++) r = r * RSHELL;
+    return r;
+}
+
+}  // namespace swg

@@ -1,0 +1,311 @@
+"""CPU checks that pin the restatement (oracle/) itself.
+
+The reference ships no golden vectors or known-answer outputs for this path,
+so each oracle function is pinned by mathematics independent of its own
+formulas: closed forms, exact rational arithmetic, analytic two-body motion,
+conservation laws, and the reference test suite's conservation thresholds
+(tests/test_swiftest.py:118-121).
+"""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from helpers import compact_disk
+from swiftest_amd import ics
+
+
+# ---------------------------------------------------------------- gravity --
+
+def test_two_body_closed_form():
+    r = np.array([[0.3, -0.2, 0.1], [1.3, 0.5, -0.4]])
+    Gm = np.array([2e-4, 3e-3])
+    acc = O.accel_plpl(r, Gm)
+    d = r[1] - r[0]
+    dist = np.sqrt(d @ d)
+    np.testing.assert_allclose(acc[0], Gm[1] * d / dist ** 3, rtol=2e-16 * 8)
+    np.testing.assert_allclose(acc[1], -Gm[0] * d / dist ** 3, rtol=2e-16 * 8)
+
+
+def test_pair_sum_momentum_and_extended_precision():
+    rng = np.random.Generator(np.random.PCG64(1))
+    n = 64
+    r = rng.normal(0, 1, (n, 3))
+    Gm = rng.uniform(1e-6, 1e-3, n)
+    acc = O.accel_plpl(r, Gm)
+    # sum_i m_i a_i = 0 (Newton's third law), to rounding
+    mom = (Gm[:, None] * acc).sum(0)
+    scale = (Gm[:, None] * np.abs(acc)).sum(0)
+    assert np.all(np.abs(mom) <= 1e-13 * scale)
+    # against an extended-precision direct sum
+    rl = r.astype(np.longdouble)
+    ref = np.zeros((n, 3), np.longdouble)
+    for i in range(n):
+        d = rl - rl[i]
+        d2 = (d * d).sum(1)
+        d2[i] = 1
+        f = Gm.astype(np.longdouble) / (d2 * np.sqrt(d2))
+        f[i] = 0
+        ref[i] = (f[:, None] * d).sum(0)
+    np.testing.assert_allclose(acc, ref.astype(np.float64), rtol=1e-12, atol=1e-16 * np.abs(acc).max())
+
+
+def test_radius_overlap_excluded():
+    r = np.array([[0.0, 0.0, 0.0], [1e-3, 0.0, 0.0], [1.0, 0.0, 0.0]])
+    Gm = np.array([1e-5, 1e-5, 1e-5])
+    R = np.array([1e-3, 1e-3, 1e-4])   # bodies 0 and 1 overlap
+    acc = O.accel_plpl(r, Gm, R)
+    assert np.isclose(acc[0][0], Gm[2] / 1.0, rtol=1e-15)
+    acc_n = O.accel_plpl(r, Gm)
+    assert acc_n[0][0] > acc[0][0]
+
+
+def test_pair_list_removal_equals_full_rows():
+    rng = np.random.Generator(np.random.PCG64(2))
+    n = 20
+    r = rng.normal(0, 1, (n, 3))
+    Gm = rng.uniform(1e-6, 1e-4, n)
+    k = np.array([(i + 1, j + 1) for i in range(n) for j in range(i + 1, n)], np.int32)
+    full = O.accel_plpl(r, Gm)
+    pairs = O.accel_pairs(k, r, Gm)
+    np.testing.assert_allclose(pairs, full, rtol=1e-12, atol=1e-15)
+
+
+def test_tp_accel_extended_precision():
+    rng = np.random.Generator(np.random.PCG64(3))
+    rpl = rng.normal(0, 1, (30, 3))
+    Gm = rng.uniform(1e-6, 1e-3, 30)
+    rtp = rng.normal(0, 2, (50, 3))
+    acc = O.accel_pltp(rtp, rpl, Gm)
+    d = rtp[:, None, :].astype(np.longdouble) - rpl[None].astype(np.longdouble)
+    d2 = (d * d).sum(-1)
+    ref = -(Gm[None, :, None] * d / (d2 * np.sqrt(d2))[..., None]).sum(1)
+    np.testing.assert_allclose(acc, ref.astype(np.float64), rtol=1e-13, atol=1e-18)
+
+
+# ------------------------------------------------------------------ drift --
+
+def _kepler_analytic(mu, r0, v0, dt):
+    """Independent two-body propagation: elements, mean anomaly advance,
+    Kepler's equation by bisection+Newton in extended precision, back to xv."""
+    r0 = np.asarray(r0, np.longdouble)
+    v0 = np.asarray(v0, np.longdouble)
+    mu = np.longdouble(mu)
+    rr = np.sqrt(r0 @ r0)
+    a = 1 / (2 / rr - (v0 @ v0) / mu)
+    n = np.sqrt(mu / a ** 3)
+    ecosE = 1 - rr / a
+    esinE = (r0 @ v0) / (n * a * a)
+    E0 = np.arctan2(esinE, ecosE)
+    e = np.sqrt(ecosE ** 2 + esinE ** 2)
+    M = E0 - esinE + n * np.longdouble(dt)
+    E = M
+    for _ in range(100):
+        E = E - (E - e * np.sin(E) - M) / (1 - e * np.cos(E))
+    dE = E - E0
+    f = a / rr * (np.cos(dE) - 1) + 1
+    g = np.longdouble(dt) + (np.sin(dE) - dE) / n
+    r1 = f * r0 + g * v0
+    r1m = np.sqrt(r1 @ r1)
+    fd = -a * a * n * np.sin(dE) / (rr * r1m)
+    gd = 1 + a / r1m * (np.cos(dE) - 1)
+    return (r1).astype(np.float64), (fd * r0 + gd * v0).astype(np.float64)
+
+
+@pytest.mark.parametrize("dt", [0.005, 0.3, 5.0])
+def test_drift_matches_analytic_kepler(dt):
+    rng = np.random.Generator(np.random.PCG64(4))
+    a = rng.uniform(0.3, 20, 40)
+    e = rng.uniform(0, 0.6, 40)
+    inc = rng.uniform(0, 1, 40)
+    r, v = ics.el2xv(ics.GMSUN, a, e, inc, *(rng.uniform(0, 2 * np.pi, 40) for _ in range(3)))
+    x, vv, fl = O.drift(ics.GMSUN, r, v, dt)
+    assert not fl.any()
+    for i in range(len(r)):
+        ra, va = _kepler_analytic(ics.GMSUN, r[i], v[i], dt)
+        np.testing.assert_allclose(x[i], ra, rtol=0, atol=1e-11 * np.linalg.norm(ra))
+        np.testing.assert_allclose(vv[i], va, rtol=0, atol=1e-11 * np.linalg.norm(va))
+
+
+def test_drift_conserves_energy_and_momentum():
+    rng = np.random.Generator(np.random.PCG64(5))
+    r, v = ics.el2xv(ics.GMSUN, rng.uniform(0.5, 5, 200), rng.uniform(0, 0.9, 200), rng.uniform(0, 1, 200),
+                     *(rng.uniform(0, 2 * np.pi, 200) for _ in range(3)))
+    v = v * rng.uniform(0.5, 1.8, (200, 1))  # includes hyperbolic orbits (Laguerre / cubic guess)
+    x, vv, fl = O.drift(ics.GMSUN, r, v, 0.7)
+    ok = fl == 0
+    E0 = 0.5 * (v * v).sum(1) - ics.GMSUN / np.linalg.norm(r, axis=1)
+    E1 = 0.5 * (vv * vv).sum(1) - ics.GMSUN / np.linalg.norm(x, axis=1)
+    np.testing.assert_allclose(E1[ok], E0[ok], rtol=1e-11, atol=1e-12)
+    L0, L1 = np.cross(r, v), np.cross(x, vv)
+    np.testing.assert_allclose(L1[ok], L0[ok], rtol=1e-11, atol=1e-13)
+
+
+@pytest.mark.parametrize("x", [1e-3, 0.05, 0.7, 3.0, 40.0, -0.3, -5.0, -30.0])
+def test_stumpff_closed_forms(x):
+    c0, c1, c2, c3 = O.stumpff(x)
+    s = np.sqrt(abs(x))
+    if x > 0:
+        e0, e1, e2, e3 = np.cos(s), np.sin(s) / s, (1 - np.cos(s)) / x, (s - np.sin(s)) / (x * s)
+    else:
+        e0, e1, e2, e3 = np.cosh(s), np.sinh(s) / s, (np.cosh(s) - 1) / -x, (np.sinh(s) - s) / (-x * s)
+    for a, b in ((c0, e0), (c1, e1), (c2, e2), (c3, e3)):
+        assert abs(a - b) <= 1e-13 * max(1.0, abs(b))
+
+
+# -------------------------------------------------------------- encounters --
+
+def _exact_encounter(xr, vr, R, dt):
+    """Closed-form minimum distance of linear relative motion over [0, dt] in
+    exact rational arithmetic, with the approach condition v.r < 0."""
+    x = [Fraction(float(q)) for q in xr]
+    v = [Fraction(float(q)) for q in vr]
+    R2 = Fraction(float(R)) ** 2
+    r2 = sum(q * q for q in x)
+    vdotr = sum(a * b for a, b in zip(x, v))
+    v2 = sum(q * q for q in v)
+    if r2 <= R2:
+        return True, 0
+    if vdotr >= 0 or v2 == 0:
+        return False, abs(float(r2 - R2))
+    tmin = -vdotr / v2
+    t = min(tmin, Fraction(float(dt)))
+    rmin2 = r2 + 2 * vdotr * t + v2 * t * t
+    return rmin2 <= R2, abs(float((rmin2 - R2) / R2))
+
+
+def test_encounter_branches():
+    # inside the critical radius: encounter whatever the velocity
+    assert O.encounter_one((0.1, 0, 0), (1, 0, 0), 0.5, 0.01) == (True, True)
+    # receding: no encounter
+    assert O.encounter_one((2, 0, 0), (1, 0, 0), 1.0, 10.0) == (False, False)
+    # closest approach inside the step (tmin < dt)
+    assert O.encounter_one((2, 0.5, 0), (-1, 0, 0), 0.6, 5.0) == (True, True)
+    # approaching but the step ends first (tmin >= dt), endpoint distance 1.0 == R: inclusive
+    assert O.encounter_one((2, 0, 0), (-1, 0, 0), 1.0, 1.0) == (True, True)
+    assert O.encounter_one((2, 0, 0), (-1, 0, 0), 0.999, 1.0) == (False, True)
+    # no relative motion
+    assert O.encounter_one((2, 0, 0), (0, 0, 0), 1.0, 1.0) == (False, False)
+
+
+def test_encounter_list_exact_rational_brute_force():
+    rng = np.random.Generator(np.random.PCG64(6))
+    n = 60
+    r = rng.uniform(-1, 1, (n, 3))
+    v = rng.normal(0, 1, (n, 3))
+    renc = rng.uniform(0.02, 0.15, n)
+    dt = 0.1
+    i1, i2, lv = O.encounters_plpl(r, v, renc, dt)
+    got = set(zip(i1.tolist(), i2.tolist()))
+    assert list(zip(i1, i2)) == sorted(got)   # ordered index1 then index2
+    for i in range(n):
+        for j in range(i + 1, n):
+            enc, margin = _exact_encounter(r[j] - r[i], v[j] - v[i], renc[i] + renc[j], dt)
+            if margin < 1e-12:
+                continue
+            assert ((i + 1, j + 1) in got) == enc, (i, j)
+    assert lv.all()
+
+
+# ---------------------------------------------------------- shell kicks --
+
+def test_pow_m1p5_correctly_rounded():
+    from decimal import Decimal, getcontext
+    getcontext().prec = 60
+    rng = np.random.Generator(np.random.PCG64(7))
+    xs = np.exp(rng.uniform(-20, 10, 3000))
+    for x in xs:
+        got = O.pow_m1p5(x)
+        exact = Decimal(float(x)) ** Decimal(-1.5)
+        lo, hi = np.nextafter(got, 0), np.nextafter(got, np.inf)
+        # the exact value is closer to got than to either neighbour
+        assert abs(Decimal(float(got)) - exact) <= abs(Decimal(float(lo)) - exact)
+        assert abs(Decimal(float(got)) - exact) <= abs(Decimal(float(hi)) - exact)
+        assert abs(got - x ** -1.5) <= np.spacing(got)   # within 1 ulp of libm pow
+
+
+def test_powi_and_shell_continuity():
+    for n in range(0, 40):
+        assert O.powi(0.48075, n) == pytest.approx(0.48075 ** n, rel=1e-15)
+    assert O.powi(0.48075, 0) == 1.0
+    # fac(r) is continuous at both shell radii: r^-3 at r^2 = ri and 0 at r^2 = rim1
+    ri = 0.3
+    r = np.sqrt(ri)
+    rr = (np.sqrt(ri) - r) / (np.sqrt(ri) * (1 - 0.48075))
+    assert 1 - 3 * rr ** 2 + 2 * rr ** 3 == 1.0
+    rr = 1.0
+    assert 1 - 3 * rr ** 2 + 2 * rr ** 3 == 0.0
+
+
+# ------------------------------------------------------------ whole runs --
+
+def _energy(GMcb, rh, vh, Gm):
+    Mt = GMcb + Gm.sum()
+    vcb = -(Gm[:, None] * vh).sum(0) / Mt
+    vb = vh + vcb
+    ke = 0.5 * (Gm * (vb * vb).sum(1)).sum() + 0.5 * GMcb * (vcb @ vcb)
+    pe = -(GMcb * Gm / np.linalg.norm(rh, axis=1)).sum()
+    d = np.linalg.norm(rh[:, None] - rh[None], axis=-1)
+    iu = np.triu_indices(len(Gm), 1)
+    pe -= (Gm[:, None] * Gm[None] / np.where(d > 0, d, np.inf))[iu].sum()
+    # barycentric angular momentum; the central body's and the frame-offset
+    # terms cancel because the total barycentric momentum is zero
+    Lv = (Gm[:, None] * np.cross(rh, vb)).sum(0)
+    return ke + pe, np.linalg.norm(Lv)
+
+
+@pytest.mark.parametrize("integ", [O.WHM, O.HELIO, O.SYMBA])
+def test_conservation_thresholds(integ):
+    """The reference's own invariant test (tests/test_swiftest.py:112-170):
+    |dE/E0| slope < 1e-8 /yr and |dL/L0| slope < 1e-10 /yr, here on an
+    offline synthetic 8-planet system over 200 yr."""
+    # the eight planets' masses (M_sun), semi-major axes (AU), eccentricities
+    # and inclinations (deg), rounded; orientation angles drawn with a fixed seed
+    m = np.array([1.66e-7, 2.45e-6, 3.0e-6, 3.2e-7, 9.55e-4, 2.86e-4, 4.37e-5, 5.15e-5])
+    a = np.array([0.387, 0.723, 1.0, 1.524, 5.203, 9.537, 19.19, 30.07])
+    e = np.array([0.2056, 0.0068, 0.0167, 0.0934, 0.0489, 0.0565, 0.0457, 0.0113])
+    inc = np.radians([7.0, 3.39, 0.0, 1.85, 1.30, 2.49, 0.77, 1.77])
+    rng = np.random.Generator(np.random.PCG64(7))
+    Gm = ics.GU * m
+    r, v = ics.el2xv(ics.GMSUN + Gm, a, e, inc, *(rng.uniform(0, 2 * np.pi, 8) for _ in range(3)))
+    s = ics.System(ics.GMSUN, ics.RSUN_AU, r, v, Gm, ics._radius_from_mass(m), np.zeros((0, 3)), np.zeros((0, 3)))
+    o = O.OracleSystem(integ, s, 0.01, istep_out=100)
+    st = o.state()
+    Gm = s.Gmass[st["pl_id"] - 1]
+    E0, L0 = _energy(s.GMcb, st["pl_rh"], st["pl_vh"], Gm)
+    ts, dE, dL = [], [], []
+    for k in range(20):
+        assert o.run(1000) == 0
+        st = o.state()
+        E, L = _energy(s.GMcb, st["pl_rh"], st["pl_vh"], Gm)
+        ts.append((k + 1) * 10.0)
+        dE.append(abs((E - E0) / E0))
+        dL.append(abs((L - L0) / L0))
+    assert abs(np.polyfit(ts, dE, 1)[0]) < 1e-8
+    assert abs(np.polyfit(ts, dL, 1)[0]) < 1e-10
+
+
+def test_symba_disk_energy_bounded():
+    d = compact_disk(seed=9, npl=80, mhi=3e-6, alo=0.9, ahi=1.1)
+    o = O.OracleSystem(O.SYMBA, d, 0.005, istep_out=20)
+    st = o.state()
+    Gm = d.Gmass[st["pl_id"] - 1]
+    E0, L0 = _energy(d.GMcb, st["pl_rh"], st["pl_vh"], Gm)
+    assert o.run(200) == 0
+    assert o.stats()["max_depth"] >= 0
+    st = o.state()
+    E, L = _energy(d.GMcb, st["pl_rh"], st["pl_vh"], Gm)
+    assert abs((E - E0) / E0) < 1e-7
+    assert abs((L - L0) / L0) < 1e-10
+
+
+def test_short_circuit_skips_tp_check_when_plpl_encounters():
+    d = compact_disk(seed=10, npl=60, ntp=40)
+    on = O.OracleSystem(O.SYMBA, d, 0.005, short_circuit=True)
+    off = O.OracleSystem(O.SYMBA, d, 0.005, short_circuit=False)
+    on.step()
+    off.step()
+    assert on.stats()["nplpl_enc"] > 0
+    assert on.stats()["npltp_enc"] == 0
+    assert off.stats()["npltp_enc"] > 0
