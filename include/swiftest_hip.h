@@ -36,6 +36,7 @@ extern "C" {
 #define SWG_ERR_COLLISION (-3) /* a collision was detected; its resolution is host work */
 #define SWG_ERR_DRIFT (-4)     /* a massive body failed the Danby drift */
 #define SWG_ERR_DEPTH (-5)     /* recursion step below double precision (symba_step.f90:196-201) */
+#define SWG_ERR_SCRATCH (-6)   /* the recursion's working-set arena was too small */
 
 #define SWG_WHM 0
 #define SWG_HELIO 1

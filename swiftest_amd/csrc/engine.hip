@@ -112,6 +112,8 @@ Ensemble::~Ensemble()
     if (ev1) (void)hipEventDestroy(ev1);
     for (hipEvent_t e : ev_tp) (void)hipEventDestroy(e);
     for (hipEvent_t e : ev_pl) (void)hipEventDestroy(e);
+    if (d_arena) (void)hipFree(d_arena);
+    if (h_subargs) (void)hipHostFree(h_subargs);
 }
 
 template <class T>
@@ -315,6 +317,16 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
     int mx = std::max(npl, ntp);
     HIPCHK(hipMalloc(&d_scan, scan_scratch_bytes(mx)));
     owned.push_back(d_scan);
+    HIPCHK(hipMalloc(&d_scan_sys, scan_scratch_bytes(nsys)));
+    owned.push_back(d_scan_sys);
+    if (own(&d_caps, nsys) || own(&d_arena_off, nsys + 1) || own(&d_sub_stats, 4)) return SWG_ERR_HIP;
+    HIPCHK(hipHostMalloc(&h_subargs, subtree_args_bytes()));
+    HIPCHK(hipMalloc(&d_subargs, subtree_args_bytes()));
+    owned.push_back(d_subargs);
+    {
+        const char *e = getenv("SWG_HOST_RECURSION");
+        host_recursion = e && e[0] == '1';
+    }
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));
 
@@ -506,12 +518,42 @@ int Ensemble::recur(int ireci)
         launch_drift(st, pl.rh, pl.vb, pl.status, pl.levelg, pl.sys, sy.GMcb, sy.err, npl, dtl, ireci, 1, 0);
         launch_drift(st, tp.rh, tp.vb, tp.status, tp.levelg, tp.sys, sy.GMcb, sy.err, ntp, dtl, ireci, 1, 1);
         if (lenc) {
-            if (int rc = recur(irecp)) return rc;
+            // below level 0 the subtree of every system runs in one persistent
+            // workgroup (k_subtree.hip) unless every level is host-driven
+            if (ireci == 0 && !host_recursion) {
+                if (int rc = run_subtree()) return rc;
+            } else {
+                if (int rc = recur(irecp)) return rc;
+            }
         }
         kicks();
         if (prm.lclose) launch_collision(st, pp, pt, pl, tp, sy, npp, npt, ireci, dtl, d_any);
         launch_set_levels(st, pl, tp, pp, pt, npl, ntp, npp, npt, ireci);
     }
+    return 0;
+}
+
+// Size the per-system arenas (room for nested working sets, see
+// k_subtree.hip), then run the recursion of every system from level 1 down.
+int Ensemble::run_subtree()
+{
+    hipStream_t st = stream;
+    const int mult = 6;
+    launch_arena_caps(st, pp, pt, sy, nsys, npp > 0, npt > 0, d_caps, mult);
+    launch_scan_i32_i64(st, d_caps, d_arena_off, nsys, d_arena_off + nsys, d_scan_sys);
+    int64_t *h = (int64_t *)(h_small + 48);
+    HIPCHK(hipMemcpyAsync(h, d_arena_off + nsys, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (*h > arena_words) {
+        if (d_arena) (void)hipFree(d_arena);
+        d_arena = nullptr;
+        int64_t w = *h + *h / 4;
+        HIPCHK(dalloc(&d_arena, (size_t)w));
+        arena_words = w;
+    }
+    launch_subtree(st, pl, tp, pp, pt, sy, nsys, prm.dt, prm.lclose, prm.short_circuit, npp > 0, npt > 0, d_arena,
+                   d_arena_off, d_sub_stats, h_subargs, d_subargs);
+    HIPCHK(hipGetLastError());
     return 0;
 }
 
@@ -529,6 +571,9 @@ int Ensemble::err_code()
     for (int s = 0; s < nsys; s++) {
         if (h_err[s] & 1) return fail(SWG_ERR_COLLISION, "collision detected in system " + std::to_string(s));
         if (h_err[s] & 2) return fail(SWG_ERR_DRIFT, "massive body drift failure in system " + std::to_string(s));
+        if (h_err[s] & 4)
+            return fail(SWG_ERR_SCRATCH, "recursion working sets exceeded their arena in system " + std::to_string(s));
+        if (h_err[s] & 8) return fail(SWG_ERR_DEPTH, "recursion step below double precision in system " + std::to_string(s));
     }
     return 0;
 }
@@ -567,6 +612,7 @@ int Ensemble::step_helio_symba()
     st_.max_depth = -1;
     st_.nsubsteps = 0;
     launch_reset_levels(st, pl, tp, npl, ntp);
+    HIPCHK(hipMemsetAsync(d_sub_stats, 0, 4 * sizeof(int32_t), st));
     npp = npt = 0;
     bool any = false;
     if (symba) {
@@ -624,6 +670,7 @@ int Ensemble::step_helio_symba()
     if (any) {
         if (int rc = recur(0)) return rc;
         trace_mark(st, "recursion");
+        if (trace_on()) subtree_profile_report(st);
     }
     // second half
     if (int rc = kick_pl(0)) return rc;
@@ -902,7 +949,13 @@ int swg_system_get_misc(swg_system *sys, double *rhill, double *cb_vb, double *p
 int swg_system_stats(swg_system *sys, swg_stats *st)
 {
     if (!sys || !st) return fail(SWG_ERR_ARG, "NULL argument");
-    *st = sys->e->st_;
+    swg::Ensemble *e = sys->e;
+    int32_t dev[4] = {0, 0, 0, 0};
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(dev, e->d_sub_stats, sizeof dev, hipMemcpyDeviceToHost));
+    *st = e->st_;
+    st->nsubsteps += dev[0];  // substeps run by the persistent recursion kernel
+    if (dev[1] > st->max_depth) st->max_depth = dev[1];
     return 0;
 }
 

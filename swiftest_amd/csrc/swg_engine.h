@@ -42,6 +42,14 @@ struct Ensemble {
     int32_t *d_any = nullptr, *d_pt_run = nullptr;
     int64_t *d_total = nullptr;
     void *d_scan = nullptr;
+    // recursion below level 0: per-system arenas of the persistent workgroups
+    int32_t *d_caps = nullptr, *d_sub_stats = nullptr, *d_arena = nullptr;
+    int64_t *d_arena_off = nullptr;
+    void *d_scan_sys = nullptr;
+    int64_t arena_words = 0;
+    void *h_subargs = nullptr, *d_subargs = nullptr;
+    bool host_recursion = false;  // SWG_HOST_RECURSION=1: every level driven from the host
+    int run_subtree();
     char *h_small = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<void *> owned;

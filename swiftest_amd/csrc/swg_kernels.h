@@ -69,4 +69,13 @@ void launch_set_levels(hipStream_t st, PlDev pl, TpDev tp, ListDev pp, ListDev p
                        int64_t npp, int64_t npt, int ireci);
 void launch_clear_flags(hipStream_t st, SysDev sy, int nsys, int32_t *any);
 
+// ---- SyMBA recursion below level 0, one workgroup per system (k_subtree.hip) ----
+void launch_arena_caps(hipStream_t st, ListDev pp, ListDev pt, SysDev sy, int nsys, int has_pp, int has_pt,
+                       int32_t *caps, int mult);
+size_t subtree_args_bytes();
+void subtree_profile_report(hipStream_t st);
+void launch_subtree(hipStream_t st, PlDev pl, TpDev tp, ListDev pp, ListDev pt, SysDev sy, int nsys, double dt,
+                    int lclose, int short_circuit, int has_pp, int has_pt, int32_t *arena, const int64_t *arena_off,
+                    int32_t *stats, void *h_stage, void *d_args);
+
 }  // namespace swg

@@ -201,6 +201,25 @@ def test_level0_lists_match():
     _compare_systems(o, g, "lists")
 
 
+def test_recursion_kernel_matches_host_driven_recursion(monkeypatch):
+    """The persistent-workgroup recursion (levels >= 1 on the device) and the
+    level-by-level host-driven recursion give bit-identical states."""
+    sysic = compact_disk(seed=12, npl=200, ntp=150, mhi=2e-5)
+    monkeypatch.setenv("SWG_HOST_RECURSION", "1")
+    h = S.GpuSystem(S.SYMBA, sysic, 0.005, short_circuit=False, lclose=False, istep_out=5)
+    monkeypatch.setenv("SWG_HOST_RECURSION", "0")
+    d = S.GpuSystem(S.SYMBA, sysic, 0.005, short_circuit=False, lclose=False, istep_out=5)
+    for blk in range(3):
+        h.run(5)
+        d.run(5)
+        sh, sd = h.stats(), d.stats()
+        assert (sh["max_depth"], sh["nsubsteps"]) == (sd["max_depth"], sd["nsubsteps"])
+        a, b = h.state(), d.state()
+        for k in ("pl_rh", "pl_vb", "tp_rh", "tp_vb"):
+            assert np.array_equal(a[k], b[k]), (blk, k)
+    assert sh["max_depth"] >= 2
+
+
 def test_ensemble_equals_single_runs():
     systems = [compact_disk(seed=s, npl=n, ntp=t, mhi=2e-6, alo=0.9, ahi=1.1)
                for s, n, t in ((1, 40, 30), (2, 70, 0), (3, 5, 90), (6, 120, 10))]
