@@ -113,6 +113,8 @@ Ensemble::~Ensemble()
     for (hipEvent_t e : ev_tp) (void)hipEventDestroy(e);
     for (hipEvent_t e : ev_pl) (void)hipEventDestroy(e);
     if (d_arena) (void)hipFree(d_arena);
+    for (void *q : lv_buf)
+        if (q) (void)hipFree(q);
     if (h_subargs) (void)hipHostFree(h_subargs);
 }
 
@@ -319,14 +321,18 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
     owned.push_back(d_scan);
     HIPCHK(hipMalloc(&d_scan_sys, scan_scratch_bytes(nsys)));
     owned.push_back(d_scan_sys);
-    if (own(&d_caps, nsys) || own(&d_arena_off, nsys + 1) || own(&d_sub_stats, 4)) return SWG_ERR_HIP;
+    if (own(&bw.keepB, npl) || own(&bw.cntE, npl) || own(&bw.posB, npl) || own(&bw.posE, npl) ||
+        own(&bw.keepBt, ntp) || own(&bw.cntEt, ntp) || own(&bw.posBt, ntp) || own(&bw.posEt, ntp) ||
+        own(&bw.tot, 4) || own(&bw.rng, 4 * (size_t)nsys) || own(&bw.caps, nsys) || own(&bw.maxsz, 1) ||
+        own(&d_arena_off, nsys + 1) || own(&d_sub_stats, 4))
+        return SWG_ERR_HIP;
+    bw.scan = d_scan;
+    lv_buf.assign(MAXLEV + 2, nullptr);
+    lv_cap.assign(MAXLEV + 2, 0);
     HIPCHK(hipHostMalloc(&h_subargs, subtree_args_bytes()));
     HIPCHK(hipMalloc(&d_subargs, subtree_args_bytes()));
     owned.push_back(d_subargs);
-    {
-        const char *e = getenv("SWG_HOST_RECURSION");
-        host_recursion = e && e[0] == '1';
-    }
+    if (const char *e = getenv("SWG_SUBTREE_MAX")) subtree_max = strtoll(e, nullptr, 10);
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));
 
@@ -409,7 +415,7 @@ int Ensemble::detect_pp(double dt)
     launch_scan_i32_i64(st, pp.rowcnt, pp.rowstart, npl, d_total, d_scan);
     launch_sys_flags(st, pp, sy.pl_off, sy.enc_pp, nsys);
     int64_t *ht = (int64_t *)h_small;
-    int32_t *hf = (int32_t *)(h_small + 64);
+    int32_t *hf = (int32_t *)(h_small + 1024);
     HIPCHK(hipMemcpyAsync(ht, d_total, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(hf, sy.enc_pp, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(hf + nsys, sy.err, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -449,7 +455,7 @@ int Ensemble::detect_pt(double dt, const std::vector<int32_t> &run)
     launch_scan_i32_i64(st, pt.rowcnt, pt.rowstart, npl, d_total, d_scan);
     launch_sys_flags(st, pt, sy.pl_off, sy.enc_pt, nsys);
     int64_t *ht = (int64_t *)h_small;
-    int32_t *hf = (int32_t *)(h_small + 64);
+    int32_t *hf = (int32_t *)(h_small + 1024);
     HIPCHK(hipMemcpyAsync(ht, d_total, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(hf, sy.enc_pt, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -486,80 +492,163 @@ int Ensemble::read_any(int32_t &v)
     return 0;
 }
 
-// symba_step_recur_system (symba_step.f90:163-262)
-int Ensemble::recur(int ireci)
+LevelArgs Ensemble::level_args() const
+{
+    LevelArgs la;
+    la.pl = pl;
+    la.tp = tp;
+    la.pp = pp;
+    la.pt = pt;
+    la.sy = sy;
+    la.nsys = nsys;
+    la.has_pp = npp > 0;
+    la.has_pt = npt > 0;
+    return la;
+}
+
+// The single level-0 substep of symba_step_recur_system (symba_step.f90:
+// 163-262 with ireci = 0) over the whole encounter lists.
+int Ensemble::recur0()
+{
+    hipStream_t st = stream;
+    const double dt = prm.dt, dth = 0.5 * dt;
+    const LevelArgs la = level_args();
+    if (st_.max_depth < 0) st_.max_depth = 0;
+    st_.nsubsteps++;
+    launch_clear_flags(st, sy, nsys, d_any);
+    launch_list_check_pp(st, pp, pl, sy, npp, 1, dt, d_any);
+    launch_list_check_pt(st, pt, pl, tp, sy, npt, 1, dt, prm.short_circuit, d_any);
+    int32_t lenc = 0;
+    if (int rc = read_any(lenc)) return rc;
+    launch_level_kicks(st, la, nullptr, npl, ntp, 0, dth);
+    launch_drift(st, pl.rh, pl.vb, pl.status, pl.levelg, pl.sys, sy.GMcb, sy.err, npl, dt, 0, 1, 0);
+    launch_drift(st, tp.rh, tp.vb, tp.status, tp.levelg, tp.sys, sy.GMcb, sy.err, ntp, dt, 0, 1, 1);
+    if (lenc) {
+        LevelView v;
+        LevelCounts c;
+        bool handoff = false;
+        if (int rc = build_level(1, nullptr, v, c, handoff)) return rc;
+        tr_hand[1] += handoff;
+        if (handoff) {
+            if (int rc = run_subtree(1, v)) return rc;
+        } else {
+            if (int rc = recur_level(1, v, c)) return rc;
+        }
+    }
+    launch_level_kicks(st, la, nullptr, npl, ntp, 0, dth);
+    if (prm.lclose) launch_collision(st, pp, pt, pl, tp, sy, npp, npt, 0, dt, d_any);
+    launch_set_levels(st, pl, tp, pp, pt, npl, ntp, npp, npt, 0);
+    return 0;
+}
+
+// Build the whole-device working set of level dnew from src (nullptr: the
+// level-0 adjacency) into the buffer of depth dnew.  handoff tells whether
+// every system's set is small enough for the persistent workgroups; then the
+// arena they need is sized too.
+int Ensemble::build_level(int dnew, const LevelView *src, LevelView &v, LevelCounts &c, bool &handoff)
+{
+    hipStream_t st = stream;
+    const LevelArgs la = level_args();
+    launch_build_count(st, la, src, npl, ntp, dnew, bw);
+    launch_scan_i32_i64(st, bw.caps, d_arena_off, nsys, d_arena_off + nsys, d_scan_sys);
+    int64_t *h = (int64_t *)(h_small + 128);
+    HIPCHK(hipMemcpyAsync(h, bw.tot, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(h + 4, d_arena_off + nsys, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(h + 5, bw.maxsz, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    c.nB = h[0];
+    c.nE = h[1];
+    c.nBt = h[2];
+    c.nEt = h[3];
+    arena_need = h[4];
+    const int32_t maxsz = *(int32_t *)(h + 5);
+    if (c.nE > INT32_MAX || c.nEt > INT32_MAX)
+        return fail(SWG_ERR_SCRATCH, "recursion working set of level " + std::to_string(dnew) + " exceeds 2^31 entries");
+    const int64_t bytes = level_bytes(c);
+    if (bytes > lv_cap[dnew]) {
+        if (lv_buf[dnew]) (void)hipFree(lv_buf[dnew]);
+        lv_buf[dnew] = nullptr;
+        lv_cap[dnew] = 0;
+        const int64_t cap = bytes + bytes / 4 + 256;
+        HIPCHK(hipMalloc(&lv_buf[dnew], (size_t)cap));
+        lv_cap[dnew] = cap;
+    }
+    launch_build_fill(st, la, src, npl, ntp, dnew, bw, lv_buf[dnew], c);
+    v = level_view(lv_buf[dnew], c);
+    handoff = subtree_max >= 0 && maxsz <= subtree_max;
+    return 0;
+}
+
+// symba_step_recur_system (symba_step.f90:163-262) at level d >= 1 on the
+// whole device, all systems in lockstep: a system with no pair at level d has
+// an empty share of the working set, and every launch is a no-op for it.
+int Ensemble::recur_level(int d, const LevelView &v, const LevelCounts &c)
 {
     hipStream_t st = stream;
     const double dt = prm.dt;
     double p3 = 1.0;
-    for (int q = 0; q < ireci; q++) p3 *= NTENC;
+    for (int q = 0; q < d; q++) p3 *= NTENC;
     const double dtl = dt / p3, dth = 0.5 * dtl;
     if (dtl / dt < DBL_EPSILON) return fail(SWG_ERR_DEPTH, "recursion step below double precision");
-    if (ireci > st_.max_depth) st_.max_depth = ireci;
-    const int irecp = ireci + 1;
-    const int nloops = ireci == 0 ? 1 : NTENC;
-    for (int j = 1; j <= nloops; j++) {
+    if (d > st_.max_depth) st_.max_depth = d;
+    const LevelArgs la = level_args();
+    const int nB = (int)c.nB, nBt = (int)c.nBt;
+    for (int j = 1; j <= NTENC; j++) {
         st_.nsubsteps++;
+        tr_sub[d]++;
+        tr_B[d] += c.nB + c.nBt;
+        tr_E[d] += c.nE + c.nEt;
         launch_clear_flags(st, sy, nsys, d_any);
-        launch_list_check_pp(st, pp, pl, sy, npp, irecp, dtl, d_any);
-        launch_list_check_pt(st, pt, pl, tp, sy, npt, irecp, dtl, prm.short_circuit, d_any);
+        launch_level_check(st, la, v, c, d, dtl, prm.short_circuit, d_any);
         int32_t lenc = 0;
         if (int rc = read_any(lenc)) return rc;
-        // the adjacency arrays of a list exist only when the list is non-empty
-        auto kicks = [&]() {
-            if (npp) launch_kick_list_pl(st, pp, pl, npl, irecp, 1, dth);
-            if (npt) launch_kick_list_tp(st, pt, pl, tp, ntp, irecp, 1, dth);
-            if (ireci != 0) {
-                if (npp) launch_kick_list_pl(st, pp, pl, npl, irecp, -1, dth);
-                if (npt) launch_kick_list_tp(st, pt, pl, tp, ntp, irecp, -1, dth);
-            }
-        };
-        kicks();
-        launch_drift(st, pl.rh, pl.vb, pl.status, pl.levelg, pl.sys, sy.GMcb, sy.err, npl, dtl, ireci, 1, 0);
-        launch_drift(st, tp.rh, tp.vb, tp.status, tp.levelg, tp.sys, sy.GMcb, sy.err, ntp, dtl, ireci, 1, 1);
+        launch_level_kicks(st, la, &v, nB, nBt, d, dth);
+        launch_level_drift(st, la, v, nB, nBt, d, dtl);
         if (lenc) {
-            // below level 0 the subtree of every system runs in one persistent
-            // workgroup (k_subtree.hip) unless every level is host-driven
-            if (ireci == 0 && !host_recursion) {
-                if (int rc = run_subtree()) return rc;
+            if (d + 1 > MAXLEV) return fail(SWG_ERR_DEPTH, "recursion deeper than the level stack");
+            LevelView nv;
+            LevelCounts nc;
+            bool handoff = false;
+            if (int rc = build_level(d + 1, &v, nv, nc, handoff)) return rc;
+            tr_hand[d + 1] += handoff;
+            if (handoff) {
+                if (int rc = run_subtree(d + 1, nv)) return rc;
             } else {
-                if (int rc = recur(irecp)) return rc;
+                if (int rc = recur_level(d + 1, nv, nc)) return rc;
             }
         }
-        kicks();
-        if (prm.lclose) launch_collision(st, pp, pt, pl, tp, sy, npp, npt, ireci, dtl, d_any);
-        launch_set_levels(st, pl, tp, pp, pt, npl, ntp, npp, npt, ireci);
+        launch_level_kicks(st, la, &v, nB, nBt, d, dth);
+        if (prm.lclose) launch_level_collision(st, la, v, nB, nBt, dtl);
+        launch_level_set_levels(st, la, v, c, d);
     }
     return 0;
 }
 
-// Size the per-system arenas (room for nested working sets, see
-// k_subtree.hip), then run the recursion of every system from level 1 down.
-int Ensemble::run_subtree()
+// Run the recursion of every system from level d0 down in one persistent
+// workgroup each (k_subtree.hip), from the level-d0 set just built.
+int Ensemble::run_subtree(int d0, const LevelView &v)
 {
     hipStream_t st = stream;
-    const int mult = 6;
-    launch_arena_caps(st, pp, pt, sy, nsys, npp > 0, npt > 0, d_caps, mult);
-    launch_scan_i32_i64(st, d_caps, d_arena_off, nsys, d_arena_off + nsys, d_scan_sys);
-    int64_t *h = (int64_t *)(h_small + 48);
-    HIPCHK(hipMemcpyAsync(h, d_arena_off + nsys, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    if (*h > arena_words) {
+    if (arena_need > arena_words) {
         if (d_arena) (void)hipFree(d_arena);
         d_arena = nullptr;
-        int64_t w = *h + *h / 4;
+        arena_words = 0;
+        const int64_t w = arena_need + arena_need / 4 + 64;
         HIPCHK(dalloc(&d_arena, (size_t)w));
         arena_words = w;
     }
-    launch_subtree(st, pl, tp, pp, pt, sy, nsys, prm.dt, prm.lclose, prm.short_circuit, npp > 0, npt > 0, d_arena,
-                   d_arena_off, d_sub_stats, h_subargs, d_subargs);
+    // the staging buffer of the argument block is rewritten only after the
+    // next stream synchronization (build_level or read_any), so the previous
+    // copy out of it has completed
+    launch_subtree(st, level_args(), v, bw.rng, d0, prm.dt, prm.lclose, prm.short_circuit, d_arena, d_arena_off,
+                   d_sub_stats, h_subargs, d_subargs);
     HIPCHK(hipGetLastError());
     return 0;
 }
 
 int Ensemble::check_errors()
 {
-    int32_t *hf = (int32_t *)(h_small + 64);
+    int32_t *hf = (int32_t *)(h_small + 1024);
     HIPCHK(hipMemcpyAsync(hf, sy.err, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     h_err.assign(hf, hf + nsys);
@@ -598,6 +687,18 @@ static void trace_mark(hipStream_t st, const char *what)
     auto now = std::chrono::steady_clock::now();
     fprintf(stderr, "[swg] %-18s %10.3f ms\n", what, std::chrono::duration<double, std::milli>(now - last).count());
     last = now;
+}
+
+void Ensemble::trace_levels()
+{
+    fprintf(stderr, "[swg] host level substeps   avg_nB   avg_nE handoffs_in\n");
+    for (int d = 1; d <= MAXLEV; d++) {
+        if (tr_sub[d] || tr_hand[d])
+            fprintf(stderr, "[swg] %10d %8lld %8.0f %8.0f %8lld\n", d, (long long)tr_sub[d],
+                    tr_sub[d] ? (double)tr_B[d] / tr_sub[d] : 0.0, tr_sub[d] ? (double)tr_E[d] / tr_sub[d] : 0.0,
+                    (long long)tr_hand[d]);
+        tr_sub[d] = tr_B[d] = tr_E[d] = tr_hand[d] = 0;
+    }
 }
 
 // One SyMBA / HELIO step of every system.
@@ -668,9 +769,12 @@ int Ensemble::step_helio_symba()
     launch_drift(st, tp.rh, tp.vb, tp.status, tp.levelg, tp.sys, sy.GMcb, sy.err, ntp, dt, -1, symba, 1);
     trace_mark(st, "first half");
     if (any) {
-        if (int rc = recur(0)) return rc;
+        if (int rc = recur0()) return rc;
         trace_mark(st, "recursion");
-        if (trace_on()) subtree_profile_report(st);
+        if (trace_on()) {
+            trace_levels();
+            subtree_profile_report(st);
+        }
     }
     // second half
     if (int rc = kick_pl(0)) return rc;

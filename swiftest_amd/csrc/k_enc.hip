@@ -9,12 +9,10 @@
 // reference's list order (index1 ascending, then index2) with no sort.  The
 // same bit matrix gives every body its index2-side adjacency in list order.
 //
-// Recursion kernels restate symba_encounter_check_list_* (symba_encounter_
-// check.f90:90-235), symba_kick_list_* (symba_kick.f90:126-340),
-// collision_check_one (collision_check.f90:16-50) and the level bookkeeping
-// of symba_step.f90:118-160.  The shell kicks are computed per body by
-// gathering that body's pairs in list order, which reproduces the reference's
-// serial accumulation exactly.
+// The level-0 substep of the recursion runs over the whole lists: list checks
+// (symba_encounter_check_list_*, symba_encounter_check.f90:90-235),
+// collision detection (collision_check.f90:16-230) and the level bookkeeping
+// of symba_step.f90:118-160; its shell kicks are in k_level.hip.
 #include "swg_kernels.h"
 #include "swg_math.h"
 
@@ -419,99 +417,6 @@ __global__ void k_list_check_pt(ListDev pt, PlDev pl, TpDev tp, SysDev sy, int64
     }
 }
 
-// symba_kick_list_plpl (symba_kick.f90:126-232), gathered per body: pairs
-// with index2 == b (partners p < b) then index1 == b (partners j > b), i.e.
-// the body's pairs in list order.
-__global__ void k_kick_list_pl(ListDev pp, PlDev pl, int npl, int irec, int sgn, double dt)
-{
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= npl) return;
-    const int irm1 = irec - 1;
-    if (pl.levelg[b] < irm1) return;
-    const int irecl = sgn < 0 ? irec - 1 : irec;
-    const double shellpow = powi(RSHELL, 2 * irecl);
-    const double xb = pl.rh.x[b], yb = pl.rh.y[b], zb = pl.rh.z[b], hb = pl.rhill[b];
-    double ax = 0.0, ay = 0.0, az = 0.0;
-    bool any = false;
-    const int64_t c0 = pp.colstart[b], c1 = c0 + pp.colcnt[b];
-    for (int64_t c = c0; c < c1; c++) {
-        int k = pp.col[c];
-        int p = pp.i1[k];
-        if (pp.status[k] != ACTIVE || pl.levelg[p] < irm1) continue;
-        double sh = pl.rhill[p] + hb;
-        double ri = ((sh * sh) * (RHSCALE * RHSCALE)) * shellpow;
-        double dx = xb - pl.rh.x[p], dy = yb - pl.rh.y[p], dz = zb - pl.rh.z[p];
-        double r2 = dot3(dx, dy, dz, dx, dy, dz);
-        double fac;
-        if (!shell_fac(ri, r2, fac)) continue;
-        any = true;
-        double faci = fac * pl.Gm[p];
-        ax = fma(-faci, dx, ax);
-        ay = fma(-faci, dy, ay);
-        az = fma(-faci, dz, az);
-    }
-    const int64_t r0 = pp.rowstart[b], r1 = r0 + pp.rowcnt[b];
-    for (int64_t k = r0; k < r1; k++) {
-        int j = pp.i2[k];
-        if (pp.status[k] != ACTIVE || pl.levelg[j] < irm1) continue;
-        double sh = hb + pl.rhill[j];
-        double ri = ((sh * sh) * (RHSCALE * RHSCALE)) * shellpow;
-        double dx = pl.rh.x[j] - xb, dy = pl.rh.y[j] - yb, dz = pl.rh.z[j] - zb;
-        double r2 = dot3(dx, dy, dz, dx, dy, dz);
-        double fac;
-        if (!shell_fac(ri, r2, fac)) continue;
-        any = true;
-        double facj = fac * pl.Gm[j];
-        ax = fma(facj, dx, ax);
-        ay = fma(facj, dy, ay);
-        az = fma(facj, dz, az);
-    }
-    if (any) {
-        const double sdt = (double)sgn * dt;
-        pl.vb.x[b] = fma(sdt, ax, pl.vb.x[b]);
-        pl.vb.y[b] = fma(sdt, ay, pl.vb.y[b]);
-        pl.vb.z[b] = fma(sdt, az, pl.vb.z[b]);
-    }
-}
-
-// symba_kick_list_pltp (symba_kick.f90:235-340), gathered per particle
-__global__ void k_kick_list_tp(ListDev pt, PlDev pl, TpDev tp, int ntp, int irec, int sgn, double dt)
-{
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ntp) return;
-    const int irm1 = irec - 1;
-    if (tp.levelg[j] < irm1) return;
-    const int64_t c0 = pt.colstart[j], c1 = c0 + pt.colcnt[j];
-    if (c1 == c0) return;
-    const int irecl = sgn < 0 ? irec - 1 : irec;
-    const double shellpow = powi(RSHELL, 2 * irecl);
-    const double xt = tp.rh.x[j], yt = tp.rh.y[j], zt = tp.rh.z[j];
-    double ax = 0.0, ay = 0.0, az = 0.0;
-    bool any = false;
-    for (int64_t c = c0; c < c1; c++) {
-        int k = pt.col[c];
-        int i = pt.i1[k];
-        if (pt.status[k] != ACTIVE || pl.levelg[i] < irm1) continue;
-        double hi = pl.rhill[i];
-        double ri = ((hi * hi) * (RHSCALE * RHSCALE)) * shellpow;
-        double dx = xt - pl.rh.x[i], dy = yt - pl.rh.y[i], dz = zt - pl.rh.z[i];
-        double r2 = dot3(dx, dy, dz, dx, dy, dz);
-        double fac;
-        if (!shell_fac(ri, r2, fac)) continue;
-        any = true;
-        double faci = fac * pl.Gm[i];
-        ax = fma(-faci, dx, ax);
-        ay = fma(-faci, dy, ay);
-        az = fma(-faci, dz, az);
-    }
-    if (any) {
-        const double sdt = (double)sgn * dt;
-        tp.vb.x[j] = fma(sdt, ax, tp.vb.x[j]);
-        tp.vb.y[j] = fma(sdt, ay, tp.vb.y[j]);
-        tp.vb.z[j] = fma(sdt, az, tp.vb.z[j]);
-    }
-}
-
 // collision_check_plpl / _pltp detection (collision_check.f90:53-230)
 __global__ void k_collision(ListDev pp, ListDev pt, PlDev pl, TpDev tp, SysDev sy, int64_t npp, int64_t npt,
                             int irec, double dt, int32_t *any)
@@ -615,14 +520,6 @@ void launch_list_check_pt(hipStream_t st, ListDev pt, PlDev pl, TpDev tp, SysDev
                           int short_circuit, int32_t *any)
 {
     if (n) k_list_check_pt<<<cdiv(n, 256), 256, 0, st>>>(pt, pl, tp, sy, n, irec, dt, short_circuit, any);
-}
-void launch_kick_list_pl(hipStream_t st, ListDev pp, PlDev pl, int npl, int irec, int sgn, double dt)
-{
-    if (npl) k_kick_list_pl<<<cdiv(npl, 128), 128, 0, st>>>(pp, pl, npl, irec, sgn, dt);
-}
-void launch_kick_list_tp(hipStream_t st, ListDev pt, PlDev pl, TpDev tp, int ntp, int irec, int sgn, double dt)
-{
-    if (ntp) k_kick_list_tp<<<cdiv(ntp, 128), 128, 0, st>>>(pt, pl, tp, ntp, irec, sgn, dt);
 }
 void launch_collision(hipStream_t st, ListDev pp, ListDev pt, PlDev pl, TpDev tp, SysDev sy, int64_t npp,
                       int64_t npt, int irec, double dt, int32_t *any)

@@ -42,14 +42,26 @@ struct Ensemble {
     int32_t *d_any = nullptr, *d_pt_run = nullptr;
     int64_t *d_total = nullptr;
     void *d_scan = nullptr;
-    // recursion below level 0: per-system arenas of the persistent workgroups
-    int32_t *d_caps = nullptr, *d_sub_stats = nullptr, *d_arena = nullptr;
+    // recursion below level 0: whole-device working sets, one buffer per
+    // depth, until every system's set is at most subtree_max bodies + entries;
+    // then per-system arenas of the persistent workgroups
+    BuildScratch bw{};
+    std::vector<void *> lv_buf;
+    std::vector<int64_t> lv_cap;
+    int32_t *d_sub_stats = nullptr, *d_arena = nullptr;
     int64_t *d_arena_off = nullptr;
     void *d_scan_sys = nullptr;
-    int64_t arena_words = 0;
+    int64_t arena_words = 0, arena_need = 0;
     void *h_subargs = nullptr, *d_subargs = nullptr;
-    bool host_recursion = false;  // SWG_HOST_RECURSION=1: every level driven from the host
-    int run_subtree();
+    int64_t subtree_max = 16384;  // SWG_SUBTREE_MAX; < 0: every level on the whole device
+    LevelArgs level_args() const;
+    int build_level(int dnew, const LevelView *src, LevelView &v, LevelCounts &c, bool &handoff);
+    int recur_level(int d, const LevelView &v, const LevelCounts &c);
+    int run_subtree(int d0, const LevelView &v);
+    // host-driven recursion counters per level (substeps, bodies, entries,
+    // hand-offs), reported with SWG_TRACE=1
+    int64_t tr_sub[MAXLEV + 2] = {}, tr_B[MAXLEV + 2] = {}, tr_E[MAXLEV + 2] = {}, tr_hand[MAXLEV + 2] = {};
+    void trace_levels();
     char *h_small = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<void *> owned;
@@ -91,7 +103,7 @@ struct Ensemble {
     int detect_pt(double dt, const std::vector<int32_t> &run);
     int build_adjacency();
     int read_any(int32_t &v);
-    int recur(int ireci);
+    int recur0();
     int step_helio_symba();
     int whm_setup();
     int step_whm();

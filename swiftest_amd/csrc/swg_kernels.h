@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "swg_level.h"
 #include "swg_state.h"
 
 namespace swg {
@@ -55,27 +56,58 @@ void launch_col_count(hipStream_t st, ListDev L, const int32_t *row_off, const i
 void launch_col_fill(hipStream_t st, ListDev L, const int32_t *row_off, const int32_t *col_sys,
                      const int32_t *col_off, const int32_t *run, int ncols, int upper);
 
-// ---- SyMBA recursion (k_enc.hip) ----
+// ---- SyMBA recursion, level 0 (k_enc.hip) ----
 void launch_list_check_pp(hipStream_t st, ListDev pp, PlDev pl, SysDev sy, int64_t n, int irec, double dt,
                           int32_t *any);
 void launch_list_check_pt(hipStream_t st, ListDev pt, PlDev pl, TpDev tp, SysDev sy, int64_t n, int irec,
                           double dt, int short_circuit, int32_t *any);
-void launch_kick_list_pl(hipStream_t st, ListDev pp, PlDev pl, int npl, int irec, int sgn, double dt);
-void launch_kick_list_tp(hipStream_t st, ListDev pt, PlDev pl, TpDev tp, int ntp, int irec, int sgn,
-                         double dt);
 void launch_collision(hipStream_t st, ListDev pp, ListDev pt, PlDev pl, TpDev tp, SysDev sy, int64_t npp,
                       int64_t npt, int irec, double dt, int32_t *any);
 void launch_set_levels(hipStream_t st, PlDev pl, TpDev tp, ListDev pp, ListDev pt, int npl, int ntp,
                        int64_t npp, int64_t npt, int ireci);
 void launch_clear_flags(hipStream_t st, SysDev sy, int nsys, int32_t *any);
 
-// ---- SyMBA recursion below level 0, one workgroup per system (k_subtree.hip) ----
-void launch_arena_caps(hipStream_t st, ListDev pp, ListDev pt, SysDev sy, int nsys, int has_pp, int has_pt,
-                       int32_t *caps, int mult);
+// ---- SyMBA recursion levels on the whole device (k_level.hip) ----
+struct LevelCounts {
+    int64_t nB = 0, nE = 0, nBt = 0, nEt = 0;  // pl bodies, pl entries, tp bodies, tp entries
+};
+struct LevelArgs {
+    PlDev pl;
+    TpDev tp;
+    ListDev pp, pt;
+    SysDev sy;
+    int nsys, has_pp, has_pt;
+};
+// temporaries of one level build (sized for all bodies) and its per-system results
+struct BuildScratch {
+    int32_t *keepB, *cntE, *keepBt, *cntEt;
+    int64_t *posB, *posE, *posBt, *posEt;
+    int64_t *tot;    // [4]: nB, nE, nBt, nEt of the new level
+    void *scan;      // scan_scratch_bytes(max(npl, ntp))
+    int32_t *rng;    // [4*nsys] new level's per-system body ranges
+    int32_t *caps;   // [nsys] arena words of each system's persistent workgroup
+    int32_t *maxsz;  // largest per-system set (bodies + entries)
+};
+LevelView level_view(void *base, const LevelCounts &c);
+int64_t level_bytes(const LevelCounts &c);
+// src == nullptr: build level 1 from the level-0 adjacency of all npl / ntp bodies
+void launch_build_count(hipStream_t st, const LevelArgs &la, const LevelView *src, int npl, int ntp, int dnew,
+                        const BuildScratch &w);
+void launch_build_fill(hipStream_t st, const LevelArgs &la, const LevelView *src, int npl, int ntp, int dnew,
+                       const BuildScratch &w, void *dst_base, const LevelCounts &c);
+void launch_level_check(hipStream_t st, const LevelArgs &la, const LevelView &v, const LevelCounts &c, int d,
+                        double dtl, int short_circuit, int32_t *any);
+// v == nullptr: the level-0 kicks over the full adjacency of nB / nBt bodies
+void launch_level_kicks(hipStream_t st, const LevelArgs &la, const LevelView *v, int nB, int nBt, int d, double dth);
+void launch_level_drift(hipStream_t st, const LevelArgs &la, const LevelView &v, int nB, int nBt, int d, double dtl);
+void launch_level_collision(hipStream_t st, const LevelArgs &la, const LevelView &v, int nB, int nBt, double dtl);
+void launch_level_set_levels(hipStream_t st, const LevelArgs &la, const LevelView &v, const LevelCounts &c, int d);
+
+// ---- SyMBA recursion from level d0 down, one workgroup per system (k_subtree.hip) ----
 size_t subtree_args_bytes();
 void subtree_profile_report(hipStream_t st);
-void launch_subtree(hipStream_t st, PlDev pl, TpDev tp, ListDev pp, ListDev pt, SysDev sy, int nsys, double dt,
-                    int lclose, int short_circuit, int has_pp, int has_pt, int32_t *arena, const int64_t *arena_off,
+void launch_subtree(hipStream_t st, const LevelArgs &la, const LevelView &start, const int32_t *rng, int d0,
+                    double dt, int lclose, int short_circuit, int32_t *arena, const int64_t *arena_off,
                     int32_t *stats, void *h_stage, void *d_args);
 
 }  // namespace swg

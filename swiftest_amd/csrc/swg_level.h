@@ -1,0 +1,351 @@
+// swg_level.h -- SyMBA recursion working sets and the per-body / per-pair
+// operations of one recursion substep, shared by the whole-device level
+// kernels (k_level.hip) and the persistent per-system kernel (k_subtree.hip).
+//
+// While the recursion is at level d (symba_step_recur_system with ireci = d,
+// symba_step.f90:163-262) only bodies with levelg >= d and pairs whose two
+// bodies both have levelg >= d can change.  A LevelView lists them: bodies in
+// ascending index order, and for each body its pairs in list order (so a
+// per-body sum over its entries follows the reference's serial order).
+#pragma once
+#include "swg_math.h"
+#include "swg_state.h"
+
+namespace swg {
+
+constexpr int MAXLEV = 36;  // deeper than the recursion depth guard (3^-33 < 2^-52)
+
+// One pl entry: x = (pair << 1) | role (role 1: the body is index1), y = the
+// partner body.  One tp entry: x = pair, y = the massive body.
+struct LevelView {
+    const int32_t *B, *O, *Bt, *Ot;  // bodies; entry offsets (into E / Et), n+1 each
+    const int2 *E, *Et;
+    int nBpl, nBtp;
+};
+
+// Word layout of one stored level, from an 8-byte aligned base:
+//   B[nB] O[nB+1] E[nE] Bt[nBt] Ot[nBt+1] Et[nEt]
+// with each entry array (int2) starting at an even word.
+struct LevelLayout {
+    int64_t oB, oO, oE, oBt, oOt, oEt, words;
+};
+
+__host__ __device__ inline LevelLayout level_layout(int64_t nB, int64_t nE, int64_t nBt, int64_t nEt)
+{
+    LevelLayout L;
+    int64_t w = 0;
+    L.oB = w;
+    w += nB;
+    L.oO = w;
+    w += nB + 1;
+    w += w & 1;
+    L.oE = w;
+    w += 2 * nE;
+    L.oBt = w;
+    w += nBt;
+    L.oOt = w;
+    w += nBt + 1;
+    w += w & 1;
+    L.oEt = w;
+    w += 2 * nEt;
+    L.words = w;
+    return L;
+}
+
+struct RecState {
+    PlDev pl;
+    TpDev tp;
+    ListDev pp, pt;
+    SysDev sy;
+};
+
+// symba_encounter_check_list_plpl for one pair (symba_encounter_check.f90:
+// 121-154); returns true when the pair enters level d+1.
+__device__ inline bool rec_check_pp(const RecState &a, int k, int d, double dtl)
+{
+    if (a.pp.status[k] != ACTIVE || a.pp.level[k] != d) return false;
+    const int irec = d + 1;
+    const int i = a.pp.i1[k], j = a.pp.i2[k];
+    const PlDev &pl = a.pl;
+    double xr = pl.rh.x[j] - pl.rh.x[i], yr = pl.rh.y[j] - pl.rh.y[i], zr = pl.rh.z[j] - pl.rh.z[i];
+    double vxr = pl.vb.x[j] - pl.vb.x[i], vyr = pl.vb.y[j] - pl.vb.y[i], vzr = pl.vb.z[j] - pl.vb.z[i];
+    const double rsh = rshell_pow(irec);
+    double rci = (pl.rhill[i] * RHSCALE) * rsh, rcj = (pl.rhill[j] * RHSCALE) * rsh;
+    bool lv;
+    bool le = encounter_one(xr, yr, zr, vxr, vyr, vzr, rci + rcj, dtl, lv);
+    a.pp.lvdotr[k] = lv;
+    if (le) {
+        double rji2 = dot3(xr, yr, zr, xr, yr, zr);
+        double rs = pl.R[i] + pl.R[j];
+        le = rji2 > rs * rs;
+    }
+    if (le) {
+        a.pp.level[k] = (int8_t)irec;
+        pl.levelg[i] = irec;
+        pl.levelg[j] = irec;
+        atomicMax(&pl.levelm[i], irec);
+        atomicMax(&pl.levelm[j], irec);
+    }
+    return le;
+}
+
+// symba_encounter_check_list_pltp for one pair (symba_encounter_check.f90:
+// 195-228)
+__device__ inline bool rec_check_pt(const RecState &a, int k, int d, double dtl)
+{
+    if (a.pt.status[k] != ACTIVE || a.pt.level[k] != d) return false;
+    const int irec = d + 1;
+    const int i = a.pt.i1[k], j = a.pt.i2[k];
+    double xr = a.tp.rh.x[j] - a.pl.rh.x[i], yr = a.tp.rh.y[j] - a.pl.rh.y[i], zr = a.tp.rh.z[j] - a.pl.rh.z[i];
+    double vxr = a.tp.vb.x[j] - a.pl.vb.x[i], vyr = a.tp.vb.y[j] - a.pl.vb.y[i], vzr = a.tp.vb.z[j] - a.pl.vb.z[i];
+    bool lv;
+    bool le = encounter_one(xr, yr, zr, vxr, vyr, vzr, (a.pl.rhill[i] * RHSCALE) * rshell_pow(irec), dtl, lv);
+    a.pt.lvdotr[k] = lv;
+    if (le) {
+        double rji2 = dot3(xr, yr, zr, xr, yr, zr);
+        le = rji2 > a.pl.R[i] * a.pl.R[i];
+    }
+    if (le) {
+        a.pt.level[k] = (int8_t)irec;
+        a.pl.levelg[i] = irec;
+        atomicMax(&a.pl.levelm[i], irec);
+        a.tp.levelg[j] = irec;
+        atomicMax(&a.tp.levelm[j], irec);
+    }
+    return le;
+}
+
+// One pl-pl shell term of body b (symba_kick.f90:150-200): the pair's
+// contribution f*(dx,dy,dz) to b's sum at the outer shell (sign +1, irecl =
+// d+1) and, when `minus`, at the inner shell (sign -1, irecl = d).  tp / tm
+// tell whether the pair lies inside the respective shell.
+struct ShellTerm {
+    double dx, dy, dz, fp, fm;
+    bool tp, tm;
+};
+
+__device__ inline void pl_shell_term(const RecState &a, int2 en, double xb, double yb, double zb, double hb,
+                                     double spp, double spm, bool minus, ShellTerm &t)
+{
+    const PlDev &pl = a.pl;
+    t.tp = t.tm = false;
+    const int k = (int)((uint32_t)en.x >> 1);
+    if (a.pp.status[k] != ACTIVE) return;
+    const int p = en.y;
+    const bool bi = en.x & 1;  // b is index1
+    double sh = bi ? hb + pl.rhill[p] : pl.rhill[p] + hb;
+    double s2 = (sh * sh) * (RHSCALE * RHSCALE);
+    if (bi) {
+        t.dx = pl.rh.x[p] - xb;
+        t.dy = pl.rh.y[p] - yb;
+        t.dz = pl.rh.z[p] - zb;
+    } else {
+        t.dx = xb - pl.rh.x[p];
+        t.dy = yb - pl.rh.y[p];
+        t.dz = zb - pl.rh.z[p];
+    }
+    double r2 = dot3(t.dx, t.dy, t.dz, t.dx, t.dy, t.dz);
+    const double gp = pl.Gm[p];
+    double fac;
+    if (shell_fac(s2 * spp, r2, fac)) {
+        t.tp = true;
+        t.fp = bi ? fac * gp : -(fac * gp);
+    }
+    if (minus && shell_fac(s2 * spm, r2, fac)) {
+        t.tm = true;
+        t.fm = bi ? fac * gp : -(fac * gp);
+    }
+}
+
+// One pl-tp shell term of test particle t (symba_kick.f90:255-300)
+__device__ inline void tp_shell_term(const RecState &a, int2 en, double xt, double yt, double zt, double spp,
+                                     double spm, bool minus, ShellTerm &t)
+{
+    t.tp = t.tm = false;
+    if (a.pt.status[en.x] != ACTIVE) return;
+    const int i = en.y;
+    double hi = a.pl.rhill[i];
+    double s2 = (hi * hi) * (RHSCALE * RHSCALE);
+    t.dx = xt - a.pl.rh.x[i];
+    t.dy = yt - a.pl.rh.y[i];
+    t.dz = zt - a.pl.rh.z[i];
+    double r2 = dot3(t.dx, t.dy, t.dz, t.dx, t.dy, t.dz);
+    const double gi = a.pl.Gm[i];
+    double fac;
+    if (shell_fac(s2 * spp, r2, fac)) {
+        t.tp = true;
+        t.fp = -(fac * gi);
+    }
+    if (minus && shell_fac(s2 * spm, r2, fac)) {
+        t.tm = true;
+        t.fm = -(fac * gi);
+    }
+}
+
+// vb += dth * (outer-shell sum), then vb -= dth * (inner-shell sum), each only
+// when its sum had a term (symba_kick.f90:210-230)
+__device__ inline void apply_shell_kicks(Vec3p v, int b, double dth, bool anyp, const double3 &p, bool anym,
+                                         const double3 &m)
+{
+    if (!(anyp || anym)) return;
+    double vx = v.x[b], vy = v.y[b], vz = v.z[b];
+    if (anyp) {
+        vx = fma(dth, p.x, vx);
+        vy = fma(dth, p.y, vy);
+        vz = fma(dth, p.z, vz);
+    }
+    if (anym) {
+        vx = fma(-dth, m.x, vx);
+        vy = fma(-dth, m.y, vy);
+        vz = fma(-dth, m.z, vz);
+    }
+    v.x[b] = vx;
+    v.y[b] = vy;
+    v.z[b] = vz;
+}
+
+// The shell kicks of level d on massive body b (symba_kick_list_plpl,
+// symba_kick.f90:126-232): at d >= 1 the call with sgn = +1 at irecl = d+1
+// and the one with sgn = -1 at irecl = d; at d = 0 only the first.  Both sums
+// are gathered in one pass over b's pairs in list order (the reference's
+// serial order) and applied in the reference's order.  They read positions
+// only, so fusing the two calls changes no result.
+__device__ inline void rec_kick_pl(const RecState &a, int b, const int2 *E, int e0, int e1, int d, double dth)
+{
+    const double spp = powi(RSHELL, 2 * (d + 1)), spm = powi(RSHELL, 2 * d);
+    const bool minus = d > 0;
+    const double xb = a.pl.rh.x[b], yb = a.pl.rh.y[b], zb = a.pl.rh.z[b], hb = a.pl.rhill[b];
+    double3 p = make_double3(0.0, 0.0, 0.0), m = p;
+    bool anyp = false, anym = false;
+    for (int u = e0; u < e1; u++) {
+        ShellTerm t;
+        pl_shell_term(a, E[u], xb, yb, zb, hb, spp, spm, minus, t);
+        if (t.tp) {
+            anyp = true;
+            p.x = fma(t.fp, t.dx, p.x);
+            p.y = fma(t.fp, t.dy, p.y);
+            p.z = fma(t.fp, t.dz, p.z);
+        }
+        if (t.tm) {
+            anym = true;
+            m.x = fma(t.fm, t.dx, m.x);
+            m.y = fma(t.fm, t.dy, m.y);
+            m.z = fma(t.fm, t.dz, m.z);
+        }
+    }
+    apply_shell_kicks(a.pl.vb, b, dth, anyp, p, anym, m);
+}
+
+// The shell kicks of level d on test particle t (symba_kick_list_pltp,
+// symba_kick.f90:235-340)
+__device__ inline void rec_kick_tp(const RecState &a, int t, const int2 *E, int e0, int e1, int d, double dth)
+{
+    const double spp = powi(RSHELL, 2 * (d + 1)), spm = powi(RSHELL, 2 * d);
+    const bool minus = d > 0;
+    const double xt = a.tp.rh.x[t], yt = a.tp.rh.y[t], zt = a.tp.rh.z[t];
+    double3 p = make_double3(0.0, 0.0, 0.0), m = p;
+    bool anyp = false, anym = false;
+    for (int u = e0; u < e1; u++) {
+        ShellTerm q;
+        tp_shell_term(a, E[u], xt, yt, zt, spp, spm, minus, q);
+        if (q.tp) {
+            anyp = true;
+            p.x = fma(q.fp, q.dx, p.x);
+            p.y = fma(q.fp, q.dy, p.y);
+            p.z = fma(q.fp, q.dz, p.z);
+        }
+        if (q.tm) {
+            anym = true;
+            m.x = fma(q.fm, q.dx, m.x);
+            m.y = fma(q.fm, q.dy, m.y);
+            m.z = fma(q.fm, q.dz, m.z);
+        }
+    }
+    apply_shell_kicks(a.tp.vb, t, dth, anyp, p, anym, m);
+}
+
+// level-d drift of one body (symba_drift.f90:14-61 over swiftest_drift_one)
+__device__ inline void rec_drift(const RecState &a, int g, bool ispl, int d, double dtl, int s)
+{
+    Vec3p r = ispl ? a.pl.rh : a.tp.rh, v = ispl ? a.pl.vb : a.tp.vb;
+    int32_t *status = ispl ? a.pl.status : a.tp.status;
+    const int32_t *levelg = ispl ? a.pl.levelg : a.tp.levelg;
+    bool m = ispl ? status[g] != INACTIVE : status[g] == ACTIVE;
+    if (!m || levelg[g] != d) return;
+    double rx = r.x[g], ry = r.y[g], rz = r.z[g], vx = v.x[g], vy = v.y[g], vz = v.z[g];
+    int fl = drift_one(a.sy.GMcb[s], rx, ry, rz, vx, vy, vz, dtl);
+    r.x[g] = rx;
+    r.y[g] = ry;
+    r.z[g] = rz;
+    v.x[g] = vx;
+    v.y[g] = vy;
+    v.z[g] = vz;
+    if (fl) {
+        status[g] = DISCARDED_DRIFTERR;
+        if (ispl) atomicOr(&a.sy.err[s], 2);
+    }
+}
+
+// collision detection of the pl-pl pairs whose index1 is body i
+// (collision_check_plpl, collision_check.f90:53-140)
+__device__ inline void rec_collide_pl(const RecState &a, int i, double dtl, int s)
+{
+    const PlDev &pl = a.pl;
+    const int64_t r0 = a.pp.rowstart[i], r1 = r0 + a.pp.rowcnt[i];
+    for (int64_t k = r0; k < r1; k++) {
+        if (a.pp.status[k] != ACTIVE) continue;
+        int j = a.pp.i2[k];
+        bool c = collision_one(pl.rh.x[i] - pl.rh.x[j], pl.rh.y[i] - pl.rh.y[j], pl.rh.z[i] - pl.rh.z[j],
+                               pl.vb.x[i] - pl.vb.x[j], pl.vb.y[i] - pl.vb.y[j], pl.vb.z[i] - pl.vb.z[j],
+                               pl.Gm[i] + pl.Gm[j], pl.R[i] + pl.R[j], dtl, a.pp.lvdotr[k] != 0);
+        if (c) {
+            a.pp.status[k] = (int8_t)COLLIDED;
+            atomicOr(&a.sy.err[s], 1);
+        }
+    }
+}
+
+// collision detection of the pl-tp pairs of particle j (collision_check_pltp)
+__device__ inline void rec_collide_tp(const RecState &a, int j, double dtl, int s)
+{
+    const int64_t c0 = a.pt.colstart[j], c1 = c0 + a.pt.colcnt[j];
+    for (int64_t c = c0; c < c1; c++) {
+        int k = a.pt.col[c];
+        if (a.pt.status[k] != ACTIVE) continue;
+        int i = a.pt.i1[k];
+        bool col = collision_one(a.pl.rh.x[i] - a.tp.rh.x[j], a.pl.rh.y[i] - a.tp.rh.y[j],
+                                 a.pl.rh.z[i] - a.tp.rh.z[j], a.pl.vb.x[i] - a.tp.vb.x[j],
+                                 a.pl.vb.y[i] - a.tp.vb.y[j], a.pl.vb.z[i] - a.tp.vb.z[j], a.pl.Gm[i], a.pl.R[i],
+                                 dtl, a.pt.lvdotr[k] != 0);
+        if (col) {
+            a.pt.status[k] = (int8_t)COLLIDED;
+            atomicOr(&a.sy.err[s], 1);
+        }
+    }
+}
+
+// ---- building the next level's sets ----------------------------------------
+
+// entries of a body at level 0: its whole adjacency, index2 side first
+__device__ inline int l0_pl_count(const RecState &a, int has_pp, int g)
+{
+    return has_pp ? a.pp.colcnt[g] + a.pp.rowcnt[g] : 0;
+}
+__device__ inline int2 l0_pl_entry(const RecState &a, int g, int q)
+{
+    int cc = a.pp.colcnt[g];
+    if (q < cc) {
+        int k = a.pp.col[a.pp.colstart[g] + q];
+        return make_int2((int)((uint32_t)k << 1), a.pp.i1[k]);
+    }
+    int k = (int)(a.pp.rowstart[g] + (q - cc));
+    return make_int2((int)(((uint32_t)k << 1) | 1u), a.pp.i2[k]);
+}
+__device__ inline int l0_tp_count(const RecState &a, int has_pt, int g) { return has_pt ? a.pt.colcnt[g] : 0; }
+__device__ inline int2 l0_tp_entry(const RecState &a, int g, int q)
+{
+    int k = a.pt.col[a.pt.colstart[g] + q];
+    return make_int2(k, a.pt.i1[k]);
+}
+
+}  // namespace swg
