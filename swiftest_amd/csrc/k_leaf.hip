@@ -181,7 +181,7 @@ __global__ __launch_bounds__(BLK) void k_leaf_tp(int ntp, int npl, const double 
 }
 
 // swiftest_drift_all (swiftest_drift.f90:60-108), GR off
-__global__ void k_leaf_drift(const double *mu, double *x, double *v, int n, double dt, const int32_t *lmask,
+__global__ __launch_bounds__(256) void k_leaf_drift(const double *mu, double *x, double *v, int n, double dt, const int32_t *lmask,
                              int32_t *iflag)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -11,9 +11,10 @@
 //
 // The next level's set is built from the current one (or from the level-0
 // adjacency) by an order-preserving compaction: count, scan, fill.  Massive
-// bodies carry hundreds of pairs each in crowded systems and are handled one
-// wavefront per body; test particles carry few and are handled one thread
-// each.
+// bodies carry hundreds of pairs each in crowded systems; their builds run
+// one wavefront per body and their shell kicks stage the pair terms of a
+// block of bodies through LDS.  Test particles carry few pairs and are
+// handled one thread each.
 #include <algorithm>
 
 #include "swg_kernels.h"
@@ -33,14 +34,6 @@ __device__ inline uint64_t lanes_below()
 {
     const int lane = threadIdx.x & 63;
     return lane ? (~0ull >> (64 - lane)) : 0ull;
-}
-
-// value of lane l of a wave-uniform index l
-__device__ inline double lane_value(double v, int l)
-{
-    int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
-    int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
-    return __hiloint2double(hi, lo);
 }
 
 // ---- building the next level ---------------------------------------------------
@@ -252,25 +245,113 @@ __global__ __launch_bounds__(256) void k_lv_check_pt(RecState a, const int2 *Et,
     }
 }
 
-// Shell kicks of one massive body per wavefront.  The lanes evaluate 64
-// consecutive entries at once; the sums are then folded in entry order by
-// every lane alike, so each body's sum is the reference's serial sum.  At
-// level 0 the source is the full adjacency and only the +1 kick is made.
+// Shell kicks of the massive bodies, KB_TB bodies per workgroup.  The
+// workgroup walks the bodies' entries in chunks of KB_K per body: all threads
+// evaluate the chunk's terms in parallel into LDS, then one lane per body
+// folds its terms in entry order, so each body's sum is the reference's
+// serial sum.  At level 0 the source is the full adjacency and only the +1
+// kick is made.
+constexpr int KB_TB = 32, KB_K = 32, KB_T = 256, KB_LD = KB_TB + 1;
+constexpr int KB_MIN_BODIES = 16384;  // below this, one wavefront per body fills the device better
+
 template <bool L0>
-__global__ __launch_bounds__(256) void k_lv_kick_pl(RecState a, LevelView v, int nB, int d, double dth, int has_pp)
+__global__ __launch_bounds__(KB_T) void k_lv_kick_pl(RecState a, LevelView v, int nB, int d, double dth, int has_pp)
+{
+    __shared__ double sdx[KB_K * KB_LD], sdy[KB_K * KB_LD], sdz[KB_K * KB_LD], sfp[KB_K * KB_LD],
+        sfm[KB_K * KB_LD];
+    __shared__ uint8_t sfl[KB_K * KB_LD];
+    __shared__ int sb[KB_TB], se0[KB_TB], sn[KB_TB];
+    __shared__ double sxb[KB_TB], syb[KB_TB], szb[KB_TB], shb[KB_TB];
+    __shared__ int smax;
+    const int tid = threadIdx.x;
+    const int q0 = blockIdx.x * KB_TB;
+    if (tid == 0) smax = 0;
+    __syncthreads();
+    if (tid < KB_TB) {
+        const int q = q0 + tid;
+        int n = 0;
+        if (q < nB) {
+            const int b = L0 ? q : v.B[q];
+            const int e0 = L0 ? 0 : v.O[q];
+            n = L0 ? l0_pl_count(a, has_pp, b) : v.O[q + 1] - e0;
+            sb[tid] = b;
+            se0[tid] = e0;
+            sxb[tid] = a.pl.rh.x[b];
+            syb[tid] = a.pl.rh.y[b];
+            szb[tid] = a.pl.rh.z[b];
+            shb[tid] = a.pl.rhill[b];
+        }
+        sn[tid] = n;
+        atomicMax(&smax, n);
+    }
+    __syncthreads();
+    const int nmax = smax;
+    if (nmax == 0) return;
+    const double spp = powi(RSHELL, 2 * (d + 1)), spm = powi(RSHELL, 2 * d);
+    const bool minus = d > 0;
+    double3 p = make_double3(0.0, 0.0, 0.0), m = p;
+    bool anyp = false, anym = false;
+    for (int c0 = 0; c0 < nmax; c0 += KB_K) {
+        for (int sidx = tid; sidx < KB_TB * KB_K; sidx += KB_T) {
+            const int tb = sidx / KB_K, k = sidx % KB_K, u = c0 + k;
+            ShellTerm t;
+            t.tp = t.tm = false;
+            if (u < sn[tb]) {
+                const int2 en = L0 ? l0_pl_entry(a, sb[tb], u) : v.E[se0[tb] + u];
+                pl_shell_term(a, en, sxb[tb], syb[tb], szb[tb], shb[tb], spp, spm, minus, t);
+            }
+            const int w = k * KB_LD + tb;
+            sfl[w] = (t.tp ? 1 : 0) | (t.tm ? 2 : 0);
+            if (t.tp || t.tm) {
+                sdx[w] = t.dx;
+                sdy[w] = t.dy;
+                sdz[w] = t.dz;
+                sfp[w] = t.fp;
+                sfm[w] = t.fm;
+            }
+        }
+        __syncthreads();
+        if (tid < KB_TB) {
+            const int kn = min(KB_K, sn[tid] - c0);
+            for (int k = 0; k < kn; k++) {
+                const int w = k * KB_LD + tid;
+                const int fl = sfl[w];
+                if (fl & 1) {
+                    anyp = true;
+                    p.x = fma(sfp[w], sdx[w], p.x);
+                    p.y = fma(sfp[w], sdy[w], p.y);
+                    p.z = fma(sfp[w], sdz[w], p.z);
+                }
+                if (fl & 2) {
+                    anym = true;
+                    m.x = fma(sfm[w], sdx[w], m.x);
+                    m.y = fma(sfm[w], sdy[w], m.y);
+                    m.z = fma(sfm[w], sdz[w], m.z);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid < KB_TB && q0 + tid < nB) apply_shell_kicks(a.pl.vb, sb[tid], dth, anyp, p, anym, m);
+}
+
+// For smaller sets (too few bodies to fill the device in blocks of KB_TB):
+// one wavefront per body; the lanes evaluate 64 consecutive entries at once
+// and every lane folds the terms in entry order (readlane broadcast).
+__device__ inline double lane_value(double v, int l)
+{
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+
+__global__ __launch_bounds__(256) void k_lv_kick_pl_wave(RecState a, LevelView v, int nB, int d, double dth)
 {
     const int q = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int lane = threadIdx.x & 63;
     if (q >= nB) return;
-    const int b = L0 ? q : v.B[q];
-    int e0, e1;
-    if (L0) {
-        e0 = 0;
-        e1 = l0_pl_count(a, has_pp, b);
-    } else {
-        e0 = v.O[q];
-        e1 = v.O[q + 1];
-    }
+    const int b = v.B[q];
+    const int e0 = v.O[q], e1 = v.O[q + 1];
     if (e1 <= e0) return;
     const double spp = powi(RSHELL, 2 * (d + 1)), spm = powi(RSHELL, 2 * d);
     const bool minus = d > 0;
@@ -282,7 +363,7 @@ __global__ __launch_bounds__(256) void k_lv_kick_pl(RecState a, LevelView v, int
         ShellTerm t;
         t.tp = t.tm = false;
         t.dx = t.dy = t.dz = t.fp = t.fm = 0.0;
-        if (u < e1) pl_shell_term(a, L0 ? l0_pl_entry(a, b, u) : v.E[u], xb, yb, zb, hb, spp, spm, minus, t);
+        if (u < e1) pl_shell_term(a, v.E[u], xb, yb, zb, hb, spp, spm, minus, t);
         const uint64_t mp = __ballot(t.tp), mm = __ballot(t.tm);
         anyp |= mp != 0;
         anym |= mm != 0;
@@ -468,11 +549,14 @@ void launch_level_kicks(hipStream_t st, const LevelArgs &la, const LevelView *v,
     LevelView vv{};
     if (v) vv = *v;
     if (v) {
-        if (nB) k_lv_kick_pl<false><<<cdiv((int64_t)nB * 64, 256), 256, 0, st>>>(a, vv, nB, d, dth, la.has_pp);
+        if (nB >= KB_MIN_BODIES)
+            k_lv_kick_pl<false><<<cdiv(nB, KB_TB), KB_T, 0, st>>>(a, vv, nB, d, dth, la.has_pp);
+        else if (nB)
+            k_lv_kick_pl_wave<<<cdiv((int64_t)nB * 64, 256), 256, 0, st>>>(a, vv, nB, d, dth);
         if (nBt) k_lv_kick_tp<false><<<cdiv(nBt, 256), 256, 0, st>>>(a, vv, nBt, d, dth, la.has_pt);
     } else {  // level 0: the full adjacency of every body
         if (nB && la.has_pp)
-            k_lv_kick_pl<true><<<cdiv((int64_t)nB * 64, 256), 256, 0, st>>>(a, vv, nB, d, dth, la.has_pp);
+            k_lv_kick_pl<true><<<cdiv(nB, KB_TB), KB_T, 0, st>>>(a, vv, nB, d, dth, la.has_pp);
         if (nBt && la.has_pt) k_lv_kick_tp<true><<<cdiv(nBt, 256), 256, 0, st>>>(a, vv, nBt, d, dth, la.has_pt);
     }
 }

@@ -30,37 +30,81 @@ __global__ void k_reset_levels(PlDev pl, TpDev tp, int npl, int ntp)
     }
 }
 
-// Per-system sequential sums at the start of a step, one thread per
-// component, in the reference's summation order:
+// Per-system sums, one workgroup per system.  Each sum is a serial chain in
+// the reference's order, run by one lane over chunks of the system's bodies
+// that the whole workgroup first stages in LDS (a chain streaming from HBM
+// would wait on every load).
+constexpr int SUM_T = 256, SUM_CH = 1024;
+
+struct SumStage {
+    double gm[SUM_CH], v[3][SUM_CH];
+    uint8_t act[SUM_CH];
+};
+
+// stage bodies [c0, c0+n) of Gm, active flags and one 3-vector
+__device__ inline void sum_stage(SumStage &S, const PlDev &pl, const Vec3p &v, int c0, int n)
+{
+    for (int k = threadIdx.x; k < n; k += SUM_T) {
+        const int i = c0 + k;
+        S.gm[k] = pl.Gm[i];
+        S.act[k] = pl.status[i] != INACTIVE;
+        S.v[0][k] = v.x[i];
+        S.v[1][k] = v.y[i];
+        S.v[2][k] = v.z[i];
+    }
+}
+
+// Start of a step:
 //   vh->vb (swiftest_util.f90:424-459) when lfirst: cb%vb = -sum(Gm vh) / (GM_cb + sum(Gm))
 //   lindrift pt (helio_drift.f90:129-163): pt = sum_masked(Gm vb) / GM_cb
-__global__ void k_sums_begin(PlDev pl, SysDev sy, int nsys, int force)
+// Lane 0 sums Gm, lanes 1-3 the components of sum(Gm vh); then lanes 0-2 the
+// components of pt.
+__global__ __launch_bounds__(SUM_T) void k_sums_begin(PlDev pl, SysDev sy, int nsys, int force)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 3 * nsys) return;
-    int s = t / 3, d = t % 3;
-    int p0 = sy.pl_off[s], p1 = sy.pl_off[s + 1];
+    __shared__ SumStage S;
+    __shared__ double shr[4];
+    const int s = blockIdx.x, tid = threadIdx.x;
+    const int p0 = sy.pl_off[s], p1 = sy.pl_off[s + 1];
     if (p1 == p0) return;
-    const double *vh = d == 0 ? pl.vh.x : d == 1 ? pl.vh.y : pl.vh.z;
-    const double *vb = d == 0 ? pl.vb.x : d == 1 ? pl.vb.y : pl.vb.z;
-    bool first = force || sy.lfirst[s];
+    const bool first = force || sy.lfirst[s];
     double cb = 0.0;
     if (first) {
-        double sm = 0.0;
-        for (int i = p0; i < p1; i++) sm = sm + pl.Gm[i];
-        double Gmtot = sy.GMcb[s] + sm;
-        double c = 0.0;
-        for (int i = p0; i < p1; i++) c = c - pl.Gm[i] * vh[i];
-        cb = c / Gmtot;
-        sy.cbvb[3 * s + d] = cb;
+        double acc = 0.0;
+        for (int c0 = p0; c0 < p1; c0 += SUM_CH) {
+            const int n = min(SUM_CH, p1 - c0);
+            sum_stage(S, pl, pl.vh, c0, n);
+            __syncthreads();
+            if (tid == 0) {
+                for (int k = 0; k < n; k++) acc = acc + S.gm[k];
+            } else if (tid <= 3) {
+                const double *v = S.v[tid - 1];
+                for (int k = 0; k < n; k++) acc = acc - S.gm[k] * v[k];
+            }
+            __syncthreads();
+        }
+        if (tid < 4) shr[tid] = acc;
+        __syncthreads();
+        if (tid < 3) {
+            cb = shr[tid + 1] / (sy.GMcb[s] + shr[0]);
+            sy.cbvb[3 * s + tid] = cb;
+        }
     }
     double pt = 0.0;
-    for (int i = p0; i < p1; i++) {
-        if (!pl_active(pl, i)) continue;
-        double v = first ? vh[i] + cb : vb[i];
-        pt = pt + pl.Gm[i] * v;
+    for (int c0 = p0; c0 < p1; c0 += SUM_CH) {
+        const int n = min(SUM_CH, p1 - c0);
+        sum_stage(S, pl, first ? pl.vh : pl.vb, c0, n);
+        __syncthreads();
+        if (tid < 3) {
+            const double *v = S.v[tid];
+            for (int k = 0; k < n; k++) {
+                if (!S.act[k]) continue;
+                const double w = first ? v[k] + cb : v[k];
+                pt = pt + S.gm[k] * w;
+            }
+        }
+        __syncthreads();
     }
-    sy.ptbeg[3 * s + d] = pt / sy.GMcb[s];
+    if (tid < 3) sy.ptbeg[3 * s + tid] = pt / sy.GMcb[s];
 }
 
 // vb = vh + cb%vb when lfirst; then the linear drift rh += pt*dt
@@ -181,7 +225,7 @@ __global__ __launch_bounds__(BLK) void k_kick_pl(PlDev pl, SysDev sy, ListDev pp
 
 // Danby drift of the masked bodies (helio_drift_body, helio_drift.f90:14-54,
 // with the SyMBA level mask of symba_drift.f90:14-61): mu = GM_cb, (rh, vb).
-__global__ void k_drift(Vec3p r, Vec3p v, int32_t *status, const int32_t *levelg, const int32_t *sys,
+__global__ __launch_bounds__(256) void k_drift(Vec3p r, Vec3p v, int32_t *status, const int32_t *levelg, const int32_t *sys,
                         const double *GMcb, int32_t *err, int n, double dt, int irec, int use_level, int is_tp)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -287,30 +331,43 @@ __global__ __launch_bounds__(BLK) void k_kick_tp(TpDev tp, PlDev pl, SysDev sy, 
     tp.vb.z[j] = fma(az, dt, tp.vb.z[j]);
 }
 
-// Per-system sums at the end of a step (6 threads per system):
-//   threads 0-2: lindrift pt = sum_masked(Gm vb) / GM_cb  -> ptend
-//   threads 3-5: vb->vh (swiftest_util.f90:363-395): cb%vb -= Gm vb / GM_cb
-//                over i = npl..1 (active only)
-__global__ void k_sums_end(PlDev pl, SysDev sy, int nsys)
+// Per-system sums at the end of a step:
+//   lindrift pt = sum_masked(Gm vb) / GM_cb -> ptend (forward, lanes 0-2)
+//   vb->vh (swiftest_util.f90:363-395): cb%vb -= Gm vb / GM_cb over
+//   i = npl..1, active only (backward, lanes 0-2)
+__global__ __launch_bounds__(SUM_T) void k_sums_end(PlDev pl, SysDev sy, int nsys)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 6 * nsys) return;
-    int s = t / 6, w = t % 6, d = w % 3;
-    int p0 = sy.pl_off[s], p1 = sy.pl_off[s + 1];
+    __shared__ SumStage S;
+    const int s = blockIdx.x, tid = threadIdx.x;
+    const int p0 = sy.pl_off[s], p1 = sy.pl_off[s + 1];
     if (p1 == p0) return;
-    const double *vb = d == 0 ? pl.vb.x : d == 1 ? pl.vb.y : pl.vb.z;
     const double gm = sy.GMcb[s];
-    if (w < 3) {
-        double pt = 0.0;
-        for (int i = p0; i < p1; i++)
-            if (pl_active(pl, i)) pt = pt + pl.Gm[i] * vb[i];
-        sy.ptend[3 * s + d] = pt / gm;
-    } else {
-        double c = 0.0;
-        for (int i = p1 - 1; i >= p0; i--)
-            if (pl_active(pl, i)) c = c - (pl.Gm[i] * vb[i]) / gm;
-        sy.cbvb[3 * s + d] = c;
+    double pt = 0.0;
+    for (int c0 = p0; c0 < p1; c0 += SUM_CH) {
+        const int n = min(SUM_CH, p1 - c0);
+        sum_stage(S, pl, pl.vb, c0, n);
+        __syncthreads();
+        if (tid < 3) {
+            const double *v = S.v[tid];
+            for (int k = 0; k < n; k++)
+                if (S.act[k]) pt = pt + S.gm[k] * v[k];
+        }
+        __syncthreads();
     }
+    if (tid < 3) sy.ptend[3 * s + tid] = pt / gm;
+    double c = 0.0;
+    for (int c1 = p1; c1 > p0; c1 -= SUM_CH) {
+        const int c0 = max(p0, c1 - SUM_CH), n = c1 - c0;
+        sum_stage(S, pl, pl.vb, c0, n);
+        __syncthreads();
+        if (tid < 3) {
+            const double *v = S.v[tid];
+            for (int k = n - 1; k >= 0; k--)
+                if (S.act[k]) c = c - (S.gm[k] * v[k]) / gm;
+        }
+        __syncthreads();
+    }
+    if (tid < 3) sy.cbvb[3 * s + tid] = c;
 }
 
 // rh += ptend*dt (masked); vh = vb - cb%vb (all bodies)
@@ -366,20 +423,29 @@ __global__ void k_snapshot_tp(TpDev tp, SysDev sy, int ntp)
     tp.vb.z[j] = tp.vh.z[j] + (-sy.ptbeg[3 * s + 2]);
 }
 
-__global__ void k_vh2vb_sum(PlDev pl, SysDev sy, int nsys)
+__global__ __launch_bounds__(SUM_T) void k_vh2vb_sum(PlDev pl, SysDev sy, int nsys)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 3 * nsys) return;
-    int s = t / 3, d = t % 3;
-    int p0 = sy.pl_off[s], p1 = sy.pl_off[s + 1];
+    __shared__ SumStage S;
+    __shared__ double shr[4];
+    const int s = blockIdx.x, tid = threadIdx.x;
+    const int p0 = sy.pl_off[s], p1 = sy.pl_off[s + 1];
     if (p1 == p0) return;
-    const double *vh = d == 0 ? pl.vh.x : d == 1 ? pl.vh.y : pl.vh.z;
-    double sm = 0.0;
-    for (int i = p0; i < p1; i++) sm = sm + pl.Gm[i];
-    double Gmtot = sy.GMcb[s] + sm;
-    double c = 0.0;
-    for (int i = p0; i < p1; i++) c = c - pl.Gm[i] * vh[i];
-    sy.cbvb[3 * s + d] = c / Gmtot;
+    double acc = 0.0;
+    for (int c0 = p0; c0 < p1; c0 += SUM_CH) {
+        const int n = min(SUM_CH, p1 - c0);
+        sum_stage(S, pl, pl.vh, c0, n);
+        __syncthreads();
+        if (tid == 0) {
+            for (int k = 0; k < n; k++) acc = acc + S.gm[k];
+        } else if (tid <= 3) {
+            const double *v = S.v[tid - 1];
+            for (int k = 0; k < n; k++) acc = acc - S.gm[k] * v[k];
+        }
+        __syncthreads();
+    }
+    if (tid < 4) shr[tid] = acc;
+    __syncthreads();
+    if (tid < 3) sy.cbvb[3 * s + tid] = shr[tid + 1] / (sy.GMcb[s] + shr[0]);
 }
 
 __global__ void k_vh2vb_apply(PlDev pl, SysDev sy, int npl)
@@ -413,7 +479,7 @@ void launch_reset_levels(hipStream_t st, PlDev pl, TpDev tp, int npl, int ntp)
 }
 void launch_sums_begin(hipStream_t st, PlDev pl, SysDev sy, int nsys, int force)
 {
-    k_sums_begin<<<cdiv(3 * nsys, 64), 64, 0, st>>>(pl, sy, nsys, force);
+    k_sums_begin<<<nsys, SUM_T, 0, st>>>(pl, sy, nsys, force);
 }
 void launch_pl_begin_apply(hipStream_t st, PlDev pl, SysDev sy, int npl, double dth, int force)
 {
@@ -450,7 +516,7 @@ void launch_kick_tp(hipStream_t st, const Tuning &tu, TpDev tp, PlDev pl, SysDev
 }
 void launch_sums_end(hipStream_t st, PlDev pl, SysDev sy, int nsys)
 {
-    k_sums_end<<<cdiv(6 * nsys, 64), 64, 0, st>>>(pl, sy, nsys);
+    k_sums_end<<<nsys, SUM_T, 0, st>>>(pl, sy, nsys);
 }
 void launch_pl_end_apply(hipStream_t st, PlDev pl, SysDev sy, int npl, double dth)
 {
@@ -466,7 +532,7 @@ void launch_lfirst_update(hipStream_t st, SysDev sy, int nsys)
 }
 void launch_snapshot(hipStream_t st, PlDev pl, TpDev tp, SysDev sy, int nsys, int npl, int ntp)
 {
-    k_vh2vb_sum<<<cdiv(3 * nsys, 64), 64, 0, st>>>(pl, sy, nsys);
+    k_vh2vb_sum<<<nsys, SUM_T, 0, st>>>(pl, sy, nsys);
     if (npl) k_vh2vb_apply<<<cdiv(npl, 256), 256, 0, st>>>(pl, sy, npl);
     if (ntp) k_snapshot_tp<<<cdiv(ntp, 256), 256, 0, st>>>(tp, sy, ntp);
 }

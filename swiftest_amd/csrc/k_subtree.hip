@@ -66,10 +66,22 @@ __device__ inline int st_block_scan(int v, int *sh, int *tot)
     return r;
 }
 
+// Slices small enough for the LDS-resident kernel (k_subtree_lds)
+constexpr int L_T = 256;                  // threads of its workgroup
+constexpr int L_NB = 512, L_NE = 4096;    // pl bodies, pl entries (pairs = entries / 2)
+constexpr int L_NP = L_NE / 2;
+constexpr int L_NBT = 256, L_NET = 1024;  // tp bodies, tp entries
+
+__device__ inline bool lds_eligible(int nB, int nE, int nBt, int nEt)
+{
+    return nB <= L_NB && nE <= L_NE && nBt <= L_NBT && nEt <= L_NET;
+}
+
 struct SubArgs {
     RecState st;
     double dt;  // param%dt
     int lclose, short_circuit, has_pp, has_pt;
+    int lds;             // 0: never use the LDS-resident kernel
     LevelView start;    // whole-device level d0; system s owns a slice of it
     const int32_t *rng;  // per system: pl begin, pl end, tp begin, tp end in `start`
     int d0;
@@ -296,6 +308,7 @@ __global__ __launch_bounds__(ST_T) void k_subtree(const SubArgs *__restrict__ ap
     cur.end = A.arena_off[s];
     cur.end += cur.end & 1;
     if (cur.nB + cur.nBt == 0) return;
+    if (A.lds && lds_eligible(cur.nB, cur.O[cur.nB] - cur.O[0], cur.nBt, cur.Ot[cur.nBt] - cur.Ot[0])) return;
     const int d0 = A.d0;
     int d = d0, j = 1;
     bool resume = false;
@@ -385,9 +398,536 @@ __global__ __launch_bounds__(ST_T) void k_subtree(const SubArgs *__restrict__ ap
     }
 }
 
+// ---- the LDS-resident variant -------------------------------------------------
+//
+// A system whose slice of the level-d0 set fits in LDS is run from LDS: the
+// bodies' state, the entries (with local indices) and the pairs' status,
+// level and lvdotr are loaded once, the whole subtree runs on them, and they
+// are written back at the end.  No nested set is built: during level d the
+// working set is exactly the bodies with levelg >= d and the entries whose
+// partner has levelg >= d, so every phase filters the d0 slice on the fly.
+// Entry word: (local pair << 10) | (role << 9) | local partner.
+
+struct LdsSet {
+    double px[L_NB], py[L_NB], pz[L_NB], pvx[L_NB], pvy[L_NB], pvz[L_NB], ph[L_NB], pgm[L_NB], pR[L_NB];
+    int pg[L_NB], plev[L_NB], plm[L_NB], pst[L_NB], pO[L_NB + 1];
+    uint32_t pE[L_NE];
+    int16_t ppi[L_NP], ppj[L_NP];
+    int ppg[L_NP];
+    int8_t pps[L_NP], ppl[L_NP], ppv[L_NP];
+    double tx[L_NBT], ty[L_NBT], tz[L_NBT], tvx[L_NBT], tvy[L_NBT], tvz[L_NBT];
+    int tg[L_NBT], tlev[L_NBT], tlm[L_NBT], tst[L_NBT], tO[L_NBT + 1];
+    int16_t tpl[L_NET], tpt[L_NET];
+    int tpg[L_NET];
+    int8_t tps[L_NET], tpl_lev[L_NET], tpv[L_NET];
+    // shell kicks: the level's bodies and one term slot per thread
+    int16_t act[L_NB + L_NBT];
+    double kdx[L_T], kdy[L_T], kdz[L_T], kfp[L_T], kfm[L_T];
+    uint8_t kfl[L_T];
+    int jst[MAXLEV + 2];
+    int npair, flag_pp, flag_pt, bad, nact;
+    unsigned long long prof[(MAXLEV + 2) * 8];  // per-level phase ticks when profiling
+};
+
+__device__ inline int lds_lower_bound(const int *a, int n, int key)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int m = (lo + hi) >> 1;
+        if (a[m] < key) lo = m + 1;
+        else hi = m;
+    }
+    return lo;
+}
+
+// index of the segment [o[q], o[q+1]) that holds u
+__device__ inline int lds_owner(const int *o, int n, int u)
+{
+    int lo = 0, hi = n;  // o[0] = 0 <= u < o[n]
+    while (hi - lo > 1) {
+        int m = (lo + hi) >> 1;
+        if (o[m] <= u) lo = m;
+        else hi = m;
+    }
+    return lo;
+}
+
+__device__ bool lds_check(LdsSet &S, const SubArgs &A, int nB, int nEt, int d, double dtl)
+{
+    const int tid = threadIdx.x, irec = d + 1;
+    if (tid == 0) {
+        S.flag_pp = 0;
+        S.flag_pt = 0;
+    }
+    __syncthreads();
+    for (int lp = tid; lp < S.npair; lp += L_T) {
+        if (S.pps[lp] != ACTIVE || S.ppl[lp] != d) continue;
+        const int i = S.ppi[lp], j = S.ppj[lp];
+        bool lv;
+        bool le = check_pp_v(S.px[i], S.py[i], S.pz[i], S.pvx[i], S.pvy[i], S.pvz[i], S.ph[i], S.pR[i], S.px[j],
+                             S.py[j], S.pz[j], S.pvx[j], S.pvy[j], S.pvz[j], S.ph[j], S.pR[j], irec, dtl, lv);
+        S.ppv[lp] = lv;
+        if (le) {
+            S.ppl[lp] = (int8_t)irec;
+            S.plev[i] = irec;
+            S.plev[j] = irec;
+            atomicMax(&S.plm[i], irec);
+            atomicMax(&S.plm[j], irec);
+            S.flag_pp = 1;
+        }
+    }
+    __syncthreads();
+    if (!(A.short_circuit && S.flag_pp)) {  // symba_step.f90:210 (.or. short-circuited)
+        for (int u = tid; u < nEt; u += L_T) {
+            if (S.tps[u] != ACTIVE || S.tpl_lev[u] != d) continue;
+            const int i = S.tpl[u], t = S.tpt[u];
+            bool lv;
+            bool le = check_pt_v(S.px[i], S.py[i], S.pz[i], S.pvx[i], S.pvy[i], S.pvz[i], S.ph[i], S.pR[i], S.tx[t],
+                                 S.ty[t], S.tz[t], S.tvx[t], S.tvy[t], S.tvz[t], irec, dtl, lv);
+            S.tpv[u] = lv;
+            if (le) {
+                S.tpl_lev[u] = (int8_t)irec;
+                S.plev[i] = irec;
+                atomicMax(&S.plm[i], irec);
+                S.tlev[t] = irec;
+                atomicMax(&S.tlm[t], irec);
+                S.flag_pt = 1;
+            }
+        }
+    }
+    __syncthreads();
+    const bool lenc = S.flag_pp || S.flag_pt;
+    __syncthreads();
+    return lenc;
+}
+
+__device__ inline void lds_apply(double &vx, double &vy, double &vz, double dth, bool anyp, const double3 &p,
+                                 bool anym, const double3 &m)
+{
+    if (anyp) {
+        vx = fma(dth, p.x, vx);
+        vy = fma(dth, p.y, vy);
+        vz = fma(dth, p.z, vz);
+    }
+    if (anym) {
+        vx = fma(-dth, m.x, vx);
+        vy = fma(-dth, m.y, vy);
+        vz = fma(-dth, m.z, vz);
+    }
+}
+
+// The +1 / -1 shell kicks of level d (the sums of rec_kick_pl / rec_kick_tp).
+// Each body of the level's set gets a group of G lanes (G as large as the
+// number of such bodies allows, at most a wavefront): the group evaluates G of
+// the body's entries at once into LDS slots, and the group's first lane folds
+// the ones that contribute, in entry order.
+__device__ void lds_kicks(LdsSet &S, int nB, int nBt, int nE, int nEt, int d, double dth)
+{
+    const int tid = threadIdx.x;
+    if (tid == 0) S.nact = 0;
+    __syncthreads();
+    for (int q = tid; q < nB + nBt; q += L_T) {
+        const bool on = q < nB ? (S.plev[q] >= d && S.pO[q + 1] > S.pO[q])
+                               : (S.tlev[q - nB] >= d && S.tO[q - nB + 1] > S.tO[q - nB]);
+        if (on) S.act[atomicAdd(&S.nact, 1)] = (int16_t)q;
+    }
+    __syncthreads();
+    const int nact = S.nact;
+    int G = 64;
+    while (G > 1 && G * nact > L_T) G >>= 1;
+    const int lane = tid & 63, gl = tid % G, grp = tid / G, ngrp = L_T / G, base = tid - gl;
+    const uint64_t gmask = (G == 64 ? ~0ull : ((1ull << G) - 1)) << (lane - gl);
+    const double spp = powi(RSHELL, 2 * (d + 1)), spm = powi(RSHELL, 2 * d);
+    for (int ai = grp; ai < nact; ai += ngrp) {
+        const int q = S.act[ai];
+        const bool ispl = q < nB;
+        const int t = q - nB;
+        const int e0 = ispl ? S.pO[q] : S.tO[t], e1 = ispl ? S.pO[q + 1] : S.tO[t + 1];
+        const double xb = ispl ? S.px[q] : S.tx[t], yb = ispl ? S.py[q] : S.ty[t], zb = ispl ? S.pz[q] : S.tz[t];
+        const double hb = ispl ? S.ph[q] : 0.0;
+        double3 p = make_double3(0.0, 0.0, 0.0), m = p;
+        bool anyp = false, anym = false;
+        for (int c0 = e0; c0 < e1; c0 += G) {
+            const int u = c0 + gl;
+            ShellTerm tm;
+            tm.tp = tm.tm = false;
+            tm.dx = tm.dy = tm.dz = tm.fp = tm.fm = 0.0;
+            if (u < e1) {
+                if (ispl) {
+                    const uint32_t e = S.pE[u];
+                    const int pr = (int)(e & 511u);
+                    if (S.plev[pr] >= d && S.pps[e >> 10] == ACTIVE)
+                        pl_shell_term_v((e >> 9) & 1u, xb, yb, zb, hb, S.px[pr], S.py[pr], S.pz[pr], S.ph[pr],
+                                        S.pgm[pr], spp, spm, true, tm);
+                } else {
+                    const int i = S.tpl[u];
+                    if (S.plev[i] >= d && S.tps[u] == ACTIVE)
+                        tp_shell_term_v(xb, yb, zb, S.px[i], S.py[i], S.pz[i], S.ph[i], S.pgm[i], spp, spm, true, tm);
+                }
+            }
+            const uint64_t live = __ballot(tm.tp || tm.tm) & gmask;
+            if (tm.tp || tm.tm) {
+                S.kfl[tid] = (tm.tp ? 1 : 0) | (tm.tm ? 2 : 0);
+                S.kdx[tid] = tm.dx;
+                S.kdy[tid] = tm.dy;
+                S.kdz[tid] = tm.dz;
+                S.kfp[tid] = tm.fp;
+                S.kfm[tid] = tm.fm;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (gl == 0) {
+                uint64_t bits = live >> lane;  // the group's lanes, in entry order
+                while (bits) {
+                    const int k = __builtin_ctzll(bits);
+                    bits &= bits - 1;
+                    const int w = base + k;
+                    const int fl = S.kfl[w];
+                    if (fl & 1) {
+                        anyp = true;
+                        p.x = fma(S.kfp[w], S.kdx[w], p.x);
+                        p.y = fma(S.kfp[w], S.kdy[w], p.y);
+                        p.z = fma(S.kfp[w], S.kdz[w], p.z);
+                    }
+                    if (fl & 2) {
+                        anym = true;
+                        m.x = fma(S.kfm[w], S.kdx[w], m.x);
+                        m.y = fma(S.kfm[w], S.kdy[w], m.y);
+                        m.z = fma(S.kfm[w], S.kdz[w], m.z);
+                    }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (gl == 0) {
+            if (ispl) lds_apply(S.pvx[q], S.pvy[q], S.pvz[q], dth, anyp, p, anym, m);
+            else lds_apply(S.tvx[t], S.tvy[t], S.tvz[t], dth, anyp, p, anym, m);
+        }
+    }
+    __syncthreads();
+}
+
+__device__ void lds_drift(LdsSet &S, const SubArgs &A, int nB, int nBt, int d, double dtl, int s)
+{
+    const double mu = A.st.sy.GMcb[s];
+    for (int q = threadIdx.x; q < nB + nBt; q += L_T) {
+        const bool ispl = q < nB;
+        const int t = q - nB;
+        const bool on = ispl ? (S.pst[q] != INACTIVE && S.plev[q] == d) : (S.tst[t] == ACTIVE && S.tlev[t] == d);
+        if (!on) continue;
+        double *x = ispl ? S.px : S.tx, *y = ispl ? S.py : S.ty, *z = ispl ? S.pz : S.tz;
+        double *vx = ispl ? S.pvx : S.tvx, *vy = ispl ? S.pvy : S.tvy, *vz = ispl ? S.pvz : S.tvz;
+        const int k = ispl ? q : t;
+        double rx = x[k], ry = y[k], rz = z[k], ux = vx[k], uy = vy[k], uz = vz[k];
+        const int fl = drift_one(mu, rx, ry, rz, ux, uy, uz, dtl);
+        x[k] = rx;
+        y[k] = ry;
+        z[k] = rz;
+        vx[k] = ux;
+        vy[k] = uy;
+        vz[k] = uz;
+        if (fl) {
+            if (ispl) {
+                S.pst[q] = DISCARDED_DRIFTERR;
+                atomicOr(&A.st.sy.err[s], 2);
+            } else {
+                S.tst[t] = DISCARDED_DRIFTERR;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// positions, velocities and pair flags to HBM (before the collision test
+// and at the end)
+__device__ void lds_store(LdsSet &S, const SubArgs &A, int nB, int nBt, int nEt, bool all)
+{
+    const RecState &a = A.st;
+    for (int q = threadIdx.x; q < nB; q += L_T) {
+        const int g = S.pg[q];
+        a.pl.rh.x[g] = S.px[q];
+        a.pl.rh.y[g] = S.py[q];
+        a.pl.rh.z[g] = S.pz[q];
+        a.pl.vb.x[g] = S.pvx[q];
+        a.pl.vb.y[g] = S.pvy[q];
+        a.pl.vb.z[g] = S.pvz[q];
+        if (all) {
+            a.pl.levelg[g] = S.plev[q];
+            a.pl.levelm[g] = S.plm[q];
+            a.pl.status[g] = S.pst[q];
+        }
+    }
+    for (int q = threadIdx.x; q < nBt; q += L_T) {
+        const int g = S.tg[q];
+        a.tp.rh.x[g] = S.tx[q];
+        a.tp.rh.y[g] = S.ty[q];
+        a.tp.rh.z[g] = S.tz[q];
+        a.tp.vb.x[g] = S.tvx[q];
+        a.tp.vb.y[g] = S.tvy[q];
+        a.tp.vb.z[g] = S.tvz[q];
+        if (all) {
+            a.tp.levelg[g] = S.tlev[q];
+            a.tp.levelm[g] = S.tlm[q];
+            a.tp.status[g] = S.tst[q];
+        }
+    }
+    for (int lp = threadIdx.x; lp < S.npair; lp += L_T) {
+        const int k = S.ppg[lp];
+        a.pp.status[k] = S.pps[lp];
+        a.pp.lvdotr[k] = S.ppv[lp];
+        if (all) a.pp.level[k] = S.ppl[lp];
+    }
+    for (int u = threadIdx.x; u < nEt; u += L_T) {
+        const int k = S.tpg[u];
+        a.pt.status[k] = S.tps[u];
+        a.pt.lvdotr[k] = S.tpv[u];
+        if (all) a.pt.level[k] = S.tpl_lev[u];
+    }
+}
+
+// collision detection at level d: the test reads whole rows, including
+// partners outside the slice, so it runs on HBM after a store; the pair
+// status it may change is then reloaded
+__device__ void lds_collision(LdsSet &S, const SubArgs &A, int nB, int nBt, int nEt, int d, double dtl, int s)
+{
+    lds_store(S, A, nB, nBt, nEt, false);
+    __syncthreads();
+    for (int q = threadIdx.x; q < nB + nBt; q += L_T) {
+        if (q < nB) {
+            if (A.has_pp && S.plev[q] >= d) rec_collide_pl(A.st, S.pg[q], dtl, s);
+        } else if (S.tlev[q - nB] >= d) {
+            rec_collide_tp(A.st, S.tg[q - nB], dtl, s);
+        }
+    }
+    __syncthreads();
+    for (int lp = threadIdx.x; lp < S.npair; lp += L_T) S.pps[lp] = A.st.pp.status[S.ppg[lp]];
+    for (int u = threadIdx.x; u < nEt; u += L_T) S.tps[u] = A.st.pt.status[S.tpg[u]];
+    __syncthreads();
+}
+
+__device__ void lds_set_levels(LdsSet &S, int nB, int nBt, int nEt, int d)
+{
+    const int tid = threadIdx.x, irecp = d + 1;
+    for (int q = tid; q < nB; q += L_T)
+        if (S.plev[q] == irecp) S.plev[q] = d;
+    for (int q = tid; q < nBt; q += L_T)
+        if (S.tlev[q] == irecp) S.tlev[q] = d;
+    for (int lp = tid; lp < S.npair; lp += L_T)
+        if (S.ppl[lp] == irecp) S.ppl[lp] = (int8_t)d;
+    for (int u = tid; u < nEt; u += L_T)
+        if (S.tpl_lev[u] == irecp) S.tpl_lev[u] = (int8_t)d;
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(L_T) void k_subtree_lds(const SubArgs *__restrict__ ap)
+{
+    const SubArgs &A = *ap;
+    const RecState &a = A.st;
+    __shared__ LdsSet S;
+    const int s = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int32_t *r = A.rng + 4 * s;
+    const int nB = r[1] - r[0], nBt = r[3] - r[2];
+    if (nB + nBt == 0) return;
+    const int32_t *gB = A.start.B + r[0], *gO = A.start.O + r[0];
+    const int32_t *gBt = A.start.Bt + r[2], *gOt = A.start.Ot + r[2];
+    const int e0 = gO[0], nE = gO[nB] - e0, et0 = gOt[0], nEt = gOt[nBt] - et0;
+    if (!A.lds || !lds_eligible(nB, nE, nBt, nEt)) return;
+    // ---- load the slice ----
+    for (int q = tid; q < nB; q += L_T) {
+        const int g = gB[q];
+        S.pg[q] = g;
+        S.px[q] = a.pl.rh.x[g];
+        S.py[q] = a.pl.rh.y[g];
+        S.pz[q] = a.pl.rh.z[g];
+        S.pvx[q] = a.pl.vb.x[g];
+        S.pvy[q] = a.pl.vb.y[g];
+        S.pvz[q] = a.pl.vb.z[g];
+        S.ph[q] = a.pl.rhill[g];
+        S.pgm[q] = a.pl.Gm[g];
+        S.pR[q] = a.pl.R[g];
+        S.plev[q] = a.pl.levelg[g];
+        S.plm[q] = a.pl.levelm[g];
+        S.pst[q] = a.pl.status[g];
+        S.pO[q] = gO[q] - e0;
+    }
+    for (int q = tid; q < nBt; q += L_T) {
+        const int g = gBt[q];
+        S.tg[q] = g;
+        S.tx[q] = a.tp.rh.x[g];
+        S.ty[q] = a.tp.rh.y[g];
+        S.tz[q] = a.tp.rh.z[g];
+        S.tvx[q] = a.tp.vb.x[g];
+        S.tvy[q] = a.tp.vb.y[g];
+        S.tvz[q] = a.tp.vb.z[g];
+        S.tlev[q] = a.tp.levelg[g];
+        S.tlm[q] = a.tp.levelm[g];
+        S.tst[q] = a.tp.status[g];
+        S.tO[q] = gOt[q] - et0;
+    }
+    if (tid == 0) {
+        S.pO[nB] = nE;
+        S.tO[nBt] = nEt;
+        S.npair = 0;
+        S.bad = 0;
+    }
+    if (A.prof)
+        for (int k = tid; k < (MAXLEV + 2) * NPROF; k += L_T) S.prof[k] = 0;
+    __syncthreads();
+    // pl entries: local partners; each pair gets a local id from its index1
+    // entry.  A partner outside the slice or a pair without both entries
+    // cannot occur in a level set; if one did, the system reports an error
+    // before touching any state.
+    for (int u = tid; u < nE; u += L_T) {
+        const int2 en = A.start.E[e0 + u];
+        const uint32_t pr = (uint32_t)lds_lower_bound(S.pg, nB, en.y);
+        if (pr >= (uint32_t)nB || S.pg[pr] != en.y) {
+            S.bad = 1;
+            S.pE[u] = 0;
+            continue;
+        }
+        if (en.x & 1) {
+            const int k = (int)((uint32_t)en.x >> 1);
+            const int lp = atomicAdd(&S.npair, 1);
+            if (lp >= L_NP) {
+                S.bad = 1;
+                S.pE[u] = 0;
+                continue;
+            }
+            S.ppg[lp] = k;
+            S.ppi[lp] = (int16_t)lds_owner(S.pO, nB, u);
+            S.ppj[lp] = (int16_t)pr;
+            S.pps[lp] = a.pp.status[k];
+            S.ppl[lp] = a.pp.level[k];
+            S.ppv[lp] = a.pp.lvdotr[k];
+            S.pE[u] = ((uint32_t)lp << 10) | (1u << 9) | pr;
+        } else {
+            S.pE[u] = pr;
+        }
+    }
+    for (int u = tid; u < nEt; u += L_T) {
+        const int2 en = A.start.Et[et0 + u];
+        const int pl = lds_lower_bound(S.pg, nB, en.y);
+        if (pl >= nB || S.pg[pl] != en.y) {
+            S.bad = 1;
+            S.tpl[u] = 0;
+        } else {
+            S.tpl[u] = (int16_t)pl;
+        }
+        S.tpt[u] = (int16_t)lds_owner(S.tO, nBt, u);
+        S.tpg[u] = en.x;
+        S.tps[u] = a.pt.status[en.x];
+        S.tpl_lev[u] = a.pt.level[en.x];
+        S.tpv[u] = a.pt.lvdotr[en.x];
+    }
+    __syncthreads();
+    // index2-side entries take the local id of the same pair from the index1
+    // body's entries
+    for (int u = tid; u < nE; u += L_T) {
+        if (S.bad) break;
+        const uint32_t e = S.pE[u];
+        if (e & (1u << 9)) continue;
+        const int k = (int)((uint32_t)A.start.E[e0 + u].x >> 1);
+        const int p = (int)(e & 511u);
+        bool found = false;
+        for (int w = S.pO[p]; w < S.pO[p + 1]; w++) {
+            const uint32_t f = S.pE[w];
+            if ((f & (1u << 9)) && S.ppg[f >> 10] == k) {
+                S.pE[u] = (f & ~1023u) | e;
+                found = true;
+                break;
+            }
+        }
+        if (!found) S.bad = 1;
+    }
+    __syncthreads();
+    if (S.bad) {
+        if (tid == 0) atomicOr(&a.sy.err[s], 4);
+        return;
+    }
+    // ---- the subtree ----
+    const int d0 = A.d0;
+    int d = d0, j = 1;
+    bool resume = false;
+    int nsub = 0, maxd = d0;
+    for (;;) {
+        double p3 = 1.0;
+        for (int q = 0; q < d; q++) p3 *= NTENC;
+        const double dtl = A.dt / p3, dth = 0.5 * dtl;
+        if (!resume) {
+            if (j > NTENC) {  // level finished
+                if (d == d0) break;
+                d--;
+                j = S.jst[d];
+                resume = true;
+                continue;
+            }
+            if (dtl / A.dt < 2.220446049250313e-16) {  // symba_step.f90:196-201
+                if (tid == 0) atomicOr(&a.sy.err[s], 8);
+                break;
+            }
+            nsub++;
+            const bool prof = A.prof && tid == 0;
+            unsigned long long t0 = prof ? wall_clock64() : 0, t1 = 0, t2 = 0, t3 = 0;
+            const bool lenc = lds_check(S, A, nB, nEt, d, dtl);
+            if (prof) t1 = wall_clock64();
+            lds_kicks(S, nB, nBt, nE, nEt, d, dth);
+            if (prof) t2 = wall_clock64();
+            lds_drift(S, A, nB, nBt, d, dtl, s);
+            if (prof) {
+                t3 = wall_clock64();
+                unsigned long long *P = S.prof + d * NPROF;
+                P[0] += 1;
+                P[1] += nB;
+                P[2] += nE;
+                P[3] += t1 - t0;
+                P[4] += t2 - t1;
+                P[5] += t3 - t2;
+            }
+            if (lenc) {
+                if (d + 1 > MAXLEV) {
+                    if (tid == 0) atomicOr(&a.sy.err[s], 8);
+                    break;
+                }
+                if (tid == 0) S.jst[d] = j;
+                __syncthreads();
+                d++;
+                if (d > maxd) maxd = d;
+                j = 1;
+                continue;
+            }
+        }
+        resume = false;
+        {
+            const bool prof = A.prof && tid == 0;
+            unsigned long long t0 = prof ? wall_clock64() : 0;
+            lds_kicks(S, nB, nBt, nE, nEt, d, dth);
+            unsigned long long t1 = prof ? wall_clock64() : 0;
+            if (A.lclose) lds_collision(S, A, nB, nBt, nEt, d, dtl, s);
+            lds_set_levels(S, nB, nBt, nEt, d);
+            if (prof) {
+                S.prof[d * NPROF + 6] += t1 - t0;
+                S.prof[d * NPROF + 7] += wall_clock64() - t1;
+            }
+        }
+        j++;
+    }
+    __syncthreads();
+    lds_store(S, A, nB, nBt, nEt, true);
+    if (tid == 0) {
+        atomicAdd(&A.stats[0], nsub);
+        atomicMax(&A.stats[1], maxd);
+    }
+    if (A.prof)
+        for (int k = tid; k < (MAXLEV + 2) * NPROF; k += L_T)
+            if (S.prof[k]) atomicAdd(&A.prof[k], S.prof[k]);
+}
+
 size_t subtree_args_bytes() { return sizeof(SubArgs); }
 
-// diagnostic profile of the last launch (SWG_PROF_SUBTREE=1), printed to stderr
+// diagnostic profile of the launches since the last report
+// (SWG_PROF_SUBTREE=1), printed to stderr
 static unsigned long long *g_prof = nullptr;
 void subtree_profile_report(hipStream_t st)
 {
@@ -396,7 +936,7 @@ void subtree_profile_report(hipStream_t st)
     (void)hipStreamSynchronize(st);
     (void)hipMemcpy(h, g_prof, sizeof h, hipMemcpyDeviceToHost);
     const double tick_us = 0.01;  // wall_clock64 runs at 100 MHz
-    fprintf(stderr, "[swg] level substeps   avg_nB   avg_nE  check_ms  kicks_ms  drift_ms  build_ms levels_ms\n");
+    fprintf(stderr, "[swg] level substeps   avg_nB   avg_nE  check_ms  kick1_ms  drift_ms kick2/bld levels_ms\n");
     for (int d = 1; d <= MAXLEV; d++) {
         const unsigned long long *p = h + d * NPROF;
         if (!p[0]) continue;
@@ -404,6 +944,7 @@ void subtree_profile_report(hipStream_t st)
                 (double)p[2] / p[0], p[3] * tick_us * 1e-3, p[4] * tick_us * 1e-3, p[5] * tick_us * 1e-3,
                 p[6] * tick_us * 1e-3, p[7] * tick_us * 1e-3);
     }
+    (void)hipMemsetAsync(g_prof, 0, sizeof h, st);
 }
 
 // h_stage: pinned host memory, d_args: device memory, both of
@@ -424,6 +965,10 @@ void launch_subtree(hipStream_t st, const LevelArgs &la, const LevelView &start,
     a.short_circuit = short_circuit;
     a.has_pp = la.has_pp;
     a.has_pt = la.has_pt;
+    {
+        const char *e = getenv("SWG_SUBTREE_LDS");
+        a.lds = !(e && e[0] == '0');
+    }
     a.start = start;
     a.rng = rng;
     a.d0 = d0;
@@ -432,13 +977,12 @@ void launch_subtree(hipStream_t st, const LevelArgs &la, const LevelView &start,
     a.stats = stats;
     a.prof = nullptr;
     if (getenv("SWG_PROF_SUBTREE") && getenv("SWG_PROF_SUBTREE")[0] == '1') {
-        if (!g_prof) (void)hipMalloc((void **)&g_prof, sizeof(unsigned long long) * NPROF * (MAXLEV + 2));
-        if (g_prof) {
-            (void)hipMemsetAsync(g_prof, 0, sizeof(unsigned long long) * NPROF * (MAXLEV + 2), st);
-            a.prof = g_prof;
-        }
+        if (!g_prof && hipMalloc((void **)&g_prof, sizeof(unsigned long long) * NPROF * (MAXLEV + 2)) == hipSuccess)
+            (void)hipMemset(g_prof, 0, sizeof(unsigned long long) * NPROF * (MAXLEV + 2));
+        if (g_prof) a.prof = g_prof;
     }
     (void)hipMemcpyAsync(d_args, h_stage, sizeof(SubArgs), hipMemcpyHostToDevice, st);
+    k_subtree_lds<<<la.nsys, L_T, 0, st>>>((const SubArgs *)d_args);
     k_subtree<<<la.nsys, ST_T, 0, st>>>((const SubArgs *)d_args);
 }
 

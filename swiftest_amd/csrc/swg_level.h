@@ -59,58 +59,40 @@ struct RecState {
     SysDev sy;
 };
 
-// symba_encounter_check_list_plpl for one pair (symba_encounter_check.f90:
-// 121-154); returns true when the pair enters level d+1.
-__device__ inline bool rec_check_pp(const RecState &a, int k, int d, double dtl)
+// ---- per-pair arithmetic on values (shared by the HBM and LDS paths) --------
+
+// The level-(d+1) test of one pl-pl pair (symba_encounter_check.f90:121-154):
+// relative motion of j with respect to i, the pair's shell radius at d+1 and
+// the physical-overlap exclusion.  Writes lvdotr; returns true when the pair
+// enters level d+1.
+__device__ inline bool check_pp_v(double xi, double yi, double zi, double vxi, double vyi, double vzi, double hi,
+                                  double Ri, double xj, double yj, double zj, double vxj, double vyj, double vzj,
+                                  double hj, double Rj, int irec, double dtl, bool &lv)
 {
-    if (a.pp.status[k] != ACTIVE || a.pp.level[k] != d) return false;
-    const int irec = d + 1;
-    const int i = a.pp.i1[k], j = a.pp.i2[k];
-    const PlDev &pl = a.pl;
-    double xr = pl.rh.x[j] - pl.rh.x[i], yr = pl.rh.y[j] - pl.rh.y[i], zr = pl.rh.z[j] - pl.rh.z[i];
-    double vxr = pl.vb.x[j] - pl.vb.x[i], vyr = pl.vb.y[j] - pl.vb.y[i], vzr = pl.vb.z[j] - pl.vb.z[i];
+    double xr = xj - xi, yr = yj - yi, zr = zj - zi;
+    double vxr = vxj - vxi, vyr = vyj - vyi, vzr = vzj - vzi;
     const double rsh = rshell_pow(irec);
-    double rci = (pl.rhill[i] * RHSCALE) * rsh, rcj = (pl.rhill[j] * RHSCALE) * rsh;
-    bool lv;
+    double rci = (hi * RHSCALE) * rsh, rcj = (hj * RHSCALE) * rsh;
     bool le = encounter_one(xr, yr, zr, vxr, vyr, vzr, rci + rcj, dtl, lv);
-    a.pp.lvdotr[k] = lv;
     if (le) {
         double rji2 = dot3(xr, yr, zr, xr, yr, zr);
-        double rs = pl.R[i] + pl.R[j];
+        double rs = Ri + Rj;
         le = rji2 > rs * rs;
-    }
-    if (le) {
-        a.pp.level[k] = (int8_t)irec;
-        pl.levelg[i] = irec;
-        pl.levelg[j] = irec;
-        atomicMax(&pl.levelm[i], irec);
-        atomicMax(&pl.levelm[j], irec);
     }
     return le;
 }
 
-// symba_encounter_check_list_pltp for one pair (symba_encounter_check.f90:
-// 195-228)
-__device__ inline bool rec_check_pt(const RecState &a, int k, int d, double dtl)
+// The level-(d+1) test of one pl-tp pair (symba_encounter_check.f90:195-228)
+__device__ inline bool check_pt_v(double xi, double yi, double zi, double vxi, double vyi, double vzi, double hi,
+                                  double Ri, double xt, double yt, double zt, double vxt, double vyt, double vzt,
+                                  int irec, double dtl, bool &lv)
 {
-    if (a.pt.status[k] != ACTIVE || a.pt.level[k] != d) return false;
-    const int irec = d + 1;
-    const int i = a.pt.i1[k], j = a.pt.i2[k];
-    double xr = a.tp.rh.x[j] - a.pl.rh.x[i], yr = a.tp.rh.y[j] - a.pl.rh.y[i], zr = a.tp.rh.z[j] - a.pl.rh.z[i];
-    double vxr = a.tp.vb.x[j] - a.pl.vb.x[i], vyr = a.tp.vb.y[j] - a.pl.vb.y[i], vzr = a.tp.vb.z[j] - a.pl.vb.z[i];
-    bool lv;
-    bool le = encounter_one(xr, yr, zr, vxr, vyr, vzr, (a.pl.rhill[i] * RHSCALE) * rshell_pow(irec), dtl, lv);
-    a.pt.lvdotr[k] = lv;
+    double xr = xt - xi, yr = yt - yi, zr = zt - zi;
+    double vxr = vxt - vxi, vyr = vyt - vyi, vzr = vzt - vzi;
+    bool le = encounter_one(xr, yr, zr, vxr, vyr, vzr, (hi * RHSCALE) * rshell_pow(irec), dtl, lv);
     if (le) {
         double rji2 = dot3(xr, yr, zr, xr, yr, zr);
-        le = rji2 > a.pl.R[i] * a.pl.R[i];
-    }
-    if (le) {
-        a.pt.level[k] = (int8_t)irec;
-        a.pl.levelg[i] = irec;
-        atomicMax(&a.pl.levelm[i], irec);
-        a.tp.levelg[j] = irec;
-        atomicMax(&a.tp.levelm[j], irec);
+        le = rji2 > Ri * Ri;
     }
     return le;
 }
@@ -118,34 +100,29 @@ __device__ inline bool rec_check_pt(const RecState &a, int k, int d, double dtl)
 // One pl-pl shell term of body b (symba_kick.f90:150-200): the pair's
 // contribution f*(dx,dy,dz) to b's sum at the outer shell (sign +1, irecl =
 // d+1) and, when `minus`, at the inner shell (sign -1, irecl = d).  tp / tm
-// tell whether the pair lies inside the respective shell.
+// tell whether the pair lies inside the respective shell.  bi: b is index1;
+// (xp, yp, zp, hp, gp) is the partner.
 struct ShellTerm {
     double dx, dy, dz, fp, fm;
     bool tp, tm;
 };
 
-__device__ inline void pl_shell_term(const RecState &a, int2 en, double xb, double yb, double zb, double hb,
-                                     double spp, double spm, bool minus, ShellTerm &t)
+__device__ inline void pl_shell_term_v(bool bi, double xb, double yb, double zb, double hb, double xp, double yp,
+                                       double zp, double hp, double gp, double spp, double spm, bool minus,
+                                       ShellTerm &t)
 {
-    const PlDev &pl = a.pl;
-    t.tp = t.tm = false;
-    const int k = (int)((uint32_t)en.x >> 1);
-    if (a.pp.status[k] != ACTIVE) return;
-    const int p = en.y;
-    const bool bi = en.x & 1;  // b is index1
-    double sh = bi ? hb + pl.rhill[p] : pl.rhill[p] + hb;
+    double sh = bi ? hb + hp : hp + hb;
     double s2 = (sh * sh) * (RHSCALE * RHSCALE);
     if (bi) {
-        t.dx = pl.rh.x[p] - xb;
-        t.dy = pl.rh.y[p] - yb;
-        t.dz = pl.rh.z[p] - zb;
+        t.dx = xp - xb;
+        t.dy = yp - yb;
+        t.dz = zp - zb;
     } else {
-        t.dx = xb - pl.rh.x[p];
-        t.dy = yb - pl.rh.y[p];
-        t.dz = zb - pl.rh.z[p];
+        t.dx = xb - xp;
+        t.dy = yb - yp;
+        t.dz = zb - zp;
     }
     double r2 = dot3(t.dx, t.dy, t.dz, t.dx, t.dy, t.dz);
-    const double gp = pl.Gm[p];
     double fac;
     if (shell_fac(s2 * spp, r2, fac)) {
         t.tp = true;
@@ -157,20 +134,15 @@ __device__ inline void pl_shell_term(const RecState &a, int2 en, double xb, doub
     }
 }
 
-// One pl-tp shell term of test particle t (symba_kick.f90:255-300)
-__device__ inline void tp_shell_term(const RecState &a, int2 en, double xt, double yt, double zt, double spp,
-                                     double spm, bool minus, ShellTerm &t)
+// One pl-tp shell term of a test particle (symba_kick.f90:255-300)
+__device__ inline void tp_shell_term_v(double xt, double yt, double zt, double xi, double yi, double zi, double hi,
+                                       double gi, double spp, double spm, bool minus, ShellTerm &t)
 {
-    t.tp = t.tm = false;
-    if (a.pt.status[en.x] != ACTIVE) return;
-    const int i = en.y;
-    double hi = a.pl.rhill[i];
     double s2 = (hi * hi) * (RHSCALE * RHSCALE);
-    t.dx = xt - a.pl.rh.x[i];
-    t.dy = yt - a.pl.rh.y[i];
-    t.dz = zt - a.pl.rh.z[i];
+    t.dx = xt - xi;
+    t.dy = yt - yi;
+    t.dz = zt - zi;
     double r2 = dot3(t.dx, t.dy, t.dz, t.dx, t.dy, t.dz);
-    const double gi = a.pl.Gm[i];
     double fac;
     if (shell_fac(s2 * spp, r2, fac)) {
         t.tp = true;
@@ -180,6 +152,75 @@ __device__ inline void tp_shell_term(const RecState &a, int2 en, double xt, doub
         t.tm = true;
         t.fm = -(fac * gi);
     }
+}
+
+// ---- the same on the HBM-resident state --------------------------------------
+
+// symba_encounter_check_list_plpl for one pair; returns true when the pair
+// enters level d+1
+__device__ inline bool rec_check_pp(const RecState &a, int k, int d, double dtl)
+{
+    if (a.pp.status[k] != ACTIVE || a.pp.level[k] != d) return false;
+    const int irec = d + 1;
+    const int i = a.pp.i1[k], j = a.pp.i2[k];
+    const PlDev &pl = a.pl;
+    bool lv;
+    bool le = check_pp_v(pl.rh.x[i], pl.rh.y[i], pl.rh.z[i], pl.vb.x[i], pl.vb.y[i], pl.vb.z[i], pl.rhill[i],
+                         pl.R[i], pl.rh.x[j], pl.rh.y[j], pl.rh.z[j], pl.vb.x[j], pl.vb.y[j], pl.vb.z[j],
+                         pl.rhill[j], pl.R[j], irec, dtl, lv);
+    a.pp.lvdotr[k] = lv;
+    if (le) {
+        a.pp.level[k] = (int8_t)irec;
+        pl.levelg[i] = irec;
+        pl.levelg[j] = irec;
+        atomicMax(&pl.levelm[i], irec);
+        atomicMax(&pl.levelm[j], irec);
+    }
+    return le;
+}
+
+// symba_encounter_check_list_pltp for one pair
+__device__ inline bool rec_check_pt(const RecState &a, int k, int d, double dtl)
+{
+    if (a.pt.status[k] != ACTIVE || a.pt.level[k] != d) return false;
+    const int irec = d + 1;
+    const int i = a.pt.i1[k], j = a.pt.i2[k];
+    const PlDev &pl = a.pl;
+    bool lv;
+    bool le = check_pt_v(pl.rh.x[i], pl.rh.y[i], pl.rh.z[i], pl.vb.x[i], pl.vb.y[i], pl.vb.z[i], pl.rhill[i],
+                         pl.R[i], a.tp.rh.x[j], a.tp.rh.y[j], a.tp.rh.z[j], a.tp.vb.x[j], a.tp.vb.y[j],
+                         a.tp.vb.z[j], irec, dtl, lv);
+    a.pt.lvdotr[k] = lv;
+    if (le) {
+        a.pt.level[k] = (int8_t)irec;
+        a.pl.levelg[i] = irec;
+        atomicMax(&a.pl.levelm[i], irec);
+        a.tp.levelg[j] = irec;
+        atomicMax(&a.tp.levelm[j], irec);
+    }
+    return le;
+}
+
+__device__ inline void pl_shell_term(const RecState &a, int2 en, double xb, double yb, double zb, double hb,
+                                     double spp, double spm, bool minus, ShellTerm &t)
+{
+    t.tp = t.tm = false;
+    const int k = (int)((uint32_t)en.x >> 1);
+    if (a.pp.status[k] != ACTIVE) return;
+    const int p = en.y;
+    const PlDev &pl = a.pl;
+    pl_shell_term_v(en.x & 1, xb, yb, zb, hb, pl.rh.x[p], pl.rh.y[p], pl.rh.z[p], pl.rhill[p], pl.Gm[p], spp, spm,
+                    minus, t);
+}
+
+__device__ inline void tp_shell_term(const RecState &a, int2 en, double xt, double yt, double zt, double spp,
+                                     double spm, bool minus, ShellTerm &t)
+{
+    t.tp = t.tm = false;
+    if (a.pt.status[en.x] != ACTIVE) return;
+    const int i = en.y;
+    tp_shell_term_v(xt, yt, zt, a.pl.rh.x[i], a.pl.rh.y[i], a.pl.rh.z[i], a.pl.rhill[i], a.pl.Gm[i], spp, spm,
+                    minus, t);
 }
 
 // vb += dth * (outer-shell sum), then vb -= dth * (inner-shell sum), each only

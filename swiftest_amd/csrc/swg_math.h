@@ -215,7 +215,7 @@ __device__ inline int kepu_lag(double &s, double dt, double r0, double mu, doubl
 }
 
 // one Danby drift of (r, v) over dt0 (:141-231); returns iflag
-__device__ inline int drift_dan(double mu, double &rx0, double &ry0, double &rz0, double &vx0, double &vy0,
+__device__ __attribute__((always_inline)) inline int drift_dan(double mu, double &rx0, double &ry0, double &rz0, double &vx0, double &vy0,
                                 double &vz0, double dt0)
 {
     double dt = dt0;
@@ -273,17 +273,23 @@ __device__ inline int drift_dan(double mu, double &rx0, double &ry0, double &rz0
 }
 
 // swiftest_drift_one (:111-138): on failure retry as ten dt/10 substeps,
-// stopping at the first substep that fails.
+// stopping at the first substep that fails.  Written with a single call of
+// drift_dan so that the compiler inlines it (a call would save and restore
+// the caller's live registers around every drift).
 __device__ inline int drift_one(double mu, double &rx, double &ry, double &rz, double &vx, double &vy, double &vz,
                                 double dt)
 {
-    int fl = drift_dan(mu, rx, ry, rz, vx, vy, vz, dt);
-    if (fl != 0) {
-        double dttmp = 0.1 * dt;
-        for (int i = 0; i < 10; i++) {
-            fl = drift_dan(mu, rx, ry, rz, vx, vy, vz, dttmp);
-            if (fl != 0) break;
+    double h = dt;
+    int i = -1, fl;
+    for (;;) {
+        fl = drift_dan(mu, rx, ry, rz, vx, vy, vz, h);
+        if (i < 0) {  // the full step
+            if (fl == 0) break;
+            h = 0.1 * dt;
+            i = 0;
+            continue;
         }
+        if (fl != 0 || ++i >= 10) break;
     }
     return fl;
 }
