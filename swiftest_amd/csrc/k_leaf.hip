@@ -89,7 +89,7 @@ __global__ void k_leaf_tri(int npl, int nplm, const double *r, const double *Gm,
             double s = R[b] + R[j];
             if (!(rji2 > s * s)) continue;
         }
-        double fac = Gm[j] / (rji2 * sqrt(rji2));
+        double fac = gm_over_r3(Gm[j], rji2, pair_g_ok(Gm[j]));
         ax = fma(fac, rx, ax);
         ay = fma(fac, ry, ay);
         az = fma(fac, rz, az);
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(BLK) void k_leaf_tp(int ntp, int npl, const double 
             for (int q = 0; q < jn; q++) {
                 double rx = xt - sx[q], ry = yt - sy[q], rz = zt - sz[q];
                 double rji2 = dot3(rx, ry, rz, rx, ry, rz);
-                double fac = sg[q] / (rji2 * sqrt(rji2));
+                double fac = gm_over_r3(sg[q], rji2, pair_g_ok(sg[q]));
                 px = fma(-fac, rx, px);
                 py = fma(-fac, ry, py);
                 pz = fma(-fac, rz, pz);
@@ -347,7 +347,7 @@ __device__ void whm_getacch_pl(const WhmArgs &a, int s, int p0, int n)
                 double rs = pl.R[i] + pl.R[j];
                 if (!(rji2 > rs * rs)) continue;
             }
-            double fac = pl.Gm[j] / (rji2 * sqrt(rji2));
+            double fac = gm_over_r3(pl.Gm[j], rji2, pair_g_ok(pl.Gm[j]));
             ax = fma(fac, rx, ax);
             ay = fma(fac, ry, ay);
             az = fma(fac, rz, az);
@@ -500,7 +500,7 @@ __global__ __launch_bounds__(BLK) void k_whm_tp_kick(WhmArgs a, TpDev tp, const 
             for (int q = 0; q < qn; q++) {
                 double rx = xt - sx[q], ry = yt - sy[q], rz = zt - sz[q];
                 double rji2 = dot3(rx, ry, rz, rx, ry, rz);
-                double fac = sg[q] / (rji2 * sqrt(rji2));
+                double fac = gm_over_r3(sg[q], rji2, pair_g_ok(sg[q]));
                 px = fma(-fac, rx, px);
                 py = fma(-fac, ry, py);
                 pz = fma(-fac, rz, pz);

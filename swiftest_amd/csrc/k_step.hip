@@ -8,6 +8,14 @@
 #include "swg_kernels.h"
 #include "swg_math.h"
 
+// unroll factors of the pair loops (independent pairs in flight per lane)
+#ifndef SWG_UNROLL_PL
+#define SWG_UNROLL_PL 4
+#endif
+#ifndef SWG_UNROLL_TP
+#define SWG_UNROLL_TP 2
+#endif
+
 namespace swg {
 
 static inline int cdiv(int64_t a, int b) { return (int)((a + b - 1) / b); }
@@ -153,16 +161,19 @@ __global__ __launch_bounds__(BLK) void k_kick_pl(PlDev pl, SysDev sy, ListDev pp
     double ax = 0.0, ay = 0.0, az = 0.0;
     for (int jt = p0; jt < p1; jt += BLK) {
         int j = jt + (int)threadIdx.x;
+        bool gok = true;
         if (j < p1) {
             sx[threadIdx.x] = pl.rh.x[j];
             syy[threadIdx.x] = pl.rh.y[j];
             sz[threadIdx.x] = pl.rh.z[j];
             sgm[threadIdx.x] = pl.Gm[j];
             sr[threadIdx.x] = pl.R[j];
+            gok = pair_g_ok(pl.Gm[j]);
         }
-        __syncthreads();
+        const bool tile_gok = __syncthreads_and(gok);
         const int jn = min(BLK, p1 - jt);
         if (valid) {
+#pragma unroll SWG_UNROLL_PL
             for (int q = 0; q < jn; q++) {
                 if (jt + q == i) continue;
                 double rx = sx[q] - xi, ry = syy[q] - yi, rz = sz[q] - zi;
@@ -171,7 +182,7 @@ __global__ __launch_bounds__(BLK) void k_kick_pl(PlDev pl, SysDev sy, ListDev pp
                     double rs = Ri + sr[q];
                     if (!(rji2 > rs * rs)) continue;
                 }
-                double fac = sgm[q] / (rji2 * sqrt(rji2));
+                double fac = gm_over_r3(sgm[q], rji2, tile_gok);
                 ax = fma(fac, rx, ax);
                 ay = fma(fac, ry, ay);
                 az = fma(fac, rz, az);
@@ -292,19 +303,22 @@ __global__ __launch_bounds__(BLK) void k_kick_tp(TpDev tp, PlDev pl, SysDev sy, 
     double px = 0.0, py = 0.0, pz = 0.0;
     for (int it = p0; it < p1; it += BLK) {
         int q0 = it + (int)threadIdx.x;
+        bool gok = true;
         if (q0 < p1) {
             sx[threadIdx.x] = rp.x[q0];
             syy[threadIdx.x] = rp.y[q0];
             sz[threadIdx.x] = rp.z[q0];
             sgm[threadIdx.x] = pl.Gm[q0];
+            gok = pair_g_ok(pl.Gm[q0]);
         }
-        __syncthreads();
+        const bool tile_gok = __syncthreads_and(gok);
         const int qn = min(BLK, p1 - it);
         if (valid) {
+#pragma unroll SWG_UNROLL_TP
             for (int q = 0; q < qn; q++) {
                 double rx = xt - sx[q], ry = yt - syy[q], rz = zt - sz[q];
                 double rji2 = dot3(rx, ry, rz, rx, ry, rz);
-                double fac = sgm[q] / (rji2 * sqrt(rji2));
+                double fac = gm_over_r3(sgm[q], rji2, tile_gok);
                 px = fma(-fac, rx, px);
                 py = fma(-fac, ry, py);
                 pz = fma(-fac, rz, pz);

@@ -40,6 +40,60 @@ __host__ __device__ inline double dot3(double x1, double y1, double z1, double x
 }
 
 // ---------------------------------------------------------------------------
+// The pair factor Gm / (r2 * sqrt(r2)) of the direct sums
+// ---------------------------------------------------------------------------
+
+// For gfx950 this compiler lowers IEEE sqrt and '/' to v_rsq / v_rcp followed
+// by Newton and correction steps, wrapped in range scaling (ldexp for tiny
+// operands, v_div_scale / v_div_fmas) and special-value fix-ups.  The two
+// cores below are those same steps in the same order, without the wrapping.
+// Where the wrapping is the identity they give the IEEE results bit for bit:
+//   sqrt_core(x): normal x >= 2^-767 (below that the operand is rescaled);
+//   div_core(a, b): a = 0, or a and b normal with a, b, a/b and 1/b far from
+//   the exponent limits and exponent(a) - exponent(b) < 768.
+__device__ inline double sqrt_core(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+}
+
+__device__ inline double div_core(double a, double b)
+{
+    double y = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    double q = a * y;
+    double r = fma(-b, q, a);
+    return fma(r, y, q);
+}
+
+// g / (r2 * sqrt(r2)), bit for bit.  The cores are used when r2 lies in
+// [2^-400, 2^400] and gok says g is 0 or within [2^-500, 2^500] (so r2*sqrt(r2)
+// is within [2^-600, 2^600] and every condition above holds); otherwise the
+// IEEE operations run.  Separations of an N-body system in AU are far inside
+// that range.
+constexpr double PAIR_R2_LO = 0x1p-400, PAIR_R2_HI = 0x1p400;
+constexpr double PAIR_G_LO = 0x1p-500, PAIR_G_HI = 0x1p500;
+
+__host__ __device__ inline bool pair_g_ok(double g) { return g == 0.0 || (g >= PAIR_G_LO && g <= PAIR_G_HI); }
+
+__device__ inline double gm_over_r3(double g, double r2, bool gok)
+{
+    if (gok && r2 >= PAIR_R2_LO && r2 <= PAIR_R2_HI) return div_core(g, r2 * sqrt_core(r2));
+    return g / (r2 * sqrt(r2));
+}
+
+// ---------------------------------------------------------------------------
 // Danby / universal-variable Kepler drift (swiftest_drift.f90:111-580).
 // ---------------------------------------------------------------------------
 

@@ -71,6 +71,32 @@ def test_kick_tp_bit_exact(npl, ntp):
     assert np.array_equal(ref, got), ulp_diff(ref, got)
 
 
+def test_pair_factor_outside_the_fast_range_is_still_ieee():
+    """The pair factor Gm / (r2 sqrt(r2)) takes a shortened sequence only for
+    separations and masses well inside the double range; separations from
+    1e-150 to 1e150 AU and masses of 0, 1e-170 and 1e170 (both sides of that
+    range) must give the IEEE result bit for bit."""
+    rng = np.random.Generator(np.random.PCG64(31))
+    npl, ntp = 70, 600
+    rpl = rng.normal(0, 1.0, (npl, 3))
+    Gm = ics.GU * np.exp(rng.uniform(np.log(1e-9), np.log(1e-3), npl))
+    Gm[3], Gm[10], Gm[40] = 0.0, 1e-170, 1e170
+    scale = 10.0 ** rng.uniform(-150, 150, ntp)
+    rtp = rpl[rng.integers(0, npl, ntp)] + rng.normal(0, 1.0, (ntp, 3)) * scale[:, None]
+    lmask = np.ones(ntp, np.int32)
+    acc0 = np.zeros((ntp, 3))
+    with np.errstate(all="ignore"):
+        ref = O.accel_pltp(rtp, rpl, Gm, lmask, acc0)
+    got = S.swiftest_kick_getacch_int_all_tp(ntp, npl, rtp, rpl, Gm, lmask, acc0)
+    assert np.array_equal(ref, got, equal_nan=True), ulp_diff(ref, got)
+    rtri = np.concatenate([rpl[:30], rpl[:30] + rng.normal(0, 1.0, (30, 3)) * (10.0 ** rng.uniform(-150, 150, 30))[:, None]])
+    gtri = np.concatenate([Gm[:30], Gm[30:60]])
+    with np.errstate(all="ignore"):
+        ref = O.accel_plpl(rtri, gtri)
+    got = S.swiftest_kick_getacch_int_all_tri(60, 60, rtri, gtri, None, np.zeros((60, 3)))
+    assert np.array_equal(ref, got, equal_nan=True), ulp_diff(ref, got)
+
+
 def _orbits(seed, n, amin, amax, emax, hyper=False):
     rng = np.random.Generator(np.random.PCG64(seed))
     a = rng.uniform(amin, amax, n)
