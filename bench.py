@@ -68,6 +68,81 @@ def cpu_baseline(sysic, target_s):
                        f"encounter detection, recursion or drift")}
 
 
+def ensemble_leg(nsys, steps, warmup, sample, log):
+    """Batched ensemble (config C4): nsys independent SyMBA systems of uneven
+    size advanced together, one launch per step phase for all of them, with
+    the pair kernels' tiles picked by the autotuner.  Compared with the same
+    systems run one at a time through the same library (the plain GPU path),
+    measured on a seeded sample of `sample` systems and scaled."""
+    import numpy as np
+    from swiftest_amd import ics
+    import swiftest_amd as S
+    dt = 0.01
+    systems = ics.ensemble(seed=4, nsys=nsys)
+    g = S.GpuSystem(S.SYMBA, systems, dt, istep_out=0, lclose=False)
+    blocks = g.autotune(3)
+    g.run(warmup)
+    t0 = time.perf_counter()
+    g.run(steps)
+    t_ens = time.perf_counter() - t0
+    st = g.stats()
+    del g
+    rate = nsys * steps / t_ens
+    log(f"[bench] ensemble {nsys} systems x {steps} steps in {t_ens:.3f} s -> {rate:.1f} system-steps/s "
+        f"(tiles {blocks}, last step {st})")
+    rng = np.random.Generator(np.random.PCG64(44))
+    idx = rng.choice(nsys, size=min(sample, nsys), replace=False)
+    t_seq = 0.0
+    for i in idx:
+        one = S.GpuSystem(S.SYMBA, [systems[int(i)]], dt, istep_out=0, lclose=False)
+        one.run(warmup)
+        t0 = time.perf_counter()
+        one.run(steps)
+        t_seq += time.perf_counter() - t0
+        del one
+    seq_rate = len(idx) * steps / t_seq
+    log(f"[bench] sequential sample {len(idx)} systems -> {seq_rate:.1f} system-steps/s")
+    return {"metric": "ensemble system-steps/s", "value": rate, "unit": "system-steps/s",
+            "systems": nsys, "steps": steps, "warmup": warmup, "seconds": t_ens,
+            "npl_total": int(sum(s.npl for s in systems)), "ntp_total": int(sum(s.ntp for s in systems)),
+            "tiles": {"kick_pl_block": blocks[0], "kick_tp_block": blocks[1]},
+            "sequential": {"value": seq_rate, "sample": int(len(idx)),
+                           "how": "the same library, one system per handle, systems run one after another"},
+            "speedup_vs_sequential": rate / seq_rate,
+            "config": "SyMBA, dt 0.01 yr, N log-uniform 4-512, m log-uniform 1e-8..1e-4 Msun, a 0.3-30 AU, "
+                      "ntp U{0..2000}, seed 4, CHK_CLOSE off"}
+
+
+def serving_leg(nreq, rate, nmax, tpmax, smin, smax, lanes, log):
+    """Served stream (config C5, scaled down so the default run stays short):
+    C3-law systems (m log-uniform 1e-7..1e-3 Msun over a U(0.5,5) AU, test
+    particles a U(0.3,10) AU) with N log-uniform 8..nmax, U{0..tpmax} test
+    particles and log-uniform smin..smax steps, arriving as a Poisson stream
+    at `rate` per second; admitted by device-memory budget into batches on
+    `lanes` concurrent streams."""
+    import numpy as np
+    from swiftest_amd import ics, serve
+    rng = np.random.Generator(np.random.PCG64(5))
+    n = np.floor(np.exp(rng.uniform(np.log(8), np.log(nmax + 1), nreq))).astype(int)
+    m = rng.integers(0, tpmax + 1, nreq)
+    steps = np.floor(np.exp(rng.uniform(np.log(smin), np.log(smax + 1), nreq))).astype(int)
+    systems = [ics.symba_big(seed=1000 + k, npl=int(n[k]), ntp=int(m[k])) for k in range(nreq)]
+    reqs = serve.poisson_requests(systems, steps, rate, seed=6)
+    srv = serve.Server(dt=0.005, lanes=lanes, batch_max=64)
+    wall = srv.run(reqs)
+    srv.close()
+    out = serve.summary(reqs, wall)
+    out.update({"metric": "served completion latency p50/p99 and system-steps/s", "arrival_rate_per_s": rate,
+                "lanes": lanes, "batches": len(srv.batches),
+                "mean_batch": float(np.mean([b for _, b in srv.batches])),
+                "config": (f"SyMBA dt 0.005, C3 mass/orbit laws, N log-uniform 8-{nmax}, ntp U{{0..{tpmax}}}, "
+                           f"steps log-uniform {smin}-{smax}, {nreq} Poisson requests at {rate}/s, seed 5 "
+                           f"(scaled down from N 8-4096, ntp 0-100000, steps 1e3-1e5), CHK_CLOSE off")})
+    log(f"[bench] served {nreq} requests in {wall:.2f} s: p50 {out['p50_s']:.3f} s, p99 {out['p99_s']:.3f} s, "
+        f"{out['system_steps_per_s']:.0f} system-steps/s over {len(srv.batches)} batches")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +157,17 @@ def main():
                          "this build does not do, so a detected collision stops the run)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--ens-systems", type=int, default=4096)
+    ap.add_argument("--ens-steps", type=int, default=10)
+    ap.add_argument("--ens-sample", type=int, default=32)
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
+    ap.add_argument("--serve-requests", type=int, default=128)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-nmax", type=int, default=512)
+    ap.add_argument("--serve-tpmax", type=int, default=2000)
+    ap.add_argument("--serve-steps", type=int, nargs=2, default=[20, 200])
+    ap.add_argument("--serve-lanes", type=int, default=4)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -97,6 +183,8 @@ def main():
     g = S.GpuSystem(S.SYMBA, sysic, dt, istep_out=args.istep_out, lclose=bool(args.lclose))
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
     log(f"[bench] created system npl={sysic.npl} ntp={sysic.ntp}")
+    tiles = g.autotune(2)
+    log(f"[bench] autotuned tiles (kick_pl, kick_tp) = {tiles}")
     for k in range(args.warmup):
         t0 = time.perf_counter()
         g.run(1)
@@ -151,6 +239,12 @@ def main():
                               f"{FLOPS_PER_PLTP_PAIR} flops per pl-tp pair x npl x ntp per launch")},
         "cpu_baseline": None,
     }
+    del g
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_leg(args.ens_systems, args.ens_steps, 1, args.ens_sample, log)
+    if not args.no_serve:
+        out["serving"] = serving_leg(args.serve_requests, args.serve_rate, args.serve_nmax, args.serve_tpmax,
+                                     args.serve_steps[0], args.serve_steps[1], args.serve_lanes, log)
     if not args.no_cpu:
         log("[bench] timing the CPU baseline sample")
         out["cpu_baseline"] = cpu_baseline(sysic, args.cpu_seconds)

@@ -125,7 +125,8 @@ int swg_system_run(swg_system *sys, int64_t nsteps);
 /* One step, no snapshot. */
 int swg_system_step(swg_system *sys);
 /* Copy the state to host arrays (any may be NULL).  pl_id gives, for each
- * stored body, its 1-based position in the creation input. */
+ * stored body, its 1-based position among its own system's bodies in the
+ * creation input. */
 int swg_system_get(swg_system *sys, double *pl_rh, double *pl_vh, double *pl_vb, double *tp_rh, double *tp_vh,
                    double *tp_vb, int32_t *pl_status, int32_t *tp_status, int32_t *pl_id);
 int swg_system_get_misc(swg_system *sys, double *rhill, double *cb_vb, double *ptbeg, double *ptend,
@@ -139,6 +140,17 @@ int swg_system_enc_list(swg_system *sys, int32_t which, int64_t *n, int32_t *ind
  * duration of one launch in ms, measured with HIP events on the system's
  * stream. */
 int swg_system_time_kernels(swg_system *sys, int32_t reps, double *ms_kick_tp, double *ms_kick_pl);
+/* Leave systems out of the following steps: frozen[s] = 1 keeps system s's
+ * state exactly as it is while the others advance; 0 puts it back.  Lets a
+ * batch of systems with different step counts run together, each stopping at
+ * its own count.  HELIO and SyMBA only.  No counterpart in the reference
+ * (one system per run there). */
+int swg_system_set_frozen(swg_system *sys, const int32_t *frozen);
+/* Autotuner: time the pl-tp and pl-pl acceleration kernels at each
+ * workgroup size (64, 128, 256 bodies per tile) on the current state (no state
+ * change) and keep the fastest for the following steps.  Reports the choice.
+ * No counterpart in the reference (its loops are OpenMP-scheduled). */
+int swg_system_autotune(swg_system *sys, int32_t reps, int32_t *pl_block, int32_t *tp_block);
 /* Live timing: while enabled, every launch of the pl-tp and pl-pl
  * acceleration kernels during steps is bracketed by HIP events on the
  * system's stream (enabling resets the record).  swg_system_kernel_times()

@@ -102,6 +102,15 @@ def context(device: int = 0):
     return _ctx
 
 
+def new_context(device: int = 0):
+    """A further library context on `device`: its own HIP stream, so work
+    issued through it overlaps with other contexts' work on the device."""
+    L = library()
+    h = vp()
+    _check(L.swg_init(device, C.byref(h)))
+    return h
+
+
 def _c(a, dt=np.float64):
     return np.ascontiguousarray(a, dtype=dt)
 
@@ -181,7 +190,7 @@ class GpuSystem:
     integrator and dt.  Body arrays of the systems are concatenated in order.
     """
 
-    def __init__(self, integrator, systems, dt, lclose=True, short_circuit=True, istep_out=0, rhill=None):
+    def __init__(self, integrator, systems, dt, lclose=True, short_circuit=True, istep_out=0, rhill=None, ctx=None):
         if not isinstance(systems, (list, tuple)):
             systems = [systems]
         self.L = library()
@@ -195,7 +204,8 @@ class GpuSystem:
                      int(rhill is not None))
         rh = None if rhill is None else _c(rhill)
         h = vp()
-        _check(self.L.swg_system_create(context(), C.byref(prm), self.nsys, self.npl_s, self.ntp_s,
+        _check(self.L.swg_system_create(ctx if ctx is not None else context(), C.byref(prm), self.nsys,
+                                        self.npl_s, self.ntp_s,
                                         _c([s.GMcb for s in systems]), cat("pl_rh", 3), cat("pl_vh", 3),
                                         _c(np.concatenate([s.Gmass for s in systems])),
                                         _c(np.concatenate([s.radius for s in systems])), _p(rh),
@@ -251,6 +261,18 @@ class GpuSystem:
         na, nb = C.c_int64(), C.c_int64()
         _check(self.L.swg_system_kernel_times(self.h, C.byref(a), C.byref(na), C.byref(b), C.byref(nb)))
         return a.value, na.value, b.value, nb.value
+
+    def set_frozen(self, frozen):
+        """frozen[s] = 1 leaves system s out of the following steps."""
+        f = np.ascontiguousarray(np.asarray(frozen, np.int32))
+        assert f.shape == (self.nsys,)
+        _check(self.L.swg_system_set_frozen(self.h, _p(f)))
+
+    def autotune(self, reps=3):
+        """Pick the pair kernels' workgroup sizes by timing; returns them."""
+        b_pl, b_tp = C.c_int32(), C.c_int32()
+        _check(self.L.swg_system_autotune(self.h, int(reps), C.byref(b_pl), C.byref(b_tp)))
+        return b_pl.value, b_tp.value
 
     def time_kernels(self, reps=5):
         tp_ms, pl_ms = C.c_double(), C.c_double()

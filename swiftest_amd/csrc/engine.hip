@@ -248,7 +248,7 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
     if (own(&sy.pl_off, nsys + 1) || own(&sy.tp_off, nsys + 1) || own(&sy.GMcb, nsys) || own(&sy.cbvb, 3 * nsys) ||
         own(&sy.ptbeg, 3 * nsys) || own(&sy.ptend, 3 * nsys) || own(&sy.lfirst, nsys) || own(&sy.enc_pp, nsys) ||
         own(&sy.enc_pt, nsys) || own(&sy.flag_pp, nsys) || own(&sy.flag_pt, nsys) || own(&sy.err, nsys) ||
-        own(&d_any, 4) || own(&d_total, 4) || own(&d_pt_run, nsys))
+        own(&sy.frozen, nsys) || own(&d_active, nsys) || own(&d_any, 4) || own(&d_total, 4) || own(&d_pt_run, nsys))
         return SWG_ERR_HIP;
     hipStream_t st = stream;
     std::vector<double> tmp;
@@ -275,7 +275,8 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
     HIPCHK(hipMemcpy(sy.ptend, dz.data(), 3 * nsys * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(sy.cbvb, dz.data(), 3 * nsys * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(sy.lfirst, ones.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice));
-    for (int32_t *q : {sy.enc_pp, sy.enc_pt, sy.flag_pp, sy.flag_pt, sy.err})
+    h_frozen.assign(nsys, 0);
+    for (int32_t *q : {sy.enc_pp, sy.enc_pt, sy.flag_pp, sy.flag_pt, sy.err, sy.frozen})
         HIPCHK(hipMemcpy(q, zeros.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice));
 
     // ---- tiles ----
@@ -361,6 +362,7 @@ void Ensemble::build_tiles()
     // kick tiles: blocks of tuning.kick_*_block bodies of one system
     std::vector<Tile> tpl, ttp;
     for (int s = 0; s < nsys; s++) {
+        if (h_frozen[s]) continue;
         for (int b = h_pl_off[s]; b < h_pl_off[s + 1]; b += tune.kick_pl_block)
             tpl.push_back(Tile{s, b, std::min(b + tune.kick_pl_block, h_pl_off[s + 1]), 0});
         for (int b = h_tp_off[s]; b < h_tp_off[s + 1]; b += tune.kick_tp_block)
@@ -368,6 +370,7 @@ void Ensemble::build_tiles()
     }
     std::vector<int4> tpp;
     for (int s = 0; s < nsys; s++) {
+        if (h_frozen[s]) continue;
         int n = h_pl_off[s + 1] - h_pl_off[s], nb = (n + 63) / 64;
         for (int ib = 0; ib < nb; ib++)
             for (int jb = ib; jb < nb; jb++) tpp.push_back(make_int4(s, ib, jb, 0));
@@ -411,7 +414,7 @@ int Ensemble::detect_pp(double dt)
 {
     hipStream_t st = stream;
     launch_pp_tiles(st, pl, sy, pp, d_tiles_pp, ntiles_pp, dt);
-    launch_rows_count(st, pp, pl.sys, 1, sy.pl_off, nullptr, npl);
+    launch_rows_count(st, pp, pl.sys, 1, sy.pl_off, nfrozen ? d_active : nullptr, npl);
     launch_scan_i32_i64(st, pp.rowcnt, pp.rowstart, npl, d_total, d_scan);
     launch_sys_flags(st, pp, sy.pl_off, sy.enc_pp, nsys);
     int64_t *ht = (int64_t *)h_small;
@@ -471,9 +474,9 @@ int Ensemble::build_adjacency()
 {
     hipStream_t st = stream;
     if (npp) {
-        launch_col_count(st, pp, sy.pl_off, pl.sys, sy.pl_off, nullptr, npl, 1);
+        launch_col_count(st, pp, sy.pl_off, pl.sys, sy.pl_off, nfrozen ? d_active : nullptr, npl, 1);
         launch_scan_i32_i64(st, pp.colcnt, pp.colstart, npl, nullptr, d_scan);
-        launch_col_fill(st, pp, sy.pl_off, pl.sys, sy.pl_off, nullptr, npl, 1);
+        launch_col_fill(st, pp, sy.pl_off, pl.sys, sy.pl_off, nfrozen ? d_active : nullptr, npl, 1);
     }
     if (npt) {
         launch_col_count(st, pt, sy.pl_off, tp.sys, sy.tp_off, d_pt_run, ntp, 0);
@@ -521,8 +524,8 @@ int Ensemble::recur0()
     int32_t lenc = 0;
     if (int rc = read_any(lenc)) return rc;
     launch_level_kicks(st, la, nullptr, npl, ntp, 0, dth);
-    launch_drift(st, pl.rh, pl.vb, pl.status, pl.levelg, pl.sys, sy.GMcb, sy.err, npl, dt, 0, 1, 0);
-    launch_drift(st, tp.rh, tp.vb, tp.status, tp.levelg, tp.sys, sy.GMcb, sy.err, ntp, dt, 0, 1, 1);
+    launch_drift(st, pl.rh, pl.vb, pl.status, pl.levelg, pl.sys, sy.GMcb, sy.err, sy.frozen, npl, dt, 0, 1, 0);
+    launch_drift(st, tp.rh, tp.vb, tp.status, tp.levelg, tp.sys, sy.GMcb, sy.err, sy.frozen, ntp, dt, 0, 1, 1);
     if (lenc) {
         LevelView v;
         LevelCounts c;
@@ -724,7 +727,7 @@ int Ensemble::step_helio_symba()
         bool anyrun = false;
         for (int s = 0; s < nsys; s++) {
             run[s] = (h_tp_off[s + 1] > h_tp_off[s]) && (h_pl_off[s + 1] > h_pl_off[s]) &&
-                     !(prm.short_circuit && h_enc_pp[s]);
+                     !(prm.short_circuit && h_enc_pp[s]) && !h_frozen[s];
             anyrun |= run[s] != 0;
         }
         h_enc_pt.assign(nsys, 0);
@@ -763,10 +766,10 @@ int Ensemble::step_helio_symba()
     launch_sums_begin(st, pl, sy, nsys, 0);
     launch_pl_begin_apply(st, pl, sy, npl, dth, 0);
     if (int rc = kick_pl(1)) return rc;
-    launch_drift(st, pl.rh, pl.vb, pl.status, pl.levelg, pl.sys, sy.GMcb, sy.err, npl, dt, -1, symba, 0);
+    launch_drift(st, pl.rh, pl.vb, pl.status, pl.levelg, pl.sys, sy.GMcb, sy.err, sy.frozen, npl, dt, -1, symba, 0);
     launch_tp_pre(st, tp, sy, ntp, dth);
     if (int rc = kick_tp(1)) return rc;
-    launch_drift(st, tp.rh, tp.vb, tp.status, tp.levelg, tp.sys, sy.GMcb, sy.err, ntp, dt, -1, symba, 1);
+    launch_drift(st, tp.rh, tp.vb, tp.status, tp.levelg, tp.sys, sy.GMcb, sy.err, sy.frozen, ntp, dt, -1, symba, 1);
     trace_mark(st, "first half");
     if (any) {
         if (int rc = recur0()) return rc;
@@ -796,7 +799,9 @@ int Ensemble::snapshot()
     if (!prm.rhill_present) {
         std::vector<double> rh(3 * (size_t)npl), vh(3 * (size_t)npl), rhill(npl);
         if (download_aos(st, pl.rh, rh.data(), npl) || download_aos(st, pl.vh, vh.data(), npl)) return SWG_ERR_HIP;
+        if (nfrozen) HIPCHK(hipMemcpyAsync(rhill.data(), pl.rhill, npl * sizeof(double), hipMemcpyDeviceToHost, st));
         for (int s = 0; s < nsys; s++) {
+            if (h_frozen[s]) continue;  // a frozen system keeps its state, rhill included
             int a = h_pl_off[s], n = h_pl_off[s + 1] - a;
             host_set_rhill(h_GMcb[s], n, &rh[3 * (size_t)a], &vh[3 * (size_t)a], &h_Gm[a], &rhill[a]);
         }
@@ -917,8 +922,7 @@ int Ensemble::time_kernels(int reps, double *ms_tp, double *ms_pl)
     // Launch the two acceleration kernels on copies of the velocity state so
     // that the timing runs leave the system unchanged.
     hipStream_t st = stream;
-    Vec3p save_pl{}, save_tp{};
-    if (alloc_vec(save_pl, npl) || alloc_vec(save_tp, ntp)) return SWG_ERR_HIP;
+    if (!save_pl.x && (alloc_vec(save_pl, npl) || alloc_vec(save_tp, ntp))) return SWG_ERR_HIP;
     for (int d = 0; d < 3; d++) {
         double *src = d == 0 ? pl.vb.x : d == 1 ? pl.vb.y : pl.vb.z;
         double *dst = d == 0 ? save_pl.x : d == 1 ? save_pl.y : save_pl.z;
@@ -930,6 +934,7 @@ int Ensemble::time_kernels(int reps, double *ms_tp, double *ms_pl)
     const int use_radius = prm.integrator == SWG_SYMBA || prm.lclose;
     float ms = 0.f;
     if (ms_tp) {
+        launch_kick_tp(st, tune, tp, pl, sy, pt, d_tiles_tp, ntiles_tp, 0.0, 1, 0);  // warm-up
         HIPCHK(hipEventRecord(ev0, st));
         for (int r = 0; r < reps; r++) launch_kick_tp(st, tune, tp, pl, sy, pt, d_tiles_tp, ntiles_tp, 0.0, 1, 0);
         HIPCHK(hipEventRecord(ev1, st));
@@ -938,6 +943,7 @@ int Ensemble::time_kernels(int reps, double *ms_tp, double *ms_pl)
         *ms_tp = ms / reps;
     }
     if (ms_pl) {
+        launch_kick_pl(st, tune, pl, sy, pp, d_tiles_pl, ntiles_pl, 0.0, 1, use_radius, 0);  // warm-up
         HIPCHK(hipEventRecord(ev0, st));
         for (int r = 0; r < reps; r++)
             launch_kick_pl(st, tune, pl, sy, pp, d_tiles_pl, ntiles_pl, 0.0, 1, use_radius, 0);
@@ -956,6 +962,56 @@ int Ensemble::time_kernels(int reps, double *ms_tp, double *ms_pl)
     }
     HIPCHK(hipStreamSynchronize(st));
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// Leave systems out of the following steps (frozen[s] = 1) or put them back.
+// A frozen system's state stays exactly as it is; the other systems' steps
+// are unaffected.
+int Ensemble::set_frozen(const int32_t *frozen)
+{
+    if (prm.integrator == SWG_WHM) return fail(SWG_ERR_ARG, "freezing systems is supported for HELIO and SyMBA");
+    nfrozen = 0;
+    std::vector<int32_t> act(nsys);
+    for (int s = 0; s < nsys; s++) {
+        h_frozen[s] = frozen[s] ? 1 : 0;
+        act[s] = !h_frozen[s];
+        nfrozen += h_frozen[s];
+    }
+    HIPCHK(hipMemcpyAsync(sy.frozen, h_frozen.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_active, act.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    build_tiles();
+    return 0;
+}
+
+// Pick the workgroup size of the two pair kernels (and with it the body
+// tiles) by timing each candidate on the current state.
+int Ensemble::autotune(int reps, int32_t *pl_block, int32_t *tp_block)
+{
+    const int cand[3] = {64, 128, 256};
+    double best_tp = 1e300, best_pl = 1e300;
+    int btp = tune.kick_tp_block, bpl = tune.kick_pl_block;
+    for (int c : cand) {
+        tune.kick_tp_block = c;
+        tune.kick_pl_block = c;
+        build_tiles();
+        double tms = 0.0, pms = 0.0;
+        if (int rc = time_kernels(reps, ntp ? &tms : nullptr, npl ? &pms : nullptr)) return rc;
+        if (ntp && tms < best_tp) {
+            best_tp = tms;
+            btp = c;
+        }
+        if (npl && pms < best_pl) {
+            best_pl = pms;
+            bpl = c;
+        }
+    }
+    tune.kick_tp_block = btp;
+    tune.kick_pl_block = bpl;
+    build_tiles();
+    if (pl_block) *pl_block = bpl;
+    if (tp_block) *tp_block = btp;
     return 0;
 }
 
@@ -1073,6 +1129,18 @@ int swg_system_time_kernels(swg_system *sys, int32_t reps, double *ms_kick_tp, d
 {
     if (!sys || reps <= 0) return fail(SWG_ERR_ARG, "bad argument");
     return sys->e->time_kernels(reps, ms_kick_tp, ms_kick_pl);
+}
+
+int swg_system_set_frozen(swg_system *sys, const int32_t *frozen)
+{
+    if (!sys || !frozen) return fail(SWG_ERR_ARG, "NULL argument");
+    return sys->e->set_frozen(frozen);
+}
+
+int swg_system_autotune(swg_system *sys, int32_t reps, int32_t *pl_block, int32_t *tp_block)
+{
+    if (!sys || reps <= 0) return fail(SWG_ERR_ARG, "bad argument");
+    return sys->e->autotune(reps, pl_block, tp_block);
 }
 
 int swg_system_kernel_timing(swg_system *sys, int32_t enable)

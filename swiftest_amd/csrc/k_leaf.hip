@@ -576,7 +576,7 @@ int Ensemble::step_whm()
     k_whm_pl<<<cdiv(nsys, 64), 64, 0, st>>>(a, nsys, dt);
     k_whm_tp_ah0<<<cdiv(nsys, 64), 64, 0, st>>>(a, nsys, 1);
     if (ntiles_tp) k_whm_tp_kick<256><<<ntiles_tp, 256, 0, st>>>(a, tp, d_tiles_tp, dth, 1);
-    launch_drift(st, tp.rh, tp.vh, tp.status, tp.levelg, tp.sys, sy.GMcb, sy.err, ntp, dt, -1, 0, 1);
+    launch_drift(st, tp.rh, tp.vh, tp.status, tp.levelg, tp.sys, sy.GMcb, sy.err, sy.frozen, ntp, dt, -1, 0, 1);
     k_whm_tp_ah0<<<cdiv(nsys, 64), 64, 0, st>>>(a, nsys, 0);
     if (ntiles_tp) k_whm_tp_kick<256><<<ntiles_tp, 256, 0, st>>>(a, tp, d_tiles_tp, dth, 0);
     HIPCHK(hipGetLastError());

@@ -73,7 +73,7 @@ __global__ __launch_bounds__(SUM_T) void k_sums_begin(PlDev pl, SysDev sy, int n
     __shared__ double shr[4];
     const int s = blockIdx.x, tid = threadIdx.x;
     const int p0 = sy.pl_off[s], p1 = sy.pl_off[s + 1];
-    if (p1 == p0) return;
+    if (p1 == p0 || sy.frozen[s]) return;
     const bool first = force || sy.lfirst[s];
     double cb = 0.0;
     if (first) {
@@ -121,6 +121,7 @@ __global__ void k_pl_begin_apply(PlDev pl, SysDev sy, int npl, double dth, int f
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npl) return;
     int s = pl.sys[i];
+    if (sy.frozen[s]) return;
     if (force || sy.lfirst[s]) {
         pl.vb.x[i] = pl.vh.x[i] + sy.cbvb[3 * s];
         pl.vb.y[i] = pl.vh.y[i] + sy.cbvb[3 * s + 1];
@@ -237,7 +238,8 @@ __global__ __launch_bounds__(BLK) void k_kick_pl(PlDev pl, SysDev sy, ListDev pp
 // Danby drift of the masked bodies (helio_drift_body, helio_drift.f90:14-54,
 // with the SyMBA level mask of symba_drift.f90:14-61): mu = GM_cb, (rh, vb).
 __global__ __launch_bounds__(256) void k_drift(Vec3p r, Vec3p v, int32_t *status, const int32_t *levelg, const int32_t *sys,
-                        const double *GMcb, int32_t *err, int n, double dt, int irec, int use_level, int is_tp)
+                        const double *GMcb, int32_t *err, const int32_t *frozen, int n, double dt, int irec,
+                        int use_level, int is_tp)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -246,6 +248,7 @@ __global__ __launch_bounds__(256) void k_drift(Vec3p r, Vec3p v, int32_t *status
     if (use_level) m = m && (levelg[i] == irec);
     if (!m) return;
     int s = sys[i];
+    if (frozen[s]) return;
     double rx = r.x[i], ry = r.y[i], rz = r.z[i], vx = v.x[i], vy = v.y[i], vz = v.z[i];
     int fl = drift_one(GMcb[s], rx, ry, rz, vx, vy, vz, dt);
     r.x[i] = rx;
@@ -267,6 +270,7 @@ __global__ void k_tp_pre(TpDev tp, SysDev sy, int ntp, double dth)
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= ntp || !tp_active(tp, j)) return;
     int s = tp.sys[j];
+    if (sy.frozen[s]) return;
     double px = sy.ptbeg[3 * s], py = sy.ptbeg[3 * s + 1], pz = sy.ptbeg[3 * s + 2];
     if (sy.lfirst[s]) {
         tp.vb.x[j] = tp.vh.x[j] + (-px);
@@ -354,7 +358,7 @@ __global__ __launch_bounds__(SUM_T) void k_sums_end(PlDev pl, SysDev sy, int nsy
     __shared__ SumStage S;
     const int s = blockIdx.x, tid = threadIdx.x;
     const int p0 = sy.pl_off[s], p1 = sy.pl_off[s + 1];
-    if (p1 == p0) return;
+    if (p1 == p0 || sy.frozen[s]) return;
     const double gm = sy.GMcb[s];
     double pt = 0.0;
     for (int c0 = p0; c0 < p1; c0 += SUM_CH) {
@@ -390,6 +394,7 @@ __global__ void k_pl_end_apply(PlDev pl, SysDev sy, int npl, double dth)
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npl) return;
     int s = pl.sys[i];
+    if (sy.frozen[s]) return;
     if (pl_active(pl, i)) {
         pl.rh.x[i] = fma(sy.ptend[3 * s], dth, pl.rh.x[i]);
         pl.rh.y[i] = fma(sy.ptend[3 * s + 1], dth, pl.rh.y[i]);
@@ -407,6 +412,7 @@ __global__ void k_tp_post(TpDev tp, SysDev sy, int ntp, double dth)
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= ntp || !tp_active(tp, j)) return;
     int s = tp.sys[j];
+    if (sy.frozen[s]) return;
     double px = sy.ptend[3 * s], py = sy.ptend[3 * s + 1], pz = sy.ptend[3 * s + 2];
     tp.rh.x[j] = fma(px, dth, tp.rh.x[j]);
     tp.rh.y[j] = fma(py, dth, tp.rh.y[j]);
@@ -421,7 +427,7 @@ __global__ void k_tp_post(TpDev tp, SysDev sy, int ntp, double dth)
 __global__ void k_lfirst_update(SysDev sy, int nsys)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nsys) return;
+    if (s >= nsys || sy.frozen[s]) return;
     if (sy.lfirst[s] && !(sy.enc_pp[s] || sy.enc_pt[s])) sy.lfirst[s] = 0;
 }
 
@@ -432,6 +438,7 @@ __global__ void k_snapshot_tp(TpDev tp, SysDev sy, int ntp)
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= ntp || !tp_active(tp, j)) return;
     int s = tp.sys[j];
+    if (sy.frozen[s]) return;
     tp.vb.x[j] = tp.vh.x[j] + (-sy.ptbeg[3 * s]);
     tp.vb.y[j] = tp.vh.y[j] + (-sy.ptbeg[3 * s + 1]);
     tp.vb.z[j] = tp.vh.z[j] + (-sy.ptbeg[3 * s + 2]);
@@ -443,7 +450,7 @@ __global__ __launch_bounds__(SUM_T) void k_vh2vb_sum(PlDev pl, SysDev sy, int ns
     __shared__ double shr[4];
     const int s = blockIdx.x, tid = threadIdx.x;
     const int p0 = sy.pl_off[s], p1 = sy.pl_off[s + 1];
-    if (p1 == p0) return;
+    if (p1 == p0 || sy.frozen[s]) return;
     double acc = 0.0;
     for (int c0 = p0; c0 < p1; c0 += SUM_CH) {
         const int n = min(SUM_CH, p1 - c0);
@@ -467,6 +474,7 @@ __global__ void k_vh2vb_apply(PlDev pl, SysDev sy, int npl)
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npl) return;
     int s = pl.sys[i];
+    if (sy.frozen[s]) return;
     pl.vb.x[i] = pl.vh.x[i] + sy.cbvb[3 * s];
     pl.vb.y[i] = pl.vh.y[i] + sy.cbvb[3 * s + 1];
     pl.vb.z[i] = pl.vh.z[i] + sy.cbvb[3 * s + 2];
@@ -510,9 +518,12 @@ void launch_kick_pl(hipStream_t st, const Tuning &tu, PlDev pl, SysDev sy, ListD
     }
 }
 void launch_drift(hipStream_t st, Vec3p r, Vec3p v, int32_t *status, int32_t *levelg, const int32_t *sys,
-                  const double *GMcb, int32_t *err, int n, double dt, int irec, int use_level, int is_tp)
+                  const double *GMcb, int32_t *err, const int32_t *frozen, int n, double dt, int irec, int use_level,
+                  int is_tp)
 {
-    if (n) k_drift<<<cdiv(n, 128), 128, 0, st>>>(r, v, status, levelg, sys, GMcb, err, n, dt, irec, use_level, is_tp);
+    if (n)
+        k_drift<<<cdiv(n, 128), 128, 0, st>>>(r, v, status, levelg, sys, GMcb, err, frozen, n, dt, irec, use_level,
+                                              is_tp);
 }
 void launch_tp_pre(hipStream_t st, TpDev tp, SysDev sy, int ntp, double dth)
 {

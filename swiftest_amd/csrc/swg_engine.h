@@ -90,6 +90,12 @@ struct Ensemble {
     int get_misc(double *rhill, double *cb_vb, double *ptbeg, double *ptend, int32_t *lfirst);
     int enc_list(int which, int64_t *n, int32_t *i1, int32_t *i2);
     int time_kernels(int reps, double *ms_tp, double *ms_pl);
+    int autotune(int reps, int32_t *pl_block, int32_t *tp_block);
+    int set_frozen(const int32_t *frozen);
+    std::vector<int32_t> h_frozen;
+    int nfrozen = 0;
+    int32_t *d_active = nullptr;  // 1 - frozen, per system
+    Vec3p save_pl{}, save_tp{};  // velocity copies kept across kernel timings
     int check_errors();
     int err_code();
 

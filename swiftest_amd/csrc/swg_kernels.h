@@ -26,7 +26,8 @@ void launch_pl_begin_apply(hipStream_t st, PlDev pl, SysDev sy, int npl, double 
 void launch_kick_pl(hipStream_t st, const Tuning &tu, PlDev pl, SysDev sy, ListDev pp, const Tile *tiles,
                     int ntiles, double dt, int lbeg, int use_radius, int do_pairs);
 void launch_drift(hipStream_t st, Vec3p r, Vec3p v, int32_t *status, int32_t *levelg, const int32_t *sys,
-                  const double *GMcb, int32_t *err, int n, double dt, int irec, int use_level, int is_tp);
+                  const double *GMcb, int32_t *err, const int32_t *frozen, int n, double dt, int irec, int use_level,
+                  int is_tp);
 void launch_tp_pre(hipStream_t st, TpDev tp, SysDev sy, int ntp, double dth);
 void launch_kick_tp(hipStream_t st, const Tuning &tu, TpDev tp, PlDev pl, SysDev sy, ListDev pt,
                     const Tile *tiles, int ntiles, double dt, int lbeg, int do_pairs);

@@ -44,6 +44,7 @@ struct SysDev {
     int32_t *enc_pp, *enc_pt;     // level-0 encounter flags of the current step
     int32_t *flag_pp, *flag_pt;   // per-substep list-check flags
     int32_t *err;                 // error bits: 1 collision, 2 pl drift failure
+    int32_t *frozen;              // 1 = system left out of the steps (its state is kept as is)
 };
 
 // A (pl, pl) or (pl, tp) encounter list with its bit matrix and adjacency.
