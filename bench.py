@@ -68,6 +68,31 @@ def cpu_baseline(sysic, target_s):
                        f"encounter detection, recursion or drift")}
 
 
+def detection_leg(sysic, log, sparse_npl=131072):
+    """Level-0 pl-pl encounter check with the triangular all-pairs sweep and
+    with the sort-and-sweep broad phase (same list), on the benchmark's
+    initial state and on a large planetesimal disk (C2 mass law, few pairs)."""
+    from swiftest_amd import ics
+    import swiftest_amd as S
+    out = {}
+    cases = [("c3", sysic), ("planetesimal_disk", ics.symba_disk(seed=2, npl=sparse_npl))]
+    for label, case in cases:
+        res = {"npl": case.npl}
+        for mode, name in ((0, "triangular"), (1, "sortsweep")):
+            g = S.GpuSystem(S.SYMBA, case, 0.005, istep_out=100, lclose=False, encounter_check=mode)
+            ms, npairs, ncand = g.time_encounters(3)
+            del g
+            res[name] = {"ms": ms, "pairs": npairs}
+            if ncand >= 0:
+                res[name]["candidates"] = ncand
+            log(f"[bench] {label} pl-pl encounter check {name}: {ms:.3f} ms, {npairs} pairs"
+                + (f", {ncand} candidates" if ncand >= 0 else ""))
+        n = case.npl
+        res["pair_checks_per_s_triangular"] = n * (n - 1) / 2 / (res["triangular"]["ms"] * 1e-3)
+        out[label] = res
+    return out
+
+
 def ensemble_leg(nsys, steps, warmup, sample, log):
     """Batched ensemble (config C4): nsys independent SyMBA systems of uneven
     size advanced together, one launch per step phase for all of them, with
@@ -240,6 +265,7 @@ def main():
         "cpu_baseline": None,
     }
     del g
+    out["encounter_detection"] = detection_leg(sysic, log)
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(args.ens_systems, args.ens_steps, 1, args.ens_sample, log)
     if not args.no_serve:

@@ -57,7 +57,8 @@ typedef struct swg_params {
     int32_t short_circuit;
     int32_t istep_out;     /* snapshot cadence in steps, 0 = none */
     int32_t rhill_present; /* RHILL_PRESENT: keep the given rhill */
-    int32_t reserved[3];
+    int32_t encounter_check; /* ENCOUNTER_CHECK for pl-pl: 0 TRIANGULAR, 1 SORTSWEEP (same list) */
+    int32_t reserved[2];
 } swg_params;
 
 typedef struct swg_stats {
@@ -140,6 +141,11 @@ int swg_system_enc_list(swg_system *sys, int32_t which, int64_t *n, int32_t *ind
  * duration of one launch in ms, measured with HIP events on the system's
  * stream. */
 int swg_system_time_kernels(swg_system *sys, int32_t reps, double *ms_kick_tp, double *ms_kick_pl);
+/* Time the level-0 pl-pl encounter check (encounter_check_all_plpl with the
+ * configured ENCOUNTER_CHECK) on the current state: average ms per call over
+ * reps calls, the number of pairs found and, for SORTSWEEP, the number of
+ * broad-phase candidates (else -1).  SyMBA only. */
+int swg_system_time_encounters(swg_system *sys, int32_t reps, double *ms, int64_t *npairs, int64_t *ncand);
 /* Leave systems out of the following steps: frozen[s] = 1 keeps system s's
  * state exactly as it is while the others advance; 0 puts it back.  Lets a
  * batch of systems with different step counts run together, each stopping at

@@ -34,7 +34,7 @@ class SwiftestHipError(RuntimeError):
 class Params(C.Structure):
     _fields_ = [("dt", C.c_double), ("integrator", C.c_int32), ("lclose", C.c_int32),
                 ("short_circuit", C.c_int32), ("istep_out", C.c_int32), ("rhill_present", C.c_int32),
-                ("reserved", C.c_int32 * 3)]
+                ("encounter_check", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 class Stats(C.Structure):
@@ -190,7 +190,8 @@ class GpuSystem:
     integrator and dt.  Body arrays of the systems are concatenated in order.
     """
 
-    def __init__(self, integrator, systems, dt, lclose=True, short_circuit=True, istep_out=0, rhill=None, ctx=None):
+    def __init__(self, integrator, systems, dt, lclose=True, short_circuit=True, istep_out=0, rhill=None, ctx=None,
+                 encounter_check=0):
         if not isinstance(systems, (list, tuple)):
             systems = [systems]
         self.L = library()
@@ -201,7 +202,7 @@ class GpuSystem:
         cat = lambda key, w: _c(np.concatenate([getattr(s, key).reshape(-1, w) for s in systems])) \
             if systems else np.zeros((0, w))
         prm = Params(float(dt), int(integrator), int(lclose), int(short_circuit), int(istep_out),
-                     int(rhill is not None))
+                     int(rhill is not None), int(encounter_check))
         rh = None if rhill is None else _c(rhill)
         h = vp()
         _check(self.L.swg_system_create(ctx if ctx is not None else context(), C.byref(prm), self.nsys,
@@ -261,6 +262,12 @@ class GpuSystem:
         na, nb = C.c_int64(), C.c_int64()
         _check(self.L.swg_system_kernel_times(self.h, C.byref(a), C.byref(na), C.byref(b), C.byref(nb)))
         return a.value, na.value, b.value, nb.value
+
+    def time_encounters(self, reps=3):
+        """(ms per level-0 pl-pl check, pairs found, broad-phase candidates or -1)."""
+        ms, npairs, ncand = C.c_double(), C.c_int64(), C.c_int64()
+        _check(self.L.swg_system_time_encounters(self.h, int(reps), C.byref(ms), C.byref(npairs), C.byref(ncand)))
+        return ms.value, npairs.value, ncand.value
 
     def set_frozen(self, frozen):
         """frozen[s] = 1 leaves system s out of the following steps."""

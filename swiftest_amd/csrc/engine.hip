@@ -293,6 +293,8 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
             woff[s] = w;
             w += (int64_t)n * nb[s];
         }
+        // the sort-and-sweep check needs no bit matrix
+        if (prm.encounter_check == 1) w = 1;
         pp_words = w;
         if (own2(&pp.words, w) || own2(&pp.wpre, w) || own2(&pp.woff, nsys) || own2(&pp.nb, nsys) ||
             own2(&pp.rowcnt, npl) || own2(&pp.rowstart, npl) || own2(&pp.colcnt, npl) || own2(&pp.colstart, npl))
@@ -432,6 +434,32 @@ int Ensemble::detect_pp(double dt)
     return 0;
 }
 
+// Level-0 pl-pl detection by sort and sweep (k_sas.hip): same list as
+// detect_pp, without the N^2 bit matrix.
+int Ensemble::detect_pp_sas(double dt)
+{
+    hipStream_t st = stream;
+    if (hipError_t e = sas_candidates(st, sas, pl, sy, npl, nsys, nfrozen ? d_active : nullptr, dt, d_total, d_scan))
+        return hipfail(e, "sort-and-sweep candidates");
+    int64_t *ht = (int64_t *)h_small;
+    HIPCHK(hipMemcpyAsync(ht, d_total, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const int64_t ncand = npl ? ht[0] : 0;
+    int64_t nsel = 0;
+    if (hipError_t e = sas_narrow(st, sas, pl, npl, ncand, dt, &nsel)) return hipfail(e, "sort-and-sweep narrow phase");
+    npp = nsel;
+    if (int rc = ensure_list(pp, npp, cap_pp)) return rc;
+    if (hipError_t e = sas_fill(st, sas, pp, pl, npl, npp)) return hipfail(e, "sort-and-sweep list");
+    launch_sys_flags(st, pp, sy.pl_off, sy.enc_pp, nsys);
+    int32_t *hf = (int32_t *)(h_small + 1024);
+    HIPCHK(hipMemcpyAsync(hf, sy.enc_pp, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hf + nsys, sy.err, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    h_enc_pp.assign(hf, hf + nsys);
+    h_err.assign(hf + nsys, hf + 2 * nsys);
+    return 0;
+}
+
 // Level-0 pl-tp detection for the systems in `run`.
 int Ensemble::detect_pt(double dt, const std::vector<int32_t> &run)
 {
@@ -473,7 +501,9 @@ int Ensemble::detect_pt(double dt, const std::vector<int32_t> &run)
 int Ensemble::build_adjacency()
 {
     hipStream_t st = stream;
-    if (npp) {
+    if (npp && prm.encounter_check == 1) {
+        if (hipError_t e = sas_adjacency(st, sas, pp, npl, npp)) return hipfail(e, "sort-and-sweep adjacency");
+    } else if (npp) {
         launch_col_count(st, pp, sy.pl_off, pl.sys, sy.pl_off, nfrozen ? d_active : nullptr, npl, 1);
         launch_scan_i32_i64(st, pp.colcnt, pp.colstart, npl, nullptr, d_scan);
         launch_col_fill(st, pp, sy.pl_off, pl.sys, sy.pl_off, nfrozen ? d_active : nullptr, npl, 1);
@@ -720,7 +750,7 @@ int Ensemble::step_helio_symba()
     npp = npt = 0;
     bool any = false;
     if (symba) {
-        if (int rc = detect_pp(dt)) return rc;
+        if (int rc = prm.encounter_check == 1 ? detect_pp_sas(dt) : detect_pp(dt)) return rc;
         trace_mark(st, "detect pl-pl");
         if (int rc = err_code()) return rc;
         std::vector<int32_t> run(nsys, 0);
@@ -965,6 +995,32 @@ int Ensemble::time_kernels(int reps, double *ms_tp, double *ms_pl)
     return 0;
 }
 
+// Time the level-0 pl-pl check of the current state (the configured broad
+// phase plus the list it produces), with the host synchronizations it
+// contains; the stored level-0 list is rebuilt at the next step anyway.
+int Ensemble::time_encounters(int reps, double *ms, int64_t *npairs, int64_t *ncand)
+{
+    if (prm.integrator != SWG_SYMBA) return fail(SWG_ERR_ARG, "encounter checks run in SyMBA only");
+    const double dt = prm.dt;
+    launch_reset_levels(stream, pl, tp, npl, ntp);  // the step's start: levels and level-0 renc
+    if (int rc = prm.encounter_check == 1 ? detect_pp_sas(dt) : detect_pp(dt)) return rc;  // warm-up
+    HIPCHK(hipStreamSynchronize(stream));
+    auto t0 = std::chrono::steady_clock::now();
+    for (int r = 0; r < reps; r++)
+        if (int rc = prm.encounter_check == 1 ? detect_pp_sas(dt) : detect_pp(dt)) return rc;
+    HIPCHK(hipStreamSynchronize(stream));
+    auto t1 = std::chrono::steady_clock::now();
+    if (ms) *ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / reps;
+    if (npairs) *npairs = npp;
+    if (ncand) {
+        *ncand = -1;
+        if (prm.encounter_check == 1) {
+            HIPCHK(hipMemcpy(ncand, d_total, sizeof(int64_t), hipMemcpyDeviceToHost));
+        }
+    }
+    return 0;
+}
+
 // Leave systems out of the following steps (frozen[s] = 1) or put them back.
 // A frozen system's state stays exactly as it is; the other systems' steps
 // are unaffected.
@@ -1129,6 +1185,12 @@ int swg_system_time_kernels(swg_system *sys, int32_t reps, double *ms_kick_tp, d
 {
     if (!sys || reps <= 0) return fail(SWG_ERR_ARG, "bad argument");
     return sys->e->time_kernels(reps, ms_kick_tp, ms_kick_pl);
+}
+
+int swg_system_time_encounters(swg_system *sys, int32_t reps, double *ms, int64_t *npairs, int64_t *ncand)
+{
+    if (!sys || reps <= 0) return fail(SWG_ERR_ARG, "bad argument");
+    return sys->e->time_encounters(reps, ms, npairs, ncand);
 }
 
 int swg_system_set_frozen(swg_system *sys, const int32_t *frozen)

@@ -92,6 +92,7 @@ struct Ensemble {
     int time_kernels(int reps, double *ms_tp, double *ms_pl);
     int autotune(int reps, int32_t *pl_block, int32_t *tp_block);
     int set_frozen(const int32_t *frozen);
+    int time_encounters(int reps, double *ms, int64_t *npairs, int64_t *ncand);
     std::vector<int32_t> h_frozen;
     int nfrozen = 0;
     int32_t *d_active = nullptr;  // 1 - frozen, per system
@@ -106,6 +107,8 @@ struct Ensemble {
     void build_tiles();
     int ensure_list(ListDev &L, int64_t n, int64_t &cap);
     int detect_pp(double dt);
+    int detect_pp_sas(double dt);
+    SasWork sas;
     int detect_pt(double dt, const std::vector<int32_t> &run);
     int build_adjacency();
     int read_any(int32_t &v);

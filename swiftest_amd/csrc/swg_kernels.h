@@ -57,6 +57,30 @@ void launch_col_count(hipStream_t st, ListDev L, const int32_t *row_off, const i
 void launch_col_fill(hipStream_t st, ListDev L, const int32_t *row_off, const int32_t *col_sys,
                      const int32_t *col_off, const int32_t *run, int ncols, int upper);
 
+// ---- sort-and-sweep broad phase of the level-0 pl-pl check (k_sas.hip) ----
+struct SasWork {
+    int n = 0;  // bodies the per-body arrays hold
+    double *lo = nullptr, *hi = nullptr, *slo = nullptr;
+    int32_t *idx = nullptr, *sidx = nullptr, *rank = nullptr, *cnt = nullptr;
+    int64_t *coff = nullptr, *nsel = nullptr;
+    uint64_t *key = nullptr, *sel = nullptr, *skey = nullptr;
+    uint8_t *flag = nullptr;
+    int32_t *pid = nullptr, *spid = nullptr, *si2 = nullptr;
+    void *temp = nullptr;
+    size_t temp_bytes = 0, key_cap = 0, flag_cap = 0, sel_cap = 0, skey_cap = 0, pid_cap = 0, spid_cap = 0,
+           si2_cap = 0;
+    ~SasWork();
+};
+// extents, per-system sort, candidate counts and their scan (total in *ncand_dev)
+hipError_t sas_candidates(hipStream_t st, SasWork &w, PlDev pl, SysDev sy, int npl, int nsys,
+                          const int32_t *active, double dt, int64_t *ncand_dev, void *scan_scratch);
+// narrow phase and compaction; returns the number of pairs (synchronizes)
+hipError_t sas_narrow(hipStream_t st, SasWork &w, PlDev pl, int npl, int64_t ncand, double dt, int64_t *nsel_host);
+// the ordered list, its row counts and the level-0 marks (list capacity >= npp)
+hipError_t sas_fill(hipStream_t st, SasWork &w, ListDev &pp, PlDev pl, int npl, int64_t npp);
+// index2-side adjacency
+hipError_t sas_adjacency(hipStream_t st, SasWork &w, ListDev &pp, int npl, int64_t npp);
+
 // ---- SyMBA recursion, level 0 (k_enc.hip) ----
 void launch_list_check_pp(hipStream_t st, ListDev pp, PlDev pl, SysDev sy, int64_t n, int irec, double dt,
                           int32_t *any);

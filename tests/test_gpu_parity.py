@@ -249,6 +249,66 @@ def test_recursion_modes_match_oracle(monkeypatch, subtree_max, lds):
     assert sg["max_depth"] >= 2
 
 
+def _sas_case(kind):
+    if kind == "disk":
+        return [compact_disk(seed=13, npl=300, ntp=0)], 0.005
+    rng = np.random.Generator(np.random.PCG64(17))
+    if kind == "clustered":
+        # a few tight clumps: many bodies share x extents
+        base = compact_disk(seed=14, npl=240, ntp=0)
+        centers = base.pl_rh[rng.integers(0, 6, base.npl)]
+        base.pl_rh[:] = centers + rng.normal(0, 2e-3, (base.npl, 3))
+        return [base], 0.005
+    if kind == "fast":
+        # large velocities and a long step: the motion over dt decides the pairs
+        base = compact_disk(seed=15, npl=200, ntp=0)
+        base.pl_vh[:] = base.pl_vh * rng.uniform(0.5, 3.0, (base.npl, 1)) + rng.normal(0, 3.0, (base.npl, 3))
+        return [base], 0.05
+    # an ensemble of unequal systems (sorted and swept per system)
+    return [compact_disk(seed=20 + k, npl=n, ntp=0) for k, n in enumerate((40, 130, 7, 90))], 0.005
+
+
+@pytest.mark.parametrize("kind", ["disk", "clustered", "fast", "ensemble"])
+def test_sortsweep_lists_equal_triangular(kind):
+    """The sort-and-sweep broad phase is conservative: after the same narrow
+    phase its level-0 pl-pl lists equal the triangular (all-pairs) lists,
+    step after step, and so do the trajectories."""
+    print(f"sortsweep {kind}", flush=True)
+    systems, dt = _sas_case(kind)
+    tri = S.GpuSystem(S.SYMBA, systems, dt, lclose=False, encounter_check=0)
+    sas = S.GpuSystem(S.SYMBA, systems, dt, lclose=False, encounter_check=1)
+    total = 0
+    for step in range(3):
+        tri.step()
+        sas.step()
+        a1, a2 = tri.enc_list(0)
+        b1, b2 = sas.enc_list(0)
+        total += len(a1)
+        assert np.array_equal(a1, b1) and np.array_equal(a2, b2), (kind, step)
+        assert tri.stats()["nsubsteps"] == sas.stats()["nsubsteps"]
+    assert total > 0
+    a, b = tri.state(), sas.state()
+    for k in ("pl_rh", "pl_vh", "pl_vb"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_sortsweep_symba_matches_oracle():
+    sysic = compact_disk(npl=160, ntp=40)
+    o = O.OracleSystem(O.SYMBA, sysic, 0.005, short_circuit=False, istep_out=10)
+    g = S.GpuSystem(S.SYMBA, sysic, 0.005, short_circuit=False, istep_out=10, encounter_check=1)
+    for blk in range(2):
+        rc = o.run(10)
+        try:
+            g.run(10)
+            grc = 0
+        except S.SwiftestHipError as e:
+            grc = e.code
+        assert (rc == -1) == (grc == -3), (rc, grc)
+        if rc:
+            break
+        _compare_systems(o, g, f"sortsweep block {blk}")
+
+
 def test_ensemble_equals_single_runs():
     systems = [compact_disk(seed=s, npl=n, ntp=t, mhi=2e-6, alo=0.9, ahi=1.1)
                for s, n, t in ((1, 40, 30), (2, 70, 0), (3, 5, 90), (6, 120, 10))]
