@@ -106,6 +106,7 @@ def ensemble_leg(nsys, steps, warmup, sample, log):
     systems = ics.ensemble(seed=4, nsys=nsys)
     g = S.GpuSystem(S.SYMBA, systems, dt, istep_out=0, lclose=False)
     blocks = g.autotune(3)
+    g.use_graphs(True)
     g.run(warmup)
     t0 = time.perf_counter()
     g.run(steps)
@@ -120,6 +121,7 @@ def ensemble_leg(nsys, steps, warmup, sample, log):
     t_seq = 0.0
     for i in idx:
         one = S.GpuSystem(S.SYMBA, [systems[int(i)]], dt, istep_out=0, lclose=False)
+        one.use_graphs(True)
         one.run(warmup)
         t0 = time.perf_counter()
         one.run(steps)
@@ -130,7 +132,7 @@ def ensemble_leg(nsys, steps, warmup, sample, log):
     return {"metric": "ensemble system-steps/s", "value": rate, "unit": "system-steps/s",
             "systems": nsys, "steps": steps, "warmup": warmup, "seconds": t_ens,
             "npl_total": int(sum(s.npl for s in systems)), "ntp_total": int(sum(s.ntp for s in systems)),
-            "tiles": {"kick_pl_block": blocks[0], "kick_tp_block": blocks[1]},
+            "tiles": {"kick_pl_block": blocks[0], "kick_tp_block": blocks[1]}, "hip_graphs": True,
             "sequential": {"value": seq_rate, "sample": int(len(idx)),
                            "how": "the same library, one system per handle, systems run one after another"},
             "speedup_vs_sequential": rate / seq_rate,

@@ -141,6 +141,10 @@ int swg_system_enc_list(swg_system *sys, int32_t which, int64_t *n, int32_t *ind
  * duration of one launch in ms, measured with HIP events on the system's
  * stream. */
 int swg_system_time_kernels(swg_system *sys, int32_t reps, double *ms_kick_tp, double *ms_kick_pl);
+/* Replay the step's two halves around the recursion (frame changes, kicks,
+ * drifts) from captured HIP graphs instead of launching them one by one;
+ * results are unchanged.  Off by default.  No counterpart in the reference. */
+int swg_system_use_graphs(swg_system *sys, int32_t enable);
 /* Time the level-0 pl-pl encounter check (encounter_check_all_plpl with the
  * configured ENCOUNTER_CHECK) on the current state: average ms per call over
  * reps calls, the number of pairs found and, for SORTSWEEP, the number of

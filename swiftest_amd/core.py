@@ -263,6 +263,10 @@ class GpuSystem:
         _check(self.L.swg_system_kernel_times(self.h, C.byref(a), C.byref(na), C.byref(b), C.byref(nb)))
         return a.value, na.value, b.value, nb.value
 
+    def use_graphs(self, enable=True):
+        """Replay the step's fixed launch sequences from captured HIP graphs."""
+        _check(self.L.swg_system_use_graphs(self.h, int(enable)))
+
     def time_encounters(self, reps=3):
         """(ms per level-0 pl-pl check, pairs found, broad-phase candidates or -1)."""
         ms, npairs, ncand = C.c_double(), C.c_int64(), C.c_int64()

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <chrono>
 #include <cstdlib>
+#include <functional>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -103,6 +104,7 @@ static void free_list(ListDev &L)
 
 Ensemble::~Ensemble()
 {
+    drop_graphs();
     for (void *p : owned)
         if (p) (void)hipFree(p);
     free_list(pp);
@@ -734,6 +736,64 @@ void Ensemble::trace_levels()
     }
 }
 
+// The two halves of the step around the recursion are fixed sequences of
+// launches whose arguments change only when the tiles or the encounter lists'
+// buffers move, or with whether the lists are empty.  With graphs on, each
+// (half, pl-pl list empty or not, pl-tp list empty or not) variant is
+// captured once as a HIP graph and replayed; the capture holds kernel nodes
+// only.  Launch timing (ktiming) records events between the launches and
+// always runs the halves directly.
+uint64_t Ensemble::graph_key() const
+{
+    const void *ptrs[] = {d_tiles_pl, d_tiles_tp, pp.i1,     pp.i2,      pp.col,    pp.colcnt, pp.colstart,
+                          pp.rowcnt,  pp.rowstart, pp.status, pt.i1,      pt.i2,     pt.col,    pt.colcnt,
+                          pt.colstart, pt.status, d_arena,   (const void *)(intptr_t)ntiles_pl,
+                          (const void *)(intptr_t)ntiles_tp, (const void *)(intptr_t)tune.kick_pl_block,
+                          (const void *)(intptr_t)tune.kick_tp_block};
+    uint64_t h = 1469598103934665603ull;
+    for (const void *p : ptrs) {
+        h ^= (uint64_t)(uintptr_t)p;
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+void Ensemble::drop_graphs()
+{
+    for (auto &g : graph_exec)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+int Ensemble::run_phase(int half, const std::function<int()> &body)
+{
+    if (!use_graphs || ktiming) return body();
+    const uint64_t key = graph_key();
+    if (key != graph_key_cached) {
+        drop_graphs();
+        graph_key_cached = key;
+    }
+    hipGraphExec_t &ex = graph_exec[half * 4 + (npp > 0) * 2 + (npt > 0)];
+    if (!ex) {
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int rc = body();
+        hipError_t e = hipStreamEndCapture(stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return hipfail(e, "hipStreamEndCapture");
+        e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return hipfail(e, "hipGraphInstantiate");
+    }
+    HIPCHK(hipGraphLaunch(ex, stream));
+    return 0;
+}
+
 // One SyMBA / HELIO step of every system.
 int Ensemble::step_helio_symba()
 {
@@ -793,13 +853,28 @@ int Ensemble::step_helio_symba()
         return 0;
     };
     // first half: pl then tp
-    launch_sums_begin(st, pl, sy, nsys, 0);
-    launch_pl_begin_apply(st, pl, sy, npl, dth, 0);
-    if (int rc = kick_pl(1)) return rc;
-    launch_drift(st, pl.rh, pl.vb, pl.status, pl.levelg, pl.sys, sy.GMcb, sy.err, sy.frozen, npl, dt, -1, symba, 0);
-    launch_tp_pre(st, tp, sy, ntp, dth);
-    if (int rc = kick_tp(1)) return rc;
-    launch_drift(st, tp.rh, tp.vb, tp.status, tp.levelg, tp.sys, sy.GMcb, sy.err, sy.frozen, ntp, dt, -1, symba, 1);
+    auto first_half = [&]() -> int {
+        launch_sums_begin(st, pl, sy, nsys, 0);
+        launch_pl_begin_apply(st, pl, sy, npl, dth, 0);
+        if (int rc = kick_pl(1)) return rc;
+        launch_drift(st, pl.rh, pl.vb, pl.status, pl.levelg, pl.sys, sy.GMcb, sy.err, sy.frozen, npl, dt, -1, symba,
+                     0);
+        launch_tp_pre(st, tp, sy, ntp, dth);
+        if (int rc = kick_tp(1)) return rc;
+        launch_drift(st, tp.rh, tp.vb, tp.status, tp.levelg, tp.sys, sy.GMcb, sy.err, sy.frozen, ntp, dt, -1, symba,
+                     1);
+        return 0;
+    };
+    auto second_half = [&]() -> int {
+        if (int rc = kick_pl(0)) return rc;
+        launch_sums_end(st, pl, sy, nsys);
+        launch_pl_end_apply(st, pl, sy, npl, dth);
+        if (int rc = kick_tp(0)) return rc;
+        launch_tp_post(st, tp, sy, ntp, dth);
+        launch_lfirst_update(st, sy, nsys);
+        return 0;
+    };
+    if (int rc = run_phase(0, first_half)) return rc;
     trace_mark(st, "first half");
     if (any) {
         if (int rc = recur0()) return rc;
@@ -810,12 +885,7 @@ int Ensemble::step_helio_symba()
         }
     }
     // second half
-    if (int rc = kick_pl(0)) return rc;
-    launch_sums_end(st, pl, sy, nsys);
-    launch_pl_end_apply(st, pl, sy, npl, dth);
-    if (int rc = kick_tp(0)) return rc;
-    launch_tp_post(st, tp, sy, ntp, dth);
-    launch_lfirst_update(st, sy, nsys);
+    if (int rc = run_phase(1, second_half)) return rc;
     trace_mark(st, "second half");
     HIPCHK(hipGetLastError());
     return 0;
@@ -1191,6 +1261,14 @@ int swg_system_time_encounters(swg_system *sys, int32_t reps, double *ms, int64_
 {
     if (!sys || reps <= 0) return fail(SWG_ERR_ARG, "bad argument");
     return sys->e->time_encounters(reps, ms, npairs, ncand);
+}
+
+int swg_system_use_graphs(swg_system *sys, int32_t enable)
+{
+    if (!sys) return fail(SWG_ERR_ARG, "NULL system");
+    sys->e->use_graphs = enable != 0;
+    if (!enable) sys->e->drop_graphs();
+    return 0;
 }
 
 int swg_system_set_frozen(swg_system *sys, const int32_t *frozen)

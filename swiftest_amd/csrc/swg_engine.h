@@ -1,5 +1,6 @@
 // swg_engine.h -- host-side objects of the library.
 #pragma once
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -93,6 +94,13 @@ struct Ensemble {
     int autotune(int reps, int32_t *pl_block, int32_t *tp_block);
     int set_frozen(const int32_t *frozen);
     int time_encounters(int reps, double *ms, int64_t *npairs, int64_t *ncand);
+    // HIP graphs of the step's two halves (see run_phase)
+    bool use_graphs = false;
+    hipGraphExec_t graph_exec[8] = {};
+    uint64_t graph_key_cached = 0;
+    uint64_t graph_key() const;
+    void drop_graphs();
+    int run_phase(int half, const std::function<int()> &body);
     std::vector<int32_t> h_frozen;
     int nfrozen = 0;
     int32_t *d_active = nullptr;  // 1 - frozen, per system

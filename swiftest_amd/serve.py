@@ -111,6 +111,8 @@ class Server:
     def _run_batch(self, lane, batch, t0):
         g = core.GpuSystem(self.integrator, [r.system for r in batch], self.dt, lclose=self.lclose,
                            istep_out=self.istep_out, ctx=self.ctx[lane])
+        if hasattr(g, "use_graphs"):
+            g.use_graphs(True)
         remaining = np.array([r.nsteps for r in batch], np.int64)
         frozen = np.zeros(len(batch), np.int32)
         while (remaining > 0).any():
