@@ -35,6 +35,19 @@ FLOPS_PER_PLTP_PAIR = 17       # swiftest_kick.f90:404-408, :464-467
 FLOPS_PER_PLPL_PAIR = 19       # swiftest_kick.f90:227-236, per ordered pair
 
 
+def measured_traffic():
+    """HBM bytes per k_kick_tp launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (profiles/kick_tp_c3_hbm_traffic.md), or None."""
+    path = os.path.join(ROOT, "profiles", "kick_tp_c3_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch"):
+                return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(sysic, target_s):
     """Pair-interaction rate of the CPU restatement's OpenMP kernels (the
     reference's loop decomposition: OpenMP over particles / rows, j in order)
@@ -260,10 +273,14 @@ def main():
                     "kick_pl_tflops": pl_tflops,
                     "kernel_pair_rate": pairs_per_eval / ((ms_tp + ms_pl) * 1e-3) if ms_tp + ms_pl > 0 else 0.0},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(),
                      "kernel": "k_kick_tp (pl-tp accelerations)",
                      "note": ("compute-bound FP64 VALU kernel, no MFMA; peak is MI355X FP64 vector; "
-                              f"{FLOPS_PER_PLTP_PAIR} flops per pl-tp pair x npl x ntp per launch")},
+                              f"{FLOPS_PER_PLTP_PAIR} flops per pl-tp pair x npl x ntp per launch; exact IEEE "
+                              "sqrt/div cores cost ~30 VALU + 2 quarter-rate ops per pair, capping it near 24% "
+                              "of the flop peak; traffic in bytes per launch from rocprofv3 FETCH_SIZE/"
+                              "WRITE_SIZE scaled by a 1 GiB copy; the SyMBA step itself is dominated by the "
+                              "recursion, reported under encounters")},
         "cpu_baseline": None,
     }
     del g
