@@ -425,6 +425,7 @@ struct LdsSet {
     double kdx[L_T], kdy[L_T], kdz[L_T], kfp[L_T], kfm[L_T];
     uint8_t kfl[L_T];
     int jst[MAXLEV + 2];
+    int lab[L_NB + L_NBT];  // connected-component labels, profiling only
     int npair, flag_pp, flag_pt, bad, nact;
     unsigned long long prof[(MAXLEV + 2) * 8];  // per-level phase ticks when profiling
 };
@@ -846,6 +847,63 @@ __global__ __launch_bounds__(L_T) void k_subtree_lds(const SubArgs *__restrict__
         if (tid == 0) atomicOr(&a.sy.err[s], 4);
         return;
     }
+    if (A.prof) {
+        // diagnostic: connected components of the slice (bodies joined by
+        // their pairs), accumulated in the profile's last row
+        for (int q = tid; q < nB + nBt; q += L_T) S.lab[q] = q;
+        __syncthreads();
+        for (int it = 0; it < nB + nBt; it++) {
+            if (tid == 0) S.flag_pp = 0;
+            __syncthreads();
+            for (int lp = tid; lp < S.npair; lp += L_T) {
+                const int i = S.ppi[lp], j = S.ppj[lp];
+                const int m = min(S.lab[i], S.lab[j]);
+                if (S.lab[i] > m) {
+                    atomicMin(&S.lab[i], m);
+                    S.flag_pp = 1;
+                }
+                if (S.lab[j] > m) {
+                    atomicMin(&S.lab[j], m);
+                    S.flag_pp = 1;
+                }
+            }
+            for (int u = tid; u < nEt; u += L_T) {
+                const int i = S.tpl[u], t = nB + S.tpt[u];
+                const int m = min(S.lab[i], S.lab[t]);
+                if (S.lab[i] > m) {
+                    atomicMin(&S.lab[i], m);
+                    S.flag_pp = 1;
+                }
+                if (S.lab[t] > m) {
+                    atomicMin(&S.lab[t], m);
+                    S.flag_pp = 1;
+                }
+            }
+            __syncthreads();
+            const bool again = S.flag_pp;
+            __syncthreads();
+            if (!again) break;
+        }
+        if (tid == 0) {
+            int ncomp = 0, big = 0;
+            for (int q = 0; q < nB + nBt; q++) {
+                if (S.lab[q] != q) continue;
+                ncomp++;
+                int sz = 0;
+                for (int w = q; w < nB + nBt; w++) sz += S.lab[w] == q;
+                big = max(big, sz);
+            }
+            unsigned long long *P = A.prof + (MAXLEV + 1) * NPROF;
+            atomicAdd(&P[0], 1ull);
+            atomicAdd(&P[1], (unsigned long long)ncomp);
+            atomicAdd(&P[2], (unsigned long long)big);
+            atomicAdd(&P[3], (unsigned long long)(nB + nBt));
+            atomicMax(&P[4], (unsigned long long)big);
+        }
+        __syncthreads();
+        if (tid == 0) S.flag_pp = 0;
+        __syncthreads();
+    }
     // ---- the subtree ----
     const int d0 = A.d0;
     int d = d0, j = 1;
@@ -944,6 +1002,10 @@ void subtree_profile_report(hipStream_t st)
                 (double)p[2] / p[0], p[3] * tick_us * 1e-3, p[4] * tick_us * 1e-3, p[5] * tick_us * 1e-3,
                 p[6] * tick_us * 1e-3, p[7] * tick_us * 1e-3);
     }
+    const unsigned long long *c = h + (MAXLEV + 1) * NPROF;
+    if (c[0])
+        fprintf(stderr, "[swg] LDS slices %llu: avg bodies %.1f, avg components %.1f, avg largest %.1f, max largest %llu\n",
+                c[0], (double)c[3] / c[0], (double)c[1] / c[0], (double)c[2] / c[0], c[4]);
     (void)hipMemsetAsync(g_prof, 0, sizeof h, st);
 }
 
