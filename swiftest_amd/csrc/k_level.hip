@@ -245,29 +245,36 @@ __global__ __launch_bounds__(256) void k_lv_check_pt(RecState a, const int2 *Et,
     }
 }
 
-// Shell kicks of the massive bodies, KB_TB bodies per workgroup.  The
-// workgroup walks the bodies' entries in chunks of KB_K per body: all threads
-// evaluate the chunk's terms in parallel into LDS, then one lane per body
-// folds its terms in entry order, so each body's sum is the reference's
-// serial sum.  At level 0 the source is the full adjacency and only the +1
-// kick is made.
-constexpr int KB_TB = 32, KB_K = 32, KB_T = 256, KB_LD = KB_TB + 1;
-constexpr int KB_MIN_BODIES = 16384;  // below this, one wavefront per body fills the device better
+// Shell kicks of the massive bodies, TB bodies per workgroup of KT_T threads.
+// The workgroup walks its bodies' entries in chunks of K = KT_C / TB entries
+// per body.  Every thread evaluates KT_R terms of a chunk into LDS, with the
+// loads of all of them issued before any term is computed; then lane tb of the
+// first wavefront folds body tb's terms of the chunk in entry order, so each
+// body's sum is the reference's serial sum.  A body's fold is serial, so the
+// folds of many bodies have to run side by side: the launch picks TB so that
+// the level's bodies make enough workgroups to fill the device.  At level 0
+// the source is the full adjacency and only the +1 kick is made.
+//
+// A listed pair leaves ACTIVE only when collision detection marks it, which
+// also sets bit 0 of its system's error word; the pair statuses (a random
+// gather over the whole list) are read only for the bodies of systems where
+// that has happened.
+constexpr int KT_T = 256, KT_C = 1024, KT_R = KT_C / KT_T;
 
-template <bool L0>
-__global__ __launch_bounds__(KB_T) void k_lv_kick_pl(RecState a, LevelView v, int nB, int d, double dth, int has_pp)
+template <int TB, bool L0>
+__global__ __launch_bounds__(KT_T) void k_lv_kick_pl(RecState a, LevelView v, int nB, int d, double dth, int has_pp)
 {
-    __shared__ double sdx[KB_K * KB_LD], sdy[KB_K * KB_LD], sdz[KB_K * KB_LD], sfp[KB_K * KB_LD],
-        sfm[KB_K * KB_LD];
-    __shared__ uint8_t sfl[KB_K * KB_LD];
-    __shared__ int sb[KB_TB], se0[KB_TB], sn[KB_TB];
-    __shared__ double sxb[KB_TB], syb[KB_TB], szb[KB_TB], shb[KB_TB];
+    constexpr int K = KT_C / TB, LD = TB + 1;  // LDS slot of (body tb, entry k): k * LD + tb
+    __shared__ double sdx[K * LD], sdy[K * LD], sdz[K * LD], sfp[K * LD], sfm[K * LD];
+    __shared__ uint8_t sfl[K * LD];
+    __shared__ int sb[TB], se0[TB], sn[TB], sck[TB];
+    __shared__ double sxb[TB], syb[TB], szb[TB], shb[TB];
     __shared__ int smax;
     const int tid = threadIdx.x;
-    const int q0 = blockIdx.x * KB_TB;
+    const int q0 = blockIdx.x * TB;
     if (tid == 0) smax = 0;
     __syncthreads();
-    if (tid < KB_TB) {
+    if (tid < TB) {
         const int q = q0 + tid;
         int n = 0;
         if (q < nB) {
@@ -276,6 +283,7 @@ __global__ __launch_bounds__(KB_T) void k_lv_kick_pl(RecState a, LevelView v, in
             n = L0 ? l0_pl_count(a, has_pp, b) : v.O[q + 1] - e0;
             sb[tid] = b;
             se0[tid] = e0;
+            sck[tid] = a.sy.err[a.pl.sys[b]] & 1;
             sxb[tid] = a.pl.rh.x[b];
             syb[tid] = a.pl.rh.y[b];
             szb[tid] = a.pl.rh.z[b];
@@ -289,104 +297,87 @@ __global__ __launch_bounds__(KB_T) void k_lv_kick_pl(RecState a, LevelView v, in
     if (nmax == 0) return;
     const double spp = powi(RSHELL, 2 * (d + 1)), spm = powi(RSHELL, 2 * d);
     const bool minus = d > 0;
+    const PlDev &pl = a.pl;
     double3 p = make_double3(0.0, 0.0, 0.0), m = p;
     bool anyp = false, anym = false;
-    for (int c0 = 0; c0 < nmax; c0 += KB_K) {
-        for (int sidx = tid; sidx < KB_TB * KB_K; sidx += KB_T) {
-            const int tb = sidx / KB_K, k = sidx % KB_K, u = c0 + k;
+    for (int c0 = 0; c0 < nmax; c0 += K) {
+        int2 en[KT_R];
+        bool live[KT_R];
+#pragma unroll
+        for (int r = 0; r < KT_R; r++) {
+            const int sidx = tid + r * KT_T, tb = sidx / K, u = c0 + sidx % K;
+            live[r] = u < sn[tb];
+            en[r] = make_int2(0, 0);
+            if (live[r]) en[r] = L0 ? l0_pl_entry(a, sb[tb], u) : v.E[se0[tb] + u];
+        }
+        int8_t stt[KT_R];
+        double xp[KT_R], yp[KT_R], zp[KT_R], hp[KT_R], gp[KT_R];
+#pragma unroll
+        for (int r = 0; r < KT_R; r++) {
+            stt[r] = INACTIVE;
+            xp[r] = yp[r] = zp[r] = hp[r] = gp[r] = 0.0;
+            if (live[r]) {
+                const int k = (int)((uint32_t)en[r].x >> 1), q = en[r].y;
+                stt[r] = sck[(tid + r * KT_T) / K] ? a.pp.status[k] : (int8_t)ACTIVE;
+                xp[r] = pl.rh.x[q];
+                yp[r] = pl.rh.y[q];
+                zp[r] = pl.rh.z[q];
+                hp[r] = pl.rhill[q];
+                gp[r] = pl.Gm[q];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < KT_R; r++) {
+            const int sidx = tid + r * KT_T, tb = sidx / K, k = sidx % K;
             ShellTerm t;
             t.tp = t.tm = false;
-            if (u < sn[tb]) {
-                const int2 en = L0 ? l0_pl_entry(a, sb[tb], u) : v.E[se0[tb] + u];
-                pl_shell_term(a, en, sxb[tb], syb[tb], szb[tb], shb[tb], spp, spm, minus, t);
-            }
-            const int w = k * KB_LD + tb;
+            if (live[r] && stt[r] == ACTIVE)
+                pl_shell_term_v(en[r].x & 1, sxb[tb], syb[tb], szb[tb], shb[tb], xp[r], yp[r], zp[r], hp[r], gp[r],
+                                spp, spm, minus, t);
+            const int w = k * LD + tb;
             sfl[w] = (t.tp ? 1 : 0) | (t.tm ? 2 : 0);
-            if (t.tp || t.tm) {
-                sdx[w] = t.dx;
-                sdy[w] = t.dy;
-                sdz[w] = t.dz;
-                sfp[w] = t.fp;
-                sfm[w] = t.fm;
-            }
+            sdx[w] = t.tp || t.tm ? t.dx : 0.0;
+            sdy[w] = t.tp || t.tm ? t.dy : 0.0;
+            sdz[w] = t.tp || t.tm ? t.dz : 0.0;
+            sfp[w] = t.tp ? t.fp : 0.0;
+            sfm[w] = t.tm ? t.fm : 0.0;
         }
         __syncthreads();
-        if (tid < KB_TB) {
-            const int kn = min(KB_K, sn[tid] - c0);
+        if (tid < TB) {
+            // every slot holds finite values, so both sums are formed and the
+            // flags select which to keep: no branches, and the loads of
+            // several entries are in flight at once
+            const int kn = min(K, sn[tid] - c0);
+#pragma unroll 4
             for (int k = 0; k < kn; k++) {
-                const int w = k * KB_LD + tid;
+                const int w = k * LD + tid;
                 const int fl = sfl[w];
-                if (fl & 1) {
-                    anyp = true;
-                    p.x = fma(sfp[w], sdx[w], p.x);
-                    p.y = fma(sfp[w], sdy[w], p.y);
-                    p.z = fma(sfp[w], sdz[w], p.z);
-                }
-                if (fl & 2) {
-                    anym = true;
-                    m.x = fma(sfm[w], sdx[w], m.x);
-                    m.y = fma(sfm[w], sdy[w], m.y);
-                    m.z = fma(sfm[w], sdz[w], m.z);
-                }
+                const double fp = sfp[w], fm = sfm[w], dx = sdx[w], dy = sdy[w], dz = sdz[w];
+                const double px = fma(fp, dx, p.x), py = fma(fp, dy, p.y), pz = fma(fp, dz, p.z);
+                const double mx = fma(fm, dx, m.x), my = fma(fm, dy, m.y), mz = fma(fm, dz, m.z);
+                if (fl & 1) p = make_double3(px, py, pz);
+                if (fl & 2) m = make_double3(mx, my, mz);
+                anyp |= (fl & 1) != 0;
+                anym |= (fl & 2) != 0;
             }
         }
         __syncthreads();
     }
-    if (tid < KB_TB && q0 + tid < nB) apply_shell_kicks(a.pl.vb, sb[tid], dth, anyp, p, anym, m);
+    if (tid < TB && q0 + tid < nB) apply_shell_kicks(a.pl.vb, sb[tid], dth, anyp, p, anym, m);
 }
 
-// For smaller sets (too few bodies to fill the device in blocks of KB_TB):
-// one wavefront per body; the lanes evaluate 64 consecutive entries at once
-// and every lane folds the terms in entry order (readlane broadcast).
-__device__ inline double lane_value(double v, int l)
+template <bool L0>
+static void launch_kick_pl_tiles(hipStream_t st, const RecState &a, const LevelView &v, int nB, int d, double dth,
+                                 int has_pp)
 {
-    int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
-    int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
-    return __hiloint2double(hi, lo);
-}
-
-__global__ __launch_bounds__(256) void k_lv_kick_pl_wave(RecState a, LevelView v, int nB, int d, double dth)
-{
-    const int q = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const int lane = threadIdx.x & 63;
-    if (q >= nB) return;
-    const int b = v.B[q];
-    const int e0 = v.O[q], e1 = v.O[q + 1];
-    if (e1 <= e0) return;
-    const double spp = powi(RSHELL, 2 * (d + 1)), spm = powi(RSHELL, 2 * d);
-    const bool minus = d > 0;
-    const double xb = a.pl.rh.x[b], yb = a.pl.rh.y[b], zb = a.pl.rh.z[b], hb = a.pl.rhill[b];
-    double3 p = make_double3(0.0, 0.0, 0.0), m = p;
-    bool anyp = false, anym = false;
-    for (int c0 = e0; c0 < e1; c0 += 64) {
-        const int u = c0 + lane;
-        ShellTerm t;
-        t.tp = t.tm = false;
-        t.dx = t.dy = t.dz = t.fp = t.fm = 0.0;
-        if (u < e1) pl_shell_term(a, v.E[u], xb, yb, zb, hb, spp, spm, minus, t);
-        const uint64_t mp = __ballot(t.tp), mm = __ballot(t.tm);
-        anyp |= mp != 0;
-        anym |= mm != 0;
-        uint64_t all = mp | mm;
-        while (all) {
-            const int l = __builtin_ctzll(all);
-            all &= all - 1;
-            const double dx = lane_value(t.dx, l), dy = lane_value(t.dy, l), dz = lane_value(t.dz, l);
-            if ((mp >> l) & 1) {
-                const double f = lane_value(t.fp, l);
-                p.x = fma(f, dx, p.x);
-                p.y = fma(f, dy, p.y);
-                p.z = fma(f, dz, p.z);
-            }
-            if ((mm >> l) & 1) {
-                const double f = lane_value(t.fm, l);
-                m.x = fma(f, dx, m.x);
-                m.y = fma(f, dy, m.y);
-                m.z = fma(f, dz, m.z);
-            }
-        }
-    }
-    if (lane == 0) apply_shell_kicks(a.pl.vb, b, dth, anyp, p, anym, m);
+    // bodies per workgroup: the largest tile that still gives two workgroups
+    // per compute unit
+    constexpr int WANT = 512;
+    if (nB >= 64 * WANT) k_lv_kick_pl<64, L0><<<cdiv(nB, 64), KT_T, 0, st>>>(a, v, nB, d, dth, has_pp);
+    else if (nB >= 32 * WANT) k_lv_kick_pl<32, L0><<<cdiv(nB, 32), KT_T, 0, st>>>(a, v, nB, d, dth, has_pp);
+    else if (nB >= 16 * WANT) k_lv_kick_pl<16, L0><<<cdiv(nB, 16), KT_T, 0, st>>>(a, v, nB, d, dth, has_pp);
+    else if (nB >= 8 * WANT) k_lv_kick_pl<8, L0><<<cdiv(nB, 8), KT_T, 0, st>>>(a, v, nB, d, dth, has_pp);
+    else k_lv_kick_pl<4, L0><<<cdiv(nB, 4), KT_T, 0, st>>>(a, v, nB, d, dth, has_pp);
 }
 
 // Shell kicks of one test particle per thread
@@ -549,14 +540,10 @@ void launch_level_kicks(hipStream_t st, const LevelArgs &la, const LevelView *v,
     LevelView vv{};
     if (v) vv = *v;
     if (v) {
-        if (nB >= KB_MIN_BODIES)
-            k_lv_kick_pl<false><<<cdiv(nB, KB_TB), KB_T, 0, st>>>(a, vv, nB, d, dth, la.has_pp);
-        else if (nB)
-            k_lv_kick_pl_wave<<<cdiv((int64_t)nB * 64, 256), 256, 0, st>>>(a, vv, nB, d, dth);
+        if (nB) launch_kick_pl_tiles<false>(st, a, vv, nB, d, dth, la.has_pp);
         if (nBt) k_lv_kick_tp<false><<<cdiv(nBt, 256), 256, 0, st>>>(a, vv, nBt, d, dth, la.has_pt);
     } else {  // level 0: the full adjacency of every body
-        if (nB && la.has_pp)
-            k_lv_kick_pl<true><<<cdiv(nB, KB_TB), KB_T, 0, st>>>(a, vv, nB, d, dth, la.has_pp);
+        if (nB && la.has_pp) launch_kick_pl_tiles<true>(st, a, vv, nB, d, dth, la.has_pp);
         if (nBt && la.has_pt) k_lv_kick_tp<true><<<cdiv(nBt, 256), 256, 0, st>>>(a, vv, nBt, d, dth, la.has_pt);
     }
 }
