@@ -89,10 +89,13 @@ struct SubArgs {
     const int64_t *arena_off;  // nsys+1 word offsets
     int32_t *stats;            // [0] substeps, [1] max depth, [2] overflow
     // optional per-level profile (NPROF words per level): substeps, sum nB,
-    // sum nE, then wall-clock ticks in check, kicks, drift, build, levels
+    // sum nE, then wall-clock ticks in check, kicks, drift, build, levels; the
+    // LDS kernel adds kick evaluate and fold ticks, sum of the largest entry
+    // count of a kicked body, sum of kicked bodies, bodies drifted, substeps
+    // where a thread drifted more than one body, and list builds
     unsigned long long *prof;
 };
-constexpr int NPROF = 8;
+constexpr int NPROF = 16;
 
 // Build level dnew of system s from src into the arena after src.  Returns
 // false (uniformly) when the arena is too small.
@@ -408,6 +411,15 @@ __global__ __launch_bounds__(ST_T) void k_subtree(const SubArgs *__restrict__ ap
 // partner has levelg >= d, so every phase filters the d0 slice on the fly.
 // Entry word: (local pair << 10) | (role << 9) | local partner.
 
+// one evaluated shell term: separation, the +1 and -1 factors (0 where that
+// shell contributes nothing) and which of the two contribute (bits 0 and 1)
+struct KTerm {
+    double dx, dy, dz, fp, fm;
+    int fl;
+};
+constexpr int L_KC = 512;                              // entry terms per kick chunk
+constexpr int L_KS = (L_NB + L_NBT + L_T - 1) / L_T;  // bodies folded per lane
+
 struct LdsSet {
     double px[L_NB], py[L_NB], pz[L_NB], pvx[L_NB], pvy[L_NB], pvz[L_NB], ph[L_NB], pgm[L_NB], pR[L_NB];
     int pg[L_NB], plev[L_NB], plm[L_NB], pst[L_NB], pO[L_NB + 1];
@@ -420,14 +432,19 @@ struct LdsSet {
     int16_t tpl[L_NET], tpt[L_NET];
     int tpg[L_NET];
     int8_t tps[L_NET], tpl_lev[L_NET], tpv[L_NET];
-    // shell kicks: the level's bodies and one term slot per thread
-    int16_t act[L_NB + L_NBT];
-    double kdx[L_T], kdy[L_T], kdz[L_T], kfp[L_T], kfm[L_T];
-    uint8_t kfl[L_T];
+    // shell kicks: the level's list (kicked bodies, their stretches of the
+    // kept entries) and one chunk of evaluated terms
+    int16_t act[L_NB + L_NBT], kst[L_NB + L_NBT], ken[L_NB + L_NBT];
+    int16_t lst[L_NE + L_NET];
+    union {
+        KTerm kt[L_KC];
+        int16_t posu[L_NE + L_NET];  // while the list is built
+        int lab[L_NB + L_NBT];       // connected-component labels, profiling only
+    };
     int jst[MAXLEV + 2];
-    int lab[L_NB + L_NBT];  // connected-component labels, profiling only
-    int npair, flag_pp, flag_pt, bad, nact;
-    unsigned long long prof[(MAXLEV + 2) * 8];  // per-level phase ticks when profiling
+    int wsum[L_T / 64 + 1];
+    int npair, flag_pp, flag_pt, bad, nact, nlst, lst_level, kmax;
+    unsigned long long prof[(MAXLEV + 2) * NPROF];  // per-level phase ticks when profiling
 };
 
 __device__ inline int lds_lower_bound(const int *a, int n, int key)
@@ -517,111 +534,227 @@ __device__ inline void lds_apply(double &vx, double &vy, double &vz, double dth,
     }
 }
 
-// The +1 / -1 shell kicks of level d (the sums of rec_kick_pl / rec_kick_tp).
-// Each body of the level's set gets a group of G lanes (G as large as the
-// number of such bodies allows, at most a wavefront): the group evaluates G of
-// the body's entries at once into LDS slots, and the group's first lane folds
-// the ones that contribute, in entry order.
-__device__ void lds_kicks(LdsSet &S, int nB, int nBt, int nE, int nEt, int d, double dth)
+// workgroup exclusive scan of one value per thread of L_T; *tot = the sum
+__device__ inline int lds_block_scan(LdsSet &S, int v, int *tot)
 {
-    const int tid = threadIdx.x;
-    if (tid == 0) S.nact = 0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int incl = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += u;
+    }
+    if (lane == 63) S.wsum[w] = incl;
     __syncthreads();
-    for (int q = tid; q < nB + nBt; q += L_T) {
-        const bool on = q < nB ? (S.plev[q] >= d && S.pO[q + 1] > S.pO[q])
-                               : (S.tlev[q - nB] >= d && S.tO[q - nB + 1] > S.tO[q - nB]);
-        if (on) S.act[atomicAdd(&S.nact, 1)] = (int16_t)q;
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < L_T / 64; k++) {
+            const int t = S.wsum[k];
+            S.wsum[k] = acc;
+            acc += t;
+        }
+        S.wsum[L_T / 64] = acc;
     }
     __syncthreads();
-    const int nact = S.nact;
-    int G = 64;
-    while (G > 1 && G * nact > L_T) G >>= 1;
-    const int lane = tid & 63, gl = tid % G, grp = tid / G, ngrp = L_T / G, base = tid - gl;
-    const uint64_t gmask = (G == 64 ? ~0ull : ((1ull << G) - 1)) << (lane - gl);
-    const double spp = powi(RSHELL, 2 * (d + 1)), spm = powi(RSHELL, 2 * d);
-    for (int ai = grp; ai < nact; ai += ngrp) {
-        const int q = S.act[ai];
+    const int r = S.wsum[w] + incl - v;
+    *tot = S.wsum[L_T / 64];
+    __syncthreads();
+    return r;
+}
+
+// whether entry u (pl entries, then test-particle entries) takes part in the
+// shell kicks of level d: its body and its partner are both in the level
+__device__ inline bool lds_entry_in_level(const LdsSet &S, int u, int nE, int d)
+{
+    if (u < nE) {
+        const uint32_t e = S.pE[u];
+        const int lp = (int)(e >> 10), pr = (int)(e & 511u);
+        const int q = ((e >> 9) & 1u) ? S.ppi[lp] : S.ppj[lp];
+        return S.plev[q] >= d && S.plev[pr] >= d;
+    }
+    const int v = u - nE;
+    return S.tlev[S.tpt[v]] >= d && S.plev[S.tpl[v]] >= d;
+}
+
+// The kick list of level d: the entries that take part, in entry order
+// (grouped by body, since each body's entries are contiguous), and the bodies
+// that have any, with their stretch of the list.  Which bodies and entries
+// take part depends only on the levels being >= d, and no body's level
+// crosses d while level d is being stepped (the list checks only raise levels
+// above d, and the level reset only lowers d+1 back to d), so the list is
+// built when level d is entered or resumed and serves all its kicks.  Pair
+// statuses can change (collisions) and are read when a term is evaluated.
+__device__ void lds_level_list(LdsSet &S, int nB, int nBt, int nE, int nEt, int d)
+{
+    const int tid = threadIdx.x, N = nE + nEt;
+    const int seg = (N + L_T - 1) / L_T, u0 = min(N, tid * seg), u1 = min(N, u0 + seg);
+    int cnt = 0;
+    for (int u = u0; u < u1; u++) {
+        const bool k = lds_entry_in_level(S, u, nE, d);
+        S.posu[u] = (int16_t)k;
+        cnt += k;
+    }
+    int tot;
+    int pos = lds_block_scan(S, cnt, &tot);
+    for (int u = u0; u < u1; u++) {
+        const bool k = S.posu[u] != 0;
+        S.posu[u] = (int16_t)pos;
+        if (k) S.lst[pos++] = (int16_t)u;
+    }
+    if (tid == 0) {
+        S.nact = 0;
+        S.nlst = tot;
+    }
+    __syncthreads();
+    for (int q = tid; q < nB + nBt; q += L_T) {
         const bool ispl = q < nB;
-        const int t = q - nB;
-        const int e0 = ispl ? S.pO[q] : S.tO[t], e1 = ispl ? S.pO[q + 1] : S.tO[t + 1];
-        const double xb = ispl ? S.px[q] : S.tx[t], yb = ispl ? S.py[q] : S.ty[t], zb = ispl ? S.pz[q] : S.tz[t];
-        const double hb = ispl ? S.ph[q] : 0.0;
-        double3 p = make_double3(0.0, 0.0, 0.0), m = p;
-        bool anyp = false, anym = false;
-        for (int c0 = e0; c0 < e1; c0 += G) {
-            const int u = c0 + gl;
-            ShellTerm tm;
-            tm.tp = tm.tm = false;
-            tm.dx = tm.dy = tm.dz = tm.fp = tm.fm = 0.0;
-            if (u < e1) {
-                if (ispl) {
-                    const uint32_t e = S.pE[u];
-                    const int pr = (int)(e & 511u);
-                    if (S.plev[pr] >= d && S.pps[e >> 10] == ACTIVE)
-                        pl_shell_term_v((e >> 9) & 1u, xb, yb, zb, hb, S.px[pr], S.py[pr], S.pz[pr], S.ph[pr],
-                                        S.pgm[pr], spp, spm, true, tm);
-                } else {
-                    const int i = S.tpl[u];
-                    if (S.plev[i] >= d && S.tps[u] == ACTIVE)
-                        tp_shell_term_v(xb, yb, zb, S.px[i], S.py[i], S.pz[i], S.ph[i], S.pgm[i], spp, spm, true, tm);
-                }
-            }
-            const uint64_t live = __ballot(tm.tp || tm.tm) & gmask;
-            if (tm.tp || tm.tm) {
-                S.kfl[tid] = (tm.tp ? 1 : 0) | (tm.tm ? 2 : 0);
-                S.kdx[tid] = tm.dx;
-                S.kdy[tid] = tm.dy;
-                S.kdz[tid] = tm.dz;
-                S.kfp[tid] = tm.fp;
-                S.kfm[tid] = tm.fm;
-            }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            if (gl == 0) {
-                uint64_t bits = live >> lane;  // the group's lanes, in entry order
-                while (bits) {
-                    const int k = __builtin_ctzll(bits);
-                    bits &= bits - 1;
-                    const int w = base + k;
-                    const int fl = S.kfl[w];
-                    if (fl & 1) {
-                        anyp = true;
-                        p.x = fma(S.kfp[w], S.kdx[w], p.x);
-                        p.y = fma(S.kfp[w], S.kdy[w], p.y);
-                        p.z = fma(S.kfp[w], S.kdz[w], p.z);
-                    }
-                    if (fl & 2) {
-                        anym = true;
-                        m.x = fma(S.kfm[w], S.kdx[w], m.x);
-                        m.y = fma(S.kfm[w], S.kdy[w], m.y);
-                        m.z = fma(S.kfm[w], S.kdz[w], m.z);
-                    }
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
+        if ((ispl ? S.plev[q] : S.tlev[q - nB]) < d) continue;
+        const int e0 = ispl ? S.pO[q] : nE + S.tO[q - nB], e1 = ispl ? S.pO[q + 1] : nE + S.tO[q - nB + 1];
+        if (e1 <= e0) continue;
+        const int f0 = S.posu[e0], f1 = e1 < N ? S.posu[e1] : tot;
+        if (f1 <= f0) continue;
+        const int ai = atomicAdd(&S.nact, 1);
+        S.act[ai] = (int16_t)q;
+        S.kst[ai] = (int16_t)f0;
+        S.ken[ai] = (int16_t)f1;
+    }
+    if (tid == 0) S.lst_level = d;
+    __syncthreads();
+}
+
+// The +1 / -1 shell kicks of level d (the sums of rec_kick_pl / rec_kick_tp).
+// The level's kick list is taken in chunks of L_KC entries: all threads
+// evaluate the chunk's terms at once into LDS, then every listed body folds
+// the terms of its own entries in the chunk, in entry order, on one lane.  The
+// running sums stay in registers across chunks (a lane serves up to L_KS
+// bodies), so each body's sum is the serial sum over its entries.  Entries
+// left out of the list (partner below the level) contribute no term, and a
+// body with no listed entry gets no kick, as in the reference.
+__device__ void lds_kicks(LdsSet &S, int nB, int nBt, int nE, int nEt, int d, double dth, bool prof)
+{
+    const int tid = threadIdx.x;
+    if (S.lst_level != d) {
+        lds_level_list(S, nB, nBt, nE, nEt, d);
+        if (prof && tid == 0) S.prof[d * NPROF + 14] += 1;
+    }
+    const int nact = S.nact, nl = S.nlst;
+    const bool pr = prof && tid == 0;
+    unsigned long long tc = 0, tf = 0, tt = pr ? wall_clock64() : 0;
+    const double spp = powi(RSHELL, 2 * (d + 1)), spm = powi(RSHELL, 2 * d);
+    int fe0[L_KS], fe1[L_KS], kmax = 0;
+    double3 p[L_KS], m[L_KS];
+    bool anyp[L_KS], anym[L_KS];
+#pragma unroll
+    for (int k = 0; k < L_KS; k++) {
+        const int ai = tid + k * L_T;
+        fe0[k] = fe1[k] = 0;
+        if (ai < nact) {
+            fe0[k] = S.kst[ai];
+            fe1[k] = S.ken[ai];
+            kmax = max(kmax, fe1[k] - fe0[k]);
         }
-        if (gl == 0) {
-            if (ispl) lds_apply(S.pvx[q], S.pvy[q], S.pvz[q], dth, anyp, p, anym, m);
-            else lds_apply(S.tvx[t], S.tvy[t], S.tvz[t], dth, anyp, p, anym, m);
+        p[k] = m[k] = make_double3(0.0, 0.0, 0.0);
+        anyp[k] = anym[k] = false;
+    }
+    for (int c0 = 0; c0 < nl; c0 += L_KC) {
+        for (int w = tid; w < L_KC && c0 + w < nl; w += L_T) {
+            const int u = S.lst[c0 + w];
+            ShellTerm t;
+            t.tp = t.tm = false;
+            t.dx = t.dy = t.dz = 0.0;
+            if (u < nE) {
+                const uint32_t e = S.pE[u];
+                const int lp = (int)(e >> 10), pr = (int)(e & 511u);
+                const bool bi = (e >> 9) & 1u;
+                const int q = bi ? S.ppi[lp] : S.ppj[lp];
+                if (S.pps[lp] == ACTIVE)
+                    pl_shell_term_v(bi, S.px[q], S.py[q], S.pz[q], S.ph[q], S.px[pr], S.py[pr], S.pz[pr], S.ph[pr],
+                                    S.pgm[pr], spp, spm, true, t);
+            } else {
+                const int v = u - nE, i = S.tpl[v], tt = S.tpt[v];
+                if (S.tps[v] == ACTIVE)
+                    tp_shell_term_v(S.tx[tt], S.ty[tt], S.tz[tt], S.px[i], S.py[i], S.pz[i], S.ph[i], S.pgm[i], spp,
+                                    spm, true, t);
+            }
+            KTerm &K = S.kt[w];
+            K.dx = t.dx;
+            K.dy = t.dy;
+            K.dz = t.dz;
+            K.fp = t.tp ? t.fp : 0.0;
+            K.fm = t.tm ? t.fm : 0.0;
+            K.fl = (t.tp ? 1 : 0) | (t.tm ? 2 : 0);
+        }
+        __syncthreads();
+        if (pr) {
+            const unsigned long long now = wall_clock64();
+            tc += now - tt;
+            tt = now;
+        }
+#pragma unroll
+        for (int k = 0; k < L_KS; k++) {
+            const int hi = min(fe1[k], c0 + L_KC);
+#pragma unroll 4
+            for (int u = max(fe0[k], c0); u < hi; u++) {
+                const KTerm &K = S.kt[u - c0];
+                const int fl = K.fl;
+                const double px = fma(K.fp, K.dx, p[k].x), py = fma(K.fp, K.dy, p[k].y),
+                             pz = fma(K.fp, K.dz, p[k].z);
+                const double mx = fma(K.fm, K.dx, m[k].x), my = fma(K.fm, K.dy, m[k].y),
+                             mz = fma(K.fm, K.dz, m[k].z);
+                if (fl & 1) p[k] = make_double3(px, py, pz);
+                if (fl & 2) m[k] = make_double3(mx, my, mz);
+                anyp[k] |= (fl & 1) != 0;
+                anym[k] |= (fl & 2) != 0;
+            }
+        }
+        __syncthreads();
+        if (pr) {
+            const unsigned long long now = wall_clock64();
+            tf += now - tt;
+            tt = now;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < L_KS; k++) {
+        const int ai = tid + k * L_T;
+        if (ai >= nact) continue;
+        const int q = S.act[ai];
+        if (q < nB) lds_apply(S.pvx[q], S.pvy[q], S.pvz[q], dth, anyp[k], p[k], anym[k], m[k]);
+        else lds_apply(S.tvx[q - nB], S.tvy[q - nB], S.tvz[q - nB], dth, anyp[k], p[k], anym[k], m[k]);
+    }
+    if (prof) {
+        if (tid == 0) S.kmax = 0;
+        __syncthreads();
+        if (kmax) atomicMax(&S.kmax, kmax);
+        __syncthreads();
+        if (pr) {
+            unsigned long long *P = S.prof + d * NPROF;
+            P[8] += tc;
+            P[9] += tf;
+            P[10] += S.kmax;
+            P[11] += nact;
         }
     }
     __syncthreads();
 }
 
-__device__ void lds_drift(LdsSet &S, const SubArgs &A, int nB, int nBt, int d, double dtl, int s)
+__device__ void lds_drift(LdsSet &S, const SubArgs &A, int nB, int nBt, int d, double dtl, int s, bool prof)
 {
     const double mu = A.st.sy.GMcb[s];
+    int ndr = 0;
+    unsigned long long tdr = 0;
     for (int q = threadIdx.x; q < nB + nBt; q += L_T) {
         const bool ispl = q < nB;
         const int t = q - nB;
         const bool on = ispl ? (S.pst[q] != INACTIVE && S.plev[q] == d) : (S.tst[t] == ACTIVE && S.tlev[t] == d);
         if (!on) continue;
+        ndr++;
         double *x = ispl ? S.px : S.tx, *y = ispl ? S.py : S.ty, *z = ispl ? S.pz : S.tz;
         double *vx = ispl ? S.pvx : S.tvx, *vy = ispl ? S.pvy : S.tvy, *vz = ispl ? S.pvz : S.tvz;
         const int k = ispl ? q : t;
         double rx = x[k], ry = y[k], rz = z[k], ux = vx[k], uy = vy[k], uz = vz[k];
+        const unsigned long long t0 = prof ? wall_clock64() : 0;
         const int fl = drift_one(mu, rx, ry, rz, ux, uy, uz, dtl);
+        if (prof) tdr += wall_clock64() - t0;
         x[k] = rx;
         y[k] = ry;
         z[k] = rz;
@@ -636,6 +769,12 @@ __device__ void lds_drift(LdsSet &S, const SubArgs &A, int nB, int nBt, int d, d
                 S.tst[t] = DISCARDED_DRIFTERR;
             }
         }
+    }
+    if (prof && ndr) {
+        unsigned long long *P = S.prof + d * NPROF;
+        atomicAdd(&P[12], (unsigned long long)ndr);
+        if (ndr > 1) atomicAdd(&P[13], 1ull);
+        atomicAdd(&P[15], tdr);
     }
     __syncthreads();
 }
@@ -772,6 +911,7 @@ __global__ __launch_bounds__(L_T) void k_subtree_lds(const SubArgs *__restrict__
         S.tO[nBt] = nEt;
         S.npair = 0;
         S.bad = 0;
+        S.lst_level = -1;
     }
     if (A.prof)
         for (int k = tid; k < (MAXLEV + 2) * NPROF; k += L_T) S.prof[k] = 0;
@@ -930,9 +1070,9 @@ __global__ __launch_bounds__(L_T) void k_subtree_lds(const SubArgs *__restrict__
             unsigned long long t0 = prof ? wall_clock64() : 0, t1 = 0, t2 = 0, t3 = 0;
             const bool lenc = lds_check(S, A, nB, nEt, d, dtl);
             if (prof) t1 = wall_clock64();
-            lds_kicks(S, nB, nBt, nE, nEt, d, dth);
+            lds_kicks(S, nB, nBt, nE, nEt, d, dth, A.prof != nullptr);
             if (prof) t2 = wall_clock64();
-            lds_drift(S, A, nB, nBt, d, dtl, s);
+            lds_drift(S, A, nB, nBt, d, dtl, s, A.prof != nullptr);
             if (prof) {
                 t3 = wall_clock64();
                 unsigned long long *P = S.prof + d * NPROF;
@@ -948,7 +1088,10 @@ __global__ __launch_bounds__(L_T) void k_subtree_lds(const SubArgs *__restrict__
                     if (tid == 0) atomicOr(&a.sy.err[s], 8);
                     break;
                 }
-                if (tid == 0) S.jst[d] = j;
+                if (tid == 0) {
+                    S.jst[d] = j;
+                    S.lst_level = -1;
+                }
                 __syncthreads();
                 d++;
                 if (d > maxd) maxd = d;
@@ -960,7 +1103,7 @@ __global__ __launch_bounds__(L_T) void k_subtree_lds(const SubArgs *__restrict__
         {
             const bool prof = A.prof && tid == 0;
             unsigned long long t0 = prof ? wall_clock64() : 0;
-            lds_kicks(S, nB, nBt, nE, nEt, d, dth);
+            lds_kicks(S, nB, nBt, nE, nEt, d, dth, A.prof != nullptr);
             unsigned long long t1 = prof ? wall_clock64() : 0;
             if (A.lclose) lds_collision(S, A, nB, nBt, nEt, d, dtl, s);
             lds_set_levels(S, nB, nBt, nEt, d);
@@ -994,13 +1137,16 @@ void subtree_profile_report(hipStream_t st)
     (void)hipStreamSynchronize(st);
     (void)hipMemcpy(h, g_prof, sizeof h, hipMemcpyDeviceToHost);
     const double tick_us = 0.01;  // wall_clock64 runs at 100 MHz
-    fprintf(stderr, "[swg] level substeps   avg_nB   avg_nE  check_ms  kick1_ms  drift_ms kick2/bld levels_ms\n");
+    fprintf(stderr, "[swg] level substeps   avg_nB   avg_nE  check_ms  kick1_ms  drift_ms kick2/bld levels_ms"
+                    "   eval_ms   fold_ms kick_nmax kick_nact  drifted drift>1 lists drift1_us\n");
     for (int d = 1; d <= MAXLEV; d++) {
         const unsigned long long *p = h + d * NPROF;
         if (!p[0]) continue;
-        fprintf(stderr, "[swg] %5d %8llu %8.0f %8.0f %9.2f %9.2f %9.2f %9.2f %9.2f\n", d, p[0], (double)p[1] / p[0],
-                (double)p[2] / p[0], p[3] * tick_us * 1e-3, p[4] * tick_us * 1e-3, p[5] * tick_us * 1e-3,
-                p[6] * tick_us * 1e-3, p[7] * tick_us * 1e-3);
+        fprintf(stderr, "[swg] %5d %8llu %8.0f %8.0f %9.2f %9.2f %9.2f %9.2f %9.2f %9.2f %9.2f %9.1f %9.1f %8.2f %7llu %5llu %9.3f\n",
+                d, p[0], (double)p[1] / p[0], (double)p[2] / p[0], p[3] * tick_us * 1e-3, p[4] * tick_us * 1e-3,
+                p[5] * tick_us * 1e-3, p[6] * tick_us * 1e-3, p[7] * tick_us * 1e-3, p[8] * tick_us * 1e-3,
+                p[9] * tick_us * 1e-3, p[10] / (2.0 * p[0]), p[11] / (2.0 * p[0]), (double)p[12] / p[0], p[13],
+                p[14], p[12] ? p[15] * tick_us / p[12] : 0.0);
     }
     const unsigned long long *c = h + (MAXLEV + 1) * NPROF;
     if (c[0])
