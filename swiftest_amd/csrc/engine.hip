@@ -778,7 +778,9 @@ int Ensemble::run_phase(int half, const std::function<int()> &body)
     hipGraphExec_t &ex = graph_exec[half * 4 + (npp > 0) * 2 + (npt > 0)];
     if (!ex) {
         hipGraph_t g = nullptr;
-        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        // relaxed: the capture holds kernel launches only, and other host
+        // threads (other contexts) may allocate or copy meanwhile
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
         int rc = body();
         hipError_t e = hipStreamEndCapture(stream, &g);
         if (rc) {
