@@ -156,6 +156,34 @@ int swg_system_time_encounters(swg_system *sys, int32_t reps, double *ms, int64_
  * its own count.  HELIO and SyMBA only.  No counterpart in the reference
  * (one system per run there). */
 int swg_system_set_frozen(swg_system *sys, const int32_t *frozen);
+
+/* Energy and angular momentum of every system, as the reference's
+ * conservation report computes them: swiftest_util_coord_rh2rb_pl
+ * (src/swiftest/swiftest_util.f90:488-518), then
+ * swiftest_util_get_energy_and_momentum_system (:1172-1291) with the
+ * triangular potential (:1350-1400), rotation and oblateness off.  WHM
+ * systems first get vb from vh (swiftest_util_coord_vh2vb_pl, :424-459), as
+ * swiftest_io_conservation_report does (src/swiftest/swiftest_io.f90:
+ * 135-141); HELIO and SyMBA use the barycentric velocities they hold.
+ *   cb_mass, cb_radius  per system (the reference's cb%mass and cb%radius)
+ *   pl_mass             per stored body (pl%mass; Gmass = GU * mass)
+ *   out                 nsys * SWG_ENERGY_NOUT doubles, fields SWG_E_*
+ * The binding energy be is formed only with CHK_CLOSE (prm.lclose).  Sums run
+ * in tree order, so the results agree with the reference's serial sums to
+ * rounding. */
+#define SWG_E_KE_ORBIT 0
+#define SWG_E_KE_SPIN 1
+#define SWG_E_PE 2
+#define SWG_E_BE 3
+#define SWG_E_BE_CB 4
+#define SWG_E_TE 5
+#define SWG_E_L_ORBIT 6  /* 3 components */
+#define SWG_E_L_SPIN 9   /* 3 components */
+#define SWG_E_L_TOTAL 12 /* 3 components */
+#define SWG_E_GMTOT 15
+#define SWG_ENERGY_NOUT 16
+int swg_system_energy(swg_system *sys, const double *cb_mass, const double *cb_radius, const double *pl_mass,
+                      double *out);
 /* Autotuner: time the pl-tp and pl-pl acceleration kernels at each
  * workgroup size (64, 128, 256 bodies per tile) on the current state (no state
  * change) and keep the fastest for the following steps.  Reports the choice.

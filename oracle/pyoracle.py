@@ -75,6 +75,10 @@ def lib(omp: bool = False):
         L.swo_sys_ntp.argtypes = [vp]
         L.swo_sys_list.argtypes = [vp, C.c_int, vp, vp, vp]
         L.swo_sys_list.restype = C.c_int64
+        L.swo_norm2.argtypes = [C.c_double] * 3
+        L.swo_norm2.restype = C.c_double
+        L.swo_coord_vh2vb.argtypes = [C.c_int, C.c_double, D, D, D, D]
+        L.swo_energy.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, D, D, D, D, D, D, I32, C.c_int, D, vp]
         _LIBS[key] = L
     return _LIBS[key]
 
@@ -241,3 +245,34 @@ class OracleSystem:
         a, b, c = (np.zeros(n, np.int32) for _ in range(3))
         self.L.swo_sys_list(self.h, which, _p(a), _p(b), _p(c))
         return a, b, c
+
+
+# ---- energy and angular momentum ---------------------------------------
+
+ENERGY_NOUT = 16
+
+
+def norm2(x, y, z):
+    return lib().swo_norm2(float(x), float(y), float(z))
+
+
+def energy(GMcb, mcb, Rcb, cb_vb, rh, vb, Gmass, mass, radius=None, status=None, lclose=False):
+    """swo_energy: the 16-field energy record of one system (and the
+    barycentric positions)."""
+    rh = _c(rh).reshape(-1, 3)
+    n = rh.shape[0]
+    st = np.zeros(n, np.int32) if status is None else _c(status, np.int32)
+    rad = np.ones(n) if radius is None else _c(radius)
+    out = np.zeros(ENERGY_NOUT)
+    rb = np.zeros((max(n, 1), 3))
+    lib().swo_energy(n, float(GMcb), float(mcb), float(Rcb), _c(cb_vb), rh, _c(vb).reshape(-1, 3), _c(Gmass),
+                     _c(mass), rad, st, int(bool(lclose)), out, _p(rb))
+    return out, rb[:n]
+
+
+def vh2vb(GMcb, Gmass, vh):
+    """swo_coord_vh2vb: (vb, cb_vb)."""
+    vh = _c(vh).reshape(-1, 3)
+    vb, cbv = np.zeros_like(vh), np.zeros(3)
+    lib().swo_coord_vh2vb(vh.shape[0], float(GMcb), _c(Gmass), vh, vb, cbv)
+    return vb, cbv

@@ -814,19 +814,21 @@ static void set_renc(swo_sys *s, int scale)
 }
 
 /* swiftest_util_coord_vh2vb_pl, swiftest_util.f90:424-459 */
-static void vh2vb_pl(swo_sys *s)
+void swo_coord_vh2vb(int npl, double GMcb, const double *Gmass, const double *vh, double *vb, double *cb_vb)
 {
-    if (s->npl == 0) return;
+    if (npl == 0) return;
     double sm = 0.0;
-    for (int i = 0; i < s->npl; i++) sm = sm + s->Gmass[i];
-    double Gmtot = s->GMcb + sm;
+    for (int i = 0; i < npl; i++) sm = sm + Gmass[i];
+    double Gmtot = GMcb + sm;
     double c[3] = {0.0, 0.0, 0.0};
-    for (int i = 0; i < s->npl; i++)
-        for (int d = 0; d < 3; d++) c[d] = c[d] - s->Gmass[i] * s->vh[3 * i + d];
-    for (int d = 0; d < 3; d++) s->cb_vb[d] = c[d] / Gmtot;
-    for (int i = 0; i < s->npl; i++)
-        for (int d = 0; d < 3; d++) s->vb[3 * i + d] = s->vh[3 * i + d] + s->cb_vb[d];
+    for (int i = 0; i < npl; i++)
+        for (int d = 0; d < 3; d++) c[d] = c[d] - Gmass[i] * vh[3 * i + d];
+    for (int d = 0; d < 3; d++) cb_vb[d] = c[d] / Gmtot;
+    for (int i = 0; i < npl; i++)
+        for (int d = 0; d < 3; d++) vb[3 * i + d] = vh[3 * i + d] + cb_vb[d];
 }
+
+static void vh2vb_pl(swo_sys *s) { swo_coord_vh2vb(s->npl, s->GMcb, s->Gmass, s->vh, s->vb, s->cb_vb); }
 
 /* swiftest_util_coord_vb2vh_pl, swiftest_util.f90:363-395 (reverse order) */
 static void vb2vh_pl(swo_sys *s)
@@ -1906,4 +1908,111 @@ int64_t swo_sys_list(const swo_sys *s, int which, int32_t *index1, int32_t *inde
     if (index2) memcpy(index2, l->i2, l->n * sizeof(int32_t));
     if (lvdotr) memcpy(lvdotr, l->lvdotr, l->n * sizeof(int32_t));
     return l->n;
+}
+
+/* ======================================================================
+ * Energy and angular momentum
+ * ====================================================================== */
+
+/* norm2 (Fortran 2008 intrinsic) of a 3-vector.  gfortran evaluates it as a
+ * scaled sum of squares: scale = max |x| so far, result accumulates
+ * (x/scale)^2 and is rescaled when a larger component appears; the norm is
+ * scale * sqrt(result). */
+double swo_norm2(double x, double y, double z)
+{
+    const double c[3] = {x, y, z};
+    double scale = 1.0, result = 0.0;
+    for (int k = 0; k < 3; k++) {
+        if (c[k] == 0.0) continue;
+        const double ax = fabs(c[k]);
+        if (scale < ax) {
+            const double val = scale / ax;
+            result = 1.0 + result * val * val;
+            scale = ax;
+        } else {
+            const double val = ax / scale;
+            result += val * val;
+        }
+    }
+    return scale * sqrt(result);
+}
+
+void swo_energy(int npl, double GMcb, double mcb, double Rcb, const double *cb_vb, const double *rh,
+                const double *vb, const double *Gmass, const double *mass, const double *radius,
+                const int32_t *status, int lclose, double *out, double *rb_out)
+{
+    double *rb = (double *)calloc(3 * (size_t)(npl > 0 ? npl : 1), sizeof(double));
+    /* swiftest_util_coord_rh2rb_pl (swiftest_util.f90:505-516) */
+    double cb_rb[3] = {0.0, 0.0, 0.0};
+    if (npl > 0) {
+        double Gmtot = GMcb, xtmp[3] = {0.0, 0.0, 0.0};
+        for (int i = 0; i < npl; i++) {
+            if (status[i] == SWO_INACTIVE) continue;
+            Gmtot = Gmtot + Gmass[i];
+            for (int d = 0; d < 3; d++) xtmp[d] = xtmp[d] + Gmass[i] * rh[3 * i + d];
+        }
+        for (int d = 0; d < 3; d++) cb_rb[d] = -xtmp[d] / Gmtot;
+        for (int i = 0; i < npl; i++) {
+            if (status[i] == SWO_INACTIVE) continue;
+            for (int d = 0; d < 3; d++) rb[3 * i + d] = rh[3 * i + d] + cb_rb[d];
+        }
+    }
+    /* swiftest_util_get_energy_and_momentum_system (:1192-1288) */
+    double GMtot = GMcb;
+    for (int i = 0; i < npl; i++)
+        if (status[i] != SWO_INACTIVE) GMtot = GMtot + Gmass[i];
+    const double kecb = mcb * dot3(cb_vb[0], cb_vb[1], cb_vb[2], cb_vb[0], cb_vb[1], cb_vb[2]);
+    const double be_cb = -(3.0 * GMcb * mcb) / (5.0 * Rcb);
+    double Lcb[3];
+    Lcb[0] = mcb * (cb_rb[1] * cb_vb[2] - cb_rb[2] * cb_vb[1]);
+    Lcb[1] = mcb * (cb_rb[2] * cb_vb[0] - cb_rb[0] * cb_vb[2]);
+    Lcb[2] = mcb * (cb_rb[0] * cb_vb[1] - cb_rb[1] * cb_vb[0]);
+    double kesum = 0.0, Lsum[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < npl; i++) {
+        if (status[i] == SWO_INACTIVE) continue;
+        const double *r = rb + 3 * i, *v = vb + 3 * i;
+        const double h0 = r[1] * v[2] - r[2] * v[1];
+        const double h1 = r[2] * v[0] - r[0] * v[2];
+        const double h2 = r[0] * v[1] - r[1] * v[0];
+        Lsum[0] = Lsum[0] + mass[i] * h0;
+        Lsum[1] = Lsum[1] + mass[i] * h1;
+        Lsum[2] = Lsum[2] + mass[i] * h2;
+        kesum = kesum + mass[i] * dot3(v[0], v[1], v[2], v[0], v[1], v[2]);
+    }
+    /* swiftest_util_get_potential_energy_triangular (:1350-1400) */
+    double pe = 0.0;
+    for (int i = 0; i < npl; i++) {
+        if (status[i] == SWO_INACTIVE) continue;
+        double row = 0.0;
+        for (int j = i + 1; j < npl; j++) {
+            if (status[j] == SWO_INACTIVE) continue;
+            const double dr = swo_norm2(rb[3 * i] - rb[3 * j], rb[3 * i + 1] - rb[3 * j + 1],
+                                        rb[3 * i + 2] - rb[3 * j + 2]);
+            row = row + (-(Gmass[i] * mass[j]) / dr);
+        }
+        pe = pe + row;
+    }
+    double pecb = 0.0;
+    for (int i = 0; i < npl; i++) {
+        if (status[i] == SWO_INACTIVE) continue;
+        pecb = pecb + (-(GMcb * mass[i]) / swo_norm2(rb[3 * i], rb[3 * i + 1], rb[3 * i + 2]));
+    }
+    pe = pe + pecb;
+    double be = 0.0;
+    if (lclose)
+        for (int i = 0; i < npl; i++)
+            if (status[i] != SWO_INACTIVE) be = be + (-(3.0 * Gmass[i] * mass[i]) / (5.0 * radius[i]));
+    for (int k = 0; k < SWO_ENERGY_NOUT; k++) out[k] = 0.0;
+    out[SWO_E_KE_ORBIT] = 0.5 * (kecb + kesum);
+    out[SWO_E_PE] = pe;
+    out[SWO_E_BE] = be;
+    out[SWO_E_BE_CB] = be_cb;
+    out[SWO_E_TE] = out[SWO_E_KE_ORBIT] + out[SWO_E_KE_SPIN] + pe + be;
+    for (int d = 0; d < 3; d++) {
+        out[SWO_E_L_ORBIT + d] = Lcb[d] + Lsum[d];
+        out[SWO_E_L_TOTAL + d] = out[SWO_E_L_ORBIT + d] + out[SWO_E_L_SPIN + d];
+    }
+    out[SWO_E_GMTOT] = GMtot;
+    if (rb_out) memcpy(rb_out, rb, 3 * (size_t)npl * sizeof(double));
+    free(rb);
 }

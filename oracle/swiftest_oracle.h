@@ -125,6 +125,33 @@ int swo_sys_ntp(const swo_sys *s);
 /* Encounter lists produced by the last step's level-0 checks (1-based). */
 int64_t swo_sys_list(const swo_sys *s, int which, int32_t *index1, int32_t *index2, int32_t *lvdotr);
 
+/* ---- energy and angular momentum (output cadence) ---- */
+/* Fields of the energy record, as swiftest_nbody_system stores them. */
+#define SWO_E_KE_ORBIT 0
+#define SWO_E_KE_SPIN 1
+#define SWO_E_PE 2
+#define SWO_E_BE 3
+#define SWO_E_BE_CB 4
+#define SWO_E_TE 5
+#define SWO_E_L_ORBIT 6  /* 3 components */
+#define SWO_E_L_SPIN 9   /* 3 components */
+#define SWO_E_L_TOTAL 12 /* 3 components */
+#define SWO_E_GMTOT 15
+#define SWO_ENERGY_NOUT 16
+/* swiftest_util_coord_vh2vb_pl (swiftest_util.f90:424-459): barycentric
+ * velocities of the massive bodies and of the central body from vh. */
+void swo_coord_vh2vb(int npl, double GMcb, const double *Gmass, const double *vh, double *vb, double *cb_vb);
+/* Fortran norm2 of a 3-vector as gfortran evaluates it (scaled sum of squares). */
+double swo_norm2(double x, double y, double z);
+/* swiftest_util_coord_rh2rb_pl (swiftest_util.f90:488-518) followed by
+ * swiftest_util_get_energy_and_momentum_system (:1172-1291) with the
+ * triangular potential (:1350-1400); rotation and oblateness off.  The
+ * velocities are the barycentric ones the integrator holds (vb, cb_vb).
+ * rb (npl x 3, optional) receives the barycentric positions. */
+void swo_energy(int npl, double GMcb, double mcb, double Rcb, const double *cb_vb, const double *rh,
+                const double *vb, const double *Gmass, const double *mass, const double *radius,
+                const int32_t *status, int lclose, double *out, double *rb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,6 +22,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libswiftest_hip.so")
 
 WHM, HELIO, SYMBA = 0, 1, 2
+# fields of an energy record (SWG_E_* in include/swiftest_hip.h)
+ENERGY_FIELDS = ("ke_orbit", "ke_spin", "pe", "be", "be_cb", "te", "L_orbit_x", "L_orbit_y", "L_orbit_z",
+                 "L_spin_x", "L_spin_y", "L_spin_z", "L_total_x", "L_total_y", "L_total_z", "GMtot")
+ENERGY_NOUT = len(ENERGY_FIELDS)
 ERRORS = {-1: "argument", -2: "hip", -3: "collision", -4: "drift", -5: "depth"}
 
 
@@ -82,6 +86,7 @@ def library():
         L.swg_system_kernel_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                               C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.swg_system_destroy.argtypes = [vp]
+        L.swg_system_energy.argtypes = [vp, D, D, D, D]
         _lib = L
     return _lib
 
@@ -212,6 +217,8 @@ class GpuSystem:
                                         _c(np.concatenate([s.radius for s in systems])), _p(rh),
                                         cat("tp_rh", 3), cat("tp_vh", 3), C.byref(h)))
         self.h = h
+        self._cb = (np.array([s.GMcb for s in systems], np.float64), np.array([s.Rcb for s in systems], np.float64))
+        self._gm_in = [np.asarray(s.Gmass, np.float64) for s in systems]
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -272,6 +279,29 @@ class GpuSystem:
         ms, npairs, ncand = C.c_double(), C.c_int64(), C.c_int64()
         _check(self.L.swg_system_time_encounters(self.h, int(reps), C.byref(ms), C.byref(npairs), C.byref(ncand)))
         return ms.value, npairs.value, ncand.value
+
+    def energy(self, cb_mass=None, cb_radius=None, pl_mass=None, GU=None):
+        """Energy and angular momentum of every system (swg_system_energy):
+        an (nsys, ENERGY_NOUT) array with the fields named in ENERGY_FIELDS.
+        Masses default to Gmass / GU (GU: G in the system's units, by default
+        that of swiftest_amd.ics), pl_mass given in the stored body order."""
+        from .ics import GU as GU_DEFAULT
+        gu = GU_DEFAULT if GU is None else GU
+        if cb_mass is None:
+            cb_mass = self._cb[0] / gu
+        if cb_radius is None:
+            cb_radius = self._cb[1]
+        if pl_mass is None:
+            ids = np.zeros(self.npl, np.int32)
+            _check(self.L.swg_system_get(self.h, *([None] * 8), _p(ids)))
+            pl_mass = np.zeros(self.npl)
+            o = 0
+            for s, n in enumerate(self.npl_s):
+                pl_mass[o:o + n] = self._gm_in[s][ids[o:o + n] - 1] / gu
+                o += n
+        out = np.zeros((self.nsys, ENERGY_NOUT))
+        _check(self.L.swg_system_energy(self.h, _c(cb_mass), _c(cb_radius), _c(pl_mass), out))
+        return out
 
     def set_frozen(self, frozen):
         """frozen[s] = 1 leaves system s out of the following steps."""

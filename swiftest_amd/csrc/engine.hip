@@ -16,6 +16,7 @@
 #include <chrono>
 #include <cstdlib>
 #include <functional>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -681,6 +682,37 @@ int Ensemble::run_subtree(int d0, const LevelView &v)
     return 0;
 }
 
+// Energy and angular momentum of every system (swg_system_energy).
+int Ensemble::energy(const double *cb_mass, const double *cb_radius, const double *pl_mass, double *out)
+{
+    hipStream_t st = stream;
+    if (!en_ready) {
+        std::vector<int4> tiles;
+        std::vector<int32_t> toff;
+        energy_tiles(nsys, h_pl_off.data(), tiles, toff);
+        en_ntiles = (int)tiles.size();
+        if (alloc_vec(en_rb, npl) || own(&en_mass, npl) || own(&en_cb, 2 * (size_t)nsys) ||
+            own(&en_part, (size_t)en_ntiles) || own(&en_out, (size_t)nsys * SWG_ENERGY_NOUT) ||
+            own(&en_tiles, (size_t)en_ntiles) || own(&en_toff, (size_t)nsys + 1))
+            return SWG_ERR_HIP;
+        if (en_ntiles)
+            HIPCHK(hipMemcpy(en_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(en_toff, toff.data(), toff.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        en_ready = true;
+    }
+    if (npl) HIPCHK(hipMemcpyAsync(en_mass, pl_mass, npl * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(en_cb, cb_mass, nsys * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(en_cb + nsys, cb_radius, nsys * sizeof(double), hipMemcpyHostToDevice, st));
+    // swiftest_io_conservation_report (swiftest_io.f90:135-141): WHM holds vh
+    if (prm.integrator == SWG_WHM) launch_vh2vb(st, pl, sy, nsys, npl);
+    launch_energy(st, pl, sy, nsys, en_mass, en_cb, en_cb + nsys, prm.lclose, en_rb, en_tiles, en_ntiles, en_toff,
+                  en_part, en_out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, en_out, (size_t)nsys * SWG_ENERGY_NOUT * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
 int Ensemble::check_errors()
 {
     int32_t *hf = (int32_t *)(h_small + 1024);
@@ -767,9 +799,17 @@ void Ensemble::drop_graphs()
         }
 }
 
+// Allocations, frees and synchronous copies of one context can invalidate a
+// graph capture in progress on another context's stream.  Captures and the
+// creation and destruction of systems take this lock, so they never overlap;
+// a capture that is invalidated all the same (by an allocation inside another
+// context's step) is dropped, and its launches, none of which ran, are issued
+// directly instead.
+static std::mutex g_capture_mu;
+
 int Ensemble::run_phase(int half, const std::function<int()> &body)
 {
-    if (!use_graphs || ktiming) return body();
+    if (!use_graphs || ktiming || capture_failures >= 4) return body();
     const uint64_t key = graph_key();
     if (key != graph_key_cached) {
         drop_graphs();
@@ -777,9 +817,9 @@ int Ensemble::run_phase(int half, const std::function<int()> &body)
     }
     hipGraphExec_t &ex = graph_exec[half * 4 + (npp > 0) * 2 + (npt > 0)];
     if (!ex) {
+        std::lock_guard<std::mutex> lk(g_capture_mu);
         hipGraph_t g = nullptr;
-        // relaxed: the capture holds kernel launches only, and other host
-        // threads (other contexts) may allocate or copy meanwhile
+        // relaxed: the capture holds kernel launches only
         HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
         int rc = body();
         hipError_t e = hipStreamEndCapture(stream, &g);
@@ -787,10 +827,16 @@ int Ensemble::run_phase(int half, const std::function<int()> &body)
             if (g) (void)hipGraphDestroy(g);
             return rc;
         }
-        if (e != hipSuccess) return hipfail(e, "hipStreamEndCapture");
-        e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) return hipfail(e, "hipGraphInstantiate");
+        if (e == hipSuccess) {
+            e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) ex = nullptr;
+        }
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            capture_failures++;
+            return body();
+        }
     }
     HIPCHK(hipGraphLaunch(ex, stream));
     return 0;
@@ -1196,6 +1242,7 @@ int swg_system_create(swg_ctx *ctx, const swg_params *prm, int32_t nsys, const i
     if (!ctx || !prm || !sys || !npl || !ntp || !GMcb) return fail(SWG_ERR_ARG, "NULL argument");
     *sys = nullptr;
     HIPCHK(hipSetDevice(ctx->device));
+    std::lock_guard<std::mutex> lk(swg::g_capture_mu);
     swg::Ensemble *e = new swg::Ensemble();
     e->stream = ctx->stream;
     int rc = e->create(*prm, nsys, npl, ntp, GMcb, pl_rh, pl_vh, Gmass, radius, rhill, tp_rh, tp_vh);
@@ -1259,6 +1306,16 @@ int swg_system_time_kernels(swg_system *sys, int32_t reps, double *ms_kick_tp, d
     return sys->e->time_kernels(reps, ms_kick_tp, ms_kick_pl);
 }
 
+int swg_system_energy(swg_system *sys, const double *cb_mass, const double *cb_radius, const double *pl_mass,
+                      double *out)
+{
+    if (!sys || !cb_mass || !cb_radius || !out || (!pl_mass && sys->e->npl))
+        return fail(SWG_ERR_ARG, "NULL argument");
+    for (int s = 0; s < sys->e->nsys; s++)
+        if (!(cb_radius[s] > 0.0)) return fail(SWG_ERR_ARG, "central body radius must be positive");
+    return sys->e->energy(cb_mass, cb_radius, pl_mass, out);
+}
+
 int swg_system_time_encounters(swg_system *sys, int32_t reps, double *ms, int64_t *npairs, int64_t *ncand)
 {
     if (!sys || reps <= 0) return fail(SWG_ERR_ARG, "bad argument");
@@ -1304,6 +1361,7 @@ int swg_system_kernel_times(swg_system *sys, double *ms_kick_tp, int64_t *n_kick
 int swg_system_destroy(swg_system *sys)
 {
     if (!sys) return 0;
+    std::lock_guard<std::mutex> lk(swg::g_capture_mu);
     delete sys->e;
     delete sys;
     return 0;

@@ -561,6 +561,11 @@ void launch_snapshot(hipStream_t st, PlDev pl, TpDev tp, SysDev sy, int nsys, in
     if (npl) k_vh2vb_apply<<<cdiv(npl, 256), 256, 0, st>>>(pl, sy, npl);
     if (ntp) k_snapshot_tp<<<cdiv(ntp, 256), 256, 0, st>>>(tp, sy, ntp);
 }
+void launch_vh2vb(hipStream_t st, PlDev pl, SysDev sy, int nsys, int npl)
+{
+    k_vh2vb_sum<<<nsys, SUM_T, 0, st>>>(pl, sy, nsys);
+    if (npl) k_vh2vb_apply<<<cdiv(npl, 256), 256, 0, st>>>(pl, sy, npl);
+}
 void launch_init_tp_vb(hipStream_t st, TpDev tp, SysDev sy, int ntp)
 {
     if (ntp) k_init_tp_vb<<<cdiv(ntp, 256), 256, 0, st>>>(tp, sy, ntp);

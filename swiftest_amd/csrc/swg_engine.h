@@ -94,10 +94,19 @@ struct Ensemble {
     int autotune(int reps, int32_t *pl_block, int32_t *tp_block);
     int set_frozen(const int32_t *frozen);
     int time_encounters(int reps, double *ms, int64_t *npairs, int64_t *ncand);
+    int energy(const double *cb_mass, const double *cb_radius, const double *pl_mass, double *out);
+    // energy scratch, allocated on the first energy call
+    bool en_ready = false;
+    Vec3p en_rb{};
+    double *en_mass = nullptr, *en_cb = nullptr, *en_part = nullptr, *en_out = nullptr;
+    int4 *en_tiles = nullptr;
+    int32_t *en_toff = nullptr;
+    int en_ntiles = 0;
     // HIP graphs of the step's two halves (see run_phase)
     bool use_graphs = false;
     hipGraphExec_t graph_exec[8] = {};
     uint64_t graph_key_cached = 0;
+    int capture_failures = 0;  // captures dropped (after 4, the halves always run directly)
     uint64_t graph_key() const;
     void drop_graphs();
     int run_phase(int half, const std::function<int()> &body);

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "swg_level.h"
 #include "swg_state.h"
 
@@ -37,6 +39,13 @@ void launch_tp_post(hipStream_t st, TpDev tp, SysDev sy, int ntp, double dth);
 void launch_lfirst_update(hipStream_t st, SysDev sy, int nsys);
 void launch_snapshot(hipStream_t st, PlDev pl, TpDev tp, SysDev sy, int nsys, int npl, int ntp);
 void launch_init_tp_vb(hipStream_t st, TpDev tp, SysDev sy, int ntp);
+
+// ---- energy and angular momentum (k_energy.hip) ----
+void energy_tiles(int nsys, const int32_t *pl_off, std::vector<int4> &tiles, std::vector<int32_t> &toff);
+void launch_energy(hipStream_t st, PlDev pl, SysDev sy, int nsys, const double *mass, const double *cbm,
+                   const double *cbR, int lclose, Vec3p rb, const int4 *tiles, int ntiles, const int32_t *toff,
+                   double *part, double *out);
+void launch_vh2vb(hipStream_t st, PlDev pl, SysDev sy, int nsys, int npl);
 
 // ---- encounter detection (k_enc.hip) ----
 void launch_pp_tiles(hipStream_t st, PlDev pl, SysDev sy, ListDev pp, const int4 *tiles, int ntiles,

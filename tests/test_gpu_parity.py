@@ -351,3 +351,42 @@ def test_ensemble_equals_single_runs():
             assert np.array_equal(so[k], se[k][off:off + n]), k
         po += s.npl
         to += s.ntp
+
+
+# ---- energy and angular momentum (swg_system_energy) ----------------------
+# Tolerance: the device sums in tree order and takes |r| as sqrt of the dot
+# product, the oracle follows the reference's serial sums and gfortran's
+# scaled norm2, so scalar fields agree to 1e-12 relative and the angular
+# momentum vector to 1e-12 of its norm.
+
+@pytest.mark.parametrize("integ", [S.SYMBA, S.HELIO, S.WHM])
+def test_energy_matches_oracle(integ):
+    print(f"energy integ={integ}", flush=True)
+    systems = [compact_disk(seed=31, npl=600, mhi=2e-6, alo=0.8, ahi=1.2), compact_disk(seed=32, npl=7),
+               sparse_system(seed=33, npl=9, ntp=20), compact_disk(seed=34, npl=2300, mhi=1e-6, alo=0.5, ahi=3.0)]
+    g = S.GpuSystem(integ, systems, 0.002, lclose=True)
+    g.run(3)
+    st = g.state()
+    rng = np.random.Generator(np.random.PCG64(35))
+    cbm = np.array([s.GMcb / ics.GU for s in systems]) * rng.uniform(0.9, 1.1, len(systems))
+    cbR = np.array([s.Rcb for s in systems])
+    gm = np.concatenate([s.Gmass[st["pl_id"][o:o + s.npl] - 1] for s, o in
+                         zip(systems, np.cumsum([0] + [s.npl for s in systems[:-1]]))])
+    mass = gm / ics.GU * rng.uniform(0.99, 1.01, gm.shape[0])
+    got = g.energy(cbm, cbR, mass)
+    assert np.array_equal(got, g.energy(cbm, cbR, mass))  # same bits every call
+    o = 0
+    for k, s in enumerate(systems):
+        sl = slice(o, o + s.npl)
+        if integ == S.WHM:
+            vb, cbv = O.vh2vb(s.GMcb, gm[sl], st["pl_vh"][sl])
+        else:
+            vb, cbv = st["pl_vb"][sl], st["cb_vb"][k]
+        ref, _ = O.energy(s.GMcb, cbm[k], cbR[k], cbv, st["pl_rh"][sl], vb, gm[sl], mass[sl], s.radius[st["pl_id"][sl] - 1],
+                          st["pl_status"][sl], lclose=True)
+        for f in (0, 2, 3, 4, 5, 15):
+            assert got[k, f] == pytest.approx(ref[f], rel=1e-12, abs=0), (k, f)
+        for a, b in ((6, 9), (12, 15)):
+            assert np.linalg.norm(got[k, a:b] - ref[a:b]) <= 1e-12 * np.linalg.norm(ref[a:b]), (k, a)
+        assert got[k, 1] == 0 and np.all(got[k, 9:12] == 0)
+        o += s.npl

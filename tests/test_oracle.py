@@ -309,3 +309,94 @@ def test_short_circuit_skips_tp_check_when_plpl_encounters():
     assert on.stats()["nplpl_enc"] > 0
     assert on.stats()["npltp_enc"] == 0
     assert off.stats()["npltp_enc"] > 0
+
+
+# ----------------------------------------------------------------- energy --
+# swo_energy restates swiftest_util_get_energy_and_momentum_system
+# (swiftest_util.f90:1172-1291) after swiftest_util_coord_rh2rb_pl (:488-518).
+# The reference takes the central-body potential of body i at its barycentric
+# distance |rb_i| (swiftest_util.f90:1368), so the pins below use
+# configurations where the facts they check hold exactly.
+
+def test_norm2_is_the_euclidean_norm_without_overflow():
+    rng = np.random.Generator(np.random.PCG64(11))
+    for _ in range(2000):
+        x, y, z = rng.normal(0, 1, 3) * 10.0 ** rng.integers(-5, 5, 3)
+        s2 = Fraction(x) ** 2 + Fraction(y) ** 2 + Fraction(z) ** 2  # exact
+        r = O.norm2(x, y, z)
+        ulp = np.spacing(r)
+        assert Fraction(r - 4 * ulp) ** 2 <= s2 <= Fraction(r + 4 * ulp) ** 2  # scaled sum: a few ulps
+    assert O.norm2(3e200, 4e200, 0.0) == pytest.approx(5e200, rel=1e-15)  # scaled: no overflow
+    assert O.norm2(0.0, 0.0, 0.0) == 0.0
+    assert O.norm2(-2.0, 0.0, 0.0) == 2.0
+
+
+def _square(a=1.3, u=5.1, Gm=2e-3, GMcb=ics.GMSUN, GU=ics.GU):
+    """Four equal bodies on a square around the central body, each moving
+    tangentially at speed u; the central body is at rest.  The positions sum
+    to zero exactly, so the barycentre is the central body."""
+    rh = np.array([[a, 0, 0], [0, a, 0], [-a, 0, 0], [0, -a, 0]], float)
+    vb = np.array([[0, u, 0], [-u, 0, 0], [0, -u, 0], [u, 0, 0]], float)
+    return rh, vb, np.full(4, Gm), np.full(4, Gm / GU), GMcb / GU
+
+
+def test_energy_closed_form_square():
+    a, u, Gm = 1.3, 5.1, 2e-3
+    rh, vb, G, m, mcb = _square(a, u, Gm)
+    out, rb = O.energy(ics.GMSUN, mcb, ics.RSUN_AU, np.zeros(3), rh, vb, G, m)
+    assert np.array_equal(rb, rh)
+    pe = -4 * ics.GMSUN * m[0] / a - Gm * m[0] * (4 / (a * np.sqrt(2)) + 2 / (2 * a))
+    assert out[O_E["pe"]] == pytest.approx(pe, rel=4e-16 * 8)
+    assert out[O_E["ke_orbit"]] == pytest.approx(0.5 * 4 * m[0] * u * u, rel=4e-16 * 4)
+    np.testing.assert_allclose(out[6:9], [0, 0, 4 * m[0] * a * u], rtol=4e-16 * 4, atol=0)
+    assert out[O_E["te"]] == pytest.approx(out[O_E["ke_orbit"]] + out[O_E["pe"]], rel=1e-16)
+    assert out[O_E["GMtot"]] == pytest.approx(ics.GMSUN + 4 * Gm, rel=1e-16)
+    assert out[O_E["be_cb"]] == pytest.approx(-3 * ics.GMSUN * mcb / (5 * ics.RSUN_AU), rel=1e-16)
+    np.testing.assert_array_equal(out[12:15], out[6:9])
+    assert out[O_E["ke_spin"]] == 0 and out[O_E["be"]] == 0
+
+
+def test_energy_barycentre_and_frame_invariance():
+    rng = np.random.Generator(np.random.PCG64(12))
+    n = 40
+    rh = rng.normal(0, 2, (n, 3))
+    rh[0] = [1e-3, -2e-3, 5e-4]  # close to the central body: rb[0] - rh[0] is the offset to ~1e-19
+    vb = rng.normal(0, 3, (n, 3))
+    Gm = ics.GU * np.exp(rng.uniform(np.log(1e-7), np.log(1e-3), n))
+    m = Gm / ics.GU
+    cb_vb = rng.normal(0, 1e-3, 3)
+    out, rb = O.energy(ics.GMSUN, 1.0, ics.RSUN_AU, cb_vb, rh, vb, Gm, m)
+    off = rb - rh                                      # one offset for every body
+    assert np.allclose(off, off[0], rtol=0, atol=1e-15)
+    bary = (Gm[:, None] * rb).sum(0) + ics.GMSUN * off[0]
+    assert np.all(np.abs(bary) <= 1e-14 * ((Gm[:, None] * np.abs(rb)).sum(0) + ics.GMSUN * np.abs(off[0])))
+    # every pair once, i < j: the pair part is -sum_{i<j} Gm_i m_j / |rb_i - rb_j|
+    pairs = sum(-Gm[i] * m[j] / np.linalg.norm(rb[i] - rb[j]) for i in range(n) for j in range(i + 1, n))
+    cb = sum(-ics.GMSUN * m[i] / np.linalg.norm(rb[i]) for i in range(n))
+    assert out[O_E["pe"]] == pytest.approx(pairs + cb, rel=1e-13)
+    ke = 0.5 * (1.0 * cb_vb @ cb_vb + sum(m[i] * vb[i] @ vb[i] for i in range(n)))
+    assert out[O_E["ke_orbit"]] == pytest.approx(ke, rel=1e-13)
+    L = np.cross(off[0], cb_vb) * 1.0 + sum(m[i] * np.cross(rb[i], vb[i]) for i in range(n))
+    assert np.linalg.norm(out[6:9] - L) <= 1e-13 * np.linalg.norm(L)
+
+
+def test_energy_inactive_bodies_are_left_out_exactly():
+    rng = np.random.Generator(np.random.PCG64(13))
+    n = 30
+    rh, vb = rng.normal(0, 2, (n, 3)), rng.normal(0, 3, (n, 3))
+    Gm = ics.GU * rng.uniform(1e-7, 1e-4, n)
+    R = rng.uniform(1e-5, 1e-4, n)
+    st = np.zeros(n, np.int32)
+    st[[3, 17, 29]] = 1  # INACTIVE
+    keep = st == 0
+    cbv = np.array([1e-4, -2e-4, 3e-5])
+    a, _ = O.energy(ics.GMSUN, 1.0, ics.RSUN_AU, cbv, rh, vb, Gm, Gm / ics.GU, R, st, lclose=True)
+    b, _ = O.energy(ics.GMSUN, 1.0, ics.RSUN_AU, cbv, rh[keep], vb[keep], Gm[keep], Gm[keep] / ics.GU, R[keep],
+                    lclose=True)
+    assert np.array_equal(a, b)
+    be = sum(-3 * Gm[i] * (Gm[i] / ics.GU) / (5 * R[i]) for i in range(n) if keep[i])
+    assert a[O_E["be"]] == pytest.approx(be, rel=1e-14)
+    assert a[O_E["te"]] == pytest.approx(a[0] + a[2] + a[3], rel=1e-16)
+
+
+O_E = {"ke_orbit": 0, "ke_spin": 1, "pe": 2, "be": 3, "be_cb": 4, "te": 5, "GMtot": 15}
