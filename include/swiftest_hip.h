@@ -184,6 +184,36 @@ int swg_system_set_frozen(swg_system *sys, const int32_t *frozen);
 #define SWG_ENERGY_NOUT 16
 int swg_system_energy(swg_system *sys, const double *cb_mass, const double *cb_radius, const double *pl_mass,
                       double *out);
+
+/* Discard checks after a step, on the resident state (swiftest_discard_system,
+ * src/swiftest/swiftest_discard.f90:13-56): for SyMBA the massive-body checks
+ * (symba_discard_pl, src/symba/symba_discard.f90:342-387, without its energy
+ * bookkeeping), then the test-particle checks (swiftest_discard_tp,
+ * swiftest_discard.f90:80-337).  The thresholds are the reference's
+ * parameters; a negative rmin/rmax/rmaxu/qmin turns that check off.  As in
+ * the reference, pl%h2b and tp%h2b overwrite the massive bodies', the central
+ * body's and the test particles' barycentric velocities when the
+ * central-body or barycentric pericentre checks run, and the pericentre
+ * checks keep their passage state between calls.  The close-approach check
+ * runs with CHK_CLOSE (prm.lclose).
+ *   cb_radius  per system (cb%radius, the floor of rmin for test particles)
+ * Discarded bodies get their DISCARDED_* status in place (statuses via
+ * swg_system_get).  Discarded test particles drop out of the following steps.
+ * A system that lost massive bodies must be rebuilt by the host (the
+ * reference's rearray): swg_system_get, then swg_system_create without them. */
+typedef struct {
+    double rmin, rmax, rmaxu;   /* CHK_RMIN, CHK_RMAX, CHK_EJECT */
+    double qmin, qmin_alo, qmin_ahi; /* CHK_QMIN, CHK_QMIN_RANGE */
+    int32_t qmin_coord;         /* CHK_QMIN_COORD: 0 HELIO, 1 BARY */
+    int32_t reserved;
+} swg_discard_params;
+#define SWG_DISCARDED_RMAX (-1)
+#define SWG_DISCARDED_RMIN (-2)
+#define SWG_DISCARDED_RMAXU (-3)
+#define SWG_DISCARDED_PERI (-4)
+#define SWG_DISCARDED_PLR (-5)
+int swg_system_discard(swg_system *sys, const swg_discard_params *dp, const double *cb_radius,
+                       int64_t *npl_discarded, int64_t *ntp_discarded);
 /* Autotuner: time the pl-tp and pl-pl acceleration kernels at each
  * workgroup size (64, 128, 256 bodies per tile) on the current state (no state
  * change) and keep the fastest for the following steps.  Reports the choice.

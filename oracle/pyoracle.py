@@ -78,6 +78,9 @@ def lib(omp: bool = False):
         L.swo_norm2.argtypes = [C.c_double] * 3
         L.swo_norm2.restype = C.c_double
         L.swo_coord_vh2vb.argtypes = [C.c_int, C.c_double, D, D, D, D]
+        L.swo_sys_discard.argtypes = [vp, D, C.c_int, C.POINTER(C.c_int64)]
+        L.swo_discard.argtypes = ([C.c_int] * 3 + [C.c_double] * 2 + [D, D, D, D, D, D, D, I32, I32, I32, D, D, I32]
+                                  + [D, D, D, I32, I32, D, D, D, C.c_int, C.c_int, C.POINTER(C.c_int64)])
         L.swo_energy.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, D, D, D, D, D, D, I32, C.c_int, D, vp]
         _LIBS[key] = L
     return _LIBS[key]
@@ -219,6 +222,13 @@ class OracleSystem:
     def step(self):
         return self.L.swo_sys_step(self.h)
 
+    def discard(self, rmin=-1.0, rmax=-1.0, rmaxu=-1.0, qmin=-1.0, qmin_alo=-1.0, qmin_ahi=-1.0, qmin_coord=0):
+        """swo_sys_discard: (massive bodies, test particles) newly discarded."""
+        prm = np.array([rmin, rmax, rmaxu, qmin, qmin_alo, qmin_ahi], np.float64)
+        counts = (C.c_int64 * 2)()
+        self.L.swo_sys_discard(self.h, prm, int(qmin_coord), counts)
+        return counts[0], counts[1]
+
     def state(self):
         npl, ntp = self.npl, self.ntp
         out = dict(pl_rh=np.zeros((npl, 3)), pl_vh=np.zeros((npl, 3)), pl_vb=np.zeros((npl, 3)),
@@ -276,3 +286,36 @@ def vh2vb(GMcb, Gmass, vh):
     vb, cbv = np.zeros_like(vh), np.zeros(3)
     lib().swo_coord_vh2vb(vh.shape[0], float(GMcb), _c(Gmass), vh, vb, cbv)
     return vb, cbv
+
+
+class DiscardState:
+    """Arrays of one system for swo_discard (the state the checks read and
+    change, pericentre-passage state included)."""
+
+    def __init__(self, integrator, GMcb, Rcb, cb_vb, pl_rh, pl_vh, pl_vb, Gmass, radius, rhill, tp_rh, tp_vh, tp_vb,
+                 pl_status=None, tp_status=None, nplenc=None):
+        self.integ, self.GMcb, self.Rcb = int(integrator), float(GMcb), float(Rcb)
+        self.cb_vb = _c(cb_vb).copy()
+        self.pl_rh, self.pl_vh, self.pl_vb = (_c(a).reshape(-1, 3).copy() for a in (pl_rh, pl_vh, pl_vb))
+        self.Gmass, self.radius, self.rhill = _c(Gmass).copy(), _c(radius).copy(), _c(rhill).copy()
+        self.tp_rh, self.tp_vh, self.tp_vb = (_c(a).reshape(-1, 3).copy() for a in (tp_rh, tp_vh, tp_vb))
+        n, m = self.pl_rh.shape[0], self.tp_rh.shape[0]
+        self.pl_status = np.zeros(n, np.int32) if pl_status is None else _c(pl_status, np.int32).copy()
+        self.tp_status = np.zeros(m, np.int32) if tp_status is None else _c(tp_status, np.int32).copy()
+        self.nplenc = np.zeros(n, np.int32) if nplenc is None else _c(nplenc, np.int32).copy()
+        self.pl_isperi, self.tp_isperi = np.ones(max(n, 1), np.int32), np.ones(max(m, 1), np.int32)
+        self.pl_atp, self.pl_peri = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        self.tp_atp, self.tp_peri = np.zeros(max(m, 1)), np.zeros(max(m, 1))
+        self.lfirst_peri = np.ones(1, np.int32)
+
+    def discard(self, dt, rmin=-1.0, rmax=-1.0, rmaxu=-1.0, qmin=-1.0, qmin_alo=-1.0, qmin_ahi=-1.0, qmin_coord=0,
+                lclose=False):
+        prm = np.array([rmin, rmax, rmaxu, qmin, qmin_alo, qmin_ahi, dt], np.float64)
+        counts = (C.c_int64 * 2)()
+        n, m = self.pl_rh.shape[0], self.tp_rh.shape[0]
+        lib().swo_discard(self.integ, n, m, self.GMcb, self.Rcb, self.cb_vb, self.pl_rh, self.pl_vh, self.pl_vb,
+                          self.Gmass, self.radius, self.rhill, self.pl_status, self.nplenc, self.pl_isperi,
+                          self.pl_atp, self.pl_peri, self.lfirst_peri, self.tp_rh, self.tp_vh, self.tp_vb,
+                          self.tp_status, self.tp_isperi, self.tp_atp, self.tp_peri, prm, int(qmin_coord),
+                          int(bool(lclose)), counts)
+        return counts[0], counts[1]

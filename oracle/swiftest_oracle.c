@@ -766,6 +766,10 @@ struct swo_sys {
     int lfirst, tlfirst, lfirstkick;
     int irec;
     int err;
+    /* pericentre-passage state of the discard checks */
+    int32_t *pisperi, *tisperi;
+    double *patp, *pperi, *tatp, *tperi;
+    int lfirst_peri, dis_ready;
     swo_params p;
     swo_stats st;
     int64_t iloop, iout;
@@ -1867,7 +1871,8 @@ void swo_sys_free(swo_sys *s)
                     s->renc, s->mu, s->eta, s->muj, s->ir3h, s->ir3j, s->status, s->levelg, s->levelm, s->lmask,
                     s->id, s->iflag, s->trh, s->tvh, s->tvb, s->tah, s->tstatus, s->tlevelg, s->tlevelm,
                     s->tlmask, s->tid, s->tiflag, s->plpl.i1, s->plpl.i2, s->plpl.lvdotr, s->plpl.status,
-                    s->plpl.level, s->pltp.i1, s->pltp.i2, s->pltp.lvdotr, s->pltp.status, s->pltp.level};
+                    s->plpl.level, s->pltp.i1, s->pltp.i2, s->pltp.lvdotr, s->pltp.status, s->pltp.level,
+                    s->pisperi, s->tisperi, s->patp, s->pperi, s->tatp, s->tperi};
     for (size_t q = 0; q < sizeof ptrs / sizeof ptrs[0]; q++) free(ptrs[q]);
     free(s);
 }
@@ -2015,4 +2020,260 @@ void swo_energy(int npl, double GMcb, double mcb, double Rcb, const double *cb_v
     out[SWO_E_GMTOT] = GMtot;
     if (rb_out) memcpy(rb_out, rb, 3 * (size_t)npl * sizeof(double));
     free(rb);
+}
+
+/* ======================================================================
+ * Discard checks
+ * ====================================================================== */
+
+/* swiftest_util_peri (swiftest_util.f90:1565-1603) for one body */
+static void peri_one(double m, const double *r, const double *v, int32_t *isperi, double *atp, double *q)
+{
+    const double vdotr = dot3(r[0], r[1], r[2], v[0], v[1], v[2]);
+    if (*isperi == -1) {
+        if (vdotr >= 0.0) {
+            double e;
+            *isperi = 0;
+            swo_orbel_xv2aeq(m, r[0], r[1], r[2], v[0], v[1], v[2], atp, &e, q);
+        }
+    } else {
+        *isperi = vdotr > 0.0 ? -1 : 1;
+    }
+}
+
+/* swiftest_discard_pl_close (swiftest_discard.f90:291-337) */
+static int pl_close(const double *dx, const double *dv, double dt, double r2crit)
+{
+    const double r2 = dot3(dx[0], dx[1], dx[2], dx[0], dx[1], dx[2]);
+    if (r2 <= r2crit) return 1;
+    const double vdotr = dot3(dx[0], dx[1], dx[2], dv[0], dv[1], dv[2]);
+    if (vdotr > 0.0) return 0;
+    const double v2 = dot3(dv[0], dv[1], dv[2], dv[0], dv[1], dv[2]);
+    const double tmin = -vdotr / v2;
+    double r2min;
+    if (tmin < dt)
+        r2min = r2 - (vdotr * vdotr) / v2;
+    else
+        r2min = (r2 + (2.0 * vdotr) * dt) + v2 * (dt * dt);
+    r2min = fmin(r2min, r2);
+    return r2min <= r2crit;
+}
+
+void swo_discard(int integrator, int npl, int ntp, double GMcb, double Rcb, double *cb_vb, const double *pl_rh,
+                 const double *pl_vh, double *pl_vb, const double *Gmass, const double *radius, const double *rhill,
+                 int32_t *pl_status, const int32_t *nplenc, int32_t *pl_isperi, double *pl_atp, double *pl_peri,
+                 int32_t *lfirst_peri, const double *tp_rh, const double *tp_vh, double *tp_vb, int32_t *tp_status,
+                 int32_t *tp_isperi, double *tp_atp, double *tp_peri, const double *prm, int qmin_coord, int lclose,
+                 int64_t *counts)
+{
+    const double rmin = prm[0], rmax = prm[1], rmaxu = prm[2], qmin = prm[3], alo = prm[4], ahi = prm[5];
+    const double dt = prm[6];
+    const int lcb = rmin >= 0.0 || rmax >= 0.0 || rmaxu >= 0.0;
+    counts[0] = counts[1] = 0;
+    /* set_msys (swiftest_util.f90:2115) */
+    double sm = 0.0;
+    for (int i = 0; i < npl; i++)
+        if (pl_status[i] != SWO_INACTIVE) sm = sm + Gmass[i];
+    const double Gmtot = GMcb + sm;
+    double *rb = (double *)calloc(3 * (size_t)(npl > 0 ? npl : 1), sizeof(double));
+    /* bodies discarded here leave the arrays (pl%rearray) before the test
+     * particles are checked */
+    uint8_t *gone = (uint8_t *)calloc((size_t)(npl > 0 ? npl : 1), 1);
+    double cb_rb[3] = {0.0, 0.0, 0.0};
+
+    /* ---- massive bodies (SyMBA) ---- */
+    if (integrator == SWO_SYMBA && npl > 0) {
+        /* pl%vb2vh repeats the step's last conversion on the same inputs, so
+         * it changes nothing; pl%rh2rb (swiftest_util.f90:505-516): */
+        double g = GMcb, xt[3] = {0.0, 0.0, 0.0};
+        for (int i = 0; i < npl; i++) {
+            if (pl_status[i] == SWO_INACTIVE) continue;
+            g = g + Gmass[i];
+            for (int d = 0; d < 3; d++) xt[d] = xt[d] + Gmass[i] * pl_rh[3 * i + d];
+        }
+        for (int d = 0; d < 3; d++) cb_rb[d] = -xt[d] / g;
+        for (int i = 0; i < npl; i++)
+            if (pl_status[i] != SWO_INACTIVE)
+                for (int d = 0; d < 3; d++) rb[3 * i + d] = pl_rh[3 * i + d] + cb_rb[d];
+        if (lcb) { /* symba_discard_cb_pl (symba_discard.f90:36-101) */
+            const double rmin2 = rmin * rmin, rmax2 = rmax * rmax, rmaxu2 = rmaxu * rmaxu;
+            for (int i = 0; i < npl; i++) {
+                if (pl_status[i] != SWO_ACTIVE) continue;
+                const double *r = pl_rh + 3 * i, *b = rb + 3 * i, *v = pl_vb + 3 * i;
+                const double rh2 = dot3(r[0], r[1], r[2], r[0], r[1], r[2]);
+                int32_t st = SWO_ACTIVE;
+                if (rmax >= 0.0 && rh2 > rmax2) {
+                    st = SWO_DISCARDED_RMAX;
+                } else if (rmin >= 0.0 && rh2 < rmin2) {
+                    st = SWO_DISCARDED_RMIN;
+                } else if (rmaxu >= 0.0) {
+                    const double rb2 = dot3(b[0], b[1], b[2], b[0], b[1], b[2]);
+                    const double vb2 = dot3(v[0], v[1], v[2], v[0], v[1], v[2]);
+                    const double energy = 0.5 * vb2 - Gmtot / sqrt(rb2);
+                    if (energy > 0.0 && rb2 > rmaxu2) st = SWO_DISCARDED_RMAXU;
+                }
+                if (st != SWO_ACTIVE) {
+                    pl_status[i] = st;
+                    gone[i] = 1;
+                    counts[0]++;
+                }
+            }
+        }
+        if (qmin >= 0.0) { /* symba_discard_peri_pl (symba_discard.f90:311-336) */
+            const int first = *lfirst_peri;
+            for (int i = 0; i < npl; i++) {
+                if (first) pl_isperi[i] = 0; /* swiftest_util_peri_body, swiftest_util.f90:1625 */
+                if (qmin_coord == 0)
+                    peri_one(GMcb + Gmass[i], pl_rh + 3 * i, pl_vh + 3 * i, &pl_isperi[i], &pl_atp[i], &pl_peri[i]);
+                else
+                    peri_one(Gmtot, rb + 3 * i, pl_vb + 3 * i, &pl_isperi[i], &pl_atp[i], &pl_peri[i]);
+            }
+            if (first) {
+                *lfirst_peri = 0;
+            } else {
+                for (int i = 0; i < npl; i++)
+                    if (pl_status[i] == SWO_ACTIVE && pl_isperi[i] == 0 && nplenc[i] == 0 && pl_atp[i] >= alo &&
+                        pl_atp[i] <= ahi && pl_peri[i] <= qmin) {
+                        pl_status[i] = SWO_DISCARDED_PERI;
+                        gone[i] = 1;
+                        counts[0]++;
+                    }
+            }
+        }
+    }
+
+    /* ---- test particles (swiftest_discard_tp) ---- */
+    if (ntp > 0) {
+        double *trb = (double *)calloc(3 * (size_t)ntp, sizeof(double));
+        if (lcb || (qmin >= 0.0 && qmin_coord == 1)) {
+            int nleft = 0;
+            for (int i = 0; i < npl; i++) nleft += !gone[i];
+            if (nleft > 0) { /* pl%h2b (swiftest_util.f90:228-265) */
+                double g = GMcb, xt[3] = {0.0, 0.0, 0.0}, vt[3] = {0.0, 0.0, 0.0};
+                for (int i = 0; i < npl; i++) {
+                    if (gone[i] || pl_status[i] == SWO_INACTIVE) continue;
+                    g = g + Gmass[i];
+                    for (int d = 0; d < 3; d++) {
+                        xt[d] = xt[d] + Gmass[i] * pl_rh[3 * i + d];
+                        vt[d] = vt[d] + Gmass[i] * pl_vh[3 * i + d];
+                    }
+                }
+                for (int d = 0; d < 3; d++) {
+                    cb_rb[d] = -xt[d] / g;
+                    cb_vb[d] = -vt[d] / g;
+                }
+                for (int i = 0; i < npl; i++)
+                    if (!gone[i] && pl_status[i] != SWO_INACTIVE)
+                        for (int d = 0; d < 3; d++) {
+                            rb[3 * i + d] = pl_rh[3 * i + d] + cb_rb[d];
+                            pl_vb[3 * i + d] = pl_vh[3 * i + d] + cb_vb[d];
+                        }
+            }
+            for (int t = 0; t < ntp; t++) /* tp%h2b (:268-296) */
+                if (tp_status[t] != SWO_INACTIVE)
+                    for (int d = 0; d < 3; d++) {
+                        trb[3 * t + d] = tp_rh[3 * t + d] + cb_rb[d];
+                        tp_vb[3 * t + d] = tp_vh[3 * t + d] + cb_vb[d];
+                    }
+        }
+        if (lcb) { /* swiftest_discard_cb_tp (swiftest_discard.f90:136-188) */
+            const double rmin2 = fmax(rmin * rmin, Rcb * Rcb), rmax2 = rmax * rmax, rmaxu2 = rmaxu * rmaxu;
+            for (int t = 0; t < ntp; t++) {
+                if (tp_status[t] != SWO_ACTIVE) continue;
+                const double *r = tp_rh + 3 * t, *b = trb + 3 * t, *v = tp_vb + 3 * t;
+                const double rh2 = dot3(r[0], r[1], r[2], r[0], r[1], r[2]);
+                int32_t st = SWO_ACTIVE;
+                if (rmax >= 0.0 && rh2 > rmax2) {
+                    st = SWO_DISCARDED_RMAX;
+                } else if (rmin >= 0.0 && rh2 < rmin2) {
+                    st = SWO_DISCARDED_RMIN;
+                } else if (rmaxu >= 0.0) {
+                    const double rb2 = dot3(b[0], b[1], b[2], b[0], b[1], b[2]);
+                    const double vb2 = dot3(v[0], v[1], v[2], v[0], v[1], v[2]);
+                    const double energy = 0.5 * vb2 - Gmtot / sqrt(rb2);
+                    if (energy > 0.0 && rb2 > rmaxu2) st = SWO_DISCARDED_RMAXU;
+                }
+                if (st != SWO_ACTIVE) {
+                    tp_status[t] = st;
+                    counts[1]++;
+                }
+            }
+        }
+        if (qmin >= 0.0) { /* swiftest_discard_peri_tp (:209-240) */
+            for (int t = 0; t < ntp; t++) {
+                if (qmin_coord == 0)
+                    peri_one(GMcb, tp_rh + 3 * t, tp_vh + 3 * t, &tp_isperi[t], &tp_atp[t], &tp_peri[t]);
+                else
+                    peri_one(Gmtot, trb + 3 * t, tp_vb + 3 * t, &tp_isperi[t], &tp_atp[t], &tp_peri[t]);
+            }
+            for (int t = 0; t < ntp; t++) {
+                if (tp_status[t] != SWO_ACTIVE || tp_isperi[t] != 0) continue;
+                int ih = 1;
+                for (int j = 0; j < npl; j++) {
+                    if (gone[j]) continue;
+                    double dx[3];
+                    for (int d = 0; d < 3; d++) dx[d] = tp_rh[3 * t + d] - pl_rh[3 * j + d];
+                    if (dot3(dx[0], dx[1], dx[2], dx[0], dx[1], dx[2]) <= rhill[j] * rhill[j]) ih = 0;
+                }
+                if (ih && tp_atp[t] >= alo && tp_atp[t] <= ahi && tp_peri[t] <= qmin) {
+                    tp_status[t] = SWO_DISCARDED_PERI;
+                    counts[1]++;
+                }
+            }
+        }
+        if (lclose) { /* swiftest_discard_pl_tp (:260-285) */
+            for (int t = 0; t < ntp; t++) {
+                if (tp_status[t] != SWO_ACTIVE) continue;
+                for (int j = 0; j < npl; j++) {
+                    if (gone[j]) continue;
+                    double dx[3], dv[3];
+                    for (int d = 0; d < 3; d++) {
+                        dx[d] = tp_rh[3 * t + d] - pl_rh[3 * j + d];
+                        dv[d] = tp_vh[3 * t + d] - pl_vh[3 * j + d];
+                    }
+                    if (pl_close(dx, dv, dt, radius[j] * radius[j])) {
+                        tp_status[t] = SWO_DISCARDED_PLR;
+                        counts[1]++;
+                        break;
+                    }
+                }
+            }
+        }
+        free(trb);
+    }
+    free(gone);
+    free(rb);
+}
+
+int swo_sys_discard(swo_sys *s, const double *prm6, int qmin_coord, int64_t *counts)
+{
+    if (!s->dis_ready) { /* swiftest_util_setup_body (swiftest_util.f90:2537): isperi = 1 */
+        const size_t np = s->npl > 0 ? s->npl : 1, nt = s->ntp > 0 ? s->ntp : 1;
+        s->pisperi = (int32_t *)malloc(np * sizeof(int32_t));
+        s->tisperi = (int32_t *)malloc(nt * sizeof(int32_t));
+        s->patp = (double *)calloc(np, sizeof(double));
+        s->pperi = (double *)calloc(np, sizeof(double));
+        s->tatp = (double *)calloc(nt, sizeof(double));
+        s->tperi = (double *)calloc(nt, sizeof(double));
+        for (size_t i = 0; i < np; i++) s->pisperi[i] = 1;
+        for (size_t i = 0; i < nt; i++) s->tisperi[i] = 1;
+        s->lfirst_peri = 1;
+        s->dis_ready = 1;
+    }
+    int32_t *nplenc = (int32_t *)calloc((size_t)(s->npl > 0 ? s->npl : 1), sizeof(int32_t));
+    for (int64_t k = 0; k < s->plpl.n; k++) { /* symba_encounter_check.f90:79-80 */
+        nplenc[s->plpl.i1[k] - 1]++;
+        nplenc[s->plpl.i2[k] - 1]++;
+    }
+    double prm[7];
+    for (int k = 0; k < 6; k++) prm[k] = prm6[k];
+    prm[6] = s->p.dt;
+    swo_discard(s->integ, s->npl, s->ntp, s->GMcb, s->Rcb, s->cb_vb, s->rh, s->vh, s->vb, s->Gmass, s->radius,
+                s->rhill, s->status, nplenc, s->pisperi, s->patp, s->pperi, &s->lfirst_peri, s->trh, s->tvh, s->tvb,
+                s->tstatus, s->tisperi, s->tatp, s->tperi, prm, qmin_coord, s->p.lclose, counts);
+    /* discarded particles leave the arrays (tp%spill, swiftest_discard.f90:113-116) */
+    for (int i = 0; i < s->ntp; i++)
+        if (s->tstatus[i] != SWO_ACTIVE) s->tlmask[i] = 0;
+    free(nplenc);
+    return 0;
 }

@@ -141,6 +141,36 @@ int64_t swo_sys_list(const swo_sys *s, int which, int32_t *index1, int32_t *inde
 /* swiftest_util_coord_vh2vb_pl (swiftest_util.f90:424-459): barycentric
  * velocities of the massive bodies and of the central body from vh. */
 void swo_coord_vh2vb(int npl, double GMcb, const double *Gmass, const double *vh, double *vb, double *cb_vb);
+/* ---- discard checks (after each step) ---- */
+#define SWO_DISCARDED_RMAX (-1)
+#define SWO_DISCARDED_RMIN (-2)
+#define SWO_DISCARDED_RMAXU (-3)
+#define SWO_DISCARDED_PERI (-4)
+#define SWO_DISCARDED_PLR (-5)
+/* The discard checks of one system after a step, in the reference's order:
+ * for SyMBA the massive-body checks (symba_discard_pl, symba_discard.f90:
+ * 342-387: rh2rb, then symba_discard_cb_pl :14-104 and symba_discard_peri_pl
+ * :292-339), then the test-particle checks (swiftest_discard_tp,
+ * swiftest_discard.f90:80-119: pl%h2b and tp%h2b, which overwrite pl_vb,
+ * cb_vb and tp_vb, then swiftest_discard_cb_tp :122-191,
+ * swiftest_discard_peri_tp :194-243 and swiftest_discard_pl_tp :246-288).
+ *   prm = {rmin, rmax, rmaxu, qmin, qmin_alo, qmin_ahi, dt}; a negative
+ *   rmin/rmax/rmaxu/qmin switches that check off.  qmin_coord: 0 HELIO, 1 BARY.
+ *   isperi/atp/peri and *lfirst_peri are the pericentre-passage state the
+ *   reference keeps between calls (isperi starts at 1, lfirst_peri at 1).
+ *   nplenc: pl-pl encounters of each body in the step's level-0 list.
+ * The total mass Gmtot is set_msys's (swiftest_util.f90:2107-2118), formed
+ * on every call.  Statuses change in place; counts[0], counts[1] receive the
+ * numbers of massive bodies and test particles newly discarded. */
+void swo_discard(int integrator, int npl, int ntp, double GMcb, double Rcb, double *cb_vb, const double *pl_rh,
+                 const double *pl_vh, double *pl_vb, const double *Gmass, const double *radius, const double *rhill,
+                 int32_t *pl_status, const int32_t *nplenc, int32_t *pl_isperi, double *pl_atp, double *pl_peri,
+                 int32_t *lfirst_peri, const double *tp_rh, const double *tp_vh, double *tp_vb, int32_t *tp_status,
+                 int32_t *tp_isperi, double *tp_atp, double *tp_peri, const double *prm, int qmin_coord, int lclose,
+                 int64_t *counts);
+/* swo_discard on a whole-system state after its last step; prm6 = {rmin,
+ * rmax, rmaxu, qmin, qmin_alo, qmin_ahi} (dt and lclose are the system's). */
+int swo_sys_discard(swo_sys *s, const double *prm6, int qmin_coord, int64_t *counts);
 /* Fortran norm2 of a 3-vector as gfortran evaluates it (scaled sum of squares). */
 double swo_norm2(double x, double y, double z);
 /* swiftest_util_coord_rh2rb_pl (swiftest_util.f90:488-518) followed by

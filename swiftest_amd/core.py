@@ -35,6 +35,12 @@ class SwiftestHipError(RuntimeError):
         self.code = code
 
 
+class DiscardParams(C.Structure):
+    _fields_ = [("rmin", C.c_double), ("rmax", C.c_double), ("rmaxu", C.c_double), ("qmin", C.c_double),
+                ("qmin_alo", C.c_double), ("qmin_ahi", C.c_double), ("qmin_coord", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
 class Params(C.Structure):
     _fields_ = [("dt", C.c_double), ("integrator", C.c_int32), ("lclose", C.c_int32),
                 ("short_circuit", C.c_int32), ("istep_out", C.c_int32), ("rhill_present", C.c_int32),
@@ -87,6 +93,7 @@ def library():
                                               C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.swg_system_destroy.argtypes = [vp]
         L.swg_system_energy.argtypes = [vp, D, D, D, D]
+        L.swg_system_discard.argtypes = [vp, C.POINTER(DiscardParams), D, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         _lib = L
     return _lib
 
@@ -302,6 +309,18 @@ class GpuSystem:
         out = np.zeros((self.nsys, ENERGY_NOUT))
         _check(self.L.swg_system_energy(self.h, _c(cb_mass), _c(cb_radius), _c(pl_mass), out))
         return out
+
+    def discard(self, rmin=-1.0, rmax=-1.0, rmaxu=-1.0, qmin=-1.0, qmin_alo=-1.0, qmin_ahi=-1.0, qmin_coord=0,
+                cb_radius=None):
+        """The reference's discard checks after a step (swg_system_discard);
+        returns (massive bodies, test particles) newly discarded.  qmin_coord:
+        0 HELIO, 1 BARY.  Discarded bodies get their status in place."""
+        dp = DiscardParams(float(rmin), float(rmax), float(rmaxu), float(qmin), float(qmin_alo), float(qmin_ahi),
+                           int(qmin_coord), 0)
+        rcb = self._cb[1] if cb_radius is None else cb_radius
+        a, b = C.c_int64(), C.c_int64()
+        _check(self.L.swg_system_discard(self.h, C.byref(dp), _c(rcb), C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def set_frozen(self, frozen):
         """frozen[s] = 1 leaves system s out of the following steps."""

@@ -713,6 +713,54 @@ int Ensemble::energy(const double *cb_mass, const double *cb_radius, const doubl
     return 0;
 }
 
+// Discard checks after a step (swg_system_discard).
+int Ensemble::discard(const swg_discard_params &dp, const double *cb_radius, int64_t *npl_dis, int64_t *ntp_dis)
+{
+    hipStream_t st = stream;
+    if (!dis_ready) {
+        std::vector<int4> tiles;
+        discard_tiles(nsys, h_tp_off.data(), tiles);
+        dis_ntiles = (int)tiles.size();
+        if (own(&dis_pl_isperi, npl) || own(&dis_tp_isperi, ntp) || own(&dis_lfirst, nsys) || own(&dis_pl_atp, npl) ||
+            own(&dis_pl_peri, npl) || own(&dis_tp_atp, ntp) || own(&dis_tp_peri, ntp) ||
+            own(&dis_sys, (size_t)nsys * discard_sys_words()) || own(&dis_rcb, nsys) || own(&dis_gone, npl) ||
+            own(&dis_count, 2) || own(&dis_tiles, (size_t)dis_ntiles))
+            return SWG_ERR_HIP;
+        // swiftest_util_setup_body (swiftest_util.f90:2537): isperi = 1; the
+        // first pericentre call of SyMBA's massive bodies is flagged per system
+        std::vector<int32_t> one(std::max(std::max(npl, ntp), nsys), 1);
+        std::vector<double> zero(std::max(npl, ntp), 0.0);
+        if (npl) {
+            HIPCHK(hipMemcpy(dis_pl_isperi, one.data(), npl * sizeof(int32_t), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(dis_pl_atp, zero.data(), npl * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(dis_pl_peri, zero.data(), npl * sizeof(double), hipMemcpyHostToDevice));
+        }
+        if (ntp) {
+            HIPCHK(hipMemcpy(dis_tp_isperi, one.data(), ntp * sizeof(int32_t), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(dis_tp_atp, zero.data(), ntp * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(dis_tp_peri, zero.data(), ntp * sizeof(double), hipMemcpyHostToDevice));
+        }
+        HIPCHK(hipMemcpy(dis_lfirst, one.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice));
+        if (dis_ntiles)
+            HIPCHK(hipMemcpy(dis_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+        dis_ready = true;
+    }
+    HIPCHK(hipMemcpyAsync(dis_rcb, cb_radius, nsys * sizeof(double), hipMemcpyHostToDevice, st));
+    const bool symba = prm.integrator == SWG_SYMBA;
+    // pl%nplenc: the pairs of each body in the last step's level-0 list
+    const bool enc = symba && npp > 0;
+    launch_discard(st, pl, tp, sy, nsys, npl, symba, dp, prm.lclose, prm.dt, dis_rcb, enc ? pp.rowcnt : nullptr,
+                   enc ? pp.colcnt : nullptr, dis_pl_isperi, dis_pl_atp, dis_pl_peri, dis_tp_isperi, dis_tp_atp,
+                   dis_tp_peri, dis_lfirst, dis_gone, dis_sys, dis_count, dis_tiles, dis_ntiles);
+    HIPCHK(hipGetLastError());
+    unsigned long long c[2];
+    HIPCHK(hipMemcpyAsync(c, dis_count, sizeof c, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (npl_dis) *npl_dis = (int64_t)c[0];
+    if (ntp_dis) *ntp_dis = (int64_t)c[1];
+    return 0;
+}
+
 int Ensemble::check_errors()
 {
     int32_t *hf = (int32_t *)(h_small + 1024);
@@ -1314,6 +1362,14 @@ int swg_system_energy(swg_system *sys, const double *cb_mass, const double *cb_r
     for (int s = 0; s < sys->e->nsys; s++)
         if (!(cb_radius[s] > 0.0)) return fail(SWG_ERR_ARG, "central body radius must be positive");
     return sys->e->energy(cb_mass, cb_radius, pl_mass, out);
+}
+
+int swg_system_discard(swg_system *sys, const swg_discard_params *dp, const double *cb_radius,
+                       int64_t *npl_discarded, int64_t *ntp_discarded)
+{
+    if (!sys || !dp || !cb_radius) return fail(SWG_ERR_ARG, "NULL argument");
+    if (dp->qmin_coord != 0 && dp->qmin_coord != 1) return fail(SWG_ERR_ARG, "qmin_coord must be 0 (HELIO) or 1 (BARY)");
+    return sys->e->discard(*dp, cb_radius, npl_discarded, ntp_discarded);
 }
 
 int swg_system_time_encounters(swg_system *sys, int32_t reps, double *ms, int64_t *npairs, int64_t *ncand)

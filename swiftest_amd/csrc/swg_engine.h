@@ -102,6 +102,16 @@ struct Ensemble {
     int4 *en_tiles = nullptr;
     int32_t *en_toff = nullptr;
     int en_ntiles = 0;
+    int discard(const swg_discard_params &dp, const double *cb_radius, int64_t *npl_dis, int64_t *ntp_dis);
+    // discard-check state, allocated on the first discard call
+    bool dis_ready = false;
+    int32_t *dis_pl_isperi = nullptr, *dis_tp_isperi = nullptr, *dis_lfirst = nullptr;
+    double *dis_pl_atp = nullptr, *dis_pl_peri = nullptr, *dis_tp_atp = nullptr, *dis_tp_peri = nullptr;
+    double *dis_sys = nullptr, *dis_rcb = nullptr;
+    uint8_t *dis_gone = nullptr;
+    unsigned long long *dis_count = nullptr;
+    int4 *dis_tiles = nullptr;
+    int dis_ntiles = 0;
     // HIP graphs of the step's two halves (see run_phase)
     bool use_graphs = false;
     hipGraphExec_t graph_exec[8] = {};

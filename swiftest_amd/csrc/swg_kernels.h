@@ -5,6 +5,7 @@
 
 #include <vector>
 
+#include "../../include/swiftest_hip.h"
 #include "swg_level.h"
 #include "swg_state.h"
 
@@ -46,6 +47,15 @@ void launch_energy(hipStream_t st, PlDev pl, SysDev sy, int nsys, const double *
                    const double *cbR, int lclose, Vec3p rb, const int4 *tiles, int ntiles, const int32_t *toff,
                    double *part, double *out);
 void launch_vh2vb(hipStream_t st, PlDev pl, SysDev sy, int nsys, int npl);
+
+// ---- discard checks (k_discard.hip) ----
+void discard_tiles(int nsys, const int32_t *tp_off, std::vector<int4> &tiles);
+size_t discard_sys_words();
+void launch_discard(hipStream_t st, PlDev pl, TpDev tp, SysDev sy, int nsys, int npl, int symba,
+                    const swg_discard_params &dp, int lclose, double dt, const double *Rcb,
+                    const int32_t *nplenc_row, const int32_t *nplenc_col, int32_t *pl_isperi, double *pl_atp,
+                    double *pl_peri, int32_t *tp_isperi, double *tp_atp, double *tp_peri, int32_t *lfirst_peri,
+                    uint8_t *gone, double *sys, unsigned long long *count, const int4 *tiles, int ntiles);
 
 // ---- encounter detection (k_enc.hip) ----
 void launch_pp_tiles(hipStream_t st, PlDev pl, SysDev sy, ListDev pp, const int4 *tiles, int ntiles,

@@ -33,6 +33,8 @@ constexpr double VSMALL = 1.4916681462400413e-154;     // sqrt(tiny(1.0_DP)), gl
 
 // status codes, globals_module.f90:86-103
 constexpr int ACTIVE = 0, INACTIVE = 1, DISCARDED_DRIFTERR = -7, COLLIDED = -14;
+constexpr int DISCARDED_RMAX = -1, DISCARDED_RMIN = -2, DISCARDED_RMAXU = -3, DISCARDED_PERI = -4,
+              DISCARDED_PLR = -5;
 
 __host__ __device__ inline double dot3(double x1, double y1, double z1, double x2, double y2, double z2)
 {
@@ -384,16 +386,20 @@ __host__ __device__ inline bool encounter_one(double xr, double yr, double zr, d
     return lvdotr && (r2min <= r2crit);
 }
 
-// swiftest_orbel_xv2aeq (swiftest_orbel.f90:700-764), pericentre only
-__device__ inline double xv2q(double mu, double rx, double ry, double rz, double vx, double vy, double vz)
+// swiftest_orbel_xv2aeq (swiftest_orbel.f90:700-764): semi-major axis and
+// pericentre
+__device__ inline void xv2aq(double mu, double rx, double ry, double rz, double vx, double vy, double vz, double &a,
+                             double &q)
 {
+    a = 0.0;
+    q = 0.0;
     double r = sqrt(dot3(rx, ry, rz, rx, ry, rz));
     double v2 = dot3(vx, vy, vz, vx, vy, vz);
     double hx = ry * vz - rz * vy, hy = rz * vx - rx * vz, hz = rx * vy - ry * vx;
     double h2 = dot3(hx, hy, hz, hx, hy, hz);
-    if (h2 < DBL_TINY) return 0.0;
+    if (h2 < DBL_TINY) return;
     double energy = 0.5 * v2 - mu / r;
-    double a = 0.0, e = 0.0, fac = 0.0;
+    double e = 0.0, fac = 0.0;
     int type;
     if (fabs((energy * r) / mu) < sqrt(TINYVALUE)) {
         type = 0;
@@ -409,11 +415,22 @@ __device__ inline double xv2q(double mu, double rx, double ry, double rz, double
     if (type == -1) {
         fac = 1.0 - h2 / (mu * a);
         if (fac > TINYVALUE) e = sqrt(fac);
-        return a * (1.0 - e);
+        q = a * (1.0 - e);
+    } else if (type == 0) {
+        a = (0.5 * h2) / mu;
+        q = a;
+    } else {
+        e = sqrt(1.0 + fac);
+        q = a * (1.0 - e);
     }
-    if (type == 0) return (0.5 * h2) / mu;
-    e = sqrt(1.0 + fac);
-    return a * (1.0 - e);
+}
+
+// the pericentre alone
+__device__ inline double xv2q(double mu, double rx, double ry, double rz, double vx, double vy, double vz)
+{
+    double a, q;
+    xv2aq(mu, rx, ry, rz, vx, vy, vz, a, q);
+    return q;
 }
 
 // collision_check_one (collision_check.f90:16-50)

@@ -390,3 +390,89 @@ def test_energy_matches_oracle(integ):
             assert np.linalg.norm(got[k, a:b] - ref[a:b]) <= 1e-12 * np.linalg.norm(ref[a:b]), (k, a)
         assert got[k, 1] == 0 and np.all(got[k, 9:12] == 0)
         o += s.npl
+
+
+# ---- discard checks (swg_system_discard) -----------------------------------
+# Against the oracle after every step: the discard counts and every status
+# exactly; the state (including the velocities the h2b conversion rewrites)
+# to 1e-12 relative.  The systems hold unbound and very eccentric particles,
+# whose drifts take the two branches that call libm transcendentals (see the
+# module docstring), so their trajectories are not bit-identical.
+
+def _discard_system(seed, npl=6, ntp=400, near=True):
+    """Planets plus test particles spread so that each check fires: some far
+    out, some close to the central body, some unbound, some on eccentric
+    orbits with small pericentres and (near=True) some next to a planet.
+    SyMBA resolves the latter as collisions inside its step, which the
+    oracle leaves to the host, so its systems use near=False."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    s = ics.whm_solar(seed=seed, npl=npl, ntp=0)
+    m = ntp
+    a = np.exp(rng.uniform(np.log(0.15), np.log(60.0), m))
+    e = np.where(rng.uniform(0, 1, m) < 0.3, rng.uniform(0.5, 0.95, m), rng.uniform(0, 0.1, m))
+    inc = rng.uniform(0, 0.1, m)
+    capom, omega, capm = (rng.uniform(0, 2 * np.pi, m) for _ in range(3))
+    r, v = ics.el2xv(np.full(m, s.GMcb), a, e, inc, capom, omega, capm)
+    v[: m // 10] *= 1.6                       # a tenth unbound
+    k = m // 10 if near else 0
+    j = rng.integers(0, npl, k)               # a tenth next to a planet
+    if k:
+        r[m - k:] = s.pl_rh[j] + rng.normal(0, 1e-3, (k, 3))
+        v[m - k:] = s.pl_vh[j] + rng.normal(0, 0.05, (k, 3))
+    s.tp_rh, s.tp_vh = r, v
+    s.radius = s.radius * 100.0
+    return s
+
+
+@pytest.mark.parametrize("integ", [S.HELIO, S.SYMBA, S.WHM])
+@pytest.mark.parametrize("qmin_coord", [0, 1])
+def test_discard_matches_oracle(integ, qmin_coord):
+    print(f"discard integ={integ} qmin_coord={qmin_coord}", flush=True)
+    sysic = _discard_system(41 + integ, near=integ != S.SYMBA)
+    kw = dict(rmin=0.2, rmax=40.0, rmaxu=30.0, qmin=0.3, qmin_alo=0.1, qmin_ahi=50.0, qmin_coord=qmin_coord)
+    g = S.GpuSystem(integ, sysic, 0.01, lclose=True)
+    o = O.OracleSystem(integ, sysic, 0.01, lclose=True)
+    tot = [0, 0]
+    for step in range(12):
+        g.step()
+        assert o.step() == 0
+        got, ref = g.discard(**kw), o.discard(**kw)
+        assert got == ref, (step, got, ref)
+        tot[0] += got[0]
+        tot[1] += got[1]
+        sg, so = g.state(), o.state()
+        sg["cb_vb"] = sg["cb_vb"][0]
+        for k in ("tp_status", "pl_status"):
+            assert np.array_equal(sg[k], so[k]), (step, k, np.argwhere(sg[k] != so[k])[:4].tolist())
+        for k in ("pl_rh", "pl_vh", "tp_rh", "tp_vh", "cb_vb", "pl_vb", "tp_vb"):
+            np.testing.assert_allclose(sg[k], so[k], rtol=1e-12, atol=1e-14, err_msg=f"step {step} {k}")
+        if got[0]:
+            break  # a system that lost massive bodies is rebuilt by the host
+    assert tot[1] > 20  # the checks fired
+
+
+def test_discard_symba_massive_bodies_and_ensemble():
+    systems = [_discard_system(51, npl=8, ntp=100, near=False), _discard_system(52, npl=5, ntp=0, near=False),
+               _discard_system(53, npl=3, ntp=60, near=False)]
+    systems[0].pl_rh[2], systems[0].pl_vh[2] = (100.0, 0.0, 0.0), (0.0, 0.63, 0.0)  # beyond rmax
+    systems[2].pl_rh[1], systems[2].pl_vh[1] = (35.0, 0.0, 0.0), (0.0, 2.0, 0.0)    # unbound, beyond rmaxu
+    kw = dict(rmin=0.1, rmax=45.0, rmaxu=30.0)
+    g = S.GpuSystem(S.SYMBA, systems, 0.01, lclose=True)
+    g.step()
+    got = g.discard(**kw)
+    sg = g.state()
+    po = to = 0
+    tot = [0, 0]
+    for s in systems:
+        o = O.OracleSystem(S.SYMBA, s, 0.01, lclose=True)
+        assert o.step() == 0
+        c = o.discard(**kw)
+        tot[0] += c[0]
+        tot[1] += c[1]
+        so = o.state()
+        assert np.array_equal(sg["pl_status"][po:po + s.npl], so["pl_status"])
+        assert np.array_equal(sg["tp_status"][to:to + s.ntp], so["tp_status"])
+        np.testing.assert_allclose(sg["tp_vb"][to:to + s.ntp], so["tp_vb"], rtol=1e-12, atol=1e-14)
+        po += s.npl
+        to += s.ntp
+    assert tuple(tot) == got and got[0] == 2

@@ -400,3 +400,111 @@ def test_energy_inactive_bodies_are_left_out_exactly():
 
 
 O_E = {"ke_orbit": 0, "ke_spin": 1, "pe": 2, "be": 3, "be_cb": 4, "te": 5, "GMtot": 15}
+
+
+# --------------------------------------------------------------- discards --
+# swo_discard restates swiftest_discard_system and its callees
+# (swiftest_discard.f90:13-337, symba_discard.f90:14-387).  Pinned by planted
+# bodies on either side of each threshold, the exact minimum distance of
+# linear relative motion for the close-approach test, a Kepler orbit's known
+# pericentre for the passage test, the reference's pericentre-flag sequence,
+# and zero barycentric momentum after the h2b conversion it performs.
+
+def _dstate(tp_rh, tp_vh, integ=O.HELIO, pl_rh=((3.0, 0, 0),), pl_vh=((0, 3.6, 0),), Gm=(1e-3,), R=(1e-3,),
+            rhill=(0.2,)):
+    pl_rh, pl_vh = np.array(pl_rh, float), np.array(pl_vh, float)
+    return O.DiscardState(integ, ics.GMSUN, 0.005, np.zeros(3), pl_rh, pl_vh, pl_vh.copy(), np.array(Gm, float),
+                          np.array(R, float), np.array(rhill, float), np.array(tp_rh, float), np.array(tp_vh, float),
+                          np.array(tp_vh, float))
+
+
+def test_discard_central_body_thresholds():
+    tp_rh = [[10.0 - 1e-12, 0, 0], [10.0 + 1e-12, 0, 0], [0.05, 0, 0], [0.003, 0, 0], [60.0, 0, 0], [60.0, 0, 0]]
+    esc = np.sqrt(2 * ics.GMSUN / 60.0)
+    tp_vh = [[0, 2, 0], [0, 2, 0], [0, 20, 0], [0, 20, 0], [0, 1.1 * esc, 0], [0, 0.9 * esc, 0]]
+    st = _dstate(tp_rh, tp_vh)
+    st.Rcb = 0.004  # rmin 0.01 floored at the central body's radius only when larger
+    n = st.discard(0.01, rmin=0.01, rmax=100.0, rmaxu=50.0)
+    assert list(st.tp_status) == [0, 0, 0, -2, -3, 0] and n == (0, 2)
+    st = _dstate(tp_rh, tp_vh)
+    n = st.discard(0.01, rmax=10.0)
+    assert list(st.tp_status) == [0, -1, 0, 0, -1, -1] and n == (0, 3)
+    st = _dstate(tp_rh, tp_vh)
+    st.Rcb = 0.06  # cb radius above rmin: the floor is the radius
+    st.discard(0.01, rmin=0.01)
+    assert list(st.tp_status) == [0, 0, -2, -2, 0, 0]
+
+
+def test_discard_h2b_zero_barycentric_momentum():
+    rng = np.random.Generator(np.random.PCG64(21))
+    n = 12
+    pl_rh, pl_vh = rng.normal(0, 3, (n, 3)), rng.normal(0, 4, (n, 3))
+    Gm = ics.GU * rng.uniform(1e-6, 1e-3, n)
+    st = _dstate(rng.normal(0, 5, (30, 3)), rng.normal(0, 3, (30, 3)), pl_rh=pl_rh, pl_vh=pl_vh, Gm=Gm,
+                 R=np.full(n, 1e-4), rhill=np.full(n, 0.1))
+    st.discard(0.01, rmax=1e9)
+    mom = (Gm[:, None] * st.pl_vb).sum(0) + ics.GMSUN * st.cb_vb
+    assert np.all(np.abs(mom) <= 1e-14 * (Gm[:, None] * np.abs(st.pl_vb)).sum(0).max())
+    np.testing.assert_array_equal(st.tp_vb, st.tp_vh + st.cb_vb)
+    np.testing.assert_array_equal(st.pl_vb, st.pl_vh + st.cb_vb)
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_discard_close_approach_exact(case):
+    # particle moving on a straight line past a body of radius R at the origin
+    R, dt = 0.01, 0.5
+    x0, v = [(0.005, 0, 0), (0.3, 0.02, 0), (0.3, 0.02, 0), (0.3, 0.005, 0), (0.3, 0.009, 0), (-0.3, 0.005, 0)][case], \
+        [(0, 1, 0), (-1, 0, 0), (-0.5, 0, 0), (-0.1, 0, 0), (-1, 0, 0), (-1, 0, 0)][case]
+    st = _dstate([x0], [v], pl_rh=[(0, 0, 0)], pl_vh=[(0, 0, 0)], R=[R])
+    st.discard(dt, lclose=True)
+    # exact minimum over [0, dt] of |x0 + v t|^2 (the reference evaluates the
+    # minimum of the line when it falls inside dt, else the end point)
+    X, V = [Fraction(q) for q in x0], [Fraction(q) for q in v]
+    r2 = sum(q * q for q in X)
+    vdotr = sum(a * b for a, b in zip(X, V))
+    v2 = sum(q * q for q in V)
+    if r2 <= Fraction(R) ** 2:
+        hit = True
+    elif vdotr > 0:
+        hit = False
+    else:
+        tmin = -vdotr / v2
+        T = tmin if tmin < Fraction(dt) else Fraction(dt)
+        hit = min(sum((a + b * T) ** 2 for a, b in zip(X, V)), r2) <= Fraction(R) ** 2
+    assert (st.tp_status[0] == -5) == hit, (case, st.tp_status[0])
+    assert [True, False, False, False, True, False][case] == hit  # both outcomes covered
+
+
+def test_discard_pericentre_sequence_and_value():
+    # a particle moving outward right after pericentre of a known ellipse
+    mu, a, e = ics.GMSUN, 1.5, 0.8
+    q = a * (1 - e)
+    r, v = ics.el2xv(np.array([mu]), np.array([a]), np.array([e]), np.zeros(1), np.zeros(1), np.zeros(1),
+                     np.array([0.01]))
+    st = _dstate(r, v, pl_rh=[(30.0, 0, 0)], pl_vh=[(0, 1.1, 0)])
+    kw = dict(qmin=0.5, qmin_alo=1.0, qmin_ahi=2.0)
+    st.discard(0.01, **kw)          # isperi 1 -> -1 (moving outward)
+    assert st.tp_isperi[0] == -1 and st.tp_status[0] == 0
+    st.discard(0.01, **kw)          # -1 and vdotr >= 0 -> 0: a and q formed, q <= qmin -> discarded
+    assert st.tp_isperi[0] == 0 and st.tp_status[0] == -4
+    assert st.tp_peri[0] == pytest.approx(q, rel=1e-12) and st.tp_atp[0] == pytest.approx(a, rel=1e-12)
+    st2 = _dstate(r, v, pl_rh=[(r[0, 0], r[0, 1] + 0.1, 0)], pl_vh=[(0, 1.1, 0)], rhill=(0.2,))
+    st2.discard(0.01, **kw)
+    st2.discard(0.01, **kw)         # inside a massive body's Hill sphere: kept
+    assert st2.tp_isperi[0] == 0 and st2.tp_status[0] == 0
+    st3 = _dstate(r, -v, pl_rh=[(30.0, 0, 0)], pl_vh=[(0, 1.1, 0)])
+    st3.discard(0.01, **kw)         # moving inward: isperi 1 -> 1
+    assert st3.tp_isperi[0] == 1
+
+
+def test_discard_symba_massive_bodies():
+    pl_rh = [(3.0, 0, 0), (120.0, 0, 0), (0.01, 0, 0), (4.0, 0, 0)]
+    pl_vh = [(0, 3.6, 0), (0, 0.5, 0), (0, 60, 0), (0, 3.1, 0)]
+    st = _dstate([(5.0, 0, 0)], [(0, 2.8, 0)], integ=O.SYMBA, pl_rh=pl_rh, pl_vh=pl_vh, Gm=(1e-4,) * 4, R=(1e-4,) * 4,
+                 rhill=(0.05,) * 4)
+    n = st.discard(0.01, rmin=0.05, rmax=100.0)
+    assert list(st.pl_status) == [0, -1, -2, 0] and n == (2, 0)
+    # the test particles then see only the bodies left: h2b over bodies 0 and 3
+    g = ics.GMSUN + 2e-4
+    cb = -(1e-4 * np.array(pl_vh[0]) + 1e-4 * np.array(pl_vh[3])) / g
+    np.testing.assert_allclose(st.cb_vb, cb, rtol=1e-15)
