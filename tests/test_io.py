@@ -1,0 +1,139 @@
+"""The reference's file formats (swiftest_amd.io): parameter file, ASCII
+initial conditions and the NetCDF output variables.  CPU only."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from swiftest_amd import io as sio
+from swiftest_amd import ics
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EXAMPLES = "/root/reference/examples/Swifter_Swiftest"
+
+PARAM_TEXT = """! a comment line
+T0               0.0
+TSTOP            1000000.0
+DT               0.005
+ISTEP_OUT        200000
+IN_FORM          xv
+IN_TYPE          ASCII
+PL_IN            pl.in
+TP_IN            tp.in
+CB_IN            cb.in
+CHK_QMIN         0.004650467260962157
+CHK_RMIN         0.004650467260962157
+CHK_RMAX         1000.0d0
+CHK_EJECT        1000.0
+CHK_QMIN_COORD   helio
+CHK_QMIN_RANGE   0.004650467260962157 1000.0
+MU2KG            1.988409870698051e+30
+TU2S             31557600.0
+DU2M             149597870700.0
+CHK_CLOSE        YES
+GR               NO
+RHILL_PRESENT    T
+ENCOUNTER_CHECK  SORTSWEEP
+GMTINY           2.1554293571575797e-06   ! trailing comment
+"""
+
+
+def test_param_reader(tmp_path):
+    p = tmp_path / "param.in"
+    p.write_text(PARAM_TEXT)
+    prm = sio.read_param(str(p))
+    assert prm["DT"] == 0.005 and prm["ISTEP_OUT"] == 200000 and prm["CHK_RMAX"] == 1000.0
+    assert prm["IN_FORM"] == "XV" and prm["CHK_QMIN_COORD"] == "HELIO"
+    assert prm["CHK_QMIN_RANGE"] == (0.004650467260962157, 1000.0)
+    assert prm["CHK_CLOSE"] is True and prm["GR"] is False and prm["RHILL_PRESENT"] is True
+    assert prm["ENCOUNTER_CHECK_PLPL"] == "SORTSWEEP" and prm["GMTINY"] == 2.1554293571575797e-06
+    assert prm["CB_IN"] == "cb.in" and prm["ROTATION"] is False  # default
+    # G in AU^3 / (M_sun yr^2) from the unit conversions, as the reference forms it
+    assert sio.gu(prm) == pytest.approx(ics.GMSUN, rel=1e-3)
+    q = tmp_path / "param2.in"
+    sio.write_param(str(q), prm)
+    assert sio.read_param(str(q)) == prm
+
+
+def _fixture_system(name):
+    with open(os.path.join(HERE, "golden", f"{name}.json")) as f:
+        d = json.load(f)
+    pl, tp = d["pl"], d["tp"]
+    s = ics.System(GMcb=d["cb"]["GM"], Rcb=d["cb"]["radius"], pl_rh=np.array([b["rh"] for b in pl]),
+                   pl_vh=np.array([b["vh"] for b in pl]), Gmass=np.array([b["Gmass"] for b in pl]),
+                   radius=np.array([b["radius"] for b in pl]),
+                   tp_rh=np.array([b["rh"] for b in tp]).reshape(-1, 3), tp_vh=np.array([b["vh"] for b in tp]).reshape(-1, 3))
+    extra = sio.Bodies(cb_name=d["cb"]["name"], pl_names=[b["name"] for b in pl], tp_names=[b["name"] for b in tp],
+                       rhill=np.array([b["rhill"] for b in pl]))
+    return s, extra
+
+
+@pytest.mark.parametrize("name", ["108pl_50tp", "8pl_0tp"])
+def test_body_files_round_trip(tmp_path, name):
+    s, extra = _fixture_system(name)
+    prm = dict(sio.DEFAULTS, IN_TYPE="ASCII", CB_IN="cb.in", PL_IN="pl.in", TP_IN="tp.in", CHK_CLOSE=True,
+               RHILL_PRESENT=True)
+    sio.write_in(prm, s, extra, str(tmp_path))
+    r, rx = sio.read_in(prm, str(tmp_path))
+    for k in ("pl_rh", "pl_vh", "Gmass", "radius", "tp_rh", "tp_vh"):
+        assert np.array_equal(getattr(r, k), getattr(s, k)), k
+    assert (r.GMcb, r.Rcb) == (s.GMcb, s.Rcb)
+    assert rx.pl_names == extra.pl_names and rx.tp_names == extra.tp_names and np.array_equal(rx.rhill, extra.rhill)
+
+
+def test_el_form_matches_el2xv(tmp_path):
+    (tmp_path / "cb.in").write_text("Sun\n39.476926408897626\n0.005\n0.0\n0.0\n")
+    (tmp_path / "pl.in").write_text("1\nP 1e-4\n1.5 0.1 5.0\n30.0 40.0 50.0\n")
+    (tmp_path / "tp.in").write_text("1\nT\n2.0 0.2 3.0\n10.0 20.0 30.0\n")
+    prm = dict(sio.DEFAULTS, IN_TYPE="ASCII", IN_FORM="EL", CB_IN="cb.in", PL_IN="pl.in", TP_IN="tp.in")
+    s, _ = sio.read_in(prm, str(tmp_path))
+    d = np.pi / 180
+    r, v = ics.el2xv(np.array([39.476926408897626 + 1e-4]), np.array([1.5]), np.array([0.1]), np.array([5.0 * d]),
+                     np.array([30.0 * d]), np.array([40.0 * d]), np.array([50.0 * d]))
+    assert np.array_equal(s.pl_rh, r) and np.array_equal(s.pl_vh, v)
+    assert s.radius[0] == 0.0 and s.ntp == 1
+
+
+@pytest.mark.skipif(not os.path.isdir(EXAMPLES), reason="the reference tree is not present")
+@pytest.mark.parametrize("name,cb", [("108pl_50tp", "cb.in"), ("8pl_0tp", "cb.swiftest.in")])
+def test_reads_the_reference_examples(name, cb):
+    prm = sio.read_param(os.path.join(EXAMPLES, name, "param.swiftest.in"))
+    prm["CB_IN"] = cb
+    s, extra = sio.read_in(prm, os.path.join(EXAMPLES, name))
+    f, fx = _fixture_system(name)
+    for k in ("pl_rh", "pl_vh", "Gmass", "tp_rh", "tp_vh"):
+        assert np.array_equal(getattr(s, k), getattr(f, k)), k
+    if prm["CHK_CLOSE"]:
+        assert np.array_equal(s.radius, f.radius)
+    assert extra.pl_names == fx.pl_names
+
+
+def test_netcdf_frames(tmp_path):
+    from scipy.io import netcdf_file
+    s, extra = _fixture_system("108pl_50tp")
+    path = str(tmp_path / "out.nc")
+    w = sio.NetCDFWriter(path, s, extra)
+    rng = np.random.Generator(np.random.PCG64(3))
+    frames = []
+    for t in (0.0, 1.0):
+        st = dict(pl_rh=rng.normal(size=(s.npl, 3)), pl_vh=rng.normal(size=(s.npl, 3)),
+                  tp_rh=rng.normal(size=(s.ntp, 3)), tp_vh=rng.normal(size=(s.ntp, 3)),
+                  pl_status=np.zeros(s.npl, np.int32), tp_status=np.zeros(s.ntp, np.int32),
+                  pl_id=np.arange(1, s.npl + 1, dtype=np.int32), rhill=rng.uniform(size=s.npl))
+        st["tp_status"][3] = -1
+        e = rng.normal(size=16)
+        w.write_frame(t, st, e)
+        frames.append((st, e))
+    w.close()
+    f = netcdf_file(path, "r", mmap=False)
+    assert f.variables["rh"].shape == (2, 1 + s.npl + s.ntp, 3)
+    for k, (st, e) in enumerate(frames):
+        assert np.array_equal(f.variables["rh"][k, 1:1 + s.npl], st["pl_rh"])
+        assert np.array_equal(f.variables["vh"][k, 1 + s.npl:], st["tp_vh"])
+        assert f.variables["PE"][k] == e[2] and np.array_equal(f.variables["L_orbit"][k], e[6:9])
+        assert f.variables["ntp"][k] == s.ntp - 1
+    names = [b"".join(r).decode().strip() for r in f.variables["name"][:]]
+    assert names[0] == extra.cb_name and names[1:1 + s.npl] == extra.pl_names
+    assert f.variables["Gmass"][0, 0] == s.GMcb
+    f.close()
