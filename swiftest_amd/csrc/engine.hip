@@ -329,7 +329,7 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
     owned.push_back(d_scan_sys);
     if (own(&bw.keepB, npl) || own(&bw.cntE, npl) || own(&bw.posB, npl) || own(&bw.posE, npl) ||
         own(&bw.keepBt, ntp) || own(&bw.cntEt, ntp) || own(&bw.posBt, ntp) || own(&bw.posEt, ntp) ||
-        own(&bw.tot, 4) || own(&bw.rng, 4 * (size_t)nsys) || own(&bw.caps, nsys) || own(&bw.maxsz, 1) ||
+        own(&bw.tot, 4) || own(&bw.rng, 4 * (size_t)nsys) || own(&bw.caps, nsys) || own(&bw.maxsz, 8) ||
         own(&d_arena_off, nsys + 1) || own(&d_sub_stats, 4))
         return SWG_ERR_HIP;
     bw.scan = d_scan;
@@ -338,7 +338,10 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
     HIPCHK(hipHostMalloc(&h_subargs, subtree_args_bytes()));
     HIPCHK(hipMalloc(&d_subargs, subtree_args_bytes()));
     owned.push_back(d_subargs);
-    if (const char *e = getenv("SWG_SUBTREE_MAX")) subtree_max = strtoll(e, nullptr, 10);
+    if (const char *e = getenv("SWG_SUBTREE_MAX")) {
+        subtree_max = strtoll(e, nullptr, 10);
+        subtree_forced = true;
+    }
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));
 
@@ -590,14 +593,15 @@ int Ensemble::build_level(int dnew, const LevelView *src, LevelView &v, LevelCou
     int64_t *h = (int64_t *)(h_small + 128);
     HIPCHK(hipMemcpyAsync(h, bw.tot, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(h + 4, d_arena_off + nsys, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(h + 5, bw.maxsz, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(h + 5, bw.maxsz, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     c.nB = h[0];
     c.nE = h[1];
     c.nBt = h[2];
     c.nEt = h[3];
     arena_need = h[4];
-    const int32_t maxsz = *(int32_t *)(h + 5);
+    const int32_t *mx = (const int32_t *)(h + 5);
+    const int32_t maxsz = mx[0];
     if (c.nE > INT32_MAX || c.nEt > INT32_MAX)
         return fail(SWG_ERR_SCRATCH, "recursion working set of level " + std::to_string(dnew) + " exceeds 2^31 entries");
     const int64_t bytes = level_bytes(c);
@@ -611,7 +615,14 @@ int Ensemble::build_level(int dnew, const LevelView *src, LevelView &v, LevelCou
     }
     launch_build_fill(st, la, src, npl, ntp, dnew, bw, lv_buf[dnew], c);
     v = level_view(lv_buf[dnew], c);
-    handoff = subtree_max >= 0 && maxsz <= subtree_max;
+    // Hand the rest of the recursion to one workgroup per system when every
+    // system's set fits the LDS-resident kernel, or when enough systems have
+    // work to fill the device with workgroups.  A single large set stays on
+    // the whole device, where its levels run many times faster than in one
+    // workgroup.  SWG_SUBTREE_MAX in the environment forces the size rule
+    // alone (hand-off modes of the tests).
+    const bool fits = subtree_lds_fits(mx[1], mx[2], mx[3], mx[4]);
+    handoff = subtree_max >= 0 && maxsz <= subtree_max && (subtree_forced || fits || mx[5] >= SUBTREE_MIN_SYSTEMS);
     return 0;
 }
 

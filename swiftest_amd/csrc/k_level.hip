@@ -95,7 +95,9 @@ __global__ __launch_bounds__(256) void k_build_count_tp(RecState a, BuildSrc src
 
 // Per system: the new level's body ranges (rng[4s .. 4s+3] = pl begin, pl
 // end, tp begin, tp end), the arena words its persistent workgroup needs for
-// the nested levels below (caps), and the largest per-system set (*maxsz).
+// the nested levels below (caps), and over the systems: the largest set
+// (maxsz[0], bodies + entries), the largest pl bodies, pl entries, tp bodies
+// and tp entries (maxsz[1..4]) and the number of systems with a set (maxsz[5]).
 __device__ inline int lower_bound_i32(const int32_t *a, int n, int key)
 {
     int lo = 0, hi = n;
@@ -142,7 +144,13 @@ __global__ void k_build_ranges(SysDev sy, BuildSrc src, int nsys, int dnew, cons
     if (nb + nbt > 0) w = (int64_t)(MAXLEV - dnew) * (level_layout(nb, e, nbt, et).words + 2 * (nb + nbt) + 4) + 8;
     caps[s] = (int32_t)(w > 0x7ffffffe ? 0x7ffffffe : w);
     const int64_t sz = nb + e + nbt + et;
-    atomicMax(maxsz, (int32_t)(sz > 0x7fffffff ? 0x7fffffff : sz));
+    auto cl = [](int64_t x) { return (int32_t)(x > 0x7fffffff ? 0x7fffffff : x); };
+    atomicMax(&maxsz[0], cl(sz));
+    atomicMax(&maxsz[1], cl(nb));
+    atomicMax(&maxsz[2], cl(e));
+    atomicMax(&maxsz[3], cl(nbt));
+    atomicMax(&maxsz[4], cl(et));
+    if (nb + nbt > 0) atomicAdd(&maxsz[5], 1);
 }
 
 // Destination of a build (pointers into a stored level)
@@ -344,22 +352,36 @@ __global__ __launch_bounds__(KT_T) void k_lv_kick_pl(RecState a, LevelView v, in
         }
         __syncthreads();
         if (tid < TB) {
-            // every slot holds finite values, so both sums are formed and the
-            // flags select which to keep: no branches, and the loads of
-            // several entries are in flight at once
+            // A term that does not add to a sum holds a zero factor for it,
+            // and fma(0, d, s) is s bit for bit (d is finite; a sum that
+            // starts at +0 is never -0), so every slot is folded without a
+            // select, with the loads two slots ahead of the arithmetic.
             const int kn = min(K, sn[tid] - c0);
-#pragma unroll 4
-            for (int k = 0; k < kn; k++) {
-                const int w = k * LD + tid;
-                const int fl = sfl[w];
-                const double fp = sfp[w], fm = sfm[w], dx = sdx[w], dy = sdy[w], dz = sdz[w];
-                const double px = fma(fp, dx, p.x), py = fma(fp, dy, p.y), pz = fma(fp, dz, p.z);
-                const double mx = fma(fm, dx, m.x), my = fma(fm, dy, m.y), mz = fma(fm, dz, m.z);
-                if (fl & 1) p = make_double3(px, py, pz);
-                if (fl & 2) m = make_double3(mx, my, mz);
-                anyp |= (fl & 1) != 0;
-                anym |= (fl & 2) != 0;
+            int fl = 0;
+            if (kn > 0) {
+                int w = tid;
+                double ap = sfp[w], am = sfm[w], ax = sdx[w], ay = sdy[w], az = sdz[w];
+                int af = sfl[w];
+                w = min(1, kn - 1) * LD + tid;
+                double bp = sfp[w], bm = sfm[w], bx = sdx[w], by = sdy[w], bz = sdz[w];
+                int bf = sfl[w];
+                for (int k = 0; k < kn; k++) {
+                    w = min(k + 2, kn - 1) * LD + tid;
+                    const double cp = sfp[w], cm = sfm[w], cx = sdx[w], cy = sdy[w], cz = sdz[w];
+                    const int cf = sfl[w];
+                    p.x = fma(ap, ax, p.x);
+                    p.y = fma(ap, ay, p.y);
+                    p.z = fma(ap, az, p.z);
+                    m.x = fma(am, ax, m.x);
+                    m.y = fma(am, ay, m.y);
+                    m.z = fma(am, az, m.z);
+                    fl |= af;
+                    ap = bp, am = bm, ax = bx, ay = by, az = bz, af = bf;
+                    bp = cp, bm = cm, bx = cx, by = cy, bz = cz, bf = cf;
+                }
             }
+            anyp |= (fl & 1) != 0;
+            anym |= (fl & 2) != 0;
         }
         __syncthreads();
     }
@@ -497,7 +519,7 @@ void launch_build_count(hipStream_t st, const LevelArgs &la, const LevelView *sr
     launch_scan_i32_i64(st, w.cntE, w.posE, s.nB, w.tot + 1, w.scan);
     launch_scan_i32_i64(st, w.keepBt, w.posBt, s.nBt, w.tot + 2, w.scan);
     launch_scan_i32_i64(st, w.cntEt, w.posEt, s.nBt, w.tot + 3, w.scan);
-    (void)hipMemsetAsync(w.maxsz, 0, sizeof(int32_t), st);
+    (void)hipMemsetAsync(w.maxsz, 0, 8 * sizeof(int32_t), st);
     k_build_ranges<<<cdiv(la.nsys, 256), 256, 0, st>>>(la.sy, s, la.nsys, dnew, w.posB, w.posE, w.posBt, w.posEt,
                                                        w.tot, w.rng, w.caps, w.maxsz);
 }

@@ -534,6 +534,29 @@ __device__ inline void lds_apply(double &vx, double &vy, double &vz, double dth,
     }
 }
 
+// Fold terms [u, hi) of a chunk into the running sums, in order.  A term
+// that does not add to a sum holds a zero factor for it, and fma(0, d, s) is
+// s bit for bit: d is finite, and a sum that starts at +0 is never -0 (an fma
+// gives -0 only from -0 operands).  So every term is folded without a
+// select, and the loads run two terms ahead of the arithmetic.
+__device__ inline void fold_terms(const KTerm *kt, int u, int hi, double3 &p, double3 &m, int &fl)
+{
+    if (u >= hi) return;
+    KTerm a = kt[u], b = kt[min(u + 1, hi - 1)];
+    for (; u < hi; u++) {
+        const KTerm c = kt[min(u + 2, hi - 1)];
+        p.x = fma(a.fp, a.dx, p.x);
+        p.y = fma(a.fp, a.dy, p.y);
+        p.z = fma(a.fp, a.dz, p.z);
+        m.x = fma(a.fm, a.dx, m.x);
+        m.y = fma(a.fm, a.dy, m.y);
+        m.z = fma(a.fm, a.dz, m.z);
+        fl |= a.fl;
+        a = b;
+        b = c;
+    }
+}
+
 // workgroup exclusive scan of one value per thread of L_T; *tot = the sum
 __device__ inline int lds_block_scan(LdsSet &S, int v, int *tot)
 {
@@ -691,20 +714,10 @@ __device__ void lds_kicks(LdsSet &S, int nB, int nBt, int nE, int nEt, int d, do
         }
 #pragma unroll
         for (int k = 0; k < L_KS; k++) {
-            const int hi = min(fe1[k], c0 + L_KC);
-#pragma unroll 4
-            for (int u = max(fe0[k], c0); u < hi; u++) {
-                const KTerm &K = S.kt[u - c0];
-                const int fl = K.fl;
-                const double px = fma(K.fp, K.dx, p[k].x), py = fma(K.fp, K.dy, p[k].y),
-                             pz = fma(K.fp, K.dz, p[k].z);
-                const double mx = fma(K.fm, K.dx, m[k].x), my = fma(K.fm, K.dy, m[k].y),
-                             mz = fma(K.fm, K.dz, m[k].z);
-                if (fl & 1) p[k] = make_double3(px, py, pz);
-                if (fl & 2) m[k] = make_double3(mx, my, mz);
-                anyp[k] |= (fl & 1) != 0;
-                anym[k] |= (fl & 2) != 0;
-            }
+            int fl = 0;
+            fold_terms(S.kt, max(fe0[k], c0) - c0, min(fe1[k], c0 + L_KC) - c0, p[k], m[k], fl);
+            anyp[k] |= (fl & 1) != 0;
+            anym[k] |= (fl & 2) != 0;
         }
         __syncthreads();
         if (pr) {
@@ -1126,6 +1139,17 @@ __global__ __launch_bounds__(L_T) void k_subtree_lds(const SubArgs *__restrict__
 }
 
 size_t subtree_args_bytes() { return sizeof(SubArgs); }
+
+static bool lds_mode()
+{
+    const char *e = getenv("SWG_SUBTREE_LDS");
+    return !(e && e[0] == '0');
+}
+
+bool subtree_lds_fits(int nb, int ne, int nbt, int net)
+{
+    return lds_mode() && nb <= L_NB && ne <= L_NE && nbt <= L_NBT && net <= L_NET;
+}
 
 // diagnostic profile of the launches since the last report
 // (SWG_PROF_SUBTREE=1), printed to stderr

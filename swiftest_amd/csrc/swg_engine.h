@@ -44,8 +44,9 @@ struct Ensemble {
     int64_t *d_total = nullptr;
     void *d_scan = nullptr;
     // recursion below level 0: whole-device working sets, one buffer per
-    // depth, until every system's set is at most subtree_max bodies + entries;
-    // then per-system arenas of the persistent workgroups
+    // depth, until every system's set is at most subtree_max bodies + entries
+    // and either fits the LDS-resident workgroup or enough systems have work
+    // to fill the device; then one persistent workgroup per system
     BuildScratch bw{};
     std::vector<void *> lv_buf;
     std::vector<int64_t> lv_cap;
@@ -55,6 +56,8 @@ struct Ensemble {
     int64_t arena_words = 0, arena_need = 0;
     void *h_subargs = nullptr, *d_subargs = nullptr;
     int64_t subtree_max = 16384;  // SWG_SUBTREE_MAX; < 0: every level on the whole device
+    bool subtree_forced = false;  // SWG_SUBTREE_MAX given: hand off on size alone
+    static constexpr int SUBTREE_MIN_SYSTEMS = 128;  // systems with work that fill the device
     LevelArgs level_args() const;
     int build_level(int dnew, const LevelView *src, LevelView &v, LevelCounts &c, bool &handoff);
     int recur_level(int d, const LevelView &v, const LevelCounts &c);

@@ -130,7 +130,7 @@ struct BuildScratch {
     void *scan;      // scan_scratch_bytes(max(npl, ntp))
     int32_t *rng;    // [4*nsys] new level's per-system body ranges
     int32_t *caps;   // [nsys] arena words of each system's persistent workgroup
-    int32_t *maxsz;  // largest per-system set (bodies + entries)
+    int32_t *maxsz;  // [8]: largest per-system set and its parts, systems with a set
 };
 LevelView level_view(void *base, const LevelCounts &c);
 int64_t level_bytes(const LevelCounts &c);
@@ -148,6 +148,8 @@ void launch_level_collision(hipStream_t st, const LevelArgs &la, const LevelView
 void launch_level_set_levels(hipStream_t st, const LevelArgs &la, const LevelView &v, const LevelCounts &c, int d);
 
 // ---- SyMBA recursion from level d0 down, one workgroup per system (k_subtree.hip) ----
+// whether every system's set of this size fits the LDS-resident subtree kernel
+bool subtree_lds_fits(int nb, int ne, int nbt, int net);
 size_t subtree_args_bytes();
 void subtree_profile_report(hipStream_t st);
 void launch_subtree(hipStream_t st, const LevelArgs &la, const LevelView &start, const int32_t *rng, int d0,
