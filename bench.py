@@ -106,6 +106,35 @@ def detection_leg(sysic, log, sparse_npl=131072):
     return out
 
 
+def output_leg(g, sysic, log):
+    """The output-cadence work on the benchmark system after its timed steps:
+    energy and angular momentum (one O(N^2) pair-potential sum) and the
+    discard checks with the thresholds of the reference's 108-body example
+    (CHK_RMIN = CHK_QMIN = solar radius, CHK_RMAX = CHK_EJECT = 1000 AU,
+    HELIO pericentres).  Each is timed over three calls after a first one
+    that allocates, host round trip included."""
+    import time
+    reps = 3
+    g.energy()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        e = g.energy()
+    ms_e = 1e3 * (time.perf_counter() - t0) / reps
+    kw = dict(rmin=sysic.Rcb, rmax=1000.0, rmaxu=1000.0, qmin=sysic.Rcb, qmin_alo=sysic.Rcb, qmin_ahi=1000.0,
+              qmin_coord=0)
+    first = g.discard(**kw)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.discard(**kw)
+    ms_d = 1e3 * (time.perf_counter() - t0) / reps
+    n = sysic.npl
+    res = {"energy_ms": ms_e, "energy_pairs_per_s": n * (n - 1) / 2 / (ms_e * 1e-3), "te": float(e[0, 5]),
+           "discard_ms": ms_d, "discarded_first_call": list(first), "discard_bodies": n + sysic.ntp}
+    log(f"[bench] energy {ms_e:.3f} ms ({res['energy_pairs_per_s']:.3e} pairs/s), discard checks {ms_d:.3f} ms "
+        f"(first call discarded {first})")
+    return res
+
+
 def ensemble_leg(nsys, steps, warmup, sample, log):
     """Batched ensemble (config C4): nsys independent SyMBA systems of uneven
     size advanced together, one launch per step phase for all of them, with
@@ -283,6 +312,7 @@ def main():
                               "recursion, reported under encounters")},
         "cpu_baseline": None,
     }
+    out["output_cadence"] = output_leg(g, sysic, log)
     del g
     out["encounter_detection"] = detection_leg(sysic, log)
     if not args.no_ensemble:
