@@ -113,6 +113,7 @@ Ensemble::~Ensemble()
     if (h_small) (void)hipHostFree(h_small);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (ev_any) (void)hipEventDestroy(ev_any);
     for (hipEvent_t e : ev_tp) (void)hipEventDestroy(e);
     for (hipEvent_t e : ev_pl) (void)hipEventDestroy(e);
     if (d_arena) (void)hipFree(d_arena);
@@ -257,30 +258,30 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
     std::vector<double> tmp;
     if (upload_aos(st, pl.rh, rh.data(), npl, tmp) || upload_aos(st, pl.vh, vh.data(), npl, tmp)) return SWG_ERR_HIP;
     if (upload_aos(st, tp.rh, tp_rh, ntp, tmp) || upload_aos(st, tp.vh, tp_vh, ntp, tmp)) return SWG_ERR_HIP;
-    HIPCHK(hipMemcpy(pl.Gm, gm.data(), npl * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(pl.R, rad.data(), npl * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(pl.rhill, rhill.data(), npl * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(copy_wait(stream, pl.Gm, gm.data(), npl * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(copy_wait(stream, pl.R, rad.data(), npl * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(copy_wait(stream, pl.rhill, rhill.data(), npl * sizeof(double), hipMemcpyHostToDevice));
     std::vector<int32_t> zeros(std::max(std::max(npl, ntp), 3 * nsys) + 1, 0);
     std::vector<int32_t> sys_pl(npl), sys_tp(ntp), ones(nsys, 1);
     for (int s = 0; s < nsys; s++) {
         for (int i = h_pl_off[s]; i < h_pl_off[s + 1]; i++) sys_pl[i] = s;
         for (int i = h_tp_off[s]; i < h_tp_off[s + 1]; i++) sys_tp[i] = s;
     }
-    HIPCHK(hipMemcpy(pl.status, zeros.data(), npl * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(tp.status, zeros.data(), ntp * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(pl.sys, sys_pl.data(), npl * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(tp.sys, sys_tp.data(), ntp * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(sy.pl_off, h_pl_off.data(), (nsys + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(sy.tp_off, h_tp_off.data(), (nsys + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(sy.GMcb, GMcb, nsys * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(copy_wait(stream, pl.status, zeros.data(), npl * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(copy_wait(stream, tp.status, zeros.data(), ntp * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(copy_wait(stream, pl.sys, sys_pl.data(), npl * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(copy_wait(stream, tp.sys, sys_tp.data(), ntp * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(copy_wait(stream, sy.pl_off, h_pl_off.data(), (nsys + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(copy_wait(stream, sy.tp_off, h_tp_off.data(), (nsys + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(copy_wait(stream, sy.GMcb, GMcb, nsys * sizeof(double), hipMemcpyHostToDevice));
     std::vector<double> dz(3 * (size_t)nsys, 0.0);
-    HIPCHK(hipMemcpy(sy.ptbeg, dz.data(), 3 * nsys * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(sy.ptend, dz.data(), 3 * nsys * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(sy.cbvb, dz.data(), 3 * nsys * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(sy.lfirst, ones.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(copy_wait(stream, sy.ptbeg, dz.data(), 3 * nsys * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(copy_wait(stream, sy.ptend, dz.data(), 3 * nsys * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(copy_wait(stream, sy.cbvb, dz.data(), 3 * nsys * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(copy_wait(stream, sy.lfirst, ones.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice));
     h_frozen.assign(nsys, 0);
     for (int32_t *q : {sy.enc_pp, sy.enc_pt, sy.flag_pp, sy.flag_pt, sy.err, sy.frozen})
-        HIPCHK(hipMemcpy(q, zeros.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHK(copy_wait(stream, q, zeros.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice));
 
     // ---- tiles ----
     build_tiles();
@@ -302,8 +303,8 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
         if (own2(&pp.words, w) || own2(&pp.wpre, w) || own2(&pp.woff, nsys) || own2(&pp.nb, nsys) ||
             own2(&pp.rowcnt, npl) || own2(&pp.rowstart, npl) || own2(&pp.colcnt, npl) || own2(&pp.colstart, npl))
             return SWG_ERR_HIP;
-        HIPCHK(hipMemcpy(pp.woff, woff.data(), nsys * sizeof(int64_t), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(pp.nb, nb.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHK(copy_wait(stream, pp.woff, woff.data(), nsys * sizeof(int64_t), hipMemcpyHostToDevice));
+        HIPCHK(copy_wait(stream, pp.nb, nb.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice));
     }
     {
         std::vector<int64_t> woff(nsys);
@@ -319,8 +320,8 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
         if (own2(&pt.woff, nsys) || own2(&pt.nb, nsys) || own2(&pt.rowcnt, npl) || own2(&pt.rowstart, npl) ||
             own2(&pt.colcnt, ntp) || own2(&pt.colstart, ntp))
             return SWG_ERR_HIP;
-        HIPCHK(hipMemcpy(pt.woff, woff.data(), nsys * sizeof(int64_t), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(pt.nb, nb.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHK(copy_wait(stream, pt.woff, woff.data(), nsys * sizeof(int64_t), hipMemcpyHostToDevice));
+        HIPCHK(copy_wait(stream, pt.nb, nb.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice));
     }
     int mx = std::max(npl, ntp);
     HIPCHK(hipMalloc(&d_scan, scan_scratch_bytes(mx)));
@@ -344,6 +345,7 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
     }
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));
+    HIPCHK(hipEventCreateWithFlags(&ev_any, hipEventDisableTiming));
 
     // ---- device part of the initialize sequence: vh->vb, tp vb = vh + cb%vb ----
     if (prm.integrator == SWG_WHM) {
@@ -389,7 +391,7 @@ void Ensemble::build_tiles()
         *dptr = nullptr;
         cnt = (int)vec.size();
         if (hipMalloc((void **)dptr, sizeof(T) * (vec.size() ? vec.size() : 1)) == hipSuccess && cnt)
-            (void)hipMemcpy(*dptr, vec.data(), sizeof(T) * vec.size(), hipMemcpyHostToDevice);
+            (void)copy_wait(stream, *dptr, vec.data(), sizeof(T) * vec.size(), hipMemcpyHostToDevice);
     };
     put(tpl, &d_tiles_pl, ntiles_pl);
     put(ttp, &d_tiles_tp, ntiles_tp);
@@ -522,12 +524,20 @@ int Ensemble::build_adjacency()
     return 0;
 }
 
-int Ensemble::read_any(int32_t &v)
+// Queue the copy of the encounter flag to the host; wait_any waits for that
+// copy alone, so work queued after post_any keeps the device busy meanwhile.
+int Ensemble::post_any()
 {
     int32_t *h = (int32_t *)(h_small + 32);
     HIPCHK(hipMemcpyAsync(h, d_any, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipStreamSynchronize(stream));
-    v = *h;
+    HIPCHK(hipEventRecord(ev_any, stream));
+    return 0;
+}
+
+int Ensemble::wait_any(int32_t &v)
+{
+    HIPCHK(hipEventSynchronize(ev_any));
+    v = *(const int32_t *)(h_small + 32);
     return 0;
 }
 
@@ -557,11 +567,14 @@ int Ensemble::recur0()
     launch_clear_flags(st, sy, nsys, d_any);
     launch_list_check_pp(st, pp, pl, sy, npp, 1, dt, d_any);
     launch_list_check_pt(st, pt, pl, tp, sy, npt, 1, dt, prm.short_circuit, d_any);
-    int32_t lenc = 0;
-    if (int rc = read_any(lenc)) return rc;
+    // the kicks and the drift depend on the check through device memory only:
+    // they are queued before the host waits for the flag
+    if (int rc = post_any()) return rc;
     launch_level_kicks(st, la, nullptr, npl, ntp, 0, dth);
     launch_drift(st, pl.rh, pl.vb, pl.status, pl.levelg, pl.sys, sy.GMcb, sy.err, sy.frozen, npl, dt, 0, 1, 0);
     launch_drift(st, tp.rh, tp.vb, tp.status, tp.levelg, tp.sys, sy.GMcb, sy.err, sy.frozen, ntp, dt, 0, 1, 1);
+    int32_t lenc = 0;
+    if (int rc = wait_any(lenc)) return rc;
     if (lenc) {
         LevelView v;
         LevelCounts c;
@@ -647,10 +660,11 @@ int Ensemble::recur_level(int d, const LevelView &v, const LevelCounts &c)
         tr_E[d] += c.nE + c.nEt;
         launch_clear_flags(st, sy, nsys, d_any);
         launch_level_check(st, la, v, c, d, dtl, prm.short_circuit, d_any);
-        int32_t lenc = 0;
-        if (int rc = read_any(lenc)) return rc;
+        if (int rc = post_any()) return rc;
         launch_level_kicks(st, la, &v, nB, nBt, d, dth);
         launch_level_drift(st, la, v, nB, nBt, d, dtl);
+        int32_t lenc = 0;
+        if (int rc = wait_any(lenc)) return rc;
         if (lenc) {
             if (d + 1 > MAXLEV) return fail(SWG_ERR_DEPTH, "recursion deeper than the level stack");
             LevelView nv;
@@ -685,8 +699,8 @@ int Ensemble::run_subtree(int d0, const LevelView &v)
         arena_words = w;
     }
     // the staging buffer of the argument block is rewritten only after the
-    // next stream synchronization (build_level or read_any), so the previous
-    // copy out of it has completed
+    // next stream synchronization of build_level, so the previous copy out of
+    // it has completed
     launch_subtree(st, level_args(), v, bw.rng, d0, prm.dt, prm.lclose, prm.short_circuit, d_arena, d_arena_off,
                    d_sub_stats, h_subargs, d_subargs);
     HIPCHK(hipGetLastError());
@@ -707,8 +721,8 @@ int Ensemble::energy(const double *cb_mass, const double *cb_radius, const doubl
             own(&en_tiles, (size_t)en_ntiles) || own(&en_toff, (size_t)nsys + 1))
             return SWG_ERR_HIP;
         if (en_ntiles)
-            HIPCHK(hipMemcpy(en_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(en_toff, toff.data(), toff.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+            HIPCHK(copy_wait(stream, en_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+        HIPCHK(copy_wait(stream, en_toff, toff.data(), toff.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         en_ready = true;
     }
     if (npl) HIPCHK(hipMemcpyAsync(en_mass, pl_mass, npl * sizeof(double), hipMemcpyHostToDevice, st));
@@ -742,18 +756,18 @@ int Ensemble::discard(const swg_discard_params &dp, const double *cb_radius, int
         std::vector<int32_t> one(std::max(std::max(npl, ntp), nsys), 1);
         std::vector<double> zero(std::max(npl, ntp), 0.0);
         if (npl) {
-            HIPCHK(hipMemcpy(dis_pl_isperi, one.data(), npl * sizeof(int32_t), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(dis_pl_atp, zero.data(), npl * sizeof(double), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(dis_pl_peri, zero.data(), npl * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(copy_wait(stream, dis_pl_isperi, one.data(), npl * sizeof(int32_t), hipMemcpyHostToDevice));
+            HIPCHK(copy_wait(stream, dis_pl_atp, zero.data(), npl * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(copy_wait(stream, dis_pl_peri, zero.data(), npl * sizeof(double), hipMemcpyHostToDevice));
         }
         if (ntp) {
-            HIPCHK(hipMemcpy(dis_tp_isperi, one.data(), ntp * sizeof(int32_t), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(dis_tp_atp, zero.data(), ntp * sizeof(double), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(dis_tp_peri, zero.data(), ntp * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(copy_wait(stream, dis_tp_isperi, one.data(), ntp * sizeof(int32_t), hipMemcpyHostToDevice));
+            HIPCHK(copy_wait(stream, dis_tp_atp, zero.data(), ntp * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(copy_wait(stream, dis_tp_peri, zero.data(), ntp * sizeof(double), hipMemcpyHostToDevice));
         }
-        HIPCHK(hipMemcpy(dis_lfirst, one.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHK(copy_wait(stream, dis_lfirst, one.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice));
         if (dis_ntiles)
-            HIPCHK(hipMemcpy(dis_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+            HIPCHK(copy_wait(stream, dis_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
         dis_ready = true;
     }
     HIPCHK(hipMemcpyAsync(dis_rcb, cb_radius, nsys * sizeof(double), hipMemcpyHostToDevice, st));
@@ -1057,8 +1071,8 @@ int Ensemble::get(double *pl_rh, double *pl_vh, double *pl_vb, double *tp_rh, do
     if (tp_rh && download_aos(st, tp.rh, tp_rh, ntp)) return SWG_ERR_HIP;
     if (tp_vh && download_aos(st, tp.vh, tp_vh, ntp)) return SWG_ERR_HIP;
     if (tp_vb && download_aos(st, tp.vb, tp_vb, ntp)) return SWG_ERR_HIP;
-    if (pl_status) HIPCHK(hipMemcpy(pl_status, pl.status, npl * sizeof(int32_t), hipMemcpyDeviceToHost));
-    if (tp_status) HIPCHK(hipMemcpy(tp_status, tp.status, ntp * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (pl_status) HIPCHK(copy_wait(stream, pl_status, pl.status, npl * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (tp_status) HIPCHK(copy_wait(stream, tp_status, tp.status, ntp * sizeof(int32_t), hipMemcpyDeviceToHost));
     if (pl_id) std::copy(h_pl_id.begin(), h_pl_id.end(), pl_id);
     return 0;
 }
@@ -1066,11 +1080,11 @@ int Ensemble::get(double *pl_rh, double *pl_vh, double *pl_vb, double *tp_rh, do
 int Ensemble::get_misc(double *rhill, double *cb_vb, double *ptbeg, double *ptend, int32_t *lfirst)
 {
     HIPCHK(hipStreamSynchronize(stream));
-    if (rhill) HIPCHK(hipMemcpy(rhill, pl.rhill, npl * sizeof(double), hipMemcpyDeviceToHost));
-    if (cb_vb) HIPCHK(hipMemcpy(cb_vb, sy.cbvb, 3 * nsys * sizeof(double), hipMemcpyDeviceToHost));
-    if (ptbeg) HIPCHK(hipMemcpy(ptbeg, sy.ptbeg, 3 * nsys * sizeof(double), hipMemcpyDeviceToHost));
-    if (ptend) HIPCHK(hipMemcpy(ptend, sy.ptend, 3 * nsys * sizeof(double), hipMemcpyDeviceToHost));
-    if (lfirst) HIPCHK(hipMemcpy(lfirst, sy.lfirst, nsys * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (rhill) HIPCHK(copy_wait(stream, rhill, pl.rhill, npl * sizeof(double), hipMemcpyDeviceToHost));
+    if (cb_vb) HIPCHK(copy_wait(stream, cb_vb, sy.cbvb, 3 * nsys * sizeof(double), hipMemcpyDeviceToHost));
+    if (ptbeg) HIPCHK(copy_wait(stream, ptbeg, sy.ptbeg, 3 * nsys * sizeof(double), hipMemcpyDeviceToHost));
+    if (ptend) HIPCHK(copy_wait(stream, ptend, sy.ptend, 3 * nsys * sizeof(double), hipMemcpyDeviceToHost));
+    if (lfirst) HIPCHK(copy_wait(stream, lfirst, sy.lfirst, nsys * sizeof(int32_t), hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -1081,8 +1095,8 @@ int Ensemble::enc_list(int which, int64_t *n, int32_t *i1, int32_t *i2)
     *n = m;
     if (!i1 || !m) return 0;
     HIPCHK(hipStreamSynchronize(stream));
-    HIPCHK(hipMemcpy(i1, L.i1, m * sizeof(int32_t), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(i2, L.i2, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHK(copy_wait(stream, i1, L.i1, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHK(copy_wait(stream, i2, L.i2, m * sizeof(int32_t), hipMemcpyDeviceToHost));
     for (int64_t k = 0; k < m; k++) {
         i1[k] += 1;
         i2[k] += 1;
@@ -1192,7 +1206,7 @@ int Ensemble::time_encounters(int reps, double *ms, int64_t *npairs, int64_t *nc
     if (ncand) {
         *ncand = -1;
         if (prm.encounter_check == 1) {
-            HIPCHK(hipMemcpy(ncand, d_total, sizeof(int64_t), hipMemcpyDeviceToHost));
+            HIPCHK(copy_wait(stream, ncand, d_total, sizeof(int64_t), hipMemcpyDeviceToHost));
         }
     }
     return 0;
@@ -1346,7 +1360,7 @@ int swg_system_stats(swg_system *sys, swg_stats *st)
     swg::Ensemble *e = sys->e;
     int32_t dev[4] = {0, 0, 0, 0};
     HIPCHK(hipStreamSynchronize(e->stream));
-    HIPCHK(hipMemcpy(dev, e->d_sub_stats, sizeof dev, hipMemcpyDeviceToHost));
+    HIPCHK(copy_wait(e->stream, dev, e->d_sub_stats, sizeof dev, hipMemcpyDeviceToHost));
     *st = e->st_;
     st->nsubsteps += dev[0];  // substeps run by the persistent recursion kernel
     if (dev[1] > st->max_depth) st->max_depth = dev[1];

@@ -558,13 +558,13 @@ int Ensemble::whm_setup()
             muj[a + q] = (h_GMcb[s] * eta[a + q]) / eta[a + q - 1];
         }
     }
-    HIPCHK(hipMemcpy(whm.eta, eta.data(), npl * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(whm.muj, muj.data(), npl * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(copy_wait(stream, whm.eta, eta.data(), npl * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(copy_wait(stream, whm.muj, muj.data(), npl * sizeof(double), hipMemcpyHostToDevice));
     std::vector<double> z(3 * (size_t)std::max(npl, ntp) + 3, 0.0);
     for (double *p : {whm.ah.x, whm.ah.y, whm.ah.z, whm.xj.x, whm.xj.y, whm.xj.z, whm.vj.x, whm.vj.y, whm.vj.z})
-        HIPCHK(hipMemcpy(p, z.data(), npl * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(copy_wait(stream, p, z.data(), npl * sizeof(double), hipMemcpyHostToDevice));
     for (double *p : {whm.tah.x, whm.tah.y, whm.tah.z})
-        HIPCHK(hipMemcpy(p, z.data(), ntp * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(copy_wait(stream, p, z.data(), ntp * sizeof(double), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -595,9 +595,9 @@ int Ensemble::whm_snapshot()
         for (int which = 0; which < 2; which++) {
             Vec3p v = which == 0 ? pl.rh : pl.vh;
             std::vector<double> &o = which == 0 ? rh : vh;
-            HIPCHK(hipMemcpy(cx.data(), v.x, npl * sizeof(double), hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(cy.data(), v.y, npl * sizeof(double), hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(cz.data(), v.z, npl * sizeof(double), hipMemcpyDeviceToHost));
+            HIPCHK(copy_wait(stream, cx.data(), v.x, npl * sizeof(double), hipMemcpyDeviceToHost));
+            HIPCHK(copy_wait(stream, cy.data(), v.y, npl * sizeof(double), hipMemcpyDeviceToHost));
+            HIPCHK(copy_wait(stream, cz.data(), v.z, npl * sizeof(double), hipMemcpyDeviceToHost));
             for (int i = 0; i < npl; i++) {
                 o[3 * i] = cx[i];
                 o[3 * i + 1] = cy[i];
@@ -608,7 +608,7 @@ int Ensemble::whm_snapshot()
             int b = h_pl_off[s], n = h_pl_off[s + 1] - b;
             host_set_rhill(h_GMcb[s], n, &rh[3 * (size_t)b], &vh[3 * (size_t)b], &h_Gm[b], &rhill[b]);
         }
-        HIPCHK(hipMemcpy(pl.rhill, rhill.data(), npl * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(copy_wait(stream, pl.rhill, rhill.data(), npl * sizeof(double), hipMemcpyHostToDevice));
     }
     return 0;
 }
@@ -830,11 +830,11 @@ static int leaf_encounter(swg_ctx *ctx, int which, int32_t npl, int32_t ntp, con
             tmp[n + i] = aos[3 * i + 1];
             tmp[2 * n + i] = aos[3 * i + 2];
         }
-        HIPCHK(hipMemcpy(dst, tmp.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(copy_wait(st, dst, tmp.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice));
         return 0;
     };
     if ((rc = soa(rpl, prx, npl)) || (rc = soa(vpl, pvx, npl))) return rc;
-    HIPCHK(hipMemcpy(pre, renc, npl * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(copy_wait(st, pre, renc, npl * sizeof(double), hipMemcpyHostToDevice));
     k_fill_i32<<<cdiv(std::max(npl, ntp), 256), 256, 0, st>>>(psys, std::max(npl, ntp), 0);
     int32_t ho[4] = {0, npl, 0, ntp};
     HIPCHK(hipMemcpyAsync(offs, ho, sizeof ho, hipMemcpyHostToDevice, st));

@@ -1159,7 +1159,7 @@ void subtree_profile_report(hipStream_t st)
     if (!g_prof) return;
     unsigned long long h[NPROF * (MAXLEV + 2)];
     (void)hipStreamSynchronize(st);
-    (void)hipMemcpy(h, g_prof, sizeof h, hipMemcpyDeviceToHost);
+    (void)copy_wait(st, h, g_prof, sizeof h, hipMemcpyDeviceToHost);
     const double tick_us = 0.01;  // wall_clock64 runs at 100 MHz
     fprintf(stderr, "[swg] level substeps   avg_nB   avg_nE  check_ms  kick1_ms  drift_ms kick2/bld levels_ms"
                     "   eval_ms   fold_ms kick_nmax kick_nact  drifted drift>1 lists drift1_us\n");
@@ -1210,7 +1210,7 @@ void launch_subtree(hipStream_t st, const LevelArgs &la, const LevelView &start,
     a.prof = nullptr;
     if (getenv("SWG_PROF_SUBTREE") && getenv("SWG_PROF_SUBTREE")[0] == '1') {
         if (!g_prof && hipMalloc((void **)&g_prof, sizeof(unsigned long long) * NPROF * (MAXLEV + 2)) == hipSuccess)
-            (void)hipMemset(g_prof, 0, sizeof(unsigned long long) * NPROF * (MAXLEV + 2));
+            (void)hipMemsetAsync(g_prof, 0, sizeof(unsigned long long) * NPROF * (MAXLEV + 2), st);
         if (g_prof) a.prof = g_prof;
     }
     (void)hipMemcpyAsync(d_args, h_stage, sizeof(SubArgs), hipMemcpyHostToDevice, st);

@@ -67,7 +67,7 @@ struct Ensemble {
     int64_t tr_sub[MAXLEV + 2] = {}, tr_B[MAXLEV + 2] = {}, tr_E[MAXLEV + 2] = {}, tr_hand[MAXLEV + 2] = {};
     void trace_levels();
     char *h_small = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_any = nullptr;
     std::vector<void *> owned;
 
     swg_stats st_{};
@@ -141,7 +141,8 @@ struct Ensemble {
     SasWork sas;
     int detect_pt(double dt, const std::vector<int32_t> &run);
     int build_adjacency();
-    int read_any(int32_t &v);
+    int post_any();
+    int wait_any(int32_t &v);
     int recur0();
     int step_helio_symba();
     int whm_setup();

@@ -11,6 +11,15 @@
 
 namespace swg {
 
+// A copy on the library's own stream, waited for.  The library never uses the
+// legacy stream: a synchronous legacy-stream call fails while any stream of
+// the process is capturing a graph, and other threads capture theirs.
+inline hipError_t copy_wait(hipStream_t st, void *dst, const void *src, size_t n, hipMemcpyKind kind)
+{
+    const hipError_t e = hipMemcpyAsync(dst, src, n, kind, st);
+    return e == hipSuccess ? hipStreamSynchronize(st) : e;
+}
+
 // A block of consecutive bodies of one system: bodies [b0, b1) of system s.
 struct Tile {
     int32_t s, b0, b1, pad;
