@@ -110,7 +110,23 @@ __device__ inline void scget(double angle, double &sx, double &cx)
     if ((x > PIBY2) && (x < PI3BY2)) cx = -cx;
 }
 
-// Stumpff c0..c3 with /4 argument reduction (:536-580)
+// x / c for a constant integer c > 0, c < 2^26, not a power of two, given
+// r = 1/c rounded: q = x*r lies within 1.5 ulp of x/c, the residual
+// fma(-q, c, x) is exact, and q + residual*r lies within 2^-52 ulp of x/c.
+// x/c is never a rounding midpoint (c has an odd factor above 1) and lies at
+// least ulp/(2c) from every midpoint, so the corrected quotient is the IEEE
+// quotient x/c whenever that quotient is a normal number.  Three dependent
+// operations instead of the division sequence.
+__host__ __device__ inline double cdiv(double x, double c, double r)
+{
+    const double q = x * r;
+    return fma(fma(-q, c, x), r, q);
+}
+
+// Stumpff c0..c3 with /4 argument reduction (:536-580).  The series
+// quotients use cdiv: each one feeds 1 - q, and a quotient too small to be
+// normal (|x| < 2^-1014) leaves 1 - q = 1 whichever way it rounds, so
+// c0..c3 equal the IEEE evaluation for every finite x.
 __host__ __device__ inline void stumpff(double x, double &c0, double &c1, double &c2, double &c3)
 {
     int n = 0;
@@ -118,18 +134,18 @@ __host__ __device__ inline void stumpff(double x, double &c0, double &c1, double
         n += 1;
         x = x / 4.0;
     }
-    double t = 1.0 - x / 182.0;
-    t = 1.0 - (x * t) / 132.0;
-    t = 1.0 - (x * t) / 90.0;
-    t = 1.0 - (x * t) / 56.0;
-    t = 1.0 - (x * t) / 30.0;
-    c2 = (1.0 - (x * t) / 12.0) / 2.0;
-    t = 1.0 - x / 210.0;
-    t = 1.0 - (x * t) / 156.0;
-    t = 1.0 - (x * t) / 110.0;
-    t = 1.0 - (x * t) / 72.0;
-    t = 1.0 - (x * t) / 42.0;
-    c3 = (1.0 - (x * t) / 20.0) / 6.0;
+    double t = 1.0 - cdiv(x, 182.0, 1.0 / 182.0);
+    t = 1.0 - cdiv(x * t, 132.0, 1.0 / 132.0);
+    t = 1.0 - cdiv(x * t, 90.0, 1.0 / 90.0);
+    t = 1.0 - cdiv(x * t, 56.0, 1.0 / 56.0);
+    t = 1.0 - cdiv(x * t, 30.0, 1.0 / 30.0);
+    c2 = (1.0 - cdiv(x * t, 12.0, 1.0 / 12.0)) / 2.0;
+    t = 1.0 - cdiv(x, 210.0, 1.0 / 210.0);
+    t = 1.0 - cdiv(x * t, 156.0, 1.0 / 156.0);
+    t = 1.0 - cdiv(x * t, 110.0, 1.0 / 110.0);
+    t = 1.0 - cdiv(x * t, 72.0, 1.0 / 72.0);
+    t = 1.0 - cdiv(x * t, 42.0, 1.0 / 42.0);
+    c3 = cdiv(1.0 - cdiv(x * t, 20.0, 1.0 / 20.0), 6.0, 1.0 / 6.0);
     c1 = 1.0 - x * c3;
     c0 = 1.0 - x * c2;
     for (int i = n; i >= 1; i--) {
@@ -146,10 +162,10 @@ __device__ inline void kepmd(double dm, double es, double ec, double &x, double 
     const double a0 = 39916800.0, a1 = 6652800.0, a2 = 332640.0, a3 = 7920.0, a4 = 110.0;
     double fac1 = 1.0 / (1.0 - ec);
     double q = fac1 * dm;
-    double fac2 = (es * es) * fac1 - ec / 3.0;
+    double fac2 = (es * es) * fac1 - cdiv(ec, 3.0, 1.0 / 3.0);  // ec is 0 or normal
     x = q * (1.0 - ((0.5 * fac1) * q) * (es - q * fac2));
     double y = x * x;
-    s = (x * (a0 - y * (a1 - y * (a2 - y * (a3 - y * (a4 - y)))))) / a0;
+    s = cdiv(x * (a0 - y * (a1 - y * (a2 - y * (a3 - y * (a4 - y))))), a0, 1.0 / a0);
     c = sqrt(1.0 - s * s);
     double f = ((x - ec * s) + es * (1.0 - c)) - dm;
     double fp = (1.0 - ec * c) + es * s;
@@ -160,7 +176,7 @@ __device__ inline void kepmd(double dm, double es, double ec, double &x, double 
     dx = -f / ((fp + (dx * fpp) / 2.0) + ((dx * dx) * fppp) * SIXTH);
     x = x + dx;
     y = x * x;
-    s = (x * (a0 - y * (a1 - y * (a2 - y * (a3 - y * (a4 - y)))))) / a0;
+    s = cdiv(x * (a0 - y * (a1 - y * (a2 - y * (a3 - y * (a4 - y))))), a0, 1.0 / a0);
     c = sqrt(1.0 - s * s);
 }
 
@@ -238,7 +254,7 @@ __device__ inline int kepu_new(double &s, double dt, double r0, double mu, doubl
         double fppp = (-(r0 * alpha) + mu) * c0 - (u * alpha) * c1;
         double ds = -f / fp;
         ds = -f / (fp + (ds * fpp) / 2.0);
-        ds = -f / ((fp + (ds * fpp) / 2.0) + ((ds * ds) * fppp) / 6.0);
+        ds = -f / ((fp + (ds * fpp) / 2.0) + cdiv((ds * ds) * fppp, 6.0, 1.0 / 6.0));
         s = s + ds;
         double fdt = f / dt;
         if (fdt * fdt < DANBYB * DANBYB) return 0;
