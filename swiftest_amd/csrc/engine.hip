@@ -330,10 +330,11 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
     owned.push_back(d_scan_sys);
     if (own(&bw.keepB, npl) || own(&bw.cntE, npl) || own(&bw.posB, npl) || own(&bw.posE, npl) ||
         own(&bw.keepBt, ntp) || own(&bw.cntEt, ntp) || own(&bw.posBt, ntp) || own(&bw.posEt, ntp) ||
-        own(&bw.tot, 4) || own(&bw.rng, 4 * (size_t)nsys) || own(&bw.caps, nsys) || own(&bw.maxsz, 8) ||
+        own(&bw.tot, 10) || own(&bw.rng, 4 * (size_t)nsys) || own(&bw.caps, nsys) ||
         own(&d_arena_off, nsys + 1) || own(&d_sub_stats, 4))
         return SWG_ERR_HIP;
     bw.scan = d_scan;
+    bw.maxsz = (int32_t *)(bw.tot + 6);
     lv_buf.assign(MAXLEV + 2, nullptr);
     lv_cap.assign(MAXLEV + 2, 0);
     HIPCHK(hipHostMalloc(&h_subargs, subtree_args_bytes()));
@@ -602,18 +603,28 @@ int Ensemble::build_level(int dnew, const LevelView *src, LevelView &v, LevelCou
     hipStream_t st = stream;
     const LevelArgs la = level_args();
     launch_build_count(st, la, src, npl, ntp, dnew, bw);
-    launch_scan_i32_i64(st, bw.caps, d_arena_off, nsys, d_arena_off + nsys, d_scan_sys);
+    if (nsys <= SCAN_SMALL_MAX) {
+        ScanSmall sc{};
+        sc.in[0] = bw.caps;
+        sc.out[0] = d_arena_off;
+        sc.n[0] = nsys;
+        sc.total[0] = d_arena_off + nsys;
+        sc.total2[0] = bw.tot + 4;
+        launch_scan_small(st, sc, 1);
+    } else {
+        launch_scan_i32_i64(st, bw.caps, d_arena_off, nsys, bw.tot + 4, d_scan_sys);
+        HIPCHK(hipMemcpyAsync(d_arena_off + nsys, bw.tot + 4, sizeof(int64_t), hipMemcpyDeviceToDevice, st));
+    }
+    // counts, arena total and the per-system maxima in one copy
     int64_t *h = (int64_t *)(h_small + 128);
-    HIPCHK(hipMemcpyAsync(h, bw.tot, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(h + 4, d_arena_off + nsys, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(h + 5, bw.maxsz, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(h, bw.tot, 10 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     c.nB = h[0];
     c.nE = h[1];
     c.nBt = h[2];
     c.nEt = h[3];
     arena_need = h[4];
-    const int32_t *mx = (const int32_t *)(h + 5);
+    const int32_t *mx = (const int32_t *)(h + 6);
     const int32_t maxsz = mx[0];
     if (c.nE > INT32_MAX || c.nEt > INT32_MAX)
         return fail(SWG_ERR_SCRATCH, "recursion working set of level " + std::to_string(dnew) + " exceeds 2^31 entries");
@@ -635,6 +646,7 @@ int Ensemble::build_level(int dnew, const LevelView *src, LevelView &v, LevelCou
     // workgroup.  SWG_SUBTREE_MAX in the environment forces the size rule
     // alone (hand-off modes of the tests).
     const bool fits = subtree_lds_fits(mx[1], mx[2], mx[3], mx[4]);
+    level_fits = fits;
     handoff = subtree_max >= 0 && maxsz <= subtree_max && (subtree_forced || fits || mx[5] >= SUBTREE_MIN_SYSTEMS);
     return 0;
 }
@@ -702,7 +714,7 @@ int Ensemble::run_subtree(int d0, const LevelView &v)
     // next stream synchronization of build_level, so the previous copy out of
     // it has completed
     launch_subtree(st, level_args(), v, bw.rng, d0, prm.dt, prm.lclose, prm.short_circuit, d_arena, d_arena_off,
-                   d_sub_stats, h_subargs, d_subargs);
+                   d_sub_stats, h_subargs, d_subargs, level_fits);
     HIPCHK(hipGetLastError());
     return 0;
 }

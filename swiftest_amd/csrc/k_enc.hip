@@ -237,6 +237,51 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_apply(const int32_t *in, int n,
 
 size_t scan_scratch_bytes(int n) { return sizeof(int64_t) * (size_t)(cdiv(n > 0 ? n : 1, SCAN_CHUNK) + 1); }
 
+// Up to four short exclusive scans in one launch, one workgroup each; block
+// 0 also clears an int32 array (the counters the next kernel accumulates).
+constexpr int SMALL_T = 1024, SMALL_ITEMS = SCAN_SMALL_MAX / SMALL_T;
+static_assert(SCAN_SMALL_MAX % SMALL_T == 0, "items per thread");
+
+__global__ __launch_bounds__(SMALL_T) void k_scan_small(ScanSmall a)
+{
+    __shared__ int64_t sh[SMALL_T];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    if (b == 0 && a.zero)
+        for (int i = tid; i < a.nzero; i += SMALL_T) a.zero[i] = 0;
+    const int n = a.n[b];
+    const int32_t *in = a.in[b];
+    const int base = tid * SMALL_ITEMS;
+    int64_t loc[SMALL_ITEMS];
+    int64_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < SMALL_ITEMS; k++) {
+        loc[k] = sum;
+        if (base + k < n) sum += in[base + k];
+    }
+    sh[tid] = sum;
+    __syncthreads();
+    for (int o = 1; o < SMALL_T; o <<= 1) {
+        const int64_t u = tid >= o ? sh[tid - o] : 0;
+        __syncthreads();
+        sh[tid] += u;
+        __syncthreads();
+    }
+    const int64_t off = sh[tid] - sum;
+    int64_t *out = a.out[b];
+#pragma unroll
+    for (int k = 0; k < SMALL_ITEMS; k++)
+        if (base + k < n) out[base + k] = off + loc[k];
+    if (tid == SMALL_T - 1) {
+        if (a.total[b]) *a.total[b] = sh[tid];
+        if (a.total2[b]) *a.total2[b] = sh[tid];
+    }
+}
+
+void launch_scan_small(hipStream_t st, const ScanSmall &a, int na)
+{
+    k_scan_small<<<na, SMALL_T, 0, st>>>(a);
+}
+
 void launch_scan_i32_i64(hipStream_t st, const int32_t *in, int64_t *out, int n, int64_t *total, void *scratch)
 {
     int64_t *part = (int64_t *)scratch;

@@ -515,11 +515,28 @@ void launch_build_count(hipStream_t st, const LevelArgs &la, const LevelView *sr
     s.nBt = src ? src->nBtp : (la.has_pt ? ntp : 0);
     if (s.nB) k_build_count_pl<<<cdiv((int64_t)s.nB * 64, 256), 256, 0, st>>>(a, s, dnew, w.keepB, w.cntE);
     if (s.nBt) k_build_count_tp<<<cdiv(s.nBt, 256), 256, 0, st>>>(a, s, dnew, w.keepBt, w.cntEt);
-    launch_scan_i32_i64(st, w.keepB, w.posB, s.nB, w.tot + 0, w.scan);
-    launch_scan_i32_i64(st, w.cntE, w.posE, s.nB, w.tot + 1, w.scan);
-    launch_scan_i32_i64(st, w.keepBt, w.posBt, s.nBt, w.tot + 2, w.scan);
-    launch_scan_i32_i64(st, w.cntEt, w.posEt, s.nBt, w.tot + 3, w.scan);
-    (void)hipMemsetAsync(w.maxsz, 0, 8 * sizeof(int32_t), st);
+    if (s.nB <= SCAN_SMALL_MAX && s.nBt <= SCAN_SMALL_MAX) {
+        // the deep levels: one launch for the four scans and the counters
+        ScanSmall sc{};
+        const int32_t *in[4] = {w.keepB, w.cntE, w.keepBt, w.cntEt};
+        int64_t *out[4] = {w.posB, w.posE, w.posBt, w.posEt};
+        const int n[4] = {s.nB, s.nB, s.nBt, s.nBt};
+        for (int k = 0; k < 4; k++) {
+            sc.in[k] = in[k];
+            sc.out[k] = out[k];
+            sc.n[k] = n[k];
+            sc.total[k] = w.tot + k;
+        }
+        sc.zero = w.maxsz;
+        sc.nzero = 8;
+        launch_scan_small(st, sc, 4);
+    } else {
+        launch_scan_i32_i64(st, w.keepB, w.posB, s.nB, w.tot + 0, w.scan);
+        launch_scan_i32_i64(st, w.cntE, w.posE, s.nB, w.tot + 1, w.scan);
+        launch_scan_i32_i64(st, w.keepBt, w.posBt, s.nBt, w.tot + 2, w.scan);
+        launch_scan_i32_i64(st, w.cntEt, w.posEt, s.nBt, w.tot + 3, w.scan);
+        (void)hipMemsetAsync(w.maxsz, 0, 8 * sizeof(int32_t), st);
+    }
     k_build_ranges<<<cdiv(la.nsys, 256), 256, 0, st>>>(la.sy, s, la.nsys, dnew, w.posB, w.posE, w.posBt, w.posEt,
                                                        w.tot, w.rng, w.caps, w.maxsz);
 }

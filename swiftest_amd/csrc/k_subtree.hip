@@ -1184,7 +1184,7 @@ void subtree_profile_report(hipStream_t st)
 // copy has executed.
 void launch_subtree(hipStream_t st, const LevelArgs &la, const LevelView &start, const int32_t *rng, int d0,
                     double dt, int lclose, int short_circuit, int32_t *arena, const int64_t *arena_off,
-                    int32_t *stats, void *h_stage, void *d_args)
+                    int32_t *stats, void *h_stage, void *d_args, bool all_lds)
 {
     SubArgs &a = *(SubArgs *)h_stage;
     a.st.pl = la.pl;
@@ -1214,8 +1214,10 @@ void launch_subtree(hipStream_t st, const LevelArgs &la, const LevelView &start,
         if (g_prof) a.prof = g_prof;
     }
     (void)hipMemcpyAsync(d_args, h_stage, sizeof(SubArgs), hipMemcpyHostToDevice, st);
-    k_subtree_lds<<<la.nsys, L_T, 0, st>>>((const SubArgs *)d_args);
-    k_subtree<<<la.nsys, ST_T, 0, st>>>((const SubArgs *)d_args);
+    // each system runs in the kernel whose limits its set fits; all_lds: every
+    // system's set fits the LDS-resident one
+    if (a.lds) k_subtree_lds<<<la.nsys, L_T, 0, st>>>((const SubArgs *)d_args);
+    if (!(a.lds && all_lds)) k_subtree<<<la.nsys, ST_T, 0, st>>>((const SubArgs *)d_args);
 }
 
 }  // namespace swg

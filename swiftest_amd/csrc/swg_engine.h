@@ -57,6 +57,7 @@ struct Ensemble {
     void *h_subargs = nullptr, *d_subargs = nullptr;
     int64_t subtree_max = 16384;  // SWG_SUBTREE_MAX; < 0: every level on the whole device
     bool subtree_forced = false;  // SWG_SUBTREE_MAX given: hand off on size alone
+    bool level_fits = false;      // every set of the level just built fits the LDS-resident kernel
     static constexpr int SUBTREE_MIN_SYSTEMS = 128;  // systems with work that fill the device
     LevelArgs level_args() const;
     int build_level(int dnew, const LevelView *src, LevelView &v, LevelCounts &c, bool &handoff);

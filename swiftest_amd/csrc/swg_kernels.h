@@ -76,6 +76,19 @@ void launch_rows_count(hipStream_t st, ListDev L, const int32_t *row_sys, int up
 void launch_scan_i32_i64(hipStream_t st, const int32_t *in, int64_t *out, int n, int64_t *total,
                          void *scratch);
 size_t scan_scratch_bytes(int n);
+// Exclusive scans of up to four int32 arrays of at most SCAN_SMALL_MAX
+// elements each in a single launch; totals go to total[k] and, when given,
+// total2[k]; zero[0, nzero) is cleared first.
+constexpr int SCAN_SMALL_MAX = 16384;
+struct ScanSmall {
+    const int32_t *in[4];
+    int64_t *out[4];
+    int n[4];
+    int64_t *total[4], *total2[4];
+    int32_t *zero;
+    int nzero;
+};
+void launch_scan_small(hipStream_t st, const ScanSmall &a, int na);
 void launch_rows_fill(hipStream_t st, ListDev L, const int32_t *row_sys, int upper, const int32_t *pl_off,
                       const int32_t *col_off, int nrows, int32_t *lev1g = nullptr, int32_t *lev1m = nullptr,
                       int32_t *lev2g = nullptr, int32_t *lev2m = nullptr);
@@ -135,11 +148,11 @@ struct LevelArgs {
 struct BuildScratch {
     int32_t *keepB, *cntE, *keepBt, *cntEt;
     int64_t *posB, *posE, *posBt, *posEt;
-    int64_t *tot;    // [4]: nB, nE, nBt, nEt of the new level
+    int64_t *tot;    // [10]: nB, nE, nBt, nEt of the new level, the arena total, then maxsz
     void *scan;      // scan_scratch_bytes(max(npl, ntp))
     int32_t *rng;    // [4*nsys] new level's per-system body ranges
     int32_t *caps;   // [nsys] arena words of each system's persistent workgroup
-    int32_t *maxsz;  // [8]: largest per-system set and its parts, systems with a set
+    int32_t *maxsz;  // [8] = (int32_t *)(tot + 6): largest per-system set and its parts, systems with a set
 };
 LevelView level_view(void *base, const LevelCounts &c);
 int64_t level_bytes(const LevelCounts &c);
@@ -163,6 +176,6 @@ size_t subtree_args_bytes();
 void subtree_profile_report(hipStream_t st);
 void launch_subtree(hipStream_t st, const LevelArgs &la, const LevelView &start, const int32_t *rng, int d0,
                     double dt, int lclose, int short_circuit, int32_t *arena, const int64_t *arena_off,
-                    int32_t *stats, void *h_stage, void *d_args);
+                    int32_t *stats, void *h_stage, void *d_args, bool all_lds);
 
 }  // namespace swg
