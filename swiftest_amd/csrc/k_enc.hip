@@ -407,7 +407,7 @@ __global__ void k_list_check_pp(ListDev pp, PlDev pl, SysDev sy, int64_t n, int 
     const int i = pp.i1[k], j = pp.i2[k];
     double xr = pl.rh.x[j] - pl.rh.x[i], yr = pl.rh.y[j] - pl.rh.y[i], zr = pl.rh.z[j] - pl.rh.z[i];
     double vxr = pl.vb.x[j] - pl.vb.x[i], vyr = pl.vb.y[j] - pl.vb.y[i], vzr = pl.vb.z[j] - pl.vb.z[i];
-    const double rsh = rshell_pow(irec);
+    const double rsh = shell_r(irec);
     double rci = (pl.rhill[i] * RHSCALE) * rsh, rcj = (pl.rhill[j] * RHSCALE) * rsh;
     bool lv;
     bool le = encounter_one(xr, yr, zr, vxr, vyr, vzr, rci + rcj, dt, lv);
@@ -443,7 +443,7 @@ __global__ void k_list_check_pt(ListDev pt, PlDev pl, TpDev tp, SysDev sy, int64
     if (short_circuit && sy.flag_pp[s]) return;
     double xr = tp.rh.x[j] - pl.rh.x[i], yr = tp.rh.y[j] - pl.rh.y[i], zr = tp.rh.z[j] - pl.rh.z[i];
     double vxr = tp.vb.x[j] - pl.vb.x[i], vyr = tp.vb.y[j] - pl.vb.y[i], vzr = tp.vb.z[j] - pl.vb.z[i];
-    double rci = (pl.rhill[i] * RHSCALE) * rshell_pow(irec);
+    double rci = (pl.rhill[i] * RHSCALE) * shell_r(irec);
     bool lv;
     bool le = encounter_one(xr, yr, zr, vxr, vyr, vzr, rci, dt, lv);
     pt.lvdotr[k] = lv;

@@ -303,7 +303,7 @@ __global__ __launch_bounds__(KT_T) void k_lv_kick_pl(RecState a, LevelView v, in
     __syncthreads();
     const int nmax = smax;
     if (nmax == 0) return;
-    const double spp = powi(RSHELL, 2 * (d + 1)), spm = powi(RSHELL, 2 * d);
+    const double spp = shell_sq(d + 1), spm = shell_sq(d);
     const bool minus = d > 0;
     const PlDev &pl = a.pl;
     double3 p = make_double3(0.0, 0.0, 0.0), m = p;
@@ -411,7 +411,7 @@ __global__ __launch_bounds__(256) void k_lv_kick_tp(RecState a, LevelView v, int
     if (L0) {
         const int t = q, n = l0_tp_count(a, has_pt, t);
         if (n == 0) return;
-        const double spp = powi(RSHELL, 2 * (d + 1));
+        const double spp = shell_sq(d + 1);
         const double xt = a.tp.rh.x[t], yt = a.tp.rh.y[t], zt = a.tp.rh.z[t];
         double3 p = make_double3(0.0, 0.0, 0.0);
         bool anyp = false;

@@ -419,6 +419,7 @@ struct KTerm {
 };
 constexpr int L_KC = 512;                              // entry terms per kick chunk
 constexpr int L_KS = (L_NB + L_NBT + L_T - 1) / L_T;  // bodies folded per lane
+static_assert(L_KS * L_T == L_NB + L_NBT, "the kick-list arrays cover every lane slot");
 
 struct LdsSet {
     double px[L_NB], py[L_NB], pz[L_NB], pvx[L_NB], pvy[L_NB], pvz[L_NB], ph[L_NB], pgm[L_NB], pR[L_NB];
@@ -662,19 +663,19 @@ __device__ void lds_kicks(LdsSet &S, int nB, int nBt, int nE, int nEt, int d, do
     const int nact = S.nact, nl = S.nlst;
     const bool pr = prof && tid == 0;
     unsigned long long tc = 0, tf = 0, tt = pr ? wall_clock64() : 0;
-    const double spp = powi(RSHELL, 2 * (d + 1)), spm = powi(RSHELL, 2 * d);
+    const double spp = shell_sq(d + 1), spm = shell_sq(d);
     int fe0[L_KS], fe1[L_KS], kmax = 0;
     double3 p[L_KS], m[L_KS];
     bool anyp[L_KS], anym[L_KS];
 #pragma unroll
     for (int k = 0; k < L_KS; k++) {
+        // read unconditionally (every ai < L_KS * L_T is inside the arrays),
+        // so the loads issue together
         const int ai = tid + k * L_T;
-        fe0[k] = fe1[k] = 0;
-        if (ai < nact) {
-            fe0[k] = S.kst[ai];
-            fe1[k] = S.ken[ai];
-            kmax = max(kmax, fe1[k] - fe0[k]);
-        }
+        const int f0 = S.kst[ai], f1 = S.ken[ai];
+        fe0[k] = ai < nact ? f0 : 0;
+        fe1[k] = ai < nact ? f1 : 0;
+        kmax = max(kmax, fe1[k] - fe0[k]);
         p[k] = m[k] = make_double3(0.0, 0.0, 0.0);
         anyp[k] = anym[k] = false;
     }

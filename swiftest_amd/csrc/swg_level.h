@@ -15,6 +15,26 @@ namespace swg {
 
 constexpr int MAXLEV = 36;  // deeper than the recursion depth guard (3^-33 < 2^-52)
 
+// The powers of RSHELL the recursion uses at level k: powi(RSHELL, 2k), the
+// shell radius factor of the kicks, and rshell_pow(k), the encounter radius
+// factor of the checks.  They are evaluated at compile time by the same
+// operations in the same order as at run time (round to nearest), so a
+// table read gives the run-time bits in one load instead of a loop of
+// dependent multiplications.
+struct ShellPowers {
+    double sq[MAXLEV + 2], r[MAXLEV + 2];
+    constexpr ShellPowers() : sq(), r()
+    {
+        for (int k = 0; k < MAXLEV + 2; k++) {
+            sq[k] = powi(RSHELL, 2 * k);
+            r[k] = rshell_pow(k);
+        }
+    }
+};
+static __constant__ const ShellPowers kShellPow{};
+__device__ inline double shell_sq(int k) { return kShellPow.sq[k]; }  // powi(RSHELL, 2k), 0 <= k <= MAXLEV+1
+__device__ inline double shell_r(int k) { return kShellPow.r[k]; }    // rshell_pow(k), 0 <= k <= MAXLEV+1
+
 // One pl entry: x = (pair << 1) | role (role 1: the body is index1), y = the
 // partner body.  One tp entry: x = pair, y = the massive body.
 struct LevelView {
@@ -71,7 +91,7 @@ __device__ inline bool check_pp_v(double xi, double yi, double zi, double vxi, d
 {
     double xr = xj - xi, yr = yj - yi, zr = zj - zi;
     double vxr = vxj - vxi, vyr = vyj - vyi, vzr = vzj - vzi;
-    const double rsh = rshell_pow(irec);
+    const double rsh = shell_r(irec);
     double rci = (hi * RHSCALE) * rsh, rcj = (hj * RHSCALE) * rsh;
     bool le = encounter_one(xr, yr, zr, vxr, vyr, vzr, rci + rcj, dtl, lv);
     if (le) {
@@ -89,7 +109,7 @@ __device__ inline bool check_pt_v(double xi, double yi, double zi, double vxi, d
 {
     double xr = xt - xi, yr = yt - yi, zr = zt - zi;
     double vxr = vxt - vxi, vyr = vyt - vyi, vzr = vzt - vzi;
-    bool le = encounter_one(xr, yr, zr, vxr, vyr, vzr, (hi * RHSCALE) * rshell_pow(irec), dtl, lv);
+    bool le = encounter_one(xr, yr, zr, vxr, vyr, vzr, (hi * RHSCALE) * shell_r(irec), dtl, lv);
     if (le) {
         double rji2 = dot3(xr, yr, zr, xr, yr, zr);
         le = rji2 > Ri * Ri;
@@ -253,7 +273,7 @@ __device__ inline void apply_shell_kicks(Vec3p v, int b, double dth, bool anyp, 
 // only, so fusing the two calls changes no result.
 __device__ inline void rec_kick_pl(const RecState &a, int b, const int2 *E, int e0, int e1, int d, double dth)
 {
-    const double spp = powi(RSHELL, 2 * (d + 1)), spm = powi(RSHELL, 2 * d);
+    const double spp = shell_sq(d + 1), spm = shell_sq(d);
     const bool minus = d > 0;
     const double xb = a.pl.rh.x[b], yb = a.pl.rh.y[b], zb = a.pl.rh.z[b], hb = a.pl.rhill[b];
     double3 p = make_double3(0.0, 0.0, 0.0), m = p;
@@ -281,7 +301,7 @@ __device__ inline void rec_kick_pl(const RecState &a, int b, const int2 *E, int 
 // symba_kick.f90:235-340)
 __device__ inline void rec_kick_tp(const RecState &a, int t, const int2 *E, int e0, int e1, int d, double dth)
 {
-    const double spp = powi(RSHELL, 2 * (d + 1)), spm = powi(RSHELL, 2 * d);
+    const double spp = shell_sq(d + 1), spm = shell_sq(d);
     const bool minus = d > 0;
     const double xt = a.tp.rh.x[t], yt = a.tp.rh.y[t], zt = a.tp.rh.z[t];
     double3 p = make_double3(0.0, 0.0, 0.0), m = p;

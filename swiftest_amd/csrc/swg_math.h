@@ -469,7 +469,7 @@ __device__ inline bool collision_one(double xr, double yr, double zr, double vxr
 
 // x**n for a run-time integer n by binary powering (what RSHELL**(2*irecl)
 // compiles to)
-__host__ __device__ inline double powi(double x, int m)
+__host__ __device__ constexpr double powi(double x, int m)
 {
     unsigned n = m < 0 ? -(unsigned)m : (unsigned)m;
     double y = (n % 2) ? x : 1.0;
@@ -512,7 +512,7 @@ __device__ inline bool shell_fac(double ri, double r2, double &fac)
 
 // renc at recursion depth irec: rhill * RHSCALE * RSHELL**irec with the
 // power built by repeated multiplication (symba_util.f90:245-267)
-__host__ __device__ inline double rshell_pow(int irec)
+__host__ __device__ constexpr double rshell_pow(int irec)
 {
     double r = 1.0;
     for (int i = 1; i <= irec; i++) r = r * RSHELL;
