@@ -568,19 +568,20 @@ int Ensemble::recur0()
     launch_clear_flags(st, sy, nsys, d_any);
     launch_list_check_pp(st, pp, pl, sy, npp, 1, dt, d_any);
     launch_list_check_pt(st, pt, pl, tp, sy, npt, 1, dt, prm.short_circuit, d_any);
-    // the kicks and the drift depend on the check through device memory only:
-    // they are queued before the host waits for the flag
-    if (int rc = post_any()) return rc;
+    // the counts of level 1 and the encounter flag are sent to the host
+    // before the kicks and the drift, which change neither, run: the host
+    // reads them while the device works
+    if (int rc = build_begin(1, nullptr)) return rc;
     launch_level_kicks(st, la, nullptr, npl, ntp, 0, dth);
     launch_drift(st, pl.rh, pl.vb, pl.status, pl.levelg, pl.sys, sy.GMcb, sy.err, sy.frozen, npl, dt, 0, 1, 0);
     launch_drift(st, tp.rh, tp.vb, tp.status, tp.levelg, tp.sys, sy.GMcb, sy.err, sy.frozen, ntp, dt, 0, 1, 1);
     int32_t lenc = 0;
-    if (int rc = wait_any(lenc)) return rc;
+    if (int rc = build_wait(lenc)) return rc;
     if (lenc) {
         LevelView v;
         LevelCounts c;
         bool handoff = false;
-        if (int rc = build_level(1, nullptr, v, c, handoff)) return rc;
+        if (int rc = build_end(1, nullptr, v, c, handoff)) return rc;
         tr_hand[1] += handoff;
         if (handoff) {
             if (int rc = run_subtree(1, v)) return rc;
@@ -595,14 +596,18 @@ int Ensemble::recur0()
 }
 
 // Build the whole-device working set of level dnew from src (nullptr: the
-// level-0 adjacency) into the buffer of depth dnew.  handoff tells whether
-// every system's set is small enough for the persistent workgroups; then the
-// arena they need is sized too.
-int Ensemble::build_level(int dnew, const LevelView *src, LevelView &v, LevelCounts &c, bool &handoff)
+// level-0 adjacency) into the buffer of depth dnew, in three parts.
+// build_begin queues the counts, the arena sizes and a copy of them and of the
+// encounter flag to the host; it depends only on the levels the check just
+// set, so it runs before the kicks and the drift of the current substep.
+// build_wait waits for that copy.  build_end (only when the flag is set)
+// sizes the buffer, queues the fill and tells whether every system's set is
+// small enough for the persistent workgroups (handoff).
+int Ensemble::build_begin(int dnew, const LevelView *src)
 {
     hipStream_t st = stream;
     const LevelArgs la = level_args();
-    launch_build_count(st, la, src, npl, ntp, dnew, bw);
+    launch_build_count(st, la, src, npl, ntp, dnew, bw, d_any);
     if (nsys <= SCAN_SMALL_MAX) {
         ScanSmall sc{};
         sc.in[0] = bw.caps;
@@ -615,10 +620,24 @@ int Ensemble::build_level(int dnew, const LevelView *src, LevelView &v, LevelCou
         launch_scan_i32_i64(st, bw.caps, d_arena_off, nsys, bw.tot + 4, d_scan_sys);
         HIPCHK(hipMemcpyAsync(d_arena_off + nsys, bw.tot + 4, sizeof(int64_t), hipMemcpyDeviceToDevice, st));
     }
-    // counts, arena total and the per-system maxima in one copy
-    int64_t *h = (int64_t *)(h_small + 128);
-    HIPCHK(hipMemcpyAsync(h, bw.tot, 10 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    // counts, arena total, flag and the per-system maxima in one copy
+    HIPCHK(hipMemcpyAsync(h_small + 128, bw.tot, 10 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(ev_any, st));
+    return 0;
+}
+
+int Ensemble::build_wait(int32_t &lenc)
+{
+    HIPCHK(hipEventSynchronize(ev_any));
+    lenc = (int32_t)((const int64_t *)(h_small + 128))[5];
+    return 0;
+}
+
+int Ensemble::build_end(int dnew, const LevelView *src, LevelView &v, LevelCounts &c, bool &handoff)
+{
+    hipStream_t st = stream;
+    const LevelArgs la = level_args();
+    const int64_t *h = (const int64_t *)(h_small + 128);
     c.nB = h[0];
     c.nE = h[1];
     c.nBt = h[2];
@@ -672,17 +691,18 @@ int Ensemble::recur_level(int d, const LevelView &v, const LevelCounts &c)
         tr_E[d] += c.nE + c.nEt;
         launch_clear_flags(st, sy, nsys, d_any);
         launch_level_check(st, la, v, c, d, dtl, prm.short_circuit, d_any);
-        if (int rc = post_any()) return rc;
+        const bool room = d + 1 <= MAXLEV;
+        if (int rc = room ? build_begin(d + 1, &v) : post_any()) return rc;
         launch_level_kicks(st, la, &v, nB, nBt, d, dth);
         launch_level_drift(st, la, v, nB, nBt, d, dtl);
         int32_t lenc = 0;
-        if (int rc = wait_any(lenc)) return rc;
+        if (int rc = room ? build_wait(lenc) : wait_any(lenc)) return rc;
         if (lenc) {
-            if (d + 1 > MAXLEV) return fail(SWG_ERR_DEPTH, "recursion deeper than the level stack");
+            if (!room) return fail(SWG_ERR_DEPTH, "recursion deeper than the level stack");
             LevelView nv;
             LevelCounts nc;
             bool handoff = false;
-            if (int rc = build_level(d + 1, &v, nv, nc, handoff)) return rc;
+            if (int rc = build_end(d + 1, &v, nv, nc, handoff)) return rc;
             tr_hand[d + 1] += handoff;
             if (handoff) {
                 if (int rc = run_subtree(d + 1, nv)) return rc;
@@ -711,8 +731,8 @@ int Ensemble::run_subtree(int d0, const LevelView &v)
         arena_words = w;
     }
     // the staging buffer of the argument block is rewritten only after the
-    // next stream synchronization of build_level, so the previous copy out of
-    // it has completed
+    // host has waited for a later event on the stream (build_wait), so the
+    // previous copy out of it has completed
     launch_subtree(st, level_args(), v, bw.rng, d0, prm.dt, prm.lclose, prm.short_circuit, d_arena, d_arena_off,
                    d_sub_stats, h_subargs, d_subargs, level_fits);
     HIPCHK(hipGetLastError());

@@ -113,9 +113,11 @@ __device__ inline int64_t scan_at(const int64_t *x, int n, int i, int64_t total)
 
 __global__ void k_build_ranges(SysDev sy, BuildSrc src, int nsys, int dnew, const int64_t *posB,
                                const int64_t *posE, const int64_t *posBt, const int64_t *posEt,
-                               const int64_t *tot, int32_t *rng, int32_t *caps, int32_t *maxsz)
+                               const int64_t *tot, int32_t *rng, int32_t *caps, int32_t *maxsz,
+                               const int32_t *any, int64_t *any_out)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0 && any) *any_out = *any;  // the encounter flag travels with the counts
     if (s >= nsys) return;
     int a0, a1, t0, t1;
     if (src.l0) {
@@ -503,7 +505,7 @@ LevelView level_view(void *base, const LevelCounts &c)
 int64_t level_bytes(const LevelCounts &c) { return 4 * level_layout(c.nB, c.nE, c.nBt, c.nEt).words; }
 
 void launch_build_count(hipStream_t st, const LevelArgs &la, const LevelView *src, int npl, int ntp, int dnew,
-                        const BuildScratch &w)
+                        const BuildScratch &w, const int32_t *any)
 {
     RecState a = rec_state(la.pl, la.tp, la.pp, la.pt, la.sy);
     BuildSrc s{};
@@ -538,7 +540,7 @@ void launch_build_count(hipStream_t st, const LevelArgs &la, const LevelView *sr
         (void)hipMemsetAsync(w.maxsz, 0, 8 * sizeof(int32_t), st);
     }
     k_build_ranges<<<cdiv(la.nsys, 256), 256, 0, st>>>(la.sy, s, la.nsys, dnew, w.posB, w.posE, w.posBt, w.posEt,
-                                                       w.tot, w.rng, w.caps, w.maxsz);
+                                                       w.tot, w.rng, w.caps, w.maxsz, any, w.tot + 5);
 }
 
 void launch_build_fill(hipStream_t st, const LevelArgs &la, const LevelView *src, int npl, int ntp, int dnew,

@@ -60,7 +60,9 @@ struct Ensemble {
     bool level_fits = false;      // every set of the level just built fits the LDS-resident kernel
     static constexpr int SUBTREE_MIN_SYSTEMS = 128;  // systems with work that fill the device
     LevelArgs level_args() const;
-    int build_level(int dnew, const LevelView *src, LevelView &v, LevelCounts &c, bool &handoff);
+    int build_begin(int dnew, const LevelView *src);
+    int build_wait(int32_t &lenc);
+    int build_end(int dnew, const LevelView *src, LevelView &v, LevelCounts &c, bool &handoff);
     int recur_level(int d, const LevelView &v, const LevelCounts &c);
     int run_subtree(int d0, const LevelView &v);
     // host-driven recursion counters per level (substeps, bodies, entries,

@@ -148,7 +148,7 @@ struct LevelArgs {
 struct BuildScratch {
     int32_t *keepB, *cntE, *keepBt, *cntEt;
     int64_t *posB, *posE, *posBt, *posEt;
-    int64_t *tot;    // [10]: nB, nE, nBt, nEt of the new level, the arena total, then maxsz
+    int64_t *tot;    // [10]: nB, nE, nBt, nEt of the new level, the arena total, the encounter flag, then maxsz
     void *scan;      // scan_scratch_bytes(max(npl, ntp))
     int32_t *rng;    // [4*nsys] new level's per-system body ranges
     int32_t *caps;   // [nsys] arena words of each system's persistent workgroup
@@ -157,8 +157,9 @@ struct BuildScratch {
 LevelView level_view(void *base, const LevelCounts &c);
 int64_t level_bytes(const LevelCounts &c);
 // src == nullptr: build level 1 from the level-0 adjacency of all npl / ntp bodies
+// any (nullable): the encounter flag, copied into w.tot[5] with the counts
 void launch_build_count(hipStream_t st, const LevelArgs &la, const LevelView *src, int npl, int ntp, int dnew,
-                        const BuildScratch &w);
+                        const BuildScratch &w, const int32_t *any);
 void launch_build_fill(hipStream_t st, const LevelArgs &la, const LevelView *src, int npl, int ntp, int dnew,
                        const BuildScratch &w, void *dst_base, const LevelCounts &c);
 void launch_level_check(hipStream_t st, const LevelArgs &la, const LevelView &v, const LevelCounts &c, int d,
