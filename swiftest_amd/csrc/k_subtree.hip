@@ -444,6 +444,12 @@ struct LdsSet {
     };
     int jst[MAXLEV + 2];
     int wsum[L_T / 64 + 1];
+    // wave mode (one wavefront runs a small sub-recursion): the level it
+    // started at, that level's bodies (pl, then tp as nB + t), pl pairs, tp
+    // entries, and the kick entries of its bodies grouped by body
+    int16_t wB[64], wP[64], wT[64], wE[64];
+    int8_t wEs[64], wEe[64];
+    int wnB, wnP, wnT, wnE, wstop;
     int npair, flag_pp, flag_pt, bad, nact, nlst, lst_level, kmax;
     unsigned long long prof[(MAXLEV + 2) * NPROF];  // per-level phase ticks when profiling
 };
@@ -874,6 +880,299 @@ __device__ void lds_set_levels(LdsSet &S, int nB, int nBt, int nEt, int d)
     __syncthreads();
 }
 
+// ---- wave mode -------------------------------------------------------------
+// Deep in the recursion a level holds a handful of bodies and pairs.  When a
+// new level's bodies, pl pairs, tp entries and the kick entries among its
+// bodies each number at most 64, the first wavefront runs that level and
+// every level below it alone, one item per lane, with no workgroup barrier
+// and no pass over the whole slice; the other wavefronts wait.  Which items
+// can change is fixed while the recursion stays at or below the level (the
+// checks raise levels above it and the level resets lower them back), so the
+// sets are collected once; each phase filters them by the current level.
+// The phases are those of the workgroup path with the same arithmetic in the
+// same order: the pair checks, the kicks (each body folds its entries in
+// entry order), the drifts and the level resets.  Collision checks are not
+// run in wave mode (the workgroup path keeps them).
+
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Collect the sets of level dn with the whole workgroup; true when all fit.
+__device__ bool wave_sets(LdsSet &S, int nB, int nBt, int nE, int nEt, int dn)
+{
+    const int tid = threadIdx.x;
+    if (tid == 0) S.wnB = S.wnP = S.wnT = S.wnE = 0;
+    __syncthreads();
+    for (int q = tid; q < nB + nBt; q += L_T)
+        if ((q < nB ? S.plev[q] : S.tlev[q - nB]) >= dn) {
+            const int k = atomicAdd(&S.wnB, 1);
+            if (k < 64) S.wB[k] = (int16_t)q;
+        }
+    for (int lp = tid; lp < S.npair; lp += L_T)
+        if (S.ppl[lp] >= dn) {
+            const int k = atomicAdd(&S.wnP, 1);
+            if (k < 64) S.wP[k] = (int16_t)lp;
+        }
+    for (int u = tid; u < nEt; u += L_T)
+        if (S.tpl_lev[u] >= dn) {
+            const int k = atomicAdd(&S.wnT, 1);
+            if (k < 64) S.wT[k] = (int16_t)u;
+        }
+    __syncthreads();
+    if (S.wnB > 64 || S.wnP > 64 || S.wnT > 64) return false;
+    if (tid < 64) {
+        // each body's entries whose partner is in the level, in entry order
+        const int k = tid;
+        int e0 = 0, e1 = 0, q = 0;
+        bool ispl = true;
+        if (k < S.wnB) {
+            q = S.wB[k];
+            ispl = q < nB;
+            e0 = ispl ? S.pO[q] : nE + S.tO[q - nB];
+            e1 = ispl ? S.pO[q + 1] : nE + S.tO[q - nB + 1];
+        }
+        int c = 0;
+        for (int u = e0; u < e1; u++) c += (ispl ? S.plev[S.pE[u] & 511u] : S.plev[S.tpl[u - nE]]) >= dn;
+        int incl = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o, 64);
+            if (k >= o) incl += t;
+        }
+        const int tot = __shfl(incl, 63, 64);
+        if (tot <= 64) {
+            int w = incl - c;
+            if (k < S.wnB) {
+                S.wEs[k] = (int8_t)w;
+                for (int u = e0; u < e1; u++)
+                    if ((ispl ? S.plev[S.pE[u] & 511u] : S.plev[S.tpl[u - nE]]) >= dn) S.wE[w++] = (int16_t)u;
+                S.wEe[k] = (int8_t)w;
+            }
+        }
+        if (k == 0) S.wnE = tot;
+    }
+    __syncthreads();
+    return S.wnE <= 64;
+}
+
+// the pair checks of level d (lds_check); returns lenc
+__device__ bool wave_check(LdsSet &S, const SubArgs &A, int d, double dtl)
+{
+    const int k = threadIdx.x, irec = d + 1;
+    bool lp_enc = false;
+    if (k < S.wnP) {
+        const int lp = S.wP[k];
+        if (S.pps[lp] == ACTIVE && S.ppl[lp] == d) {
+            const int i = S.ppi[lp], j = S.ppj[lp];
+            bool lv;
+            lp_enc = check_pp_v(S.px[i], S.py[i], S.pz[i], S.pvx[i], S.pvy[i], S.pvz[i], S.ph[i], S.pR[i], S.px[j],
+                                S.py[j], S.pz[j], S.pvx[j], S.pvy[j], S.pvz[j], S.ph[j], S.pR[j], irec, dtl, lv);
+            S.ppv[lp] = lv;
+            if (lp_enc) {
+                S.ppl[lp] = (int8_t)irec;
+                S.plev[i] = irec;
+                S.plev[j] = irec;
+                atomicMax(&S.plm[i], irec);
+                atomicMax(&S.plm[j], irec);
+            }
+        }
+    }
+    const bool fpp = __ballot(lp_enc) != 0;
+    wave_sync();
+    bool tp_enc = false;
+    if (!(A.short_circuit && fpp) && k < S.wnT) {  // symba_step.f90:210 (.or. short-circuited)
+        const int u = S.wT[k];
+        if (S.tps[u] == ACTIVE && S.tpl_lev[u] == d) {
+            const int i = S.tpl[u], t = S.tpt[u];
+            bool lv;
+            tp_enc = check_pt_v(S.px[i], S.py[i], S.pz[i], S.pvx[i], S.pvy[i], S.pvz[i], S.ph[i], S.pR[i], S.tx[t],
+                                S.ty[t], S.tz[t], S.tvx[t], S.tvy[t], S.tvz[t], irec, dtl, lv);
+            S.tpv[u] = lv;
+            if (tp_enc) {
+                S.tpl_lev[u] = (int8_t)irec;
+                S.plev[i] = irec;
+                atomicMax(&S.plm[i], irec);
+                S.tlev[t] = irec;
+                atomicMax(&S.tlm[t], irec);
+            }
+        }
+    }
+    const bool fpt = __ballot(tp_enc) != 0;
+    wave_sync();
+    return fpp || fpt;
+}
+
+// the +1 / -1 shell kicks of level d (lds_kicks)
+__device__ void wave_kicks(LdsSet &S, int nB, int nE, int d, double dth)
+{
+    const int k = threadIdx.x;
+    const double spp = shell_sq(d + 1), spm = shell_sq(d);
+    if (k < S.wnE) {
+        const int u = S.wE[k];
+        ShellTerm t;
+        t.tp = t.tm = false;
+        t.dx = t.dy = t.dz = 0.0;
+        if (u < nE) {
+            const uint32_t e = S.pE[u];
+            const int lp = (int)(e >> 10), pr = (int)(e & 511u);
+            const bool bi = (e >> 9) & 1u;
+            const int q = bi ? S.ppi[lp] : S.ppj[lp];
+            if (S.plev[q] >= d && S.plev[pr] >= d && S.pps[lp] == ACTIVE)
+                pl_shell_term_v(bi, S.px[q], S.py[q], S.pz[q], S.ph[q], S.px[pr], S.py[pr], S.pz[pr], S.ph[pr],
+                                S.pgm[pr], spp, spm, true, t);
+        } else {
+            const int v = u - nE, i = S.tpl[v], tt = S.tpt[v];
+            if (S.tlev[tt] >= d && S.plev[i] >= d && S.tps[v] == ACTIVE)
+                tp_shell_term_v(S.tx[tt], S.ty[tt], S.tz[tt], S.px[i], S.py[i], S.pz[i], S.ph[i], S.pgm[i], spp, spm,
+                                true, t);
+        }
+        KTerm &K = S.kt[k];
+        K.dx = t.dx;
+        K.dy = t.dy;
+        K.dz = t.dz;
+        K.fp = t.tp ? t.fp : 0.0;
+        K.fm = t.tm ? t.fm : 0.0;
+        K.fl = (t.tp ? 1 : 0) | (t.tm ? 2 : 0);
+    }
+    wave_sync();
+    if (k < S.wnB) {
+        const int q = S.wB[k];
+        const bool ispl = q < nB;
+        if ((ispl ? S.plev[q] : S.tlev[q - nB]) >= d) {
+            double3 p = make_double3(0.0, 0.0, 0.0), m = p;
+            int fl = 0;
+            fold_terms(S.kt, S.wEs[k], S.wEe[k], p, m, fl);
+            if (ispl) lds_apply(S.pvx[q], S.pvy[q], S.pvz[q], dth, (fl & 1) != 0, p, (fl & 2) != 0, m);
+            else lds_apply(S.tvx[q - nB], S.tvy[q - nB], S.tvz[q - nB], dth, (fl & 1) != 0, p, (fl & 2) != 0, m);
+        }
+    }
+    wave_sync();
+}
+
+// the drifts of level d (lds_drift)
+__device__ void wave_drift(LdsSet &S, const SubArgs &A, int nB, int d, double dtl, int s)
+{
+    const int k = threadIdx.x;
+    if (k < S.wnB) {
+        const int q = S.wB[k];
+        const bool ispl = q < nB;
+        const int t = q - nB;
+        if (ispl ? (S.pst[q] != INACTIVE && S.plev[q] == d) : (S.tst[t] == ACTIVE && S.tlev[t] == d)) {
+            double *x = ispl ? S.px : S.tx, *y = ispl ? S.py : S.ty, *z = ispl ? S.pz : S.tz;
+            double *vx = ispl ? S.pvx : S.tvx, *vy = ispl ? S.pvy : S.tvy, *vz = ispl ? S.pvz : S.tvz;
+            const int i = ispl ? q : t;
+            double rx = x[i], ry = y[i], rz = z[i], ux = vx[i], uy = vy[i], uz = vz[i];
+            const int fl = drift_one(A.st.sy.GMcb[s], rx, ry, rz, ux, uy, uz, dtl);
+            x[i] = rx;
+            y[i] = ry;
+            z[i] = rz;
+            vx[i] = ux;
+            vy[i] = uy;
+            vz[i] = uz;
+            if (fl) {
+                if (ispl) {
+                    S.pst[q] = DISCARDED_DRIFTERR;
+                    atomicOr(&A.st.sy.err[s], 2);
+                } else {
+                    S.tst[t] = DISCARDED_DRIFTERR;
+                }
+            }
+        }
+    }
+    wave_sync();
+}
+
+// the level reset after a substep of level d (lds_set_levels)
+__device__ void wave_set_levels(LdsSet &S, int nB, int d)
+{
+    const int k = threadIdx.x, irecp = d + 1;
+    if (k < S.wnB) {
+        const int q = S.wB[k];
+        if (q < nB) {
+            if (S.plev[q] == irecp) S.plev[q] = d;
+        } else if (S.tlev[q - nB] == irecp) {
+            S.tlev[q - nB] = d;
+        }
+    }
+    if (k < S.wnP && S.ppl[S.wP[k]] == irecp) S.ppl[S.wP[k]] = (int8_t)d;
+    if (k < S.wnT && S.tpl_lev[S.wT[k]] == irecp) S.tpl_lev[S.wT[k]] = (int8_t)d;
+    wave_sync();
+}
+
+// Levels dw and below on the first wavefront, from the first substep of dw
+// to the end of dw (the loop of k_subtree_lds).  Adds its substeps to nsub
+// and raises maxd; sets S.wstop when the recursion must stop (error flagged).
+__device__ void wave_recursion(LdsSet &S, const SubArgs &A, int nB, int nE, int dw, int s, int &nsub, int &maxd)
+{
+    const bool prof = A.prof && threadIdx.x == 0;
+    int d = dw, j = 1;
+    bool resume = false;
+    for (;;) {
+        double p3 = 1.0;
+        for (int q = 0; q < d; q++) p3 *= NTENC;
+        const double dtl = A.dt / p3, dth = 0.5 * dtl;
+        if (!resume) {
+            if (j > NTENC) {
+                if (d == dw) break;
+                d--;
+                j = S.jst[d];
+                resume = true;
+                continue;
+            }
+            if (dtl / A.dt < 2.220446049250313e-16) {  // symba_step.f90:196-201
+                if (threadIdx.x == 0) {
+                    atomicOr(&A.st.sy.err[s], 8);
+                    S.wstop = 1;
+                }
+                break;
+            }
+            nsub++;
+            const unsigned long long t0 = prof ? wall_clock64() : 0;
+            const bool lenc = wave_check(S, A, d, dtl);
+            const unsigned long long t1 = prof ? wall_clock64() : 0;
+            wave_kicks(S, nB, nE, d, dth);
+            const unsigned long long t2 = prof ? wall_clock64() : 0;
+            wave_drift(S, A, nB, d, dtl, s);
+            if (prof) {
+                unsigned long long *P = S.prof + d * NPROF;
+                P[0] += 1;
+                P[1] += S.wnB;
+                P[2] += S.wnE;
+                P[3] += t1 - t0;
+                P[4] += t2 - t1;
+                P[5] += wall_clock64() - t2;
+            }
+            if (lenc) {
+                if (d + 1 > MAXLEV) {
+                    if (threadIdx.x == 0) {
+                        atomicOr(&A.st.sy.err[s], 8);
+                        S.wstop = 1;
+                    }
+                    break;
+                }
+                if (threadIdx.x == 0) S.jst[d] = j;
+                wave_sync();
+                d++;
+                if (d > maxd) maxd = d;
+                j = 1;
+                continue;
+            }
+        }
+        resume = false;
+        const unsigned long long t0 = prof ? wall_clock64() : 0;
+        wave_kicks(S, nB, nE, d, dth);
+        const unsigned long long t1 = prof ? wall_clock64() : 0;
+        wave_set_levels(S, nB, d);
+        if (prof) {
+            S.prof[d * NPROF + 6] += t1 - t0;
+            S.prof[d * NPROF + 7] += wall_clock64() - t1;
+        }
+        j++;
+    }
+}
+
 __global__ __launch_bounds__(L_T) void k_subtree_lds(const SubArgs *__restrict__ ap)
 {
     const SubArgs &A = *ap;
@@ -1105,11 +1404,20 @@ __global__ __launch_bounds__(L_T) void k_subtree_lds(const SubArgs *__restrict__
                 if (tid == 0) {
                     S.jst[d] = j;
                     S.lst_level = -1;
+                    S.wstop = 0;
                 }
                 __syncthreads();
                 d++;
                 if (d > maxd) maxd = d;
                 j = 1;
+                if (!A.lclose && wave_sets(S, nB, nBt, nE, nEt, d)) {
+                    // level d and below on the first wavefront; then level d
+                    // is finished and the workgroup resumes level d - 1
+                    if (tid < 64) wave_recursion(S, A, nB, nE, d, s, nsub, maxd);
+                    __syncthreads();
+                    if (S.wstop) break;
+                    j = NTENC + 1;
+                }
                 continue;
             }
         }
