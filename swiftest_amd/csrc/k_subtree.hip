@@ -82,6 +82,7 @@ struct SubArgs {
     double dt;  // param%dt
     int lclose, short_circuit, has_pp, has_pt;
     int lds;             // 0: never use the LDS-resident kernel
+    int wave;            // 0: never hand small levels to one wavefront
     LevelView start;    // whole-device level d0; system s owns a slice of it
     const int32_t *rng;  // per system: pl begin, pl end, tp begin, tp end in `start`
     int d0;
@@ -1410,7 +1411,7 @@ __global__ __launch_bounds__(L_T) void k_subtree_lds(const SubArgs *__restrict__
                 d++;
                 if (d > maxd) maxd = d;
                 j = 1;
-                if (!A.lclose && wave_sets(S, nB, nBt, nE, nEt, d)) {
+                if (!A.lclose && A.wave && wave_sets(S, nB, nBt, nE, nEt, d)) {
                     // level d and below on the first wavefront; then level d
                     // is finished and the workgroup resumes level d - 1
                     if (tid < 64) wave_recursion(S, A, nB, nE, d, s, nsub, maxd);
@@ -1509,6 +1510,8 @@ void launch_subtree(hipStream_t st, const LevelArgs &la, const LevelView &start,
     {
         const char *e = getenv("SWG_SUBTREE_LDS");
         a.lds = !(e && e[0] == '0');
+        const char *w = getenv("SWG_SUBTREE_WAVE");
+        a.wave = !(w && w[0] == '0');
     }
     a.start = start;
     a.rng = rng;

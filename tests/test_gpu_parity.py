@@ -227,17 +227,20 @@ def test_level0_lists_match():
     _compare_systems(o, g, "lists")
 
 
-@pytest.mark.parametrize("subtree_max,lds", [("-1", "1"), ("60", "1"), ("60", "0"), ("1000000000", "1"),
-                                              ("1000000000", "0")])
-def test_recursion_modes_match_oracle(monkeypatch, subtree_max, lds):
+@pytest.mark.parametrize("subtree_max,lds,wave", [("-1", "1", "1"), ("60", "1", "1"), ("60", "1", "0"),
+                                                   ("60", "0", "1"), ("1000000000", "1", "1"),
+                                                   ("1000000000", "1", "0"), ("1000000000", "0", "1")])
+def test_recursion_modes_match_oracle(monkeypatch, subtree_max, lds, wave):
     """Every level on the whole device (-1), a hand-off to the per-system
     persistent workgroups part-way down (60) and right at level 1 (1e9), with
-    the LDS-resident workgroup kernel on or off, all reproduce the oracle bit
-    for bit, with the same substep count."""
-    print(f"recursion mode SWG_SUBTREE_MAX={subtree_max} SWG_SUBTREE_LDS={lds}", flush=True)
+    the LDS-resident workgroup kernel on or off and its one-wavefront mode for
+    small deep levels on or off, all reproduce the oracle bit for bit, with
+    the same substep count."""
+    print(f"recursion mode SWG_SUBTREE_MAX={subtree_max} SWG_SUBTREE_LDS={lds} SWG_SUBTREE_WAVE={wave}", flush=True)
     sysic = compact_disk(seed=12, npl=200, ntp=150, mhi=2e-5)
     monkeypatch.setenv("SWG_SUBTREE_MAX", subtree_max)
     monkeypatch.setenv("SWG_SUBTREE_LDS", lds)
+    monkeypatch.setenv("SWG_SUBTREE_WAVE", wave)
     g = S.GpuSystem(S.SYMBA, sysic, 0.005, short_circuit=False, lclose=False, istep_out=5)
     o = O.OracleSystem(O.SYMBA, sysic, 0.005, short_circuit=False, lclose=False, istep_out=5)
     for blk in range(3):
