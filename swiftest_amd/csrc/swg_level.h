@@ -143,14 +143,16 @@ __device__ inline void pl_shell_term_v(bool bi, double xb, double yb, double zb,
         t.dz = zb - zp;
     }
     double r2 = dot3(t.dx, t.dy, t.dz, t.dx, t.dy, t.dz);
-    double fac;
-    if (shell_fac(s2 * spp, r2, fac)) {
+    bool okp, okm;
+    double facp, facm;
+    shell_facs(s2 * spp, s2 * spm, r2, minus, okp, facp, okm, facm);
+    if (okp) {
         t.tp = true;
-        t.fp = bi ? fac * gp : -(fac * gp);
+        t.fp = bi ? facp * gp : -(facp * gp);
     }
-    if (minus && shell_fac(s2 * spm, r2, fac)) {
+    if (okm) {
         t.tm = true;
-        t.fm = bi ? fac * gp : -(fac * gp);
+        t.fm = bi ? facm * gp : -(facm * gp);
     }
 }
 
@@ -163,14 +165,16 @@ __device__ inline void tp_shell_term_v(double xt, double yt, double zt, double x
     t.dy = yt - yi;
     t.dz = zt - zi;
     double r2 = dot3(t.dx, t.dy, t.dz, t.dx, t.dy, t.dz);
-    double fac;
-    if (shell_fac(s2 * spp, r2, fac)) {
+    bool okp, okm;
+    double facp, facm;
+    shell_facs(s2 * spp, s2 * spm, r2, minus, okp, facp, okm, facm);
+    if (okp) {
         t.tp = true;
-        t.fp = -(fac * gi);
+        t.fp = -(facp * gi);
     }
-    if (minus && shell_fac(s2 * spm, r2, fac)) {
+    if (okm) {
         t.tm = true;
-        t.fm = -(fac * gi);
+        t.fm = -(facm * gi);
     }
 }
 

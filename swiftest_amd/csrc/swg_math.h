@@ -493,21 +493,27 @@ __host__ __device__ inline double pow_m1p5(double x)
     return fma(yh, e, yh);
 }
 
-// fac for one listed pair at shell radius^2 ri; false when r2 < rim1 (the
-// pair is dropped for this kick)
-__device__ inline bool shell_fac(double ri, double r2, double &fac)
+// fac of one listed pair at the two shell radii^2 rp (sign +1) and rm (sign
+// -1, when minus): ok is false when r2 < ri * RSHELL^2 (the pair is dropped
+// for that kick); inside ri the factor is r2^-1.5 times the switch
+// polynomial, outside it 1 / (r2 sqrt(r2)) (symba_kick.f90:163-196).  Both
+// forms are computed for both radii and selected, with no branch: the two
+// shells' chains then overlap, and sqrt(r2) and r2^-1.5 are formed once.
+// Each selected factor is the expression the reference evaluates for it.
+__device__ inline void shell_facs(double rp, double rm, double r2, bool minus, bool &okp, double &facp, bool &okm,
+                                  double &facm)
 {
-    double rim1 = ri * (RSHELL * RSHELL);
-    if (r2 < rim1) return false;
-    if (r2 < ri) {
-        double ris = sqrt(ri);
-        double r = sqrt(r2);
-        double rr = (ris - r) / (ris * (1.0 - RSHELL));
-        fac = pow_m1p5(r2) * ((1.0 - 3.0 * (rr * rr)) + 2.0 * ((rr * rr) * rr));
-    } else {
-        fac = 1.0 / (r2 * sqrt(r2));
-    }
-    return true;
+    const double r = sqrt(r2);
+    const double out = 1.0 / (r2 * r);
+    const double p15 = pow_m1p5(r2);
+    const double sp = sqrt(rp), sm = sqrt(rm);
+    const double rrp = (sp - r) / (sp * (1.0 - RSHELL)), rrm = (sm - r) / (sm * (1.0 - RSHELL));
+    const double inp = p15 * ((1.0 - 3.0 * (rrp * rrp)) + 2.0 * ((rrp * rrp) * rrp));
+    const double inm = p15 * ((1.0 - 3.0 * (rrm * rrm)) + 2.0 * ((rrm * rrm) * rrm));
+    okp = !(r2 < rp * (RSHELL * RSHELL));
+    okm = minus && !(r2 < rm * (RSHELL * RSHELL));
+    facp = r2 < rp ? inp : out;
+    facm = r2 < rm ? inm : out;
 }
 
 // renc at recursion depth irec: rhill * RHSCALE * RSHELL**irec with the
