@@ -449,8 +449,8 @@ struct LdsSet {
     // started at, that level's bodies (pl, then tp as nB + t), pl pairs, tp
     // entries, and the kick entries of its bodies grouped by body
     int16_t wB[64], wP[64], wT[64], wE[64];
-    int8_t wEs[64], wEe[64];
-    int wnB, wnP, wnT, wnE, wstop;
+    int8_t wEs[64], wEe[64], wH[64];  // wH: slots of the bodies the second wavefront drifts
+    int wnB, wnP, wnT, wnE, wstop, wnH, wlenc;
     int npair, flag_pp, flag_pt, bad, nact, nlst, lst_level, kmax;
     unsigned long long prof[(MAXLEV + 2) * NPROF];  // per-level phase ticks when profiling
 };
@@ -1052,37 +1052,73 @@ __device__ void wave_kicks(LdsSet &S, int nB, int nE, int d, double dth)
     wave_sync();
 }
 
-// the drifts of level d (lds_drift)
-__device__ void wave_drift(LdsSet &S, const SubArgs &A, int nB, int d, double dtl, int s)
+// The drifts of level d (lds_drift), called by the whole workgroup.  A
+// wavefront runs a bound body's series solution (kepmd) and an unbound one's
+// iteration (kepu) one after the other, so the first wavefront keeps the bodies
+// that look bound and hands the others to the second, and the two drift side
+// by side.  The test only places the work (each drift is drift_one either
+// way): alpha > 0 of drift_dan is 2 mu / r0 > v0^2, taken here without the
+// square root and division as 4 mu^2 > v0^4 r0^2.
+// Returns S.wlenc, read between the two barriers (it is rewritten only after
+// the second).
+__device__ bool wave_drift(LdsSet &S, const SubArgs &A, int nB, int d, double dtl, int s)
 {
-    const int k = threadIdx.x;
-    if (k < S.wnB) {
-        const int q = S.wB[k];
+    const int tid = threadIdx.x;
+    const double mu = A.st.sy.GMcb[s];
+    bool on = false, far = false;
+    int q = 0;
+    if (tid < 64) {
+        if (tid < S.wnB) {
+            q = S.wB[tid];
+            const bool ispl = q < nB;
+            const int t = q - nB;
+            on = ispl ? (S.pst[q] != INACTIVE && S.plev[q] == d) : (S.tst[t] == ACTIVE && S.tlev[t] == d);
+            if (on) {
+                const double rx = ispl ? S.px[q] : S.tx[t], ry = ispl ? S.py[q] : S.ty[t], rz = ispl ? S.pz[q] : S.tz[t];
+                const double vx = ispl ? S.pvx[q] : S.tvx[t], vy = ispl ? S.pvy[q] : S.tvy[t],
+                             vz = ispl ? S.pvz[q] : S.tvz[t];
+                const double v2 = dot3(vx, vy, vz, vx, vy, vz);
+                far = !(4.0 * mu * mu > (v2 * v2) * dot3(rx, ry, rz, rx, ry, rz));
+            }
+        }
+        const uint64_t m = __ballot(far);
+        if (far) S.wH[__popcll(m & ((1ull << tid) - 1))] = (int8_t)tid;
+        if (tid == 0) S.wnH = __popcll(m);
+    }
+    __syncthreads();
+    const bool lenc = S.wlenc != 0;
+    int k = -1;
+    if (tid < 64) {
+        if (on && !far) k = tid;
+    } else if (tid < 128 && tid - 64 < S.wnH) {
+        k = S.wH[tid - 64];
+        q = S.wB[k];
+    }
+    if (k >= 0) {
         const bool ispl = q < nB;
         const int t = q - nB;
-        if (ispl ? (S.pst[q] != INACTIVE && S.plev[q] == d) : (S.tst[t] == ACTIVE && S.tlev[t] == d)) {
-            double *x = ispl ? S.px : S.tx, *y = ispl ? S.py : S.ty, *z = ispl ? S.pz : S.tz;
-            double *vx = ispl ? S.pvx : S.tvx, *vy = ispl ? S.pvy : S.tvy, *vz = ispl ? S.pvz : S.tvz;
-            const int i = ispl ? q : t;
-            double rx = x[i], ry = y[i], rz = z[i], ux = vx[i], uy = vy[i], uz = vz[i];
-            const int fl = drift_one(A.st.sy.GMcb[s], rx, ry, rz, ux, uy, uz, dtl);
-            x[i] = rx;
-            y[i] = ry;
-            z[i] = rz;
-            vx[i] = ux;
-            vy[i] = uy;
-            vz[i] = uz;
-            if (fl) {
-                if (ispl) {
-                    S.pst[q] = DISCARDED_DRIFTERR;
-                    atomicOr(&A.st.sy.err[s], 2);
-                } else {
-                    S.tst[t] = DISCARDED_DRIFTERR;
-                }
+        double *x = ispl ? S.px : S.tx, *y = ispl ? S.py : S.ty, *z = ispl ? S.pz : S.tz;
+        double *vx = ispl ? S.pvx : S.tvx, *vy = ispl ? S.pvy : S.tvy, *vz = ispl ? S.pvz : S.tvz;
+        const int i = ispl ? q : t;
+        double rx = x[i], ry = y[i], rz = z[i], ux = vx[i], uy = vy[i], uz = vz[i];
+        const int fl = drift_one(mu, rx, ry, rz, ux, uy, uz, dtl);
+        x[i] = rx;
+        y[i] = ry;
+        z[i] = rz;
+        vx[i] = ux;
+        vy[i] = uy;
+        vz[i] = uz;
+        if (fl) {
+            if (ispl) {
+                S.pst[q] = DISCARDED_DRIFTERR;
+                atomicOr(&A.st.sy.err[s], 2);
+            } else {
+                S.tst[t] = DISCARDED_DRIFTERR;
             }
         }
     }
-    wave_sync();
+    __syncthreads();
+    return lenc;
 }
 
 // the level reset after a substep of level d (lds_set_levels)
@@ -1102,11 +1138,15 @@ __device__ void wave_set_levels(LdsSet &S, int nB, int d)
     wave_sync();
 }
 
-// Levels dw and below on the first wavefront, from the first substep of dw
-// to the end of dw (the loop of k_subtree_lds).  Adds its substeps to nsub
-// and raises maxd; sets S.wstop when the recursion must stop (error flagged).
+// Levels dw and below in wave mode, from the first substep of dw to the end
+// of dw (the loop of k_subtree_lds), called by the whole workgroup: the first
+// wavefront runs the checks, kicks and level resets, the drifts take the
+// second one as well, and the others only follow the loop.  Adds its substeps
+// to nsub and raises maxd; sets S.wstop when the recursion must stop (error
+// flagged).
 __device__ void wave_recursion(LdsSet &S, const SubArgs &A, int nB, int nE, int dw, int s, int &nsub, int &maxd)
 {
+    const bool w0 = threadIdx.x < 64;
     const bool prof = A.prof && threadIdx.x == 0;
     int d = dw, j = 1;
     bool resume = false;
@@ -1131,11 +1171,14 @@ __device__ void wave_recursion(LdsSet &S, const SubArgs &A, int nB, int nE, int 
             }
             nsub++;
             const unsigned long long t0 = prof ? wall_clock64() : 0;
-            const bool lenc = wave_check(S, A, d, dtl);
+            if (w0) {
+                const bool le = wave_check(S, A, d, dtl);
+                if (threadIdx.x == 0) S.wlenc = le;
+            }
             const unsigned long long t1 = prof ? wall_clock64() : 0;
-            wave_kicks(S, nB, nE, d, dth);
+            if (w0) wave_kicks(S, nB, nE, d, dth);
             const unsigned long long t2 = prof ? wall_clock64() : 0;
-            wave_drift(S, A, nB, d, dtl, s);
+            const bool lenc = wave_drift(S, A, nB, d, dtl, s);
             if (prof) {
                 unsigned long long *P = S.prof + d * NPROF;
                 P[0] += 1;
@@ -1163,9 +1206,9 @@ __device__ void wave_recursion(LdsSet &S, const SubArgs &A, int nB, int nE, int 
         }
         resume = false;
         const unsigned long long t0 = prof ? wall_clock64() : 0;
-        wave_kicks(S, nB, nE, d, dth);
+        if (w0) wave_kicks(S, nB, nE, d, dth);
         const unsigned long long t1 = prof ? wall_clock64() : 0;
-        wave_set_levels(S, nB, d);
+        if (w0) wave_set_levels(S, nB, d);
         if (prof) {
             S.prof[d * NPROF + 6] += t1 - t0;
             S.prof[d * NPROF + 7] += wall_clock64() - t1;
@@ -1414,7 +1457,7 @@ __global__ __launch_bounds__(L_T) void k_subtree_lds(const SubArgs *__restrict__
                 if (!A.lclose && A.wave && wave_sets(S, nB, nBt, nE, nEt, d)) {
                     // level d and below on the first wavefront; then level d
                     // is finished and the workgroup resumes level d - 1
-                    if (tid < 64) wave_recursion(S, A, nB, nE, d, s, nsub, maxd);
+                    wave_recursion(S, A, nB, nE, d, s, nsub, maxd);
                     __syncthreads();
                     if (S.wstop) break;
                     j = NTENC + 1;
