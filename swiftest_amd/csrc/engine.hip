@@ -330,7 +330,7 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
     owned.push_back(d_scan_sys);
     if (own(&bw.keepB, npl) || own(&bw.cntE, npl) || own(&bw.posB, npl) || own(&bw.posE, npl) ||
         own(&bw.keepBt, ntp) || own(&bw.cntEt, ntp) || own(&bw.posBt, ntp) || own(&bw.posEt, ntp) ||
-        own(&bw.tot, 10) || own(&bw.rng, 4 * (size_t)nsys) || own(&bw.caps, nsys) ||
+        own(&bw.tot, 10) || own(&bw.rng, 4 * (size_t)nsys) || own(&bw.caps, nsys) || own(&d_kc, 8 * (size_t)npl) ||
         own(&d_arena_off, nsys + 1) || own(&d_sub_stats, 4))
         return SWG_ERR_HIP;
     bw.scan = d_scan;
@@ -693,7 +693,10 @@ int Ensemble::recur_level(int d, const LevelView &v, const LevelCounts &c)
         launch_level_check(st, la, v, c, d, dtl, prm.short_circuit, d_any);
         const bool room = d + 1 <= MAXLEV;
         if (int rc = room ? build_begin(d + 1, &v) : post_any()) return rc;
-        launch_level_kicks(st, la, &v, nB, nBt, d, dth);
+        // opening kicks: from the previous substep's closing kicks when
+        // nothing between them can have changed the sums (no collision test)
+        const bool kept = j > 1 && !prm.lclose;
+        launch_level_kicks(st, la, &v, nB, nBt, d, dth, kept ? 2 : 0, d_kc);
         launch_level_drift(st, la, v, nB, nBt, d, dtl);
         int32_t lenc = 0;
         if (int rc = room ? build_wait(lenc) : wait_any(lenc)) return rc;
@@ -710,7 +713,7 @@ int Ensemble::recur_level(int d, const LevelView &v, const LevelCounts &c)
                 if (int rc = recur_level(d + 1, nv, nc)) return rc;
             }
         }
-        launch_level_kicks(st, la, &v, nB, nBt, d, dth);
+        launch_level_kicks(st, la, &v, nB, nBt, d, dth, (j < NTENC && !prm.lclose) ? 1 : 0, d_kc);
         if (prm.lclose) launch_level_collision(st, la, v, nB, nBt, dtl);
         launch_level_set_levels(st, la, v, c, d);
     }

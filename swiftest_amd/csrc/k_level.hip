@@ -272,7 +272,8 @@ __global__ __launch_bounds__(256) void k_lv_check_pt(RecState a, const int2 *Et,
 constexpr int KT_T = 256, KT_C = 1024, KT_R = KT_C / KT_T;
 
 template <int TB, bool L0>
-__global__ __launch_bounds__(KT_T) void k_lv_kick_pl(RecState a, LevelView v, int nB, int d, double dth, int has_pp)
+__global__ __launch_bounds__(KT_T) void k_lv_kick_pl(RecState a, LevelView v, int nB, int d, double dth, int has_pp,
+                                                     double *kc)
 {
     constexpr int K = KT_C / TB, LD = TB + 1;  // LDS slot of (body tb, entry k): k * LD + tb
     __shared__ double sdx[K * LD], sdy[K * LD], sdz[K * LD], sfp[K * LD], sfm[K * LD];
@@ -304,7 +305,10 @@ __global__ __launch_bounds__(KT_T) void k_lv_kick_pl(RecState a, LevelView v, in
     }
     __syncthreads();
     const int nmax = smax;
-    if (nmax == 0) return;
+    if (nmax == 0) {
+        if (kc && tid < TB && q0 + tid < nB) kc[8 * (size_t)(q0 + tid) + 6] = 0.0;
+        return;
+    }
     const double spp = shell_sq(d + 1), spm = shell_sq(d);
     const bool minus = d > 0;
     const PlDev &pl = a.pl;
@@ -387,21 +391,50 @@ __global__ __launch_bounds__(KT_T) void k_lv_kick_pl(RecState a, LevelView v, in
         }
         __syncthreads();
     }
-    if (tid < TB && q0 + tid < nB) apply_shell_kicks(a.pl.vb, sb[tid], dth, anyp, p, anym, m);
+    if (tid < TB && q0 + tid < nB) {
+        apply_shell_kicks(a.pl.vb, sb[tid], dth, anyp, p, anym, m);
+        if (kc) {
+            double *c = kc + 8 * (size_t)(q0 + tid);
+            c[0] = p.x;
+            c[1] = p.y;
+            c[2] = p.z;
+            c[3] = m.x;
+            c[4] = m.y;
+            c[5] = m.z;
+            c[6] = (anyp ? 1.0 : 0.0) + (anym ? 2.0 : 0.0);
+        }
+    }
+}
+
+// The opening shell kicks of a substep from the sums kept by the closing
+// kicks of the substep before it at the same level (kc, 8 words per body):
+// positions, masses, Hill radii and pair statuses have not changed in
+// between and no body's level has crossed d, so the reference computes the
+// same sums (symba_step.f90:208-247, symba_kick.f90:126-230).
+__global__ __launch_bounds__(256) void k_lv_kick_pl_kept(RecState a, LevelView v, int nB, double dth,
+                                                          const double *kc)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nB) return;
+    const double *c = kc + 8 * (size_t)q;
+    const int fl = (int)c[6];
+    if (!fl) return;
+    apply_shell_kicks(a.pl.vb, v.B[q], dth, (fl & 1) != 0, make_double3(c[0], c[1], c[2]), (fl & 2) != 0,
+                      make_double3(c[3], c[4], c[5]));
 }
 
 template <bool L0>
 static void launch_kick_pl_tiles(hipStream_t st, const RecState &a, const LevelView &v, int nB, int d, double dth,
-                                 int has_pp)
+                                 int has_pp, double *kc)
 {
     // bodies per workgroup: the largest tile that still gives two workgroups
     // per compute unit
     constexpr int WANT = 512;
-    if (nB >= 64 * WANT) k_lv_kick_pl<64, L0><<<cdiv(nB, 64), KT_T, 0, st>>>(a, v, nB, d, dth, has_pp);
-    else if (nB >= 32 * WANT) k_lv_kick_pl<32, L0><<<cdiv(nB, 32), KT_T, 0, st>>>(a, v, nB, d, dth, has_pp);
-    else if (nB >= 16 * WANT) k_lv_kick_pl<16, L0><<<cdiv(nB, 16), KT_T, 0, st>>>(a, v, nB, d, dth, has_pp);
-    else if (nB >= 8 * WANT) k_lv_kick_pl<8, L0><<<cdiv(nB, 8), KT_T, 0, st>>>(a, v, nB, d, dth, has_pp);
-    else k_lv_kick_pl<4, L0><<<cdiv(nB, 4), KT_T, 0, st>>>(a, v, nB, d, dth, has_pp);
+    if (nB >= 64 * WANT) k_lv_kick_pl<64, L0><<<cdiv(nB, 64), KT_T, 0, st>>>(a, v, nB, d, dth, has_pp, kc);
+    else if (nB >= 32 * WANT) k_lv_kick_pl<32, L0><<<cdiv(nB, 32), KT_T, 0, st>>>(a, v, nB, d, dth, has_pp, kc);
+    else if (nB >= 16 * WANT) k_lv_kick_pl<16, L0><<<cdiv(nB, 16), KT_T, 0, st>>>(a, v, nB, d, dth, has_pp, kc);
+    else if (nB >= 8 * WANT) k_lv_kick_pl<8, L0><<<cdiv(nB, 8), KT_T, 0, st>>>(a, v, nB, d, dth, has_pp, kc);
+    else k_lv_kick_pl<4, L0><<<cdiv(nB, 4), KT_T, 0, st>>>(a, v, nB, d, dth, has_pp, kc);
 }
 
 // Shell kicks of one test particle per thread
@@ -575,16 +608,18 @@ void launch_level_check(hipStream_t st, const LevelArgs &la, const LevelView &v,
     if (c.nEt) k_lv_check_pt<<<cdiv(c.nEt, 256), 256, 0, st>>>(a, v.Et, (int)c.nEt, d, dtl, short_circuit, any);
 }
 
-void launch_level_kicks(hipStream_t st, const LevelArgs &la, const LevelView *v, int nB, int nBt, int d, double dth)
+void launch_level_kicks(hipStream_t st, const LevelArgs &la, const LevelView *v, int nB, int nBt, int d, double dth,
+                        int keep, double *kc)
 {
     RecState a = rec_state(la.pl, la.tp, la.pp, la.pt, la.sy);
     LevelView vv{};
     if (v) vv = *v;
     if (v) {
-        if (nB) launch_kick_pl_tiles<false>(st, a, vv, nB, d, dth, la.has_pp);
+        if (nB && keep == 2) k_lv_kick_pl_kept<<<cdiv(nB, 256), 256, 0, st>>>(a, vv, nB, dth, kc);
+        else if (nB) launch_kick_pl_tiles<false>(st, a, vv, nB, d, dth, la.has_pp, keep == 1 ? kc : nullptr);
         if (nBt) k_lv_kick_tp<false><<<cdiv(nBt, 256), 256, 0, st>>>(a, vv, nBt, d, dth, la.has_pt);
     } else {  // level 0: the full adjacency of every body
-        if (nB && la.has_pp) launch_kick_pl_tiles<true>(st, a, vv, nB, d, dth, la.has_pp);
+        if (nB && la.has_pp) launch_kick_pl_tiles<true>(st, a, vv, nB, d, dth, la.has_pp, nullptr);
         if (nBt && la.has_pt) k_lv_kick_tp<true><<<cdiv(nBt, 256), 256, 0, st>>>(a, vv, nBt, d, dth, la.has_pt);
     }
 }

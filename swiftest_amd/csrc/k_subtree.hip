@@ -1005,10 +1005,36 @@ __device__ bool wave_check(LdsSet &S, const SubArgs &A, int d, double dtl)
     return fpp || fpt;
 }
 
-// the +1 / -1 shell kicks of level d (lds_kicks)
-__device__ void wave_kicks(LdsSet &S, int nB, int nE, int d, double dth)
+// A lane's shell-kick sums kept from the closing kicks of a substep for the
+// opening kicks of the next one at the same level.  Between the two only the
+// level reset, the encounter check and (not in wave mode) the collision test
+// run: positions, masses, Hill radii and pair statuses are unchanged, and no
+// body's level crosses the current one, so the reference computes the same
+// sums again (symba_step.f90:208-247, symba_kick.f90:126-230).
+struct KCache {
+    double3 p, m;
+    int fl;
+};
+
+// the +1 / -1 shell kicks of level d (lds_kicks).  mode 0: evaluate; 1:
+// evaluate and keep the sums in c; 2: apply the sums kept in c.
+__device__ void wave_kicks(LdsSet &S, int nB, int nE, int d, double dth, int mode, KCache &c)
 {
     const int k = threadIdx.x;
+    if (mode == 2) {
+        if (k < S.wnB) {
+            const int q = S.wB[k];
+            const bool ispl = q < nB;
+            if ((ispl ? S.plev[q] : S.tlev[q - nB]) >= d) {
+                if (ispl) lds_apply(S.pvx[q], S.pvy[q], S.pvz[q], dth, (c.fl & 1) != 0, c.p, (c.fl & 2) != 0, c.m);
+                else
+                    lds_apply(S.tvx[q - nB], S.tvy[q - nB], S.tvz[q - nB], dth, (c.fl & 1) != 0, c.p, (c.fl & 2) != 0,
+                              c.m);
+            }
+        }
+        wave_sync();
+        return;
+    }
     const double spp = shell_sq(d + 1), spm = shell_sq(d);
     if (k < S.wnE) {
         const int u = S.wE[k];
@@ -1047,6 +1073,11 @@ __device__ void wave_kicks(LdsSet &S, int nB, int nE, int d, double dth)
             fold_terms(S.kt, S.wEs[k], S.wEe[k], p, m, fl);
             if (ispl) lds_apply(S.pvx[q], S.pvy[q], S.pvz[q], dth, (fl & 1) != 0, p, (fl & 2) != 0, m);
             else lds_apply(S.tvx[q - nB], S.tvy[q - nB], S.tvz[q - nB], dth, (fl & 1) != 0, p, (fl & 2) != 0, m);
+            if (mode == 1) {
+                c.p = p;
+                c.m = m;
+                c.fl = fl;
+            }
         }
     }
     wave_sync();
@@ -1150,6 +1181,8 @@ __device__ void wave_recursion(LdsSet &S, const SubArgs &A, int nB, int nE, int 
     const bool prof = A.prof && threadIdx.x == 0;
     int d = dw, j = 1;
     bool resume = false;
+    KCache kc{};
+    int kcd = -1;  // the level whose closing-kick sums kc holds (-1: none)
     for (;;) {
         double p3 = 1.0;
         for (int q = 0; q < d; q++) p3 *= NTENC;
@@ -1176,7 +1209,8 @@ __device__ void wave_recursion(LdsSet &S, const SubArgs &A, int nB, int nE, int 
                 if (threadIdx.x == 0) S.wlenc = le;
             }
             const unsigned long long t1 = prof ? wall_clock64() : 0;
-            if (w0) wave_kicks(S, nB, nE, d, dth);
+            if (w0) wave_kicks(S, nB, nE, d, dth, kcd == d ? 2 : 0, kc);
+            kcd = -1;
             const unsigned long long t2 = prof ? wall_clock64() : 0;
             const bool lenc = wave_drift(S, A, nB, d, dtl, s);
             if (prof) {
@@ -1206,7 +1240,8 @@ __device__ void wave_recursion(LdsSet &S, const SubArgs &A, int nB, int nE, int 
         }
         resume = false;
         const unsigned long long t0 = prof ? wall_clock64() : 0;
-        if (w0) wave_kicks(S, nB, nE, d, dth);
+        if (w0) wave_kicks(S, nB, nE, d, dth, 1, kc);
+        kcd = d;
         const unsigned long long t1 = prof ? wall_clock64() : 0;
         if (w0) wave_set_levels(S, nB, d);
         if (prof) {

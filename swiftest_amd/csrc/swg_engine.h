@@ -58,6 +58,7 @@ struct Ensemble {
     int64_t subtree_max = 16384;  // SWG_SUBTREE_MAX; < 0: every level on the whole device
     bool subtree_forced = false;  // SWG_SUBTREE_MAX given: hand off on size alone
     bool level_fits = false;      // every set of the level just built fits the LDS-resident kernel
+    double *d_kc = nullptr;       // shell-kick sums kept from one substep's closing kicks for the next's opening ones
     static constexpr int SUBTREE_MIN_SYSTEMS = 128;  // systems with work that fill the device
     LevelArgs level_args() const;
     int build_begin(int dnew, const LevelView *src);

@@ -165,7 +165,12 @@ void launch_build_fill(hipStream_t st, const LevelArgs &la, const LevelView *src
 void launch_level_check(hipStream_t st, const LevelArgs &la, const LevelView &v, const LevelCounts &c, int d,
                         double dtl, int short_circuit, int32_t *any);
 // v == nullptr: the level-0 kicks over the full adjacency of nB / nBt bodies
-void launch_level_kicks(hipStream_t st, const LevelArgs &la, const LevelView *v, int nB, int nBt, int d, double dth);
+// keep 0: evaluate the shell kicks; 1: evaluate and keep the pl bodies' sums
+// in kc (8 words per body of v); 2: apply the pl sums kept in kc by the
+// closing kicks of the previous substep at the same level (the tp kicks are
+// evaluated in every mode)
+void launch_level_kicks(hipStream_t st, const LevelArgs &la, const LevelView *v, int nB, int nBt, int d, double dth,
+                        int keep = 0, double *kc = nullptr);
 void launch_level_drift(hipStream_t st, const LevelArgs &la, const LevelView &v, int nB, int nBt, int d, double dtl);
 void launch_level_collision(hipStream_t st, const LevelArgs &la, const LevelView &v, int nB, int nBt, double dtl);
 void launch_level_set_levels(hipStream_t st, const LevelArgs &la, const LevelView &v, const LevelCounts &c, int d);
