@@ -418,6 +418,18 @@ struct KTerm {
     double dx, dy, dz, fp, fm;
     int fl;
 };
+// A body's shell-kick sums kept from the closing kicks of a substep for the
+// opening kicks of the next one at the same level.  Between the two only the
+// level reset, the encounter check and (not in wave mode) the collision test
+// run: positions, masses, Hill radii and pair statuses are unchanged (the
+// collision test is off when they are kept), and no body's level crosses the
+// current one, so the reference computes the same sums again
+// (symba_step.f90:208-247, symba_kick.f90:126-230).
+struct KCache {
+    double3 p, m;
+    int fl;
+};
+
 constexpr int L_KC = 512;                              // entry terms per kick chunk
 constexpr int L_KS = (L_NB + L_NBT + L_T - 1) / L_T;  // bodies folded per lane
 static_assert(L_KS * L_T == L_NB + L_NBT, "the kick-list arrays cover every lane slot");
@@ -660,9 +672,28 @@ __device__ void lds_level_list(LdsSet &S, int nB, int nBt, int nE, int nEt, int 
 // bodies), so each body's sum is the serial sum over its entries.  Entries
 // left out of the list (partner below the level) contribute no term, and a
 // body with no listed entry gets no kick, as in the reference.
-__device__ void lds_kicks(LdsSet &S, int nB, int nBt, int nE, int nEt, int d, double dth, bool prof)
+// mode 0: evaluate; 1: evaluate and keep each folded body's sums in kc; 2:
+// apply the sums kept in kc by the previous call, the closing kicks of the
+// substep before at this level (the same lane folds the same bodies while the
+// level's list stands).
+__device__ __attribute__((always_inline)) inline void lds_kicks(LdsSet &S, int nB, int nBt, int nE, int nEt, int d,
+                                                                double dth, bool prof, int mode, KCache (&kc)[L_KS])
 {
     const int tid = threadIdx.x;
+    if (mode == 2) {
+        const int nact = S.nact;
+#pragma unroll
+        for (int k = 0; k < L_KS; k++) {
+            const int ai = tid + k * L_T;
+            if (ai >= nact) continue;
+            const int q = S.act[ai];
+            const bool ap = (kc[k].fl & 1) != 0, am = (kc[k].fl & 2) != 0;
+            if (q < nB) lds_apply(S.pvx[q], S.pvy[q], S.pvz[q], dth, ap, kc[k].p, am, kc[k].m);
+            else lds_apply(S.tvx[q - nB], S.tvy[q - nB], S.tvz[q - nB], dth, ap, kc[k].p, am, kc[k].m);
+        }
+        __syncthreads();
+        return;
+    }
     if (S.lst_level != d) {
         lds_level_list(S, nB, nBt, nE, nEt, d);
         if (prof && tid == 0) S.prof[d * NPROF + 14] += 1;
@@ -741,6 +772,11 @@ __device__ void lds_kicks(LdsSet &S, int nB, int nBt, int nE, int nEt, int d, do
         const int q = S.act[ai];
         if (q < nB) lds_apply(S.pvx[q], S.pvy[q], S.pvz[q], dth, anyp[k], p[k], anym[k], m[k]);
         else lds_apply(S.tvx[q - nB], S.tvy[q - nB], S.tvz[q - nB], dth, anyp[k], p[k], anym[k], m[k]);
+        if (mode == 1) {
+            kc[k].p = p[k];
+            kc[k].m = m[k];
+            kc[k].fl = (anyp[k] ? 1 : 0) | (anym[k] ? 2 : 0);
+        }
     }
     if (prof) {
         if (tid == 0) S.kmax = 0;
@@ -1004,17 +1040,6 @@ __device__ bool wave_check(LdsSet &S, const SubArgs &A, int d, double dtl)
     wave_sync();
     return fpp || fpt;
 }
-
-// A lane's shell-kick sums kept from the closing kicks of a substep for the
-// opening kicks of the next one at the same level.  Between the two only the
-// level reset, the encounter check and (not in wave mode) the collision test
-// run: positions, masses, Hill radii and pair statuses are unchanged, and no
-// body's level crosses the current one, so the reference computes the same
-// sums again (symba_step.f90:208-247, symba_kick.f90:126-230).
-struct KCache {
-    double3 p, m;
-    int fl;
-};
 
 // the +1 / -1 shell kicks of level d (lds_kicks).  mode 0: evaluate; 1:
 // evaluate and keep the sums in c; 2: apply the sums kept in c.
@@ -1441,6 +1466,8 @@ __global__ __launch_bounds__(L_T) void k_subtree_lds(const SubArgs *__restrict__
     int d = d0, j = 1;
     bool resume = false;
     int nsub = 0, maxd = d0;
+    KCache kc[L_KS];
+    int kcd = -1;  // the level whose closing-kick sums kc holds (-1: none)
     for (;;) {
         double p3 = 1.0;
         for (int q = 0; q < d; q++) p3 *= NTENC;
@@ -1462,7 +1489,8 @@ __global__ __launch_bounds__(L_T) void k_subtree_lds(const SubArgs *__restrict__
             unsigned long long t0 = prof ? wall_clock64() : 0, t1 = 0, t2 = 0, t3 = 0;
             const bool lenc = lds_check(S, A, nB, nEt, d, dtl);
             if (prof) t1 = wall_clock64();
-            lds_kicks(S, nB, nBt, nE, nEt, d, dth, A.prof != nullptr);
+            lds_kicks(S, nB, nBt, nE, nEt, d, dth, A.prof != nullptr, kcd == d ? 2 : 0, kc);
+            kcd = -1;
             if (prof) t2 = wall_clock64();
             lds_drift(S, A, nB, nBt, d, dtl, s, A.prof != nullptr);
             if (prof) {
@@ -1504,7 +1532,8 @@ __global__ __launch_bounds__(L_T) void k_subtree_lds(const SubArgs *__restrict__
         {
             const bool prof = A.prof && tid == 0;
             unsigned long long t0 = prof ? wall_clock64() : 0;
-            lds_kicks(S, nB, nBt, nE, nEt, d, dth, A.prof != nullptr);
+            lds_kicks(S, nB, nBt, nE, nEt, d, dth, A.prof != nullptr, A.lclose ? 0 : 1, kc);
+            kcd = A.lclose ? -1 : d;
             unsigned long long t1 = prof ? wall_clock64() : 0;
             if (A.lclose) lds_collision(S, A, nB, nBt, nEt, d, dtl, s);
             lds_set_levels(S, nB, nBt, nEt, d);
