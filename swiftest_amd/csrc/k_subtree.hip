@@ -1508,9 +1508,11 @@ __global__ __launch_bounds__(L_T) void k_subtree_lds(const SubArgs *__restrict__
                     if (tid == 0) atomicOr(&a.sy.err[s], 8);
                     break;
                 }
+                // the kick list of level d stays: a deeper level in wave mode
+                // leaves it alone, one in this path rebuilds it for its own
+                // level (and lst_level tells)
                 if (tid == 0) {
                     S.jst[d] = j;
-                    S.lst_level = -1;
                     S.wstop = 0;
                 }
                 __syncthreads();
