@@ -119,7 +119,6 @@ Ensemble::~Ensemble()
     if (d_arena) (void)hipFree(d_arena);
     for (void *q : lv_buf)
         if (q) (void)hipFree(q);
-    if (h_subargs) (void)hipHostFree(h_subargs);
 }
 
 template <class T>
@@ -337,9 +336,6 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
     bw.maxsz = (int32_t *)(bw.tot + 6);
     lv_buf.assign(MAXLEV + 2, nullptr);
     lv_cap.assign(MAXLEV + 2, 0);
-    HIPCHK(hipHostMalloc(&h_subargs, subtree_args_bytes()));
-    HIPCHK(hipMalloc(&d_subargs, subtree_args_bytes()));
-    owned.push_back(d_subargs);
     if (const char *e = getenv("SWG_SUBTREE_MAX")) {
         subtree_max = strtoll(e, nullptr, 10);
         subtree_forced = true;
@@ -689,7 +685,7 @@ int Ensemble::recur_level(int d, const LevelView &v, const LevelCounts &c)
         tr_sub[d]++;
         tr_B[d] += c.nB + c.nBt;
         tr_E[d] += c.nE + c.nEt;
-        launch_clear_flags(st, sy, nsys, d_any);
+        if (j == 1) launch_clear_flags(st, sy, nsys, d_any);  // later substeps: cleared by the level reset
         launch_level_check(st, la, v, c, d, dtl, prm.short_circuit, d_any);
         const bool room = d + 1 <= MAXLEV;
         if (int rc = room ? build_begin(d + 1, &v) : post_any()) return rc;
@@ -715,7 +711,7 @@ int Ensemble::recur_level(int d, const LevelView &v, const LevelCounts &c)
         }
         launch_level_kicks(st, la, &v, nB, nBt, d, dth, (j < NTENC && !prm.lclose) ? 1 : 0, d_kc);
         if (prm.lclose) launch_level_collision(st, la, v, nB, nBt, dtl);
-        launch_level_set_levels(st, la, v, c, d);
+        launch_level_set_levels(st, la, v, c, d, j < NTENC ? d_any : nullptr);
     }
     return 0;
 }
@@ -733,11 +729,8 @@ int Ensemble::run_subtree(int d0, const LevelView &v)
         HIPCHK(dalloc(&d_arena, (size_t)w));
         arena_words = w;
     }
-    // the staging buffer of the argument block is rewritten only after the
-    // host has waited for a later event on the stream (build_wait), so the
-    // previous copy out of it has completed
     launch_subtree(st, level_args(), v, bw.rng, d0, prm.dt, prm.lclose, prm.short_circuit, d_arena, d_arena_off,
-                   d_sub_stats, h_subargs, d_subargs, level_fits);
+                   d_sub_stats, level_fits);
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -488,10 +488,19 @@ __global__ __launch_bounds__(256) void k_lv_collision(RecState a, LevelView v, i
 }
 
 // symba_step_set_recur_levels_system (symba_step.f90:118-160) at level d
+// any (non-null): also clear the encounter flags (per system and any) for
+// the next substep's check, which follows with nothing in between
 __global__ __launch_bounds__(256) void k_lv_set_levels(RecState a, LevelView v, int nB, int nBt, int nE, int nEt,
-                                                       int d)
+                                                       int d, int nsys, int32_t *any)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (any && blockIdx.x == 0) {
+        for (int s = threadIdx.x; s < nsys; s += blockDim.x) {
+            a.sy.flag_pp[s] = 0;
+            a.sy.flag_pt[s] = 0;
+        }
+        if (threadIdx.x == 0) *any = 0;
+    }
     const int irecp = d + 1;
     if (q < nB && a.pl.levelg[v.B[q]] == irecp) a.pl.levelg[v.B[q]] = d;
     if (q < nBt && a.tp.levelg[v.Bt[q]] == irecp) a.tp.levelg[v.Bt[q]] = d;
@@ -636,12 +645,16 @@ void launch_level_collision(hipStream_t st, const LevelArgs &la, const LevelView
     if (nB + nBt) k_lv_collision<<<cdiv(nB + nBt, 256), 256, 0, st>>>(a, v, nB, nBt, dtl, la.has_pp);
 }
 
-void launch_level_set_levels(hipStream_t st, const LevelArgs &la, const LevelView &v, const LevelCounts &c, int d)
+void launch_level_set_levels(hipStream_t st, const LevelArgs &la, const LevelView &v, const LevelCounts &c, int d,
+                             int32_t *any)
 {
     RecState a = rec_state(la.pl, la.tp, la.pp, la.pt, la.sy);
     int64_t n = std::max(std::max(c.nB, c.nBt), std::max(c.nE, c.nEt));
     if (n)
-        k_lv_set_levels<<<cdiv(n, 256), 256, 0, st>>>(a, v, (int)c.nB, (int)c.nBt, (int)c.nE, (int)c.nEt, d);
+        k_lv_set_levels<<<cdiv(n, 256), 256, 0, st>>>(a, v, (int)c.nB, (int)c.nBt, (int)c.nE, (int)c.nEt, d,
+                                                      la.nsys, any);
+    else if (any)
+        launch_clear_flags(st, la.sy, la.nsys, any);
 }
 
 }  // namespace swg

@@ -290,9 +290,8 @@ __device__ void phase_set_levels(const SubArgs &A, const LevelRec &L, int d)
 // symba_step_recur_system from level d0 down, one workgroup per system.  The
 // argument block is read through a pointer to HBM: taking the address of a
 // by-value kernel argument this large would copy it to scratch.
-__global__ __launch_bounds__(ST_T) void k_subtree(const SubArgs *__restrict__ ap)
+__global__ __launch_bounds__(ST_T) void k_subtree(const SubArgs A)
 {
-    const SubArgs &A = *ap;
     __shared__ int sh[ST_W + 2];
     __shared__ int flags[2];
     __shared__ LevelRec lev[MAXLEV + 2];
@@ -1277,9 +1276,8 @@ __device__ void wave_recursion(LdsSet &S, const SubArgs &A, int nB, int nE, int 
     }
 }
 
-__global__ __launch_bounds__(L_T) void k_subtree_lds(const SubArgs *__restrict__ ap)
+__global__ __launch_bounds__(L_T) void k_subtree_lds(const SubArgs A)
 {
-    const SubArgs &A = *ap;
     const RecState &a = A.st;
     __shared__ LdsSet S;
     const int s = blockIdx.x;
@@ -1557,7 +1555,6 @@ __global__ __launch_bounds__(L_T) void k_subtree_lds(const SubArgs *__restrict__
             if (S.prof[k]) atomicAdd(&A.prof[k], S.prof[k]);
 }
 
-size_t subtree_args_bytes() { return sizeof(SubArgs); }
 
 static bool lds_mode()
 {
@@ -1598,14 +1595,11 @@ void subtree_profile_report(hipStream_t st)
     (void)hipMemsetAsync(g_prof, 0, sizeof h, st);
 }
 
-// h_stage: pinned host memory, d_args: device memory, both of
-// subtree_args_bytes(); the staging buffer must not be rewritten before the
-// copy has executed.
 void launch_subtree(hipStream_t st, const LevelArgs &la, const LevelView &start, const int32_t *rng, int d0,
                     double dt, int lclose, int short_circuit, int32_t *arena, const int64_t *arena_off,
-                    int32_t *stats, void *h_stage, void *d_args, bool all_lds)
+                    int32_t *stats, bool all_lds)
 {
-    SubArgs &a = *(SubArgs *)h_stage;
+    SubArgs a{};  // passed by value in the kernel arguments
     a.st.pl = la.pl;
     a.st.tp = la.tp;
     a.st.pp = la.pp;
@@ -1634,11 +1628,10 @@ void launch_subtree(hipStream_t st, const LevelArgs &la, const LevelView &start,
             (void)hipMemsetAsync(g_prof, 0, sizeof(unsigned long long) * NPROF * (MAXLEV + 2), st);
         if (g_prof) a.prof = g_prof;
     }
-    (void)hipMemcpyAsync(d_args, h_stage, sizeof(SubArgs), hipMemcpyHostToDevice, st);
     // each system runs in the kernel whose limits its set fits; all_lds: every
     // system's set fits the LDS-resident one
-    if (a.lds) k_subtree_lds<<<la.nsys, L_T, 0, st>>>((const SubArgs *)d_args);
-    if (!(a.lds && all_lds)) k_subtree<<<la.nsys, ST_T, 0, st>>>((const SubArgs *)d_args);
+    if (a.lds) k_subtree_lds<<<la.nsys, L_T, 0, st>>>(a);
+    if (!(a.lds && all_lds)) k_subtree<<<la.nsys, ST_T, 0, st>>>(a);
 }
 
 }  // namespace swg

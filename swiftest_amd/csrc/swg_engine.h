@@ -54,7 +54,6 @@ struct Ensemble {
     int64_t *d_arena_off = nullptr;
     void *d_scan_sys = nullptr;
     int64_t arena_words = 0, arena_need = 0;
-    void *h_subargs = nullptr, *d_subargs = nullptr;
     int64_t subtree_max = 16384;  // SWG_SUBTREE_MAX; < 0: every level on the whole device
     bool subtree_forced = false;  // SWG_SUBTREE_MAX given: hand off on size alone
     bool level_fits = false;      // every set of the level just built fits the LDS-resident kernel

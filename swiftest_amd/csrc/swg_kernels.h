@@ -173,15 +173,16 @@ void launch_level_kicks(hipStream_t st, const LevelArgs &la, const LevelView *v,
                         int keep = 0, double *kc = nullptr);
 void launch_level_drift(hipStream_t st, const LevelArgs &la, const LevelView &v, int nB, int nBt, int d, double dtl);
 void launch_level_collision(hipStream_t st, const LevelArgs &la, const LevelView &v, int nB, int nBt, double dtl);
-void launch_level_set_levels(hipStream_t st, const LevelArgs &la, const LevelView &v, const LevelCounts &c, int d);
+// any (non-null): also clear the encounter flags for the next substep
+void launch_level_set_levels(hipStream_t st, const LevelArgs &la, const LevelView &v, const LevelCounts &c, int d,
+                             int32_t *any = nullptr);
 
 // ---- SyMBA recursion from level d0 down, one workgroup per system (k_subtree.hip) ----
 // whether every system's set of this size fits the LDS-resident subtree kernel
 bool subtree_lds_fits(int nb, int ne, int nbt, int net);
-size_t subtree_args_bytes();
 void subtree_profile_report(hipStream_t st);
 void launch_subtree(hipStream_t st, const LevelArgs &la, const LevelView &start, const int32_t *rng, int d0,
                     double dt, int lclose, int short_circuit, int32_t *arena, const int64_t *arena_off,
-                    int32_t *stats, void *h_stage, void *d_args, bool all_lds);
+                    int32_t *stats, bool all_lds);
 
 }  // namespace swg
