@@ -603,8 +603,9 @@ int Ensemble::build_begin(int dnew, const LevelView *src)
 {
     hipStream_t st = stream;
     const LevelArgs la = level_args();
-    launch_build_count(st, la, src, npl, ntp, dnew, bw, d_any);
-    if (nsys <= SCAN_SMALL_MAX) {
+    if (launch_build_count(st, la, src, npl, ntp, dnew, bw, d_any, d_arena_off)) {
+        // the arena offsets came with the ranges
+    } else if (nsys <= SCAN_SMALL_MAX) {
         ScanSmall sc{};
         sc.in[0] = bw.caps;
         sc.out[0] = d_arena_off;

@@ -111,14 +111,10 @@ __device__ inline int lower_bound_i32(const int32_t *a, int n, int key)
 
 __device__ inline int64_t scan_at(const int64_t *x, int n, int i, int64_t total) { return i < n ? x[i] : total; }
 
-__global__ void k_build_ranges(SysDev sy, BuildSrc src, int nsys, int dnew, const int64_t *posB,
-                               const int64_t *posE, const int64_t *posBt, const int64_t *posEt,
-                               const int64_t *tot, int32_t *rng, int32_t *caps, int32_t *maxsz,
-                               const int32_t *any, int64_t *any_out)
+__device__ inline void build_range_sys(int s, SysDev sy, const BuildSrc &src, int dnew, const int64_t *posB,
+                                       const int64_t *posE, const int64_t *posBt, const int64_t *posEt,
+                                       const int64_t *tot, int32_t *rng, int32_t *caps, int32_t *maxsz)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s == 0 && any) *any_out = *any;  // the encounter flag travels with the counts
-    if (s >= nsys) return;
     int a0, a1, t0, t1;
     if (src.l0) {
         a0 = sy.pl_off[s];
@@ -153,6 +149,60 @@ __global__ void k_build_ranges(SysDev sy, BuildSrc src, int nsys, int dnew, cons
     atomicMax(&maxsz[3], cl(nbt));
     atomicMax(&maxsz[4], cl(et));
     if (nb + nbt > 0) atomicAdd(&maxsz[5], 1);
+}
+
+__global__ void k_build_ranges(SysDev sy, BuildSrc src, int nsys, int dnew, const int64_t *posB,
+                               const int64_t *posE, const int64_t *posBt, const int64_t *posEt,
+                               const int64_t *tot, int32_t *rng, int32_t *caps, int32_t *maxsz,
+                               const int32_t *any, int64_t *any_out)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0 && any) *any_out = *any;  // the encounter flag travels with the counts
+    if (s >= nsys) return;
+    build_range_sys(s, sy, src, dnew, posB, posE, posBt, posEt, tot, rng, caps, maxsz);
+}
+
+// The same for at most RS_T systems in one workgroup, followed by the
+// exclusive scan of their arena sizes into arena_off (arena_off[nsys] and
+// *arena_tot: the total), which otherwise takes a launch of its own.
+constexpr int RS_T = 1024;
+__global__ __launch_bounds__(RS_T) void k_build_ranges_scan(SysDev sy, BuildSrc src, int nsys, int dnew,
+                                                            const int64_t *posB, const int64_t *posE,
+                                                            const int64_t *posBt, const int64_t *posEt,
+                                                            const int64_t *tot, int32_t *rng, int32_t *caps,
+                                                            int32_t *maxsz, const int32_t *any, int64_t *any_out,
+                                                            int64_t *arena_off, int64_t *arena_tot)
+{
+    __shared__ int64_t wsum[RS_T / 64 + 1];
+    const int s = threadIdx.x, lane = s & 63, w = s >> 6, nw = (blockDim.x + 63) >> 6;
+    if (s == 0 && any) *any_out = *any;
+    int64_t c = 0;
+    if (s < nsys) {
+        build_range_sys(s, sy, src, dnew, posB, posE, posBt, posEt, tot, rng, caps, maxsz);
+        c = caps[s];
+    }
+    int64_t incl = c;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t u = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += u;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    if (s == 0) {
+        int64_t acc = 0;
+        for (int k = 0; k < nw; k++) {
+            const int64_t t = wsum[k];
+            wsum[k] = acc;
+            acc += t;
+        }
+        wsum[RS_T / 64] = acc;
+    }
+    __syncthreads();
+    if (s < nsys) arena_off[s] = wsum[w] + incl - c;
+    if (s == 0) {
+        arena_off[nsys] = wsum[RS_T / 64];
+        *arena_tot = wsum[RS_T / 64];
+    }
 }
 
 // Destination of a build (pointers into a stored level)
@@ -546,8 +596,8 @@ LevelView level_view(void *base, const LevelCounts &c)
 
 int64_t level_bytes(const LevelCounts &c) { return 4 * level_layout(c.nB, c.nE, c.nBt, c.nEt).words; }
 
-void launch_build_count(hipStream_t st, const LevelArgs &la, const LevelView *src, int npl, int ntp, int dnew,
-                        const BuildScratch &w, const int32_t *any)
+bool launch_build_count(hipStream_t st, const LevelArgs &la, const LevelView *src, int npl, int ntp, int dnew,
+                        const BuildScratch &w, const int32_t *any, int64_t *arena_off)
 {
     RecState a = rec_state(la.pl, la.tp, la.pp, la.pt, la.sy);
     BuildSrc s{};
@@ -581,8 +631,15 @@ void launch_build_count(hipStream_t st, const LevelArgs &la, const LevelView *sr
         launch_scan_i32_i64(st, w.cntEt, w.posEt, s.nBt, w.tot + 3, w.scan);
         (void)hipMemsetAsync(w.maxsz, 0, 8 * sizeof(int32_t), st);
     }
+    if (arena_off && la.nsys <= RS_T) {
+        const int nt = la.nsys <= 64 ? 64 : RS_T;
+        k_build_ranges_scan<<<1, nt, 0, st>>>(la.sy, s, la.nsys, dnew, w.posB, w.posE, w.posBt, w.posEt, w.tot,
+                                               w.rng, w.caps, w.maxsz, any, w.tot + 5, arena_off, w.tot + 4);
+        return true;
+    }
     k_build_ranges<<<cdiv(la.nsys, 256), 256, 0, st>>>(la.sy, s, la.nsys, dnew, w.posB, w.posE, w.posBt, w.posEt,
                                                        w.tot, w.rng, w.caps, w.maxsz, any, w.tot + 5);
+    return false;
 }
 
 void launch_build_fill(hipStream_t st, const LevelArgs &la, const LevelView *src, int npl, int ntp, int dnew,

@@ -157,9 +157,11 @@ struct BuildScratch {
 LevelView level_view(void *base, const LevelCounts &c);
 int64_t level_bytes(const LevelCounts &c);
 // src == nullptr: build level 1 from the level-0 adjacency of all npl / ntp bodies
-// any (nullable): the encounter flag, copied into w.tot[5] with the counts
-void launch_build_count(hipStream_t st, const LevelArgs &la, const LevelView *src, int npl, int ntp, int dnew,
-                        const BuildScratch &w, const int32_t *any);
+// any (nullable): the encounter flag, copied into w.tot[5] with the counts.
+// Returns true when it also scanned the arena sizes into arena_off (and
+// w.tot[4]), which it does for up to 1024 systems; otherwise the caller does.
+bool launch_build_count(hipStream_t st, const LevelArgs &la, const LevelView *src, int npl, int ntp, int dnew,
+                        const BuildScratch &w, const int32_t *any, int64_t *arena_off);
 void launch_build_fill(hipStream_t st, const LevelArgs &la, const LevelView *src, int npl, int ntp, int dnew,
                        const BuildScratch &w, void *dst_base, const LevelCounts &c);
 void launch_level_check(hipStream_t st, const LevelArgs &la, const LevelView &v, const LevelCounts &c, int d,
