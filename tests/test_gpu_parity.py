@@ -252,6 +252,27 @@ def test_recursion_modes_match_oracle(monkeypatch, subtree_max, lds, wave):
     assert sg["max_depth"] >= 2
 
 
+@pytest.mark.parametrize("npl,mhi,ntp", [(400, 1e-4, 100), (600, 3e-5, 200)])
+def test_deep_recursion_with_tp_bit_exact(npl, mhi, ntp):
+    """Heavier crowded disks with test particles in encounters (no short
+    circuit) reach depth 8-9: the default scheduling (whole-device levels,
+    hand-off, the LDS-resident workgroup and its one-wavefront mode, the kept
+    closing-kick sums) reproduces the oracle bit for bit, substep for
+    substep, over 20 steps."""
+    sysic = compact_disk(seed=31, npl=npl, ntp=ntp, mhi=mhi)
+    o = O.OracleSystem(O.SYMBA, sysic, 0.005, short_circuit=False, lclose=False, istep_out=10)
+    g = S.GpuSystem(S.SYMBA, sysic, 0.005, short_circuit=False, lclose=False, istep_out=10)
+    for blk in range(2):
+        print(f"deep recursion npl={npl} ntp={ntp} block {blk}", flush=True)
+        assert o.run(10) == 0
+        g.run(10)
+        so, sg = o.stats(), g.stats()
+        assert (so["max_depth"], so["nsubsteps"], so["npltp_enc"]) == (sg["max_depth"], sg["nsubsteps"],
+                                                                       sg["npltp_enc"]), (blk, so, sg)
+        _compare_systems(o, g, f"block {blk}")
+    assert sg["max_depth"] >= 6
+
+
 def _sas_case(kind):
     if kind == "disk":
         return [compact_disk(seed=13, npl=300, ntp=0)], 0.005
