@@ -191,6 +191,75 @@ __device__ inline double kepu_fchk(double dt, double r0, double mu, double alpha
     return ((r0 * c1 + u * c2) + mu * c3) - dt;
 }
 
+// 1/b to about 1 ulp for b in [1, 16]: the hardware reciprocal and two
+// Newton steps on the device (the host divides).
+__host__ __device__ inline double rcp_nr(double b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rcp(b);
+    y = fma(y, fma(-b, y, 1.0), y);
+    return fma(y, fma(-b, y, 1.0), y);
+#else
+    return 1.0 / b;
+#endif
+}
+
+// x**THIRD (THIRD = the double nearest 1/3 = 1/3 - 2^-54/3) for x in
+// [2^-1000, 2^1000], correctly rounded except where the exact power lies
+// within about 2^-90 (relative) of a rounding midpoint:
+//  1. x = M 2^(3k), M in [0.5, 4); a float log2/exp2 seed for M^(1/3)
+//     (relative error ~2^-21) and one Halley step y (y^3 + 2M) / (2y^3 + M)
+//     give y within ~2^-50 of M^(1/3).
+//  2. The residual M - y^3 is formed to ~2^-102 (y^2 and y*y^2 split
+//     exactly by fma, M - y*y^2 exact by Sterbenz), and the Newton step
+//     d = (M - y^3) / (3y^2) leaves M^(1/3) - (y + d) = O(d^2 / y) ~ 2^-100 y.
+//  3. x**THIRD = x^(1/3) exp(-(2^-54/3) ln x); the correction y (2^-54/3)
+//     ln x is at most 2^-46 y, so ln x is needed to ~2^-35 absolute: the
+//     exponent times ln 2 plus 2 atanh((m-1)/(m+1)) for m in [sqrt(1/2),
+//     sqrt(2)), summed to s^13 (truncation ~2^-41).
+// The rounded sum y + (d - corr) is then the correctly rounded power (the
+// general pow of the device library is not: it differs from it in about a
+// quarter of the cube roots), at about half the latency of that pow.
+__host__ __device__ inline bool pow_third_ok(double x) { return x >= 0x1p-1000 && x <= 0x1p1000; }
+
+__host__ __device__ inline double pow_third_core(double x)
+{
+    int e;
+    const double m = frexp(x, &e);  // x = m 2^e, m in [0.5, 1)
+    const int k = (e >= 0 ? e : e - 2) / 3;  // floor(e / 3)
+    const double M = ldexp(m, e - 3 * k);
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y = (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf((float)M) * (1.0f / 3.0f));
+#else
+    double y = (double)exp2f(log2f((float)M) * (1.0f / 3.0f));
+#endif
+    const double y3 = (y * y) * y;
+    y = y * ((y3 + 2.0 * M) * rcp_nr(2.0 * y3 + M));
+    const double s2 = y * y, s2l = fma(y, y, -s2);
+    const double p = y * s2, pl = fma(y, s2, -p);
+    const double r = fma(-y, s2l, (M - p) - pl);
+    const double d = r * rcp_nr(3.0 * s2);
+    const bool lo = m < 0.70710678118654752;
+    const double m2 = lo ? 2.0 * m : m;
+    const double s = (m2 - 1.0) * rcp_nr(m2 + 1.0), z = s * s;
+    double t = 1.0 / 13.0;
+    t = fma(t, z, 1.0 / 11.0);
+    t = fma(t, z, 1.0 / 9.0);
+    t = fma(t, z, 1.0 / 7.0);
+    t = fma(t, z, 1.0 / 5.0);
+    t = fma(t, z, 1.0 / 3.0);
+    t = fma(t, z, 1.0);
+    const double lnx = (double)(lo ? e - 1 : e) * 0.69314718055994531 + (2.0 * s) * t;
+    const double corr = y * (lnx * (0x1p-54 / 3.0));
+    return ldexp(y + (d - corr), k);
+}
+
+__host__ __device__ inline double pow_third(double x)
+{
+    if (!pow_third_ok(x)) return pow(x, THIRD);
+    return pow_third_core(x);
+}
+
 // real root of the cubic for the hyperbolic first guess (:486-533)
 __device__ inline int kepu_p3solve(double dt, double r0, double mu, double alpha, double u, double &s)
 {
@@ -203,8 +272,11 @@ __device__ inline int kepu_p3solve(double dt, double r0, double mu, double alpha
     double sq2 = (q * q) * q + r * r;
     if (sq2 >= 0.0) {
         double sq = sqrt(sq2);
-        double p1 = ((r + sq) <= 0.0) ? -pow(-(r + sq), THIRD) : pow(r + sq, THIRD);
-        double p2 = ((r - sq) <= 0.0) ? -pow(-(r - sq), THIRD) : pow(r - sq, THIRD);
+        // t <= 0 ? -(-t)**THIRD : t**THIRD, with -t == |t| in the first arm
+        const double t1 = r + sq, t2 = r - sq;
+        const double w1 = pow_third(fabs(t1)), w2 = pow_third(fabs(t2));
+        double p1 = (t1 <= 0.0) ? -w1 : w1;
+        double p2 = (t2 <= 0.0) ? -w2 : w2;
         s = (p1 + p2) - a2 * THIRD;
         return 0;
     }

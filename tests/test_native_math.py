@@ -7,7 +7,10 @@ the IEEE quotient over random operands spanning the normal range; the device
 instantiation is covered by the bit-exact drift parity tests.  The same
 program compares the compile-time table of RSHELL powers (swg_level.h; the
 shell factors of symba_kick.f90:150-200 and the encounter radii of
-symba_util.f90:245-267) with the powers evaluated at run time.
+symba_util.f90:245-267) with the powers evaluated at run time, and checks
+pow_third, the correctly rounded x**THIRD of the cubic first guess
+(swiftest_drift.f90:486-533), against 80-bit powl and on its exact cubes and
+out-of-range fallbacks.
 """
 import os
 import shutil
