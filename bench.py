@@ -53,7 +53,7 @@ def cpu_baseline(sysic, target_s):
     reference's loop decomposition: OpenMP over particles / rows, j in order)
     on a bounded sample of the benchmark state."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle as O  # the checker doubles as the CPU baseline
+    import pyoracle as O  # the test oracle doubles as the CPU baseline
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     npl = sysic.npl
     rpl, Gm = sysic.pl_rh, sysic.Gmass

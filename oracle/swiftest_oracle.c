@@ -1,7 +1,7 @@
 /*
  * swiftest_oracle.c -- CPU restatement of Swiftest's per-step hot path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker and the CPU baseline); never linked
+ * TEST INFRASTRUCTURE ONLY (the parity oracle and the CPU baseline); never linked
  * into the product.  See swiftest_oracle.h for the floating-point rule.
  * All citations are to carlislewishard/swiftest v2023.10.2 (src/...).
  *

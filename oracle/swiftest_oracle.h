@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (swiftest_amd/, include/)
  * may link, load or call this code.  Only tests/, __graft_entry__.smoke() and
- * bench.py's cpu_baseline leg use it, and only as the checker / CPU baseline.
+ * bench.py's cpu_baseline leg use it, and only as the parity oracle / CPU baseline.
  *
  * Every function restates the arithmetic of the Fortran reference
  * (carlislewishard/swiftest v2023.10.2) in plain C, in the reference's
