@@ -223,7 +223,7 @@ def main():
     ap.add_argument("--istep-out", type=int, default=100)
     ap.add_argument("--lclose", type=int, default=0,
                     help="CHK_CLOSE: 1 checks collisions inside the recursion (their merge is host work "
-                         "this build does not do, so a detected collision stops the run)")
+                         "the library leaves to the host, so a detected collision stops the run)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
@@ -241,7 +241,7 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path.", file=sys.stderr)
+        print("bench.py runs on one GPU only: the library has no multi-GPU path.", file=sys.stderr)
         return 2
 
     from swiftest_amd import ics
