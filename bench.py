@@ -222,8 +222,8 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--istep-out", type=int, default=100)
     ap.add_argument("--lclose", type=int, default=0,
-                    help="CHK_CLOSE: 1 checks collisions inside the recursion (their merge is host work "
-                         "the library leaves to the host, so a detected collision stops the run)")
+                    help="CHK_CLOSE: 1 checks collisions inside the recursion (their merge is host work, "
+                         "so a detected collision stops the run)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
