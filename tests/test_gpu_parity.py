@@ -416,6 +416,67 @@ def test_energy_matches_oracle(integ):
         o += s.npl
 
 
+# ---- 10^3 steps: positions and the energy-error trace ---------------------
+# Tolerances: positions and velocities within 1e-10 of the oracle's (relative
+# to the largest component), after every block of 100 steps up to step 1000;
+# encounter counts, depth and substeps equal.  The relative energy error
+# (E - E0) / |E0| from the device's energy reduction on the device's
+# trajectory against the oracle's on its own: within 2e-12 at every block
+# (each energy agrees to 1e-12 relative, so the two traces agree to the order
+# of the error itself or better).
+
+def _energy_of(st, sysic, integ, on_device, g=None):
+    gm = sysic.Gmass[st["pl_id"] - 1]
+    mass = gm / ics.GU
+    cbm = sysic.GMcb / ics.GU
+    if on_device:
+        return g.energy(np.array([cbm]), np.array([sysic.Rcb]), mass)[0, 5]
+    if integ == S.WHM:
+        vb, cbv = O.vh2vb(sysic.GMcb, gm, st["pl_vh"])
+    else:
+        vb, cbv = st["pl_vb"], st["cb_vb"]
+    rec, _ = O.energy(sysic.GMcb, cbm, sysic.Rcb, cbv, st["pl_rh"], vb, gm, mass,
+                      sysic.radius[st["pl_id"] - 1], st["pl_status"])
+    return rec[5]
+
+
+@pytest.mark.parametrize("integ,kind", [(S.SYMBA, "crowded"), (S.SYMBA, "crowded-tp"), (S.HELIO, "sparse"),
+                                        (S.WHM, "sparse")])
+def test_thousand_steps_positions_and_energy_trace(integ, kind):
+    if kind == "crowded":
+        sysic, dt = compact_disk(seed=41, npl=120, ntp=50), 0.005
+    elif kind == "crowded-tp":
+        sysic, dt = compact_disk(seed=44, npl=100, ntp=200), 0.005
+    else:
+        sysic, dt = sparse_system(seed=42, npl=12, ntp=300), 0.01
+    o = O.OracleSystem(integ, sysic, dt, lclose=False, istep_out=100)
+    g = S.GpuSystem(integ, sysic, dt, lclose=False, istep_out=100)
+    e0o = _energy_of(o.state(), sysic, integ, False)
+    e0g = _energy_of(g.state(), sysic, integ, True, g)
+    assert e0g == pytest.approx(e0o, rel=1e-12, abs=0)
+    depth = -1
+    for blk in range(10):
+        print(f"1000 steps integ={integ} {kind}: steps {100 * blk}..{100 * blk + 99}", flush=True)
+        assert o.run(100) == 0
+        g.run(100)
+        so, sg = o.state(), g.state()
+        if integ == S.SYMBA:  # encounter statistics of the recursion
+            ss, gs = o.stats(), g.stats()
+            assert (ss["nplpl_enc"], ss["npltp_enc"], ss["max_depth"], ss["nsubsteps"]) == \
+                (gs["nplpl_enc"], gs["npltp_enc"], gs["max_depth"], gs["nsubsteps"]), (blk, ss, gs)
+            depth = max(depth, ss["max_depth"])
+        assert np.array_equal(so["pl_id"], sg["pl_id"])
+        for k in ("pl_rh", "pl_vh", "tp_rh", "tp_vh"):
+            scale = np.max(np.abs(so[k])) if so[k].size else 1.0
+            assert np.max(np.abs(so[k] - sg[k]), initial=0.0) <= 1e-10 * scale, (blk, k)
+        assert np.array_equal(so["pl_status"], sg["pl_status"]) and np.array_equal(so["tp_status"], sg["tp_status"])
+        deo = (_energy_of(so, sysic, integ, False) - e0o) / abs(e0o)
+        deg = (_energy_of(sg, sysic, integ, True, g) - e0g) / abs(e0g)
+        assert abs(deg - deo) <= 2e-12, (blk, deo, deg)
+    if kind != "sparse":
+        assert depth >= 2
+
+
 # ---- discard checks (swg_system_discard) -----------------------------------
 # Against the oracle after every step: the discard counts and every status
 # exactly; the state (including the velocities the h2b conversion rewrites)
