@@ -127,15 +127,24 @@ def test_drift_elliptic(dt):
         np.testing.assert_allclose(gv[~exact], vx[~exact], rtol=1e-12, atol=1e-14)
 
 
-def test_drift_hyperbolic():
+@pytest.mark.parametrize("dt", [0.05, 1e-6, 2.8e-8])
+def test_drift_hyperbolic(dt):
+    """Unbound orbits take the cubic first guess, whose two cube roots the
+    device rounds correctly (pow_third) while the oracle calls libm's pow,
+    which misrounds about 0.08 % of them: all drifts agree to 1e-12, and
+    nearly all of them bit for bit."""
+    print(f"hyperbolic drift dt={dt}", flush=True)
     r, v = _orbits(13, 2000, 0.5, 5.0, 0.3, hyper=True)
     mu = np.full(len(r), ics.GMSUN)
     lmask = np.ones(len(r), np.int32)
-    rx, vx, fx = O.drift(mu, r, v, 0.05, lmask)
-    gx, gv, gf = S.swiftest_drift_all(mu, r, v, len(r), 0.05, lmask)
+    rx, vx, fx = O.drift(mu, r, v, dt, lmask)
+    gx, gv, gf = S.swiftest_drift_all(mu, r, v, len(r), dt, lmask)
     assert np.array_equal(fx, gf)
     np.testing.assert_allclose(gx, rx, rtol=1e-12, atol=1e-14)
     np.testing.assert_allclose(gv, vx, rtol=1e-12, atol=1e-14)
+    same = np.all(gx == rx, axis=1) & np.all(gv == vx, axis=1)
+    print(f"bit-identical drifts: {same.mean():.5f}", flush=True)
+    assert same.mean() >= 0.99, same.mean()
 
 
 @pytest.mark.parametrize("n", [1, 63, 64, 65, 700])
