@@ -125,6 +125,11 @@ def test_drift_elliptic(dt):
     if (~exact).any():
         np.testing.assert_allclose(gx[~exact], rx[~exact], rtol=1e-12, atol=1e-14)
         np.testing.assert_allclose(gv[~exact], vx[~exact], rtol=1e-12, atol=1e-14)
+        same = np.all(gx[~exact] == rx[~exact], axis=1) & np.all(gv[~exact] == vx[~exact], axis=1)
+        print(f"sin-guess drifts: {(~exact).sum()}, bit-identical {same.mean():.5f}", flush=True)
+        # the device's and libm's sin may differ in the guess's last bit;
+        # the converged iteration almost always removes the difference
+        assert same.mean() >= 0.99, same.mean()
 
 
 @pytest.mark.parametrize("dt", [0.05, 1e-6, 2.8e-8])
