@@ -8,10 +8,7 @@
 #include "swg_kernels.h"
 #include "swg_math.h"
 
-// unroll factors of the pair loops (independent pairs in flight per lane)
-#ifndef SWG_UNROLL_PL
-#define SWG_UNROLL_PL 4
-#endif
+// unroll factor of the pl-tp pair loop (independent pairs in flight per lane)
 #ifndef SWG_UNROLL_TP
 #define SWG_UNROLL_TP 2
 #endif
@@ -19,6 +16,9 @@
 namespace swg {
 
 static inline int cdiv(int64_t a, int b) { return (int)((a + b - 1) / b); }
+
+// pl-pl terms formed together per lane before they are folded in order
+constexpr int KPL_G = 4;
 
 __device__ inline bool pl_active(const PlDev &pl, int i) { return pl.status[i] != INACTIVE; }
 __device__ inline bool tp_active(const TpDev &tp, int j) { return tp.status[j] == ACTIVE; }
@@ -147,6 +147,7 @@ template <int BLK>
 __global__ __launch_bounds__(BLK) void k_kick_pl(PlDev pl, SysDev sy, ListDev pp, const Tile *tiles, double dt,
                                                  int lbeg, int use_radius, int do_pairs)
 {
+    static_assert(BLK % KPL_G == 0, "the grouped row walk reads LDS in whole groups");
     __shared__ double sx[BLK], syy[BLK], sz[BLK], sgm[BLK], sr[BLK];
     const Tile tl = tiles[blockIdx.x];
     const int p0 = sy.pl_off[tl.s], p1 = sy.pl_off[tl.s + 1];
@@ -174,19 +175,42 @@ __global__ __launch_bounds__(BLK) void k_kick_pl(PlDev pl, SysDev sy, ListDev pp
         const bool tile_gok = __syncthreads_and(gok);
         const int jn = min(BLK, p1 - jt);
         if (valid) {
-#pragma unroll SWG_UNROLL_PL
-            for (int q = 0; q < jn; q++) {
-                if (jt + q == i) continue;
-                double rx = sx[q] - xi, ry = syy[q] - yi, rz = sz[q] - zi;
-                double rji2 = dot3(rx, ry, rz, rx, ry, rz);
-                if (use_radius) {
-                    double rs = Ri + sr[q];
-                    if (!(rji2 > rs * rs)) continue;
+            // Groups of KPL_G bodies: the factors of a group are formed
+            // without branches (the exact cores, garbage where a term is
+            // not taken or its r2 is outside the cores' range), the IEEE
+            // operations redo the rare out-of-range terms under a
+            // wave-uniform test, and the terms are folded in j order with
+            // selects (a term not taken leaves the sum as it is).  BLK is a
+            // multiple of KPL_G, so the LDS reads stay inside the tile.
+            for (int q0 = 0; q0 < jn; q0 += KPL_G) {
+                double rx[KPL_G], ry[KPL_G], rz[KPL_G], r2[KPL_G], fac[KPL_G];
+                bool take[KPL_G], need[KPL_G];
+                bool any_need = false;
+#pragma unroll
+                for (int u = 0; u < KPL_G; u++) {
+                    const int q = q0 + u;
+                    rx[u] = sx[q] - xi;
+                    ry[u] = syy[q] - yi;
+                    rz[u] = sz[q] - zi;
+                    r2[u] = dot3(rx[u], ry[u], rz[u], rx[u], ry[u], rz[u]);
+                    const double rs = Ri + sr[q];
+                    // & and | (not && and ||): no short-circuit, no branches
+                    take[u] = (q < jn) & (jt + q != i) & (!use_radius | (r2[u] > rs * rs));
+                    fac[u] = div_core(sgm[q], r2[u] * sqrt_core(r2[u]));
+                    need[u] = take[u] & !(tile_gok & (r2[u] >= PAIR_R2_LO) & (r2[u] <= PAIR_R2_HI));
+                    any_need = any_need | need[u];
                 }
-                double fac = gm_over_r3(sgm[q], rji2, tile_gok);
-                ax = fma(fac, rx, ax);
-                ay = fma(fac, ry, ay);
-                az = fma(fac, rz, az);
+                if (__ballot(any_need)) {
+#pragma unroll
+                    for (int u = 0; u < KPL_G; u++)
+                        if (need[u]) fac[u] = sgm[q0 + u] / (r2[u] * sqrt(r2[u]));
+                }
+#pragma unroll
+                for (int u = 0; u < KPL_G; u++) {
+                    ax = take[u] ? fma(fac[u], rx[u], ax) : ax;
+                    ay = take[u] ? fma(fac[u], ry[u], ay) : ay;
+                    az = take[u] ? fma(fac[u], rz[u], az) : az;
+                }
             }
         }
         __syncthreads();
@@ -194,31 +218,56 @@ __global__ __launch_bounds__(BLK) void k_kick_pl(PlDev pl, SysDev sy, ListDev pp
     if (!valid) return;
     if (do_pairs) {
         double hix = 0.0, hiy = 0.0, hiz = 0.0, hjx = 0.0, hjy = 0.0, hjz = 0.0;
+        // Both walks gather KPL_G entries at a time (indices past the end
+        // repeat the walk's current entry, whose term is then not taken) so
+        // that the gathers overlap, and fold the terms in list order.
         const int64_t c0 = pp.colstart[i], c1 = c0 + pp.colcnt[i];
-        for (int64_t c = c0; c < c1; c++) {  // pairs (p, i), p < i
-            int p = pp.i1[pp.col[c]];
-            double rx = xi - pl.rh.x[p], ry = yi - pl.rh.y[p], rz = zi - pl.rh.z[p];
-            double rji2 = dot3(rx, ry, rz, rx, ry, rz);
-            double rs = pl.R[p] + Ri;
-            if (!(rji2 > rs * rs)) continue;
-            double irij3 = 1.0 / (rji2 * sqrt(rji2));
-            double faci = pl.Gm[p] * irij3;
-            hjx = fma(-faci, rx, hjx);
-            hjy = fma(-faci, ry, hjy);
-            hjz = fma(-faci, rz, hjz);
+        for (int64_t c = c0; c < c1; c += KPL_G) {  // pairs (p, i), p < i
+            double rx[KPL_G], ry[KPL_G], rz[KPL_G], fac[KPL_G];
+            bool take[KPL_G];
+#pragma unroll
+            for (int u = 0; u < KPL_G; u++) {
+                const int64_t cu = c + u < c1 ? c + u : c;
+                const int p = pp.i1[pp.col[cu]];
+                rx[u] = xi - pl.rh.x[p];
+                ry[u] = yi - pl.rh.y[p];
+                rz[u] = zi - pl.rh.z[p];
+                const double rji2 = dot3(rx[u], ry[u], rz[u], rx[u], ry[u], rz[u]);
+                const double rs = pl.R[p] + Ri;
+                take[u] = (c + u < c1) & (rji2 > rs * rs);
+                const double irij3 = 1.0 / (rji2 * sqrt(rji2));
+                fac[u] = pl.Gm[p] * irij3;
+            }
+#pragma unroll
+            for (int u = 0; u < KPL_G; u++) {
+                hjx = take[u] ? fma(-fac[u], rx[u], hjx) : hjx;
+                hjy = take[u] ? fma(-fac[u], ry[u], hjy) : hjy;
+                hjz = take[u] ? fma(-fac[u], rz[u], hjz) : hjz;
+            }
         }
         const int64_t r0 = pp.rowstart[i], r1 = r0 + pp.rowcnt[i];
-        for (int64_t k = r0; k < r1; k++) {  // pairs (i, j), j > i
-            int j = pp.i2[k];
-            double rx = pl.rh.x[j] - xi, ry = pl.rh.y[j] - yi, rz = pl.rh.z[j] - zi;
-            double rji2 = dot3(rx, ry, rz, rx, ry, rz);
-            double rs = Ri + pl.R[j];
-            if (!(rji2 > rs * rs)) continue;
-            double irij3 = 1.0 / (rji2 * sqrt(rji2));
-            double facj = pl.Gm[j] * irij3;
-            hix = fma(facj, rx, hix);
-            hiy = fma(facj, ry, hiy);
-            hiz = fma(facj, rz, hiz);
+        for (int64_t k = r0; k < r1; k += KPL_G) {  // pairs (i, j), j > i
+            double rx[KPL_G], ry[KPL_G], rz[KPL_G], fac[KPL_G];
+            bool take[KPL_G];
+#pragma unroll
+            for (int u = 0; u < KPL_G; u++) {
+                const int64_t ku = k + u < r1 ? k + u : k;
+                const int j = pp.i2[ku];
+                rx[u] = pl.rh.x[j] - xi;
+                ry[u] = pl.rh.y[j] - yi;
+                rz[u] = pl.rh.z[j] - zi;
+                const double rji2 = dot3(rx[u], ry[u], rz[u], rx[u], ry[u], rz[u]);
+                const double rs = Ri + pl.R[j];
+                take[u] = (k + u < r1) & (rji2 > rs * rs);
+                const double irij3 = 1.0 / (rji2 * sqrt(rji2));
+                fac[u] = pl.Gm[j] * irij3;
+            }
+#pragma unroll
+            for (int u = 0; u < KPL_G; u++) {
+                hix = take[u] ? fma(fac[u], rx[u], hix) : hix;
+                hiy = take[u] ? fma(fac[u], ry[u], hiy) : hiy;
+                hiz = take[u] ? fma(fac[u], rz[u], hiz) : hiz;
+            }
         }
         ax = ax - ((0.0 + hix) + hjx);
         ay = ay - ((0.0 + hiy) + hjy);
