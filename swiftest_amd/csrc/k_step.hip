@@ -8,17 +8,18 @@
 #include "swg_kernels.h"
 #include "swg_math.h"
 
-// unroll factor of the pl-tp pair loop (independent pairs in flight per lane)
-#ifndef SWG_UNROLL_TP
-#define SWG_UNROLL_TP 2
-#endif
 
 namespace swg {
 
 static inline int cdiv(int64_t a, int b) { return (int)((a + b - 1) / b); }
 
-// pl-pl terms formed together per lane before they are folded in order
+// pl-pl and pl-tp terms formed together per lane before they are folded in
+// order
 constexpr int KPL_G = 4;
+#ifndef SWG_KTP_G
+#define SWG_KTP_G 4
+#endif
+constexpr int KTP_G = SWG_KTP_G;
 
 __device__ inline bool pl_active(const PlDev &pl, int i) { return pl.status[i] != INACTIVE; }
 __device__ inline bool tp_active(const TpDev &tp, int j) { return tp.status[j] == ACTIVE; }
@@ -367,8 +368,38 @@ __global__ __launch_bounds__(BLK) void k_kick_tp(TpDev tp, PlDev pl, SysDev sy, 
         const bool tile_gok = __syncthreads_and(gok);
         const int qn = min(BLK, p1 - it);
         if (valid) {
-#pragma unroll SWG_UNROLL_TP
-            for (int q = 0; q < qn; q++) {
+            // Whole groups of KTP_G bodies: the factors are formed without
+            // branches (exact cores; the rare out-of-range terms redone with
+            // the IEEE operations under a wave-uniform test) and folded in
+            // j order.  The tail of a partial tile takes the plain path.
+            int q = 0;
+            for (; q + KTP_G <= qn; q += KTP_G) {
+                double rx[KTP_G], ry[KTP_G], rz[KTP_G], r2[KTP_G], fac[KTP_G];
+                bool need[KTP_G];
+                bool any_need = false;
+#pragma unroll
+                for (int u = 0; u < KTP_G; u++) {
+                    rx[u] = xt - sx[q + u];
+                    ry[u] = yt - syy[q + u];
+                    rz[u] = zt - sz[q + u];
+                    r2[u] = dot3(rx[u], ry[u], rz[u], rx[u], ry[u], rz[u]);
+                    fac[u] = div_core(sgm[q + u], r2[u] * sqrt_core(r2[u]));
+                    need[u] = !(tile_gok & (r2[u] >= PAIR_R2_LO) & (r2[u] <= PAIR_R2_HI));
+                    any_need = any_need | need[u];
+                }
+                if (__ballot(any_need)) {
+#pragma unroll
+                    for (int u = 0; u < KTP_G; u++)
+                        if (need[u]) fac[u] = sgm[q + u] / (r2[u] * sqrt(r2[u]));
+                }
+#pragma unroll
+                for (int u = 0; u < KTP_G; u++) {
+                    px = fma(-fac[u], rx[u], px);
+                    py = fma(-fac[u], ry[u], py);
+                    pz = fma(-fac[u], rz[u], pz);
+                }
+            }
+            for (; q < qn; q++) {
                 double rx = xt - sx[q], ry = yt - syy[q], rz = zt - sz[q];
                 double rji2 = dot3(rx, ry, rz, rx, ry, rz);
                 double fac = gm_over_r3(sgm[q], rji2, tile_gok);
