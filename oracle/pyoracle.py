@@ -39,51 +39,69 @@ def lib(omp: bool = False):
         path = os.path.join(HERE, name)
         if not os.path.exists(path):
             build()
-        L = C.CDLL(path)
-        vp = C.c_void_p
-        L.swo_kick_getacch_int_all_tri.argtypes = [C.c_int, C.c_int, D, D, vp, D]
-        L.swo_kick_getacch_int_all_flat.argtypes = [C.c_int, C.c_int64, I32, D, D, vp, D]
-        L.swo_kick_getacch_int_all_tp.argtypes = [C.c_int, C.c_int, D, D, D, I32, D]
-        L.swo_drift_all.argtypes = [D, D, D, C.c_int, C.c_double, I32, I32, C.c_int, C.c_double]
-        L.swo_drift_kepu_stumpff.argtypes = [C.c_double] + [C.POINTER(C.c_double)] * 4
-        L.swo_encounter_check_one.argtypes = [C.c_double] * 8 + [C.POINTER(C.c_int32)] * 2
-        pp = C.POINTER(C.POINTER(C.c_int32))
-        L.swo_encounter_check_all_plpl.argtypes = [C.c_int, D, D, D, C.c_double, pp, pp, pp]
-        L.swo_encounter_check_all_plpl.restype = C.c_int64
-        L.swo_encounter_check_all_pltp.argtypes = [C.c_int, C.c_int, D, D, D, D, D, C.c_double, pp, pp, pp]
-        L.swo_encounter_check_all_pltp.restype = C.c_int64
-        L.swo_free.argtypes = [vp]
-        L.swo_collision_check_one.argtypes = [C.c_double] * 9 + [C.c_int32] + [C.POINTER(C.c_int32)] * 2
-        L.swo_orbel_xv2aeq.argtypes = [C.c_double] * 7 + [C.POINTER(C.c_double)] * 3
-        L.swo_orbel_xv2a.argtypes = [C.c_double] * 7
-        L.swo_orbel_xv2a.restype = C.c_double
-        L.swo_pow_m1p5.argtypes = [C.c_double]
-        L.swo_pow_m1p5.restype = C.c_double
-        L.swo_powi.argtypes = [C.c_double, C.c_int]
-        L.swo_powi.restype = C.c_double
-        L.swo_sys_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, D, D, D, D, vp, D, D,
-                                  C.POINTER(Params)]
-        L.swo_sys_new.restype = vp
-        L.swo_sys_free.argtypes = [vp]
-        L.swo_sys_run.argtypes = [vp, C.c_int64]
-        L.swo_sys_step.argtypes = [vp]
-        L.swo_sys_get.argtypes = [vp] + [vp] * 8
-        L.swo_sys_get_misc.argtypes = [vp] * 6
-        L.swo_sys_get_order.argtypes = [vp, I32]
-        L.swo_sys_stats.argtypes = [vp, C.POINTER(Stats)]
-        L.swo_sys_npl.argtypes = [vp]
-        L.swo_sys_ntp.argtypes = [vp]
-        L.swo_sys_list.argtypes = [vp, C.c_int, vp, vp, vp]
-        L.swo_sys_list.restype = C.c_int64
-        L.swo_norm2.argtypes = [C.c_double] * 3
-        L.swo_norm2.restype = C.c_double
-        L.swo_coord_vh2vb.argtypes = [C.c_int, C.c_double, D, D, D, D]
-        L.swo_sys_discard.argtypes = [vp, D, C.c_int, C.POINTER(C.c_int64)]
-        L.swo_discard.argtypes = ([C.c_int] * 3 + [C.c_double] * 2 + [D, D, D, D, D, D, D, I32, I32, I32, D, D, I32]
-                                  + [D, D, D, I32, I32, D, D, D, C.c_int, C.c_int, C.POINTER(C.c_int64)])
-        L.swo_energy.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, D, D, D, D, D, D, I32, C.c_int, D, vp]
-        _LIBS[key] = L
+        _LIBS[key] = load(path)
     return _LIBS[key]
+
+
+def build_variant(outdir, defines=()):
+    """Compile swiftest_oracle.c with extra -D macros into outdir (the
+    planted-mistake builds of the pin tests); returns the library path."""
+    out = os.path.join(outdir, "libswiftest_oracle_" + "_".join(d.replace("=", "") for d in defines) + ".so")
+    subprocess.run(["gcc", "-std=c11", "-O2", "-ffp-contract=off", "-fno-fast-math", "-march=x86-64-v3", "-fPIC",
+                    "-shared", "-w", *[f"-D{d}" for d in defines], "-o", out,
+                    os.path.join(HERE, "swiftest_oracle.c"), "-lm"], check=True)
+    return out
+
+
+def load(path):
+    """Open an oracle build and declare its entry points."""
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    L.swo_kick_getacch_int_all_tri.argtypes = [C.c_int, C.c_int, D, D, vp, D]
+    L.swo_kick_getacch_int_all_flat.argtypes = [C.c_int, C.c_int64, I32, D, D, vp, D]
+    L.swo_kick_getacch_int_all_tp.argtypes = [C.c_int, C.c_int, D, D, D, I32, D]
+    L.swo_drift_all.argtypes = [D, D, D, C.c_int, C.c_double, I32, I32, C.c_int, C.c_double]
+    L.swo_drift_kepu_stumpff.argtypes = [C.c_double] + [C.POINTER(C.c_double)] * 4
+    L.swo_encounter_check_one.argtypes = [C.c_double] * 8 + [C.POINTER(C.c_int32)] * 2
+    pp = C.POINTER(C.POINTER(C.c_int32))
+    L.swo_encounter_check_all_plpl.argtypes = [C.c_int, D, D, D, C.c_double, pp, pp, pp]
+    L.swo_encounter_check_all_plpl.restype = C.c_int64
+    L.swo_encounter_check_all_pltp.argtypes = [C.c_int, C.c_int, D, D, D, D, D, C.c_double, pp, pp, pp]
+    L.swo_encounter_check_all_pltp.restype = C.c_int64
+    L.swo_free.argtypes = [vp]
+    L.swo_collision_check_one.argtypes = [C.c_double] * 9 + [C.c_int32] + [C.POINTER(C.c_int32)] * 2
+    L.swo_orbel_xv2aeq.argtypes = [C.c_double] * 7 + [C.POINTER(C.c_double)] * 3
+    L.swo_orbel_xv2a.argtypes = [C.c_double] * 7
+    L.swo_orbel_xv2a.restype = C.c_double
+    L.swo_pow_m1p5.argtypes = [C.c_double]
+    L.swo_pow_m1p5.restype = C.c_double
+    L.swo_powi.argtypes = [C.c_double, C.c_int]
+    L.swo_powi.restype = C.c_double
+    L.swo_sys_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, D, D, D, D, vp, D, D,
+                              C.POINTER(Params)]
+    L.swo_sys_new.restype = vp
+    L.swo_sys_free.argtypes = [vp]
+    L.swo_sys_run.argtypes = [vp, C.c_int64]
+    L.swo_sys_step.argtypes = [vp]
+    L.swo_sys_get.argtypes = [vp] + [vp] * 8
+    L.swo_sys_get_misc.argtypes = [vp] * 6
+    L.swo_sys_get_order.argtypes = [vp, I32]
+    L.swo_sys_stats.argtypes = [vp, C.POINTER(Stats)]
+    L.swo_sys_npl.argtypes = [vp]
+    L.swo_sys_ntp.argtypes = [vp]
+    L.swo_sys_list.argtypes = [vp, C.c_int, vp, vp, vp]
+    L.swo_sys_list.restype = C.c_int64
+    L.swo_norm2.argtypes = [C.c_double] * 3
+    L.swo_norm2.restype = C.c_double
+    L.swo_coord_vh2vb.argtypes = [C.c_int, C.c_double, D, D, D, D]
+    L.swo_sys_discard.argtypes = [vp, D, C.c_int, C.POINTER(C.c_int64)]
+    L.swo_discard.argtypes = ([C.c_int] * 3 + [C.c_double] * 2 + [D, D, D, D, D, D, D, I32, I32, I32, D, D, I32]
+                              + [D, D, D, I32, I32, D, D, D, C.c_int, C.c_int, C.POINTER(C.c_int64)])
+    L.swo_energy.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, D, D, D, D, D, D, I32, C.c_int, D, vp]
+    L.swo_sys_get_levels.argtypes = [vp] * 5
+    L.swo_shell_fac.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double)]
+    L.swo_shell_fac.restype = C.c_int
+    return L
 
 
 def _c(a, dt=np.float64):
@@ -192,6 +210,14 @@ def powi(x, n):
     return lib().swo_powi(float(x), int(n))
 
 
+def shell_fac(ri, r2, L=None):
+    """(kept, fac) of the SyMBA shell kick for shell radius^2 ri and
+    separation^2 r2 (symba_kick.f90:187-208)."""
+    f = C.c_double(0.0)
+    k = (L or lib()).swo_shell_fac(float(ri), float(r2), C.byref(f))
+    return bool(k), f.value
+
+
 # ---- whole systems --------------------------------------------------------
 
 WHM, HELIO, SYMBA = 0, 1, 2
@@ -201,8 +227,8 @@ class OracleSystem:
     """A system integrated by the CPU restatement of the reference step layer."""
 
     def __init__(self, integrator, sysic, dt, lclose=True, short_circuit=True, istep_out=0,
-                 rhill=None, nthreads=1, omp=False):
-        self.L = lib(omp)
+                 rhill=None, nthreads=1, omp=False, L=None):
+        self.L = L if L is not None else lib(omp)
         prm = Params(dt, int(lclose), int(short_circuit), int(istep_out), int(rhill is not None), int(nthreads))
         self.npl, self.ntp = sysic.npl, sysic.ntp
         rh_in = None if rhill is None else _c(rhill)
@@ -249,6 +275,14 @@ class OracleSystem:
         st = Stats()
         self.L.swo_sys_stats(self.h, C.byref(st))
         return {k: getattr(st, k) for k, _ in Stats._fields_}
+
+    def levels(self):
+        """levelg, levelm of the massive bodies and of the test particles
+        after the last step, in storage order."""
+        out = [np.zeros(self.npl, np.int32), np.zeros(self.npl, np.int32), np.zeros(self.ntp, np.int32),
+               np.zeros(self.ntp, np.int32)]
+        self.L.swo_sys_get_levels(self.h, *[_p(a) for a in out])
+        return dict(levelg=out[0], levelm=out[1], tlevelg=out[2], tlevelm=out[3])
 
     def enc_list(self, which=0):
         n = self.L.swo_sys_list(self.h, which, None, None, None)

@@ -1199,6 +1199,15 @@ static int list_check_pltp(swo_sys *s, double dt, int irec)
     return any;
 }
 
+/* Planted mistakes for the tests that check the oracle's own pins
+ * (tests/test_oracle_symba_pins.py builds this file with -DSWO_MUTANT=k into
+ * a temporary directory and expects a pin to fail): 1 drops the shell taper
+ * polynomial, 2 drops the -1 shell kicks of the recursion.  The oracle the
+ * tests compare against is always built with SWO_MUTANT = 0. */
+#ifndef SWO_MUTANT
+#define SWO_MUTANT 0
+#endif
+
 /* shell factor of symba_kick_list_plpl / _pltp (symba_kick.f90:187-208,
  * :299-318); returns 0 when r2 < rim1 (pair dropped) */
 static int shell_fac(double ri, double r2, double *fac)
@@ -1206,15 +1215,21 @@ static int shell_fac(double ri, double r2, double *fac)
     double rim1 = ri * (RSHELL * RSHELL);
     if (r2 < rim1) return 0;
     if (r2 < ri) {
+#if SWO_MUTANT == 1
+        *fac = swo_pow_m1p5(r2);
+#else
         double ris = sqrt(ri);
         double r = sqrt(r2);
         double rr = (ris - r) / (ris * (1.0 - RSHELL));
         *fac = swo_pow_m1p5(r2) * ((1.0 - 3.0 * (rr * rr)) + 2.0 * ((rr * rr) * rr));
+#endif
     } else {
         *fac = 1.0 / (r2 * sqrt(r2));
     }
     return 1;
 }
+
+int swo_shell_fac(double ri, double r2, double *fac) { return shell_fac(ri, r2, fac); }
 
 /* symba_kick_list_plpl, symba_kick.f90:126-232 */
 static void kick_list_plpl(swo_sys *s, double dt, int irec, int sgn)
@@ -1395,7 +1410,7 @@ static int recur(swo_sys *s, int ireci)
         if (!lenc || !s->p.short_circuit) lenc = list_check_pltp(s, dtl, irecp) || lenc;
         kick_list_plpl(s, dth, irecp, 1);
         kick_list_pltp(s, dth, irecp, 1);
-        if (ireci != 0) {
+        if (ireci != 0 && SWO_MUTANT != 2) {
             kick_list_plpl(s, dth, irecp, -1);
             kick_list_pltp(s, dth, irecp, -1);
         }
@@ -1408,7 +1423,7 @@ static int recur(swo_sys *s, int ireci)
         s->irec = ireci;
         kick_list_plpl(s, dth, irecp, 1);
         kick_list_pltp(s, dth, irecp, 1);
-        if (ireci != 0) {
+        if (ireci != 0 && SWO_MUTANT != 2) {
             kick_list_plpl(s, dth, irecp, -1);
             kick_list_pltp(s, dth, irecp, -1);
         }
@@ -1903,6 +1918,13 @@ void swo_sys_get_misc(const swo_sys *s, double *rhill, double *cb_vb, double *pt
 
 void swo_sys_get_order(const swo_sys *s, int32_t *pl_id) { memcpy(pl_id, s->id, s->npl * sizeof(int32_t)); }
 void swo_sys_stats(const swo_sys *s, swo_stats *st) { *st = s->st; }
+void swo_sys_get_levels(const swo_sys *s, int32_t *levelg, int32_t *levelm, int32_t *tlevelg, int32_t *tlevelm)
+{
+    if (levelg) memcpy(levelg, s->levelg, s->npl * sizeof(int32_t));
+    if (levelm) memcpy(levelm, s->levelm, s->npl * sizeof(int32_t));
+    if (tlevelg) memcpy(tlevelg, s->tlevelg, s->ntp * sizeof(int32_t));
+    if (tlevelm) memcpy(tlevelm, s->tlevelm, s->ntp * sizeof(int32_t));
+}
 int swo_sys_npl(const swo_sys *s) { return s->npl; }
 int swo_sys_ntp(const swo_sys *s) { return s->ntp; }
 

@@ -120,6 +120,12 @@ void swo_sys_get_misc(const swo_sys *s, double *rhill, double *cb_vb, double *pt
                       int32_t *lfirst);
 void swo_sys_get_order(const swo_sys *s, int32_t *pl_id);
 void swo_sys_stats(const swo_sys *s, swo_stats *st);
+/* recursion levels of the last step: levelg (current) and levelm (deepest
+ * reached) of every massive body and test particle, in storage order */
+void swo_sys_get_levels(const swo_sys *s, int32_t *levelg, int32_t *levelm, int32_t *tlevelg, int32_t *tlevelm);
+/* the shell factor of symba_kick_list_plpl / _pltp for shell radius^2 ri and
+ * separation^2 r2; returns 0 when the pair is dropped (r2 < ri*RSHELL^2) */
+int swo_shell_fac(double ri, double r2, double *fac);
 int swo_sys_npl(const swo_sys *s);
 int swo_sys_ntp(const swo_sys *s);
 /* Encounter lists produced by the last step's level-0 checks (1-based). */

@@ -283,7 +283,7 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
         HIPCHK(copy_wait(stream, q, zeros.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice));
 
     // ---- tiles ----
-    build_tiles();
+    if (int rc = build_tiles()) return rc;
 
     // ---- pl-pl bit matrix ----
     {
@@ -364,7 +364,7 @@ int Ensemble::own2(T **p, size_t n)
     return 0;
 }
 
-void Ensemble::build_tiles()
+int Ensemble::build_tiles()
 {
     // kick tiles: blocks of tuning.kick_*_block bodies of one system
     std::vector<Tile> tpl, ttp;
@@ -382,22 +382,28 @@ void Ensemble::build_tiles()
         for (int ib = 0; ib < nb; ib++)
             for (int jb = ib; jb < nb; jb++) tpp.push_back(make_int4(s, ib, jb, 0));
     }
-    auto put = [&](auto &vec, auto **dptr, int &cnt) {
+    // A tile array is published (its count set) only once it holds the
+    // tiles: after a failed allocation or copy the count stays 0, so no
+    // kernel is launched over a missing array, and the error is returned.
+    auto put = [&](auto &vec, auto **dptr, int &cnt) -> int {
         using T = typename std::remove_reference<decltype(vec)>::type::value_type;
+        cnt = 0;
         if (*dptr) (void)hipFree(*dptr);
         *dptr = nullptr;
+        HIPCHK(hipMalloc((void **)dptr, sizeof(T) * (vec.size() ? vec.size() : 1)));
+        if (!vec.empty()) HIPCHK(copy_wait(stream, *dptr, vec.data(), sizeof(T) * vec.size(), hipMemcpyHostToDevice));
         cnt = (int)vec.size();
-        if (hipMalloc((void **)dptr, sizeof(T) * (vec.size() ? vec.size() : 1)) == hipSuccess && cnt)
-            (void)copy_wait(stream, *dptr, vec.data(), sizeof(T) * vec.size(), hipMemcpyHostToDevice);
+        return 0;
     };
-    put(tpl, &d_tiles_pl, ntiles_pl);
-    put(ttp, &d_tiles_tp, ntiles_tp);
-    put(tpp, &d_tiles_pp, ntiles_pp);
+    if (int rc = put(tpl, &d_tiles_pl, ntiles_pl)) return rc;
+    if (int rc = put(ttp, &d_tiles_tp, ntiles_tp)) return rc;
+    if (int rc = put(tpp, &d_tiles_pp, ntiles_pp)) return rc;
     if (!d_tiles_pt) {
         int total = 0;
         for (int s = 0; s < nsys; s++) total += (h_tp_off[s + 1] - h_tp_off[s] + 63) / 64;
-        (void)hipMalloc((void **)&d_tiles_pt, sizeof(int4) * (total ? total : 1));
+        HIPCHK(hipMalloc((void **)&d_tiles_pt, sizeof(int4) * (total ? total : 1)));
     }
+    return 0;
 }
 
 int Ensemble::ensure_list(ListDev &L, int64_t n, int64_t &cap)
@@ -894,6 +900,11 @@ uint64_t Ensemble::graph_key() const
 
 void Ensemble::drop_graphs()
 {
+    // a replay still queued on the stream must end before its executable
+    // graph (and the kernel arguments it owns) is destroyed
+    bool any = false;
+    for (auto &g : graph_exec) any |= g != nullptr;
+    if (any && stream) (void)hipStreamSynchronize(stream);
     for (auto &g : graph_exec)
         if (g) {
             (void)hipGraphExecDestroy(g);
@@ -1257,8 +1268,7 @@ int Ensemble::set_frozen(const int32_t *frozen)
     HIPCHK(hipMemcpyAsync(sy.frozen, h_frozen.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(d_active, act.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice, stream));
     HIPCHK(hipStreamSynchronize(stream));
-    build_tiles();
-    return 0;
+    return build_tiles();
 }
 
 // Pick the workgroup size of the two pair kernels (and with it the body
@@ -1271,7 +1281,7 @@ int Ensemble::autotune(int reps, int32_t *pl_block, int32_t *tp_block)
     for (int c : cand) {
         tune.kick_tp_block = c;
         tune.kick_pl_block = c;
-        build_tiles();
+        if (int rc = build_tiles()) return rc;
         double tms = 0.0, pms = 0.0;
         if (int rc = time_kernels(reps, ntp ? &tms : nullptr, npl ? &pms : nullptr)) return rc;
         if (ntp && tms < best_tp) {
@@ -1285,7 +1295,7 @@ int Ensemble::autotune(int reps, int32_t *pl_block, int32_t *tp_block)
     }
     tune.kick_tp_block = btp;
     tune.kick_pl_block = bpl;
-    build_tiles();
+    if (int rc = build_tiles()) return rc;
     if (pl_block) *pl_block = bpl;
     if (tp_block) *tp_block = btp;
     return 0;

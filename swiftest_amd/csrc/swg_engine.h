@@ -137,7 +137,7 @@ struct Ensemble {
     template <class T> int own(T **p, size_t n);
     template <class T> int own2(T **p, size_t n);
     int alloc_vec(Vec3p &v, size_t n);
-    void build_tiles();
+    int build_tiles();
     int ensure_list(ListDev &L, int64_t n, int64_t &cap);
     int detect_pp(double dt);
     int detect_pp_sas(double dt);

@@ -225,17 +225,11 @@ def test_pow_m1p5_correctly_rounded():
         assert abs(got - x ** -1.5) <= np.spacing(got)   # within 1 ulp of libm pow
 
 
-def test_powi_and_shell_continuity():
+def test_powi():
+    # the shell factor itself is pinned in test_oracle_symba_pins.py
     for n in range(0, 40):
         assert O.powi(0.48075, n) == pytest.approx(0.48075 ** n, rel=1e-15)
     assert O.powi(0.48075, 0) == 1.0
-    # fac(r) is continuous at both shell radii: r^-3 at r^2 = ri and 0 at r^2 = rim1
-    ri = 0.3
-    r = np.sqrt(ri)
-    rr = (np.sqrt(ri) - r) / (np.sqrt(ri) * (1 - 0.48075))
-    assert 1 - 3 * rr ** 2 + 2 * rr ** 3 == 1.0
-    rr = 1.0
-    assert 1 - 3 * rr ** 2 + 2 * rr ** 3 == 0.0
 
 
 # ------------------------------------------------------------ whole runs --
@@ -296,7 +290,9 @@ def test_symba_disk_energy_bounded():
     assert o.stats()["max_depth"] >= 0
     st = o.state()
     E, L = _energy(d.GMcb, st["pl_rh"], st["pl_vh"], Gm)
-    assert abs((E - E0) / E0) < 1e-7
+    # the oracle reaches 2.7e-11; the bound is about 100x that (a dropped
+    # shell taper gives 6.5e-8, dropped -1 shell kicks 1.1e-7)
+    assert abs((E - E0) / E0) < 3e-9
     assert abs((L - L0) / L0) < 1e-10
 
 
