@@ -10,11 +10,16 @@ snapshots every 100 steps, all state resident in HBM.
 `value` is FP64 pair interactions per second of whole-step wall time:
 2 evaluations x (N(N-1)/2 + npl*ntp) pair interactions per step, divided by
 the measured time of complete steps (encounter detection, recursion, drifts
-and frame changes included).  The roofline object covers the dominant kernel
-(pl-tp accelerations, 17 flops per pair) with its launch durations measured
-by HIP events on its own stream during the timed steps.  The CPU baseline
-times the CPU restatement's OpenMP pair kernels on a bounded sample of the
-same state.
+and frame changes included).  The steps run with the fast pair factor
+(SWG_PAIR_FAST: within a few ulp per pair, accelerations within 1e-13 of the
+exact sums); the exact-factor kernels are timed on the same state beside it.
+The roofline object covers the dominant kernel (pl-tp accelerations, 17
+flops per pair) with its launch durations measured by HIP events on its own
+stream during the timed steps.  The CPU baseline times the CPU restatement's
+OpenMP pair kernels on a bounded sample of the same state (the unit of
+`value`), and whole runs of the two small workloads (WHM 8 planets + 1000
+particles, 10^4 steps; the 2000-body SyMBA disk, 10^3 steps) beside the same
+runs on the GPU.
 """
 from __future__ import annotations
 
@@ -35,10 +40,11 @@ FLOPS_PER_PLTP_PAIR = 17       # swiftest_kick.f90:404-408, :464-467
 FLOPS_PER_PLPL_PAIR = 19       # swiftest_kick.f90:227-236, per ordered pair
 
 
-def measured_traffic():
-    """HBM bytes per k_kick_tp launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (profiles/kick_tp_c3_hbm_traffic.md), or None."""
-    path = os.path.join(ROOT, "profiles", "kick_tp_c3_hbm_traffic.md")
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (profiles/<kernel>_c3_hbm_traffic.md),
+    or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_c3_hbm_traffic.md")
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch"):
@@ -48,13 +54,27 @@ def measured_traffic():
     return None
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
 def cpu_baseline(sysic, target_s):
     """Pair-interaction rate of the CPU restatement's OpenMP kernels (the
     reference's loop decomposition: OpenMP over particles / rows, j in order)
     on a bounded sample of the benchmark state."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O  # the test oracle doubles as the CPU baseline
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cores = cpu_threads()
     npl = sysic.npl
     rpl, Gm = sysic.pl_rh, sysic.Gmass
     # calibrate on a small slice, then size the timed slice to ~target_s
@@ -74,11 +94,59 @@ def cpu_baseline(sysic, target_s):
     t_pp = time.perf_counter() - t0
     pairs_pp = float(min(npl, 4096)) * (npl - 1)
     rate = (pairs + pairs_pp) / (t_tp + t_pp)
-    return {"value": rate, "unit": "pair-interactions/s", "cores": cores, "kind": "port",
+    return {"value": rate, "unit": "pair-interactions/s", "cores": cores, "kind": "port", "cpu": cpu_model(),
             "sample": (f"CPU restatement (C, -O3, OpenMP, {cores} threads, reference loop decomposition) pair "
                        f"kernels on the benchmark's initial state: pl-tp accelerations of {nt} particles x {npl} "
-                       f"bodies ({t_tp:.2f} s) plus 4096 full pl-pl rows ({t_pp:.2f} s); kernel time only, no "
-                       f"encounter detection, recursion or drift")}
+                       f"bodies ({t_tp:.2f} s) plus 4096 full pl-pl rows ({t_pp:.2f} s); kernel time only. A whole "
+                       f"step of this workload on the CPU is bound by its serial recursion and did not finish in "
+                       f"25 minutes (one core); whole-step rates of the small workloads are under small_workloads")}
+
+
+def small_workloads_leg(log, cpu=True, c2_cpu_steps_1thread=100):
+    """Whole runs of the two small workloads, on the GPU and on the host:
+    WHM 8 planets + 1000 particles (dt 0.01, 10^4 steps) and the SyMBA
+    planetesimal disk of 2000 bodies (dt 0.005, 10^3 steps, CHK_CLOSE on),
+    both in the exact pair mode (bit-identical to the CPU restatement).  The
+    CPU side is the restatement's OpenMP build at the host's threads and at
+    one thread (the disk's one-thread run over its first 100 steps)."""
+    from swiftest_amd import ics
+    import swiftest_amd as S
+    out = {}
+    cases = [("C1_whm_8pl_1000tp", S.WHM, ics.whm_solar(), 0.01, 10000, True),
+             ("C2_symba_disk_2000pl", S.SYMBA, ics.symba_disk(), 0.005, 1000, True)]
+    for name, integ, sysic, dt, nsteps, lclose in cases:
+        g = S.GpuSystem(integ, sysic, dt, istep_out=100, lclose=lclose)
+        g.run(1)
+        t0 = time.perf_counter()
+        g.run(nsteps - 1)
+        t_gpu = time.perf_counter() - t0
+        res = {"npl": sysic.npl, "ntp": sysic.ntp, "steps": nsteps, "dt": dt,
+               "gpu": {"steps_per_s": (nsteps - 1) / t_gpu, "seconds": t_gpu,
+                       "how": "one system, exact pair mode, steps 2..N timed"}}
+        del g
+        log(f"[bench] {name}: GPU {res['gpu']['steps_per_s']:.1f} steps/s")
+        if cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import pyoracle as O
+            oi = {S.WHM: O.WHM, S.HELIO: O.HELIO, S.SYMBA: O.SYMBA}[integ]
+            for nth in (cpu_threads(), 1):
+                steps = nsteps if (nth > 1 or integ == S.WHM) else c2_cpu_steps_1thread
+                o = O.OracleSystem(oi, sysic, dt, istep_out=100, lclose=lclose, omp=True, nthreads=nth)
+                t0 = time.perf_counter()
+                rc = o.run(steps)
+                t_cpu = time.perf_counter() - t0
+                del o
+                res[f"cpu_{nth}_threads"] = {"steps_per_s": steps / t_cpu, "seconds": t_cpu, "steps": steps, "rc": rc}
+                log(f"[bench] {name}: CPU {nth} threads {steps / t_cpu:.1f} steps/s ({steps} steps)")
+            # creating a system sets the OpenMP thread count of the process;
+            # put it back for the legs that follow
+            O.OracleSystem(oi, ics.whm_solar(npl=2, ntp=0), dt, omp=True, nthreads=cpu_threads())
+            res["cpu"] = {"model": cpu_model(), "kind": "port",
+                          "how": "CPU restatement, -O3 -fopenmp, reference loop decomposition, whole steps"}
+            res["gpu_over_cpu"] = res["gpu"]["steps_per_s"] / res[f"cpu_{cpu_threads()}_threads"]["steps_per_s"]
+            res["gpu_over_cpu_1thread"] = res["gpu"]["steps_per_s"] / res["cpu_1_threads"]["steps_per_s"]
+        out[name] = res
+    return out
 
 
 def detection_leg(sysic, log, sparse_npl=131072):
@@ -182,34 +250,74 @@ def ensemble_leg(nsys, steps, warmup, sample, log):
                       "ntp U{0..2000}, seed 4, CHK_CLOSE off"}
 
 
-def serving_leg(nreq, rate, nmax, tpmax, smin, smax, lanes, log):
-    """Served stream (config C5, scaled down so the default run stays short):
-    C3-law systems (m log-uniform 1e-7..1e-3 Msun over a U(0.5,5) AU, test
-    particles a U(0.3,10) AU) with N log-uniform 8..nmax, U{0..tpmax} test
-    particles and log-uniform smin..smax steps, arriving as a Poisson stream
-    at `rate` per second; admitted by device-memory budget into batches on
-    `lanes` concurrent streams."""
+def serving_leg(nreq, rate, nmax, tpmax, smin, smax, chunk, seq_sample, log):
+    """Served stream (config C5): C3-law systems (m log-uniform 1e-7..1e-3
+    Msun over a U(0.5,5) AU, test particles a U(0.3,10) AU) with N
+    log-uniform 8..nmax, U{0..tpmax} test particles and log-uniform
+    smin..smax steps, arriving as a Poisson stream at `rate` per second,
+    served with continuous batching over slot pools (swiftest_amd.serve).
+    Compared with a seeded sample of the same requests run one after another
+    through the plain path, one system per handle."""
     import numpy as np
     from swiftest_amd import ics, serve
+    import swiftest_amd as S
     rng = np.random.Generator(np.random.PCG64(5))
     n = np.floor(np.exp(rng.uniform(np.log(8), np.log(nmax + 1), nreq))).astype(int)
     m = rng.integers(0, tpmax + 1, nreq)
     steps = np.floor(np.exp(rng.uniform(np.log(smin), np.log(smax + 1), nreq))).astype(int)
     systems = [ics.symba_big(seed=1000 + k, npl=int(n[k]), ntp=int(m[k])) for k in range(nreq)]
     reqs = serve.poisson_requests(systems, steps, rate, seed=6)
-    srv = serve.Server(dt=0.005, lanes=lanes, batch_max=64)
+    srv = serve.Server(dt=0.005, chunk=chunk)
     wall = srv.run(reqs)
     srv.close()
     out = serve.summary(reqs, wall)
+    occ = [float(np.mean(o)) if o else 0.0 for o in srv.occupancy]
     out.update({"metric": "served completion latency p50/p99 and system-steps/s", "arrival_rate_per_s": rate,
-                "lanes": lanes, "batches": len(srv.batches),
-                "mean_batch": float(np.mean([b for _, b in srv.batches])),
+                "slot_classes": [{"npl_cap": c.npl_cap, "ntp_cap": c.ntp_cap, "slots": c.slots, "loads": srv.loads[k],
+                                  "mean_occupied": occ[k], "bytes": srv.pool_bytes[k]}
+                                 for k, c in enumerate(srv.classes)],
+                "chunk_steps": chunk,
                 "config": (f"SyMBA dt 0.005, C3 mass/orbit laws, N log-uniform 8-{nmax}, ntp U{{0..{tpmax}}}, "
-                           f"steps log-uniform {smin}-{smax}, {nreq} Poisson requests at {rate}/s, seed 5 "
-                           f"(scaled down from N 8-4096, ntp 0-100000, steps 1e3-1e5), CHK_CLOSE off")})
+                           f"steps log-uniform {smin}-{smax} (the law's 1e3-1e5 scaled down to keep the run "
+                           f"short), {nreq} Poisson requests at {rate}/s, seed 5, CHK_CLOSE off, exact pair mode")})
     log(f"[bench] served {nreq} requests in {wall:.2f} s: p50 {out['p50_s']:.3f} s, p99 {out['p99_s']:.3f} s, "
-        f"{out['system_steps_per_s']:.0f} system-steps/s over {len(srv.batches)} batches")
+        f"{out['system_steps_per_s']:.0f} system-steps/s, pools {out['slot_classes']}")
+    # the same requests, one after another, each alone
+    idx = np.random.Generator(np.random.PCG64(55)).choice(nreq, size=min(seq_sample, nreq), replace=False)
+    t_seq, s_seq = 0.0, 0
+    for i in idx:
+        r = reqs[int(i)]
+        t0 = time.perf_counter()
+        one = S.GpuSystem(S.SYMBA, [r.system], 0.005, istep_out=0, lclose=False)
+        one.run(r.nsteps)
+        t_seq += time.perf_counter() - t0
+        s_seq += r.nsteps
+        del one
+    out["sequential"] = {"system_steps_per_s": s_seq / t_seq, "sample": int(len(idx)),
+                         "how": "a sample of the same requests, each created and run alone, one after another"}
+    out["speedup_vs_sequential"] = out["system_steps_per_s"] / out["sequential"]["system_steps_per_s"]
+    log(f"[bench] sequential sample {len(idx)} requests -> {s_seq / t_seq:.1f} system-steps/s")
     return out
+
+
+def no_short_circuit_leg(sysic, mode, log, steps=1):
+    """One step of the benchmark workload with the pl-tp encounter check run
+    in every step (short_circuit off: the other legal evaluation of the
+    reference's `pl check .or. tp check`, symba_step.f90:38), so that the
+    pl-tp detection over npl x ntp pairs and the pl-tp recursion are timed,
+    with their encounter counts."""
+    import swiftest_amd as S
+    g = S.GpuSystem(S.SYMBA, sysic, 0.005, istep_out=100, lclose=False, short_circuit=False, pair_mode=mode)
+    g.autotune(1)
+    res = {"steps": steps, "step_seconds": [], "stats": []}
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        g.run(1)
+        res["step_seconds"].append(time.perf_counter() - t0)
+        res["stats"].append(g.stats())
+        log(f"[bench] short_circuit off: step {res['step_seconds'][-1]:.3f} s {res['stats'][-1]}")
+    del g
+    return res
 
 
 def main():
@@ -224,19 +332,26 @@ def main():
     ap.add_argument("--lclose", type=int, default=0,
                     help="CHK_CLOSE: 1 checks collisions inside the recursion (their merge is host work, "
                          "so a detected collision stops the run)")
+    ap.add_argument("--pair-mode", choices=("fast", "exact"), default="fast",
+                    help="pair factor of the direct sums in the timed steps")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-small", action="store_true", help="skip the whole-run legs of the two small workloads")
+    ap.add_argument("--no-detection", action="store_true", help="skip the encounter-detection leg")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--ens-systems", type=int, default=4096)
-    ap.add_argument("--ens-steps", type=int, default=10)
+    ap.add_argument("--ens-steps", type=int, default=100)
     ap.add_argument("--ens-sample", type=int, default=32)
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
-    ap.add_argument("--serve-requests", type=int, default=128)
+    ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=200.0)
-    ap.add_argument("--serve-nmax", type=int, default=512)
-    ap.add_argument("--serve-tpmax", type=int, default=2000)
+    ap.add_argument("--serve-nmax", type=int, default=4096)
+    ap.add_argument("--serve-tpmax", type=int, default=100000)
     ap.add_argument("--serve-steps", type=int, nargs=2, default=[20, 200])
-    ap.add_argument("--serve-lanes", type=int, default=4)
+    ap.add_argument("--serve-chunk", type=int, default=8)
+    ap.add_argument("--serve-seq-sample", type=int, default=8)
+    ap.add_argument("--no-sc-leg", action="store_true",
+                    help="skip the step with the pl-tp encounter check not short-circuited")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -249,9 +364,10 @@ def main():
 
     sysic = ics.symba_big(seed=args.seed, npl=args.npl, ntp=args.ntp)
     dt = 0.005
-    g = S.GpuSystem(S.SYMBA, sysic, dt, istep_out=args.istep_out, lclose=bool(args.lclose))
+    mode = S.PAIR_FAST if args.pair_mode == "fast" else S.PAIR_EXACT
+    g = S.GpuSystem(S.SYMBA, sysic, dt, istep_out=args.istep_out, lclose=bool(args.lclose), pair_mode=mode)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
-    log(f"[bench] created system npl={sysic.npl} ntp={sysic.ntp}")
+    log(f"[bench] created system npl={sysic.npl} ntp={sysic.ntp}, pair mode {args.pair_mode}")
     tiles = g.autotune(2)
     log(f"[bench] autotuned tiles (kick_pl, kick_tp) = {tiles}")
     for k in range(args.warmup):
@@ -275,9 +391,25 @@ def main():
 
     npl, ntp = sysic.npl, sysic.ntp
     pairs_per_eval = npl * (npl - 1) / 2 + float(npl) * ntp
+    tp_flops = FLOPS_PER_PLTP_PAIR * float(npl) * ntp
+    pl_flops = FLOPS_PER_PLPL_PAIR * float(npl) * (npl - 1)
     value = 2 * pairs_per_eval * args.steps / elapsed
-    achieved = FLOPS_PER_PLTP_PAIR * float(npl) * ntp / (ms_tp * 1e-3) / 1e12 if ms_tp > 0 else 0.0
-    pl_tflops = FLOPS_PER_PLPL_PAIR * float(npl) * (npl - 1) / (ms_pl * 1e-3) / 1e12 if ms_pl > 0 else 0.0
+    achieved = tp_flops / (ms_tp * 1e-3) / 1e12 if ms_tp > 0 else 0.0
+    pl_tflops = pl_flops / (ms_pl * 1e-3) / 1e12 if ms_pl > 0 else 0.0
+    # both pair modes' kernels on the final state, same tiles policy (no
+    # encounter-pair walks), for the side-by-side
+    kmodes = {}
+    for name, m in (("fast", S.PAIR_FAST), ("exact", S.PAIR_EXACT)):
+        g.set_pair_mode(m)
+        t = g.autotune(2)
+        tms, pms = g.time_kernels(3)
+        kmodes[name] = {"kick_tp_ms": tms, "kick_tp_frac": tp_flops / (tms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                        "kick_pl_ms": pms, "kick_pl_frac": pl_flops / (pms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                        "tiles": t}
+        log(f"[bench] {name} pair kernels: kick_tp {tms:.3f} ms ({kmodes[name]['kick_tp_frac']:.3f} of peak), "
+            f"kick_pl {pms:.3f} ms ({kmodes[name]['kick_pl_frac']:.3f} of peak), tiles {t}")
+    g.set_pair_mode(mode)
+    kname = "k_kick_tp_fast" if mode == S.PAIR_FAST else "k_kick_tp"
     out = {
         "metric": METRIC,
         "value": value,
@@ -294,32 +426,40 @@ def main():
         "config": {"workload": (f"SyMBA {npl} massive bodies + {ntp} test particles, dt={dt} yr, "
                                 f"seed {args.seed}, snapshots every {args.istep_out} steps"),
                    "npl": npl, "ntp": ntp, "dt": dt, "integrator": "SyMBA", "short_circuit": True,
-                   "chk_close": bool(args.lclose)},
+                   "chk_close": bool(args.lclose), "pair_mode": args.pair_mode},
         "steps_per_s": args.steps / elapsed,
+        "step_seconds": per_step,
         "encounters": {"nplpl_enc": [s["nplpl_enc"] for s in stats], "npltp_enc": [s["npltp_enc"] for s in stats],
                        "max_depth": [s["max_depth"] for s in stats], "nsubsteps": [s["nsubsteps"] for s in stats]},
         "kernels": {"kick_tp_ms": ms_tp, "kick_tp_launches": n_tp, "kick_pl_ms": ms_pl, "kick_pl_launches": n_pl,
-                    "kick_pl_tflops": pl_tflops,
-                    "kernel_pair_rate": pairs_per_eval / ((ms_tp + ms_pl) * 1e-3) if ms_tp + ms_pl > 0 else 0.0},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(),
-                     "kernel": "k_kick_tp (pl-tp accelerations)",
-                     "note": ("compute-bound FP64 VALU kernel, no MFMA; peak is MI355X FP64 vector; "
-                              f"{FLOPS_PER_PLTP_PAIR} flops per pl-tp pair x npl x ntp per launch; exact IEEE "
-                              "sqrt/div cores cost ~30 VALU + 2 quarter-rate ops per pair, capping it near 24% "
-                              "of the flop peak; traffic in bytes per launch from rocprofv3 FETCH_SIZE/"
-                              "WRITE_SIZE scaled by a 1 GiB copy; the SyMBA step itself is dominated by the "
-                              "recursion, reported under encounters")},
+                    "kick_pl_tflops": pl_tflops, "kick_pl_frac": pl_tflops / PEAK_FP64_TFLOPS,
+                    "kernel_pair_rate": pairs_per_eval / ((ms_tp + ms_pl) * 1e-3) if ms_tp + ms_pl > 0 else 0.0,
+                    "by_pair_mode": kmodes},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(kname),
+                     "kernel": f"{kname} (pl-tp accelerations, {args.pair_mode} pair factor)",
+                     "note": ("FP64 vector-ALU-bound kernel (no MFMA: the pair sum is not a contraction); peak is "
+                              "MI355X FP64 vector; 17 flops per pl-tp pair x npl x ntp per launch, duration from "
+                              "HIP events around each launch of the timed steps; the fast factor takes 16 VALU "
+                              "operations and one quarter-rate v_rsq_f64 per pair; traffic in bytes per launch "
+                              "from rocprofv3 FETCH_SIZE/WRITE_SIZE scaled by a copy calibration; the SyMBA step "
+                              "itself is dominated by the recursion, reported under encounters")},
         "cpu_baseline": None,
     }
     out["output_cadence"] = output_leg(g, sysic, log)
     del g
-    out["encounter_detection"] = detection_leg(sysic, log)
+    if not args.no_sc_leg:
+        out["no_short_circuit"] = no_short_circuit_leg(sysic, mode, log)
+    if not args.no_detection:
+        out["encounter_detection"] = detection_leg(sysic, log)
+    if not args.no_small:
+        out["small_workloads"] = small_workloads_leg(log, cpu=not args.no_cpu)
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(args.ens_systems, args.ens_steps, 1, args.ens_sample, log)
     if not args.no_serve:
         out["serving"] = serving_leg(args.serve_requests, args.serve_rate, args.serve_nmax, args.serve_tpmax,
-                                     args.serve_steps[0], args.serve_steps[1], args.serve_lanes, log)
+                                     args.serve_steps[0], args.serve_steps[1], args.serve_chunk,
+                                     args.serve_seq_sample, log)
     if not args.no_cpu:
         log("[bench] timing the CPU baseline sample")
         out["cpu_baseline"] = cpu_baseline(sysic, args.cpu_seconds)

@@ -42,6 +42,19 @@ extern "C" {
 #define SWG_HELIO 1
 #define SWG_SYMBA 2
 
+/* Pair factor Gm / r^3 of the direct sums (the pl-pl rows, the pl-tp sums and
+ * the encounter-pair corrections at level 0; the recursion's shell kicks
+ * always use the exact form):
+ *   SWG_PAIR_EXACT  IEEE sqrt and division in the reference's order: every
+ *                   sum is bit-identical to a strict evaluation of the
+ *                   reference's expressions (the default);
+ *   SWG_PAIR_FAST   the hardware reciprocal square root with a second-order
+ *                   correction: each factor within a few ulp of the IEEE
+ *                   one, sums in the same order, about half the
+ *                   instructions per pair.  Not bit-identical. */
+#define SWG_PAIR_EXACT 0
+#define SWG_PAIR_FAST 1
+
 typedef struct swg_ctx swg_ctx;
 typedef struct swg_system swg_system;
 
@@ -58,7 +71,8 @@ typedef struct swg_params {
     int32_t istep_out;     /* snapshot cadence in steps, 0 = none */
     int32_t rhill_present; /* RHILL_PRESENT: keep the given rhill */
     int32_t encounter_check; /* ENCOUNTER_CHECK for pl-pl: 0 TRIANGULAR, 1 SORTSWEEP (same list) */
-    int32_t reserved[2];
+    int32_t pair_mode;     /* SWG_PAIR_EXACT or SWG_PAIR_FAST (no counterpart in the reference) */
+    int32_t reserved;
 } swg_params;
 
 typedef struct swg_stats {
@@ -141,6 +155,44 @@ int swg_system_enc_list(swg_system *sys, int32_t which, int64_t *n, int32_t *ind
  * duration of one launch in ms, measured with HIP events on the system's
  * stream. */
 int swg_system_time_kernels(swg_system *sys, int32_t reps, double *ms_kick_tp, double *ms_kick_pl);
+/* Slot pools for continuous batching (no counterpart in the reference, which
+ * runs one system per process).  swg_system_create_slots makes an ensemble of
+ * nslots empty slots with room for npl_cap[s] massive bodies and ntp_cap[s]
+ * test particles each, all allocated up front; empty slots are frozen.
+ * swg_system_load puts a new system (initial conditions as for
+ * swg_system_create: heliocentric rh, vh, Gmass, radius, optional rhill) into
+ * a slot that is free or whose system has finished, runs the initialize
+ * sequence for that system alone and puts the slot into the following steps,
+ * while the other slots' systems keep their state.  The rest of a slot's
+ * capacity holds padding bodies (inactive, zero mass, far away) that add
+ * exactly nothing to any sum, so a system's trajectory in a slot is its
+ * trajectory alone, bit for bit.  swg_system_get_slot copies out the state
+ * of the system a slot holds (its own bodies only, in storage order; pl_id as
+ * in swg_system_get); freezing a slot (swg_system_set_frozen) stops it.
+ * HELIO and SyMBA. */
+int swg_system_create_slots(swg_ctx *ctx, const swg_params *prm, int32_t nslots, const int32_t *npl_cap,
+                            const int32_t *ntp_cap, swg_system **sys);
+int swg_system_load(swg_system *sys, int32_t slot, int32_t npl, int32_t ntp, double GMcb, const double *pl_rh,
+                    const double *pl_vh, const double *Gmass, const double *radius, const double *rhill,
+                    const double *tp_rh, const double *tp_vh);
+int swg_system_get_slot(swg_system *sys, int32_t slot, double *pl_rh, double *pl_vh, double *pl_vb, double *tp_rh,
+                        double *tp_vh, double *tp_vb, int32_t *pl_status, int32_t *tp_status, int32_t *pl_id);
+/* Accelerations of the two direct sums at the current positions, computed by
+ * the kernels and tiles the steps use (with the system's pair_mode), into
+ * host arrays pl_acc[npl][3] and tp_acc[ntp][3] (either may be NULL): each
+ * massive body's full row over its system with the radius test
+ * (swiftest_kick_getacch_int_all_tri_rad_pl, swiftest_kick.f90:165-271,
+ * without the encounter-pair removal), each active test particle's sum over
+ * its system's massive bodies (swiftest_kick_getacch_int_all_tp, :374-415).
+ * The state is unchanged.  HELIO and SyMBA.  A measurement and test hook
+ * with no counterpart in the reference. */
+int swg_system_accel(swg_system *sys, double *pl_acc, double *tp_acc);
+/* Switch a resident system's pair mode for the following steps (the tiles
+ * are rebuilt; the autotuner's choices of the other mode are kept). */
+int swg_system_set_pair_mode(swg_system *sys, int32_t mode);
+/* Pair mode of the leaf operators called on ctx (SWG_PAIR_EXACT by default);
+ * resident systems take theirs from swg_params.pair_mode. */
+int swg_set_pair_mode(swg_ctx *ctx, int32_t mode);
 /* Replay the step's two halves around the recursion (frame changes, kicks,
  * drifts) from captured HIP graphs instead of launching them one by one;
  * results are unchanged.  Off by default.  No counterpart in the reference. */
@@ -217,6 +269,8 @@ int swg_system_discard(swg_system *sys, const swg_discard_params *dp, const doub
 /* Autotuner: time the pl-tp and pl-pl acceleration kernels at each
  * workgroup size (64, 128, 256 bodies per tile) on the current state (no state
  * change) and keep the fastest for the following steps.  Reports the choice.
+ * With SWG_PAIR_FAST the massive-body tiles are 64 bodies and *pl_block
+ * reports the number of wavefronts sharing each tile's rows (2 ... 16).
  * No counterpart in the reference (its loops are OpenMP-scheduled). */
 int swg_system_autotune(swg_system *sys, int32_t reps, int32_t *pl_block, int32_t *tp_block);
 /* Live timing: while enabled, every launch of the pl-tp and pl-pl

@@ -114,12 +114,12 @@ def _p(a):
 
 # ---- leaf kernels -------------------------------------------------------
 
-def accel_plpl(r, Gm, radius=None, acc=None, nplm=None):
+def accel_plpl(r, Gm, radius=None, acc=None, nplm=None, omp=False):
     r = _c(r)
     n = r.shape[0]
     acc = np.zeros((n, 3)) if acc is None else _c(acc).copy()
     rad = None if radius is None else _c(radius)
-    lib().swo_kick_getacch_int_all_tri(n, n if nplm is None else nplm, r, _c(Gm), _p(rad), acc)
+    lib(omp).swo_kick_getacch_int_all_tri(n, n if nplm is None else nplm, r, _c(Gm), _p(rad), acc)
     return acc
 
 

@@ -22,6 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libswiftest_hip.so")
 
 WHM, HELIO, SYMBA = 0, 1, 2
+PAIR_EXACT, PAIR_FAST = 0, 1   # SWG_PAIR_* (pair factor of the direct sums)
 # fields of an energy record (SWG_E_* in include/swiftest_hip.h)
 ENERGY_FIELDS = ("ke_orbit", "ke_spin", "pe", "be", "be_cb", "te", "L_orbit_x", "L_orbit_y", "L_orbit_z",
                  "L_spin_x", "L_spin_y", "L_spin_z", "L_total_x", "L_total_y", "L_total_z", "GMtot")
@@ -44,7 +45,7 @@ class DiscardParams(C.Structure):
 class Params(C.Structure):
     _fields_ = [("dt", C.c_double), ("integrator", C.c_int32), ("lclose", C.c_int32),
                 ("short_circuit", C.c_int32), ("istep_out", C.c_int32), ("rhill_present", C.c_int32),
-                ("encounter_check", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("encounter_check", C.c_int32), ("pair_mode", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -92,6 +93,12 @@ def library():
         L.swg_system_kernel_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                               C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.swg_system_destroy.argtypes = [vp]
+        L.swg_system_accel.argtypes = [vp, vp, vp]
+        L.swg_system_create_slots.argtypes = [vp, C.POINTER(Params), C.c_int32, I32, I32, C.POINTER(vp)]
+        L.swg_system_load.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_double] + [vp] * 7
+        L.swg_system_get_slot.argtypes = [vp, C.c_int32] + [vp] * 9
+        L.swg_set_pair_mode.argtypes = [vp, C.c_int32]
+        L.swg_system_set_pair_mode.argtypes = [vp, C.c_int32]
         L.swg_system_energy.argtypes = [vp, D, D, D, D]
         L.swg_system_discard.argtypes = [vp, C.POINTER(DiscardParams), D, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         _lib = L
@@ -132,6 +139,11 @@ def _p(a):
 
 
 # ---- leaf operators (same names and contracts as the reference) ----------
+
+def set_pair_mode(mode):
+    """Pair factor of the leaf operators: PAIR_EXACT (default) or PAIR_FAST."""
+    _check(library().swg_set_pair_mode(context(), int(mode)))
+
 
 def swiftest_kick_getacch_int_all_tri(npl, nplm, r, Gmass, radius, acc):
     """swiftest_kick_getacch_int_all_tri_rad_pl / _norad_pl (swiftest_kick.f90:165, :274).
@@ -203,7 +215,7 @@ class GpuSystem:
     """
 
     def __init__(self, integrator, systems, dt, lclose=True, short_circuit=True, istep_out=0, rhill=None, ctx=None,
-                 encounter_check=0):
+                 encounter_check=0, pair_mode=PAIR_EXACT):
         if not isinstance(systems, (list, tuple)):
             systems = [systems]
         self.L = library()
@@ -214,7 +226,7 @@ class GpuSystem:
         cat = lambda key, w: _c(np.concatenate([getattr(s, key).reshape(-1, w) for s in systems])) \
             if systems else np.zeros((0, w))
         prm = Params(float(dt), int(integrator), int(lclose), int(short_circuit), int(istep_out),
-                     int(rhill is not None), int(encounter_check))
+                     int(rhill is not None), int(encounter_check), int(pair_mode))
         rh = None if rhill is None else _c(rhill)
         h = vp()
         _check(self.L.swg_system_create(ctx if ctx is not None else context(), C.byref(prm), self.nsys,
@@ -226,6 +238,48 @@ class GpuSystem:
         self.h = h
         self._cb = (np.array([s.GMcb for s in systems], np.float64), np.array([s.Rcb for s in systems], np.float64))
         self._gm_in = [np.asarray(s.Gmass, np.float64) for s in systems]
+
+    @classmethod
+    def slots(cls, integrator, npl_cap, ntp_cap, dt, lclose=True, short_circuit=True, ctx=None,
+              encounter_check=0, pair_mode=PAIR_EXACT):
+        """A slot pool (swg_system_create_slots): len(npl_cap) empty slots of
+        the given capacities, all frozen until a system is loaded."""
+        self = cls.__new__(cls)
+        self.L = library()
+        self.nsys = len(npl_cap)
+        self.npl_s = _c(npl_cap, np.int32)
+        self.ntp_s = _c(ntp_cap, np.int32)
+        self.npl, self.ntp = int(self.npl_s.sum()), int(self.ntp_s.sum())
+        prm = Params(float(dt), int(integrator), int(lclose), int(short_circuit), 0, 0, int(encounter_check),
+                     int(pair_mode))
+        h = vp()
+        _check(self.L.swg_system_create_slots(ctx if ctx is not None else context(), C.byref(prm), self.nsys,
+                                              self.npl_s, self.ntp_s, C.byref(h)))
+        self.h = h
+        self._cb = (np.ones(self.nsys), np.ones(self.nsys))
+        self._gm_in = [np.zeros(n) for n in self.npl_s]
+        self._held = [(0, 0)] * self.nsys
+        return self
+
+    def load(self, slot, system, rhill=None):
+        """Put `system` (an ics.System) into `slot` (swg_system_load)."""
+        rh = None if rhill is None else _c(rhill)
+        _check(self.L.swg_system_load(self.h, int(slot), system.npl, system.ntp, float(system.GMcb),
+                                      _p(_c(system.pl_rh).reshape(-1, 3)), _p(_c(system.pl_vh).reshape(-1, 3)),
+                                      _p(_c(system.Gmass)), _p(_c(system.radius)), _p(rh),
+                                      _p(_c(system.tp_rh).reshape(-1, 3)), _p(_c(system.tp_vh).reshape(-1, 3))))
+        self._held[slot] = (system.npl, system.ntp)
+        self._cb[0][slot], self._cb[1][slot] = system.GMcb, system.Rcb
+
+    def get_slot(self, slot):
+        """State of the system `slot` holds (swg_system_get_slot)."""
+        n, m = self._held[slot]
+        out = dict(pl_rh=np.zeros((n, 3)), pl_vh=np.zeros((n, 3)), pl_vb=np.zeros((n, 3)), tp_rh=np.zeros((m, 3)),
+                   tp_vh=np.zeros((m, 3)), tp_vb=np.zeros((m, 3)), pl_status=np.zeros(n, np.int32),
+                   tp_status=np.zeros(m, np.int32), pl_id=np.zeros(n, np.int32))
+        _check(self.L.swg_system_get_slot(self.h, int(slot), *[_p(out[k]) for k in (
+            "pl_rh", "pl_vh", "pl_vb", "tp_rh", "tp_vh", "tp_vb", "pl_status", "tp_status", "pl_id")]))
+        return out
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -264,6 +318,17 @@ class GpuSystem:
         a, b = np.zeros(max(n.value, 1), np.int32), np.zeros(max(n.value, 1), np.int32)
         _check(self.L.swg_system_enc_list(self.h, which, C.byref(n), _p(a), _p(b)))
         return a[:n.value], b[:n.value]
+
+    def accel(self):
+        """(pl_acc, tp_acc): the direct-sum accelerations at the current
+        positions from the step's own kernels (swg_system_accel)."""
+        a, b = np.zeros((self.npl, 3)), np.zeros((self.ntp, 3))
+        _check(self.L.swg_system_accel(self.h, _p(a), _p(b)))
+        return a, b
+
+    def set_pair_mode(self, mode):
+        """PAIR_EXACT or PAIR_FAST for the following steps."""
+        _check(self.L.swg_system_set_pair_mode(self.h, int(mode)))
 
     def kernel_timing(self, enable=True):
         """Record HIP events around every acceleration-kernel launch of the

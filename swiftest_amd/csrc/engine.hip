@@ -42,6 +42,8 @@ int hipfail(hipError_t e, const char *what)
     return SWG_ERR_HIP;
 }
 
+static int hipfail_or_ok(hipError_t e) { return e == hipSuccess ? 0 : hipfail(e, "slot pool set-up"); }
+
 #define HIPCHK(x)                                    \
     do {                                             \
         hipError_t e_ = (x);                         \
@@ -176,6 +178,9 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
     if (p.integrator != SWG_HELIO && p.integrator != SWG_SYMBA && p.integrator != SWG_WHM)
         return fail(SWG_ERR_ARG, "unknown integrator");
     if (ns <= 0) return fail(SWG_ERR_ARG, "nsys must be positive");
+    if (p.pair_mode != SWG_PAIR_EXACT && p.pair_mode != SWG_PAIR_FAST)
+        return fail(SWG_ERR_ARG, "pair_mode must be SWG_PAIR_EXACT or SWG_PAIR_FAST");
+    tune.pair_fast = p.pair_mode == SWG_PAIR_FAST;
     nsys = ns;
     h_pl_off.assign(nsys + 1, 0);
     h_tp_off.assign(nsys + 1, 0);
@@ -187,6 +192,8 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
     npl = h_pl_off[nsys];
     ntp = h_tp_off[nsys];
     h_GMcb.assign(GMcb, GMcb + nsys);
+    h_nreal_pl.assign(npl_s, npl_s + nsys);
+    h_nreal_tp.assign(ntp_s, ntp_s + nsys);
 
     // ---- the reference's initialize sequence on the host side ----
     // set_rhill (unless given), then sort each system's massive bodies by mass,
@@ -368,12 +375,15 @@ int Ensemble::build_tiles()
 {
     // kick tiles: blocks of tuning.kick_*_block bodies of one system
     std::vector<Tile> tpl, ttp;
+    const int pl_tile = tune.pair_fast ? KPL_FAST_TILE : tune.kick_pl_block;
     for (int s = 0; s < nsys; s++) {
         if (h_frozen[s]) continue;
-        for (int b = h_pl_off[s]; b < h_pl_off[s + 1]; b += tune.kick_pl_block)
-            tpl.push_back(Tile{s, b, std::min(b + tune.kick_pl_block, h_pl_off[s + 1]), 0});
-        for (int b = h_tp_off[s]; b < h_tp_off[s + 1]; b += tune.kick_tp_block)
-            ttp.push_back(Tile{s, b, std::min(b + tune.kick_tp_block, h_tp_off[s + 1]), 0});
+        // rows of the bodies a system holds; a slot's padding (inactive, at
+        // the end of its range) is only ever a partner, with zero mass
+        const int pe = h_pl_off[s] + h_nreal_pl[s], te = h_tp_off[s] + h_nreal_tp[s];
+        for (int b = h_pl_off[s]; b < pe; b += pl_tile) tpl.push_back(Tile{s, b, std::min(b + pl_tile, pe), 0});
+        for (int b = h_tp_off[s]; b < te; b += tune.kick_tp_block)
+            ttp.push_back(Tile{s, b, std::min(b + tune.kick_tp_block, te), 0});
     }
     std::vector<int4> tpp;
     for (int s = 0; s < nsys; s++) {
@@ -889,7 +899,8 @@ uint64_t Ensemble::graph_key() const
                           pp.rowcnt,  pp.rowstart, pp.status, pt.i1,      pt.i2,     pt.col,    pt.colcnt,
                           pt.colstart, pt.status, d_arena,   (const void *)(intptr_t)ntiles_pl,
                           (const void *)(intptr_t)ntiles_tp, (const void *)(intptr_t)tune.kick_pl_block,
-                          (const void *)(intptr_t)tune.kick_tp_block};
+                          (const void *)(intptr_t)tune.kick_tp_block, (const void *)(intptr_t)tune.pair_fast,
+                          (const void *)(intptr_t)tune.kick_pl_split};
     uint64_t h = 1469598103934665603ull;
     for (const void *p : ptrs) {
         h ^= (uint64_t)(uintptr_t)p;
@@ -1226,6 +1237,212 @@ int Ensemble::time_kernels(int reps, double *ms_tp, double *ms_pl)
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Slots: an ensemble whose systems are fixed-capacity slots that take new
+// systems while the others keep running (continuous batching).
+// ---------------------------------------------------------------------------
+
+// Padding of a slot beyond the system it holds: inactive massive bodies
+// with zero mass, radius and Hill radius, and inactive test particles, far
+// from everything (10^30 AU, each at its own place) and at rest.  A padding
+// body adds exactly nothing to any sum (every term it enters has a zero
+// mass factor, fma(0, d, s) = s), is never drifted, kicked or listed in an
+// encounter, and sorts after the real bodies.
+static void pad_position(int k, double *r)
+{
+    r[0] = 1e30;
+    r[1] = 1e27 * (double)(k + 1);
+    r[2] = 0.0;
+}
+
+// Replace system s by a new one of npl_new <= capacity massive bodies and
+// ntp_new <= capacity test particles (initial conditions as for create), run
+// the reference's initialize sequence for it alone, and put it back into the
+// steps.  The other systems are not touched.
+int Ensemble::load(int s, int npl_new, int ntp_new, double GMcb_new, const double *pl_rh, const double *pl_vh,
+                   const double *Gmass, const double *radius, const double *rhill_in, const double *tp_rh,
+                   const double *tp_vh)
+{
+    if (prm.integrator == SWG_WHM) return fail(SWG_ERR_ARG, "slots hold HELIO or SyMBA systems");
+    if (s < 0 || s >= nsys) return fail(SWG_ERR_ARG, "no such slot");
+    const int p0 = h_pl_off[s], cap_pl = h_pl_off[s + 1] - p0;
+    const int t0 = h_tp_off[s], cap_tp = h_tp_off[s + 1] - t0;
+    if (npl_new < 0 || ntp_new < 0 || npl_new > cap_pl || ntp_new > cap_tp)
+        return fail(SWG_ERR_ARG, "system of " + std::to_string(npl_new) + " + " + std::to_string(ntp_new) +
+                                     " bodies does not fit slot " + std::to_string(s) + " (" + std::to_string(cap_pl) +
+                                     " + " + std::to_string(cap_tp) + ")");
+    if (!(GMcb_new > 0.0)) return fail(SWG_ERR_ARG, "central body GM must be positive");
+    hipStream_t st = stream;
+    HIPCHK(hipStreamSynchronize(st));
+    // host side of the initialize sequence, as in create: rhill, then the
+    // bodies sorted by mass, descending (helio_util.f90:26)
+    std::vector<double> rh(3 * (size_t)cap_pl), vh(3 * (size_t)cap_pl, 0.0), gm(cap_pl, 0.0), rad(cap_pl, 0.0),
+        rhl(cap_pl, 0.0), rhill(npl_new ? npl_new : 1);
+    if (rhill_in && prm.rhill_present)
+        std::copy(rhill_in, rhill_in + npl_new, rhill.begin());
+    else
+        host_set_rhill(GMcb_new, npl_new, pl_rh, pl_vh, Gmass, rhill.data());
+    std::vector<int> idx(npl_new);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int u, int v) { return Gmass[u] > Gmass[v]; });
+    std::vector<int32_t> status(cap_pl, INACTIVE), tstatus(cap_tp, INACTIVE);
+    for (int q = 0; q < cap_pl; q++) {
+        if (q < npl_new) {
+            const int src = idx[q];
+            for (int d = 0; d < 3; d++) {
+                rh[3 * (size_t)q + d] = pl_rh[3 * (size_t)src + d];
+                vh[3 * (size_t)q + d] = pl_vh[3 * (size_t)src + d];
+            }
+            gm[q] = Gmass[src];
+            rad[q] = radius[src];
+            rhl[q] = rhill[src];
+            status[q] = ACTIVE;
+            h_pl_id[p0 + q] = src + 1;
+        } else {
+            pad_position(q - npl_new, &rh[3 * (size_t)q]);
+            h_pl_id[p0 + q] = 0;
+        }
+        h_Gm[p0 + q] = gm[q];
+    }
+    std::vector<double> trh(3 * (size_t)cap_tp), tvh(3 * (size_t)cap_tp, 0.0);
+    for (int q = 0; q < cap_tp; q++) {
+        if (q < ntp_new) {
+            for (int d = 0; d < 3; d++) {
+                trh[3 * (size_t)q + d] = tp_rh[3 * (size_t)q + d];
+                tvh[3 * (size_t)q + d] = tp_vh[3 * (size_t)q + d];
+            }
+            tstatus[q] = ACTIVE;
+        } else {
+            double pr[3];
+            pad_position(q - ntp_new, pr);
+            trh[3 * (size_t)q] = -pr[0];
+            trh[3 * (size_t)q + 1] = pr[1];
+            trh[3 * (size_t)q + 2] = pr[2];
+        }
+    }
+    // the slot's ranges of the SoA arrays
+    auto put3 = [&](Vec3p v, int off, const std::vector<double> &aos, int n) -> int {
+        if (!n) return 0;
+        std::vector<double> c(3 * (size_t)n);
+        for (int i = 0; i < n; i++)
+            for (int d = 0; d < 3; d++) c[(size_t)d * n + i] = aos[3 * (size_t)i + d];
+        HIPCHK(copy_wait(st, v.x + off, c.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(copy_wait(st, v.y + off, c.data() + n, n * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(copy_wait(st, v.z + off, c.data() + 2 * (size_t)n, n * sizeof(double), hipMemcpyHostToDevice));
+        return 0;
+    };
+    // rbeg / rend of the padding are read as partner positions by the
+    // particle kicks, so the whole range gets the positions
+    for (Vec3p v : {pl.rh, pl.rbeg, pl.rend})
+        if (int rc = put3(v, p0, rh, cap_pl)) return rc;
+    if (int rc = put3(pl.vh, p0, vh, cap_pl)) return rc;
+    if (int rc = put3(tp.rh, t0, trh, cap_tp)) return rc;
+    if (int rc = put3(tp.vh, t0, tvh, cap_tp)) return rc;
+    if (cap_pl) {
+        HIPCHK(copy_wait(st, pl.Gm + p0, gm.data(), cap_pl * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(copy_wait(st, pl.R + p0, rad.data(), cap_pl * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(copy_wait(st, pl.rhill + p0, rhl.data(), cap_pl * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(copy_wait(st, pl.status + p0, status.data(), cap_pl * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    if (cap_tp) HIPCHK(copy_wait(st, tp.status + t0, tstatus.data(), cap_tp * sizeof(int32_t), hipMemcpyHostToDevice));
+    // per-system scalars as create leaves them
+    const double z3[3] = {0.0, 0.0, 0.0};
+    const int32_t one = 1, zero = 0;
+    h_GMcb[s] = GMcb_new;
+    HIPCHK(copy_wait(st, sy.GMcb + s, &GMcb_new, sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(copy_wait(st, sy.ptbeg + 3 * s, z3, sizeof z3, hipMemcpyHostToDevice));
+    HIPCHK(copy_wait(st, sy.ptend + 3 * s, z3, sizeof z3, hipMemcpyHostToDevice));
+    HIPCHK(copy_wait(st, sy.lfirst + s, &one, sizeof one, hipMemcpyHostToDevice));
+    for (int32_t *q : {sy.enc_pp, sy.enc_pt, sy.flag_pp, sy.flag_pt, sy.err, sy.frozen})
+        HIPCHK(copy_wait(st, q + s, &zero, sizeof zero, hipMemcpyHostToDevice));
+    HIPCHK(copy_wait(st, d_active + s, &one, sizeof one, hipMemcpyHostToDevice));
+    if (dis_ready) {
+        // pericentre-passage state of the discard checks starts afresh
+        std::vector<int32_t> ones(std::max(std::max(cap_pl, cap_tp), 1), 1);
+        std::vector<double> zd(std::max(std::max(cap_pl, cap_tp), 1), 0.0);
+        if (cap_pl) {
+            HIPCHK(copy_wait(st, dis_pl_isperi + p0, ones.data(), cap_pl * sizeof(int32_t), hipMemcpyHostToDevice));
+            HIPCHK(copy_wait(st, dis_pl_atp + p0, zd.data(), cap_pl * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(copy_wait(st, dis_pl_peri + p0, zd.data(), cap_pl * sizeof(double), hipMemcpyHostToDevice));
+        }
+        if (cap_tp) {
+            HIPCHK(copy_wait(st, dis_tp_isperi + t0, ones.data(), cap_tp * sizeof(int32_t), hipMemcpyHostToDevice));
+            HIPCHK(copy_wait(st, dis_tp_atp + t0, zd.data(), cap_tp * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(copy_wait(st, dis_tp_peri + t0, zd.data(), cap_tp * sizeof(double), hipMemcpyHostToDevice));
+        }
+        HIPCHK(copy_wait(st, dis_lfirst + s, &one, sizeof one, hipMemcpyHostToDevice));
+    }
+    // device part of the initialize sequence for this system alone
+    launch_init_one(st, pl, tp, sy, s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    h_nreal_pl[s] = npl_new;
+    h_nreal_tp[s] = ntp_new;
+    if (h_frozen[s]) nfrozen--;
+    h_frozen[s] = 0;
+    return build_tiles();
+}
+
+// The state of the system slot s holds (its real bodies, in storage order).
+int Ensemble::get_slot(int s, double *pl_rh, double *pl_vh, double *pl_vb, double *tp_rh, double *tp_vh,
+                       double *tp_vb, int32_t *pl_status, int32_t *tp_status, int32_t *pl_id)
+{
+    if (s < 0 || s >= nsys) return fail(SWG_ERR_ARG, "no such slot");
+    hipStream_t st = stream;
+    HIPCHK(hipStreamSynchronize(st));
+    const int p0 = h_pl_off[s], n = h_nreal_pl[s], t0 = h_tp_off[s], m = h_nreal_tp[s];
+    auto sub = [](Vec3p v, int off) { return Vec3p{v.x + off, v.y + off, v.z + off}; };
+    if (pl_rh && n && download_aos(st, sub(pl.rh, p0), pl_rh, n)) return SWG_ERR_HIP;
+    if (pl_vh && n && download_aos(st, sub(pl.vh, p0), pl_vh, n)) return SWG_ERR_HIP;
+    if (pl_vb && n && download_aos(st, sub(pl.vb, p0), pl_vb, n)) return SWG_ERR_HIP;
+    if (tp_rh && m && download_aos(st, sub(tp.rh, t0), tp_rh, m)) return SWG_ERR_HIP;
+    if (tp_vh && m && download_aos(st, sub(tp.vh, t0), tp_vh, m)) return SWG_ERR_HIP;
+    if (tp_vb && m && download_aos(st, sub(tp.vb, t0), tp_vb, m)) return SWG_ERR_HIP;
+    if (pl_status && n) HIPCHK(copy_wait(st, pl_status, pl.status + p0, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (tp_status && m) HIPCHK(copy_wait(st, tp_status, tp.status + t0, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (pl_id) std::copy(h_pl_id.begin() + p0, h_pl_id.begin() + p0 + n, pl_id);
+    return 0;
+}
+
+// Accelerations of the two direct sums at the current positions, with the
+// kernels and tiles the steps use: every massive body's full row with the
+// radius test (no encounter-pair removal), every test particle's sum over
+// the massive bodies at their current positions.  The kick kernels run with
+// dt = 1 on zeroed velocities (vb = fma(a, 1, 0) = a), the velocities are
+// read back and restored.
+int Ensemble::accel(double *pl_acc, double *tp_acc)
+{
+    hipStream_t st = stream;
+    if (!save_pl.x && (alloc_vec(save_pl, npl) || alloc_vec(save_tp, ntp))) return SWG_ERR_HIP;
+    const Vec3p vs[2] = {pl.vb, tp.vb}, sv[2] = {save_pl, save_tp};
+    const int ns[2] = {npl, ntp};
+    for (int k = 0; k < 2; k++) {
+        if (!ns[k]) continue;
+        const size_t b = ns[k] * sizeof(double);
+        HIPCHK(hipMemcpyAsync(sv[k].x, vs[k].x, b, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(sv[k].y, vs[k].y, b, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(sv[k].z, vs[k].z, b, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemsetAsync(vs[k].x, 0, b, st));
+        HIPCHK(hipMemsetAsync(vs[k].y, 0, b, st));
+        HIPCHK(hipMemsetAsync(vs[k].z, 0, b, st));
+    }
+    const int use_radius = prm.integrator == SWG_SYMBA || prm.lclose;
+    launch_kick_pl(st, tune, pl, sy, pp, d_tiles_pl, ntiles_pl, 1.0, 1, use_radius, 0);
+    launch_kick_tp(st, tune, tp, pl, sy, pt, d_tiles_tp, ntiles_tp, 1.0, 1, 0);
+    HIPCHK(hipGetLastError());
+    if (pl_acc && npl && download_aos(st, pl.vb, pl_acc, npl)) return SWG_ERR_HIP;
+    if (tp_acc && ntp && download_aos(st, tp.vb, tp_acc, ntp)) return SWG_ERR_HIP;
+    for (int k = 0; k < 2; k++) {
+        if (!ns[k]) continue;
+        const size_t b = ns[k] * sizeof(double);
+        HIPCHK(hipMemcpyAsync(vs[k].x, sv[k].x, b, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(vs[k].y, sv[k].y, b, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(vs[k].z, sv[k].z, b, hipMemcpyDeviceToDevice, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
 // Time the level-0 pl-pl check of the current state (the configured broad
 // phase plus the list it produces), with the host synchronizations it
 // contains; the stored level-0 list is rebuilt at the next step anyway.
@@ -1276,6 +1493,34 @@ int Ensemble::set_frozen(const int32_t *frozen)
 int Ensemble::autotune(int reps, int32_t *pl_block, int32_t *tp_block)
 {
     const int cand[3] = {64, 128, 256};
+    if (tune.pair_fast) {
+        // fast pair mode: 64-body massive tiles shared by 2 ... 16
+        // wavefronts, and the particle tile size
+        const int split[4] = {2, 4, 8, 16};
+        double best_tp = 1e300, best_pl = 1e300;
+        int btp = tune.kick_tp_block, bsp = tune.kick_pl_split;
+        for (int k = 0; k < 4; k++) {
+            tune.kick_pl_split = split[k];
+            if (k < 3) tune.kick_tp_block = cand[k];
+            if (int rc = build_tiles()) return rc;
+            double tms = 0.0, pms = 0.0;
+            if (int rc = time_kernels(reps, (ntp && k < 3) ? &tms : nullptr, npl ? &pms : nullptr)) return rc;
+            if (ntp && k < 3 && tms < best_tp) {
+                best_tp = tms;
+                btp = cand[k];
+            }
+            if (npl && pms < best_pl) {
+                best_pl = pms;
+                bsp = split[k];
+            }
+        }
+        tune.kick_tp_block = btp;
+        tune.kick_pl_split = bsp;
+        if (int rc = build_tiles()) return rc;
+        if (pl_block) *pl_block = bsp;
+        if (tp_block) *tp_block = btp;
+        return 0;
+    }
     double best_tp = 1e300, best_pl = 1e300;
     int btp = tune.kick_tp_block, bpl = tune.kick_pl_block;
     for (int c : cand) {
@@ -1332,6 +1577,7 @@ int swg_init(int device, swg_ctx **ctx)
         return hipfail(e, "hipStreamCreate");
     }
     c->leaf = new swg::LeafState();
+    c->pair_mode = SWG_PAIR_EXACT;
     *ctx = c;
     return 0;
 }
@@ -1416,6 +1662,96 @@ int swg_system_time_kernels(swg_system *sys, int32_t reps, double *ms_kick_tp, d
 {
     if (!sys || reps <= 0) return fail(SWG_ERR_ARG, "bad argument");
     return sys->e->time_kernels(reps, ms_kick_tp, ms_kick_pl);
+}
+
+int swg_system_create_slots(swg_ctx *ctx, const swg_params *prm, int32_t nslots, const int32_t *npl_cap,
+                            const int32_t *ntp_cap, swg_system **sys)
+{
+    if (!ctx || !prm || !sys || !npl_cap || !ntp_cap || nslots <= 0) return fail(SWG_ERR_ARG, "bad argument");
+    if (prm->integrator == SWG_WHM) return fail(SWG_ERR_ARG, "slots hold HELIO or SyMBA systems");
+    *sys = nullptr;
+    int64_t np = 0, nt = 0;
+    for (int s = 0; s < nslots; s++) {
+        if (npl_cap[s] < 0 || ntp_cap[s] < 0) return fail(SWG_ERR_ARG, "negative slot capacity");
+        np += npl_cap[s];
+        nt += ntp_cap[s];
+    }
+    if (np > INT32_MAX || nt > INT32_MAX) return fail(SWG_ERR_ARG, "slot capacities too large");
+    // every slot starts as padding only, with no real body
+    std::vector<double> rh(3 * (size_t)std::max<int64_t>(np, 1)), trh(3 * (size_t)std::max<int64_t>(nt, 1));
+    std::vector<double> vh(rh.size(), 0.0), tvh(trh.size(), 0.0), z(std::max<int64_t>(np, 1), 0.0);
+    std::vector<double> GM(nslots, 1.0);
+    for (int s = 0, p = 0, t = 0; s < nslots; s++) {
+        for (int q = 0; q < npl_cap[s]; q++, p++) pad_position(q, &rh[3 * (size_t)p]);
+        for (int q = 0; q < ntp_cap[s]; q++, t++) {
+            pad_position(q, &trh[3 * (size_t)t]);
+            trh[3 * (size_t)t] = -trh[3 * (size_t)t];
+        }
+    }
+    swg_params p = *prm;
+    p.rhill_present = 1;
+    int rc = swg_system_create(ctx, &p, nslots, npl_cap, ntp_cap, GM.data(), rh.data(), vh.data(), z.data(), z.data(),
+                               z.data(), trh.data(), tvh.data(), sys);
+    if (rc) return rc;
+    swg::Ensemble *e = (*sys)->e;
+    e->prm.rhill_present = prm->rhill_present;
+    std::vector<int32_t> inact(std::max<int64_t>(std::max(np, nt), 1), INACTIVE), fr(nslots, 1);
+    for (int s = 0; s < nslots; s++) {
+        e->h_nreal_pl[s] = 0;
+        e->h_nreal_tp[s] = 0;
+    }
+    if (np) rc = hipfail_or_ok(copy_wait(e->stream, e->pl.status, inact.data(), np * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!rc && nt) rc = hipfail_or_ok(copy_wait(e->stream, e->tp.status, inact.data(), nt * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!rc) rc = e->set_frozen(fr.data());
+    if (rc) {
+        swg_system_destroy(*sys);
+        *sys = nullptr;
+    }
+    return rc;
+}
+
+int swg_system_load(swg_system *sys, int32_t slot, int32_t npl, int32_t ntp, double GMcb, const double *pl_rh,
+                    const double *pl_vh, const double *Gmass, const double *radius, const double *rhill,
+                    const double *tp_rh, const double *tp_vh)
+{
+    if (!sys) return fail(SWG_ERR_ARG, "NULL system");
+    if ((npl && (!pl_rh || !pl_vh || !Gmass || !radius)) || (ntp && (!tp_rh || !tp_vh)))
+        return fail(SWG_ERR_ARG, "NULL array");
+    std::lock_guard<std::mutex> lk(swg::g_capture_mu);
+    return sys->e->load(slot, npl, ntp, GMcb, pl_rh, pl_vh, Gmass, radius, rhill, tp_rh, tp_vh);
+}
+
+int swg_system_get_slot(swg_system *sys, int32_t slot, double *pl_rh, double *pl_vh, double *pl_vb, double *tp_rh,
+                        double *tp_vh, double *tp_vb, int32_t *pl_status, int32_t *tp_status, int32_t *pl_id)
+{
+    if (!sys) return fail(SWG_ERR_ARG, "NULL system");
+    return sys->e->get_slot(slot, pl_rh, pl_vh, pl_vb, tp_rh, tp_vh, tp_vb, pl_status, tp_status, pl_id);
+}
+
+int swg_system_accel(swg_system *sys, double *pl_acc, double *tp_acc)
+{
+    if (!sys) return fail(SWG_ERR_ARG, "NULL system");
+    if (sys->e->prm.integrator == SWG_WHM) return fail(SWG_ERR_ARG, "accelerations are evaluated for HELIO and SyMBA");
+    return sys->e->accel(pl_acc, tp_acc);
+}
+
+int swg_system_set_pair_mode(swg_system *sys, int32_t mode)
+{
+    if (!sys) return fail(SWG_ERR_ARG, "NULL system");
+    if (mode != SWG_PAIR_EXACT && mode != SWG_PAIR_FAST) return fail(SWG_ERR_ARG, "unknown pair mode");
+    swg::Ensemble *e = sys->e;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->prm.pair_mode = mode;
+    e->tune.pair_fast = mode == SWG_PAIR_FAST;
+    return e->build_tiles();
+}
+
+int swg_set_pair_mode(swg_ctx *ctx, int32_t mode)
+{
+    if (!ctx) return fail(SWG_ERR_ARG, "ctx is NULL");
+    if (mode != SWG_PAIR_EXACT && mode != SWG_PAIR_FAST) return fail(SWG_ERR_ARG, "unknown pair mode");
+    ctx->pair_mode = mode;
+    return 0;
 }
 
 int swg_system_energy(swg_system *sys, const double *cb_mass, const double *cb_radius, const double *pl_mass,

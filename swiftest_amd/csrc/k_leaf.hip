@@ -29,10 +29,15 @@ static inline int cdiv(int64_t a, int b) { return (int)((a + b - 1) / b); }
 
 // int_one_pl contribution of pair (i, j) to body b's ahi (b == i) or ahj (b == j)
 __device__ inline void one_pl_half(double rji2, double xr, double yr, double zr, double Gm_other, bool b_is_i,
-                                   double &ax, double &ay, double &az)
+                                   double &ax, double &ay, double &az, int fast)
 {
-    double irij3 = 1.0 / (rji2 * sqrt(rji2));
-    double fac = Gm_other * irij3;
+    double fac;
+    if (fast) {
+        fac = gm_over_r3_fast(Gm_other, rji2);
+    } else {
+        double irij3 = 1.0 / (rji2 * sqrt(rji2));
+        fac = Gm_other * irij3;
+    }
     if (b_is_i) {
         ax = fma(fac, xr, ax);
         ay = fma(fac, yr, ay);
@@ -45,7 +50,7 @@ __device__ inline void one_pl_half(double rji2, double xr, double yr, double zr,
 }
 
 // swiftest_kick_getacch_int_all_tri_{rad,norad}_pl (swiftest_kick.f90:165-371)
-__global__ void k_leaf_tri(int npl, int nplm, const double *r, const double *Gm, const double *R, double *acc)
+__global__ void k_leaf_tri(int npl, int nplm, const double *r, const double *Gm, const double *R, double *acc, int fast)
 {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= npl) return;
@@ -61,7 +66,7 @@ __global__ void k_leaf_tri(int npl, int nplm, const double *r, const double *Gm,
                 double s = R[i] + R[b];
                 if (!(rji2 > s * s)) continue;
             }
-            one_pl_half(rji2, xr, yr, zr, Gm[i], false, hjx, hjy, hjz);
+            one_pl_half(rji2, xr, yr, zr, Gm[i], false, hjx, hjy, hjz, fast);
         }
         if (b < nplm) {
             for (int j = b + 1; j < npl; j++) {  // pairs (b, j)
@@ -71,7 +76,7 @@ __global__ void k_leaf_tri(int npl, int nplm, const double *r, const double *Gm,
                     double s = R[b] + R[j];
                     if (!(rji2 > s * s)) continue;
                 }
-                one_pl_half(rji2, xr, yr, zr, Gm[j], true, hix, hiy, hiz);
+                one_pl_half(rji2, xr, yr, zr, Gm[j], true, hix, hiy, hiz, fast);
             }
         }
         acc[3 * b] = (acc[3 * b] + hix) + hjx;
@@ -89,7 +94,7 @@ __global__ void k_leaf_tri(int npl, int nplm, const double *r, const double *Gm,
             double s = R[b] + R[j];
             if (!(rji2 > s * s)) continue;
         }
-        double fac = gm_over_r3(Gm[j], rji2, pair_g_ok(Gm[j]));
+        double fac = fast ? gm_over_r3_fast(Gm[j], rji2) : gm_over_r3(Gm[j], rji2, pair_g_ok(Gm[j]));
         ax = fma(fac, rx, ax);
         ay = fma(fac, ry, ay);
         az = fma(fac, rz, az);
@@ -104,7 +109,7 @@ __global__ void k_leaf_tri(int npl, int nplm, const double *r, const double *Gm,
 // each in list order (built on the host).
 __global__ void k_leaf_flat(int npl, const int32_t *k, const double *r, const double *Gm, const double *R,
                             const int64_t *rs, const int64_t *cs, const int64_t *rowl, const int64_t *coll,
-                            double *acc)
+                            double *acc, int fast)
 {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= npl) return;
@@ -118,7 +123,7 @@ __global__ void k_leaf_flat(int npl, const int32_t *k, const double *r, const do
             double s = R[i] + R[j];
             if (!(rji2 > s * s)) continue;
         }
-        one_pl_half(rji2, xr, yr, zr, Gm[j], true, hix, hiy, hiz);
+        one_pl_half(rji2, xr, yr, zr, Gm[j], true, hix, hiy, hiz, fast);
     }
     for (int64_t q = cs[b]; q < cs[b + 1]; q++) {
         int64_t p = coll[q];
@@ -129,7 +134,7 @@ __global__ void k_leaf_flat(int npl, const int32_t *k, const double *r, const do
             double s = R[i] + R[j];
             if (!(rji2 > s * s)) continue;
         }
-        one_pl_half(rji2, xr, yr, zr, Gm[i], false, hjx, hjy, hjz);
+        one_pl_half(rji2, xr, yr, zr, Gm[i], false, hjx, hjy, hjz, fast);
     }
     acc[3 * b] = (acc[3 * b] + hix) + hjx;
     acc[3 * b + 1] = (acc[3 * b + 1] + hiy) + hjy;
@@ -140,7 +145,7 @@ __global__ void k_leaf_flat(int npl, const int32_t *k, const double *r, const do
 // from zero over pl ascending, added to acc once (OpenMP reduction).
 template <int BLK>
 __global__ __launch_bounds__(BLK) void k_leaf_tp(int ntp, int npl, const double *rtp, const double *rpl,
-                                                 const double *Gm, const int32_t *lmask, double *acc)
+                                                 const double *Gm, const int32_t *lmask, double *acc, int fast)
 {
     __shared__ double sx[BLK], sy[BLK], sz[BLK], sg[BLK];
     int t = blockIdx.x * BLK + threadIdx.x;
@@ -166,7 +171,7 @@ __global__ __launch_bounds__(BLK) void k_leaf_tp(int ntp, int npl, const double 
             for (int q = 0; q < jn; q++) {
                 double rx = xt - sx[q], ry = yt - sy[q], rz = zt - sz[q];
                 double rji2 = dot3(rx, ry, rz, rx, ry, rz);
-                double fac = gm_over_r3(sg[q], rji2, pair_g_ok(sg[q]));
+                double fac = fast ? gm_over_r3_fast(sg[q], rji2) : gm_over_r3(sg[q], rji2, pair_g_ok(sg[q]));
                 px = fma(-fac, rx, px);
                 py = fma(-fac, ry, py);
                 pz = fma(-fac, rz, pz);
@@ -231,6 +236,7 @@ struct WhmArgs {
     int32_t *tlfirst;
     double *ah0;  // 3*nsys
     int lclose;
+    int fast;     // fast pair mode for the test-particle sums
 };
 
 // whm_coord_h2j_pl (whm_coord.f90:14-51)
@@ -500,7 +506,7 @@ __global__ __launch_bounds__(BLK) void k_whm_tp_kick(WhmArgs a, TpDev tp, const 
             for (int q = 0; q < qn; q++) {
                 double rx = xt - sx[q], ry = yt - sy[q], rz = zt - sz[q];
                 double rji2 = dot3(rx, ry, rz, rx, ry, rz);
-                double fac = gm_over_r3(sg[q], rji2, pair_g_ok(sg[q]));
+                double fac = a.fast ? gm_over_r3_fast(sg[q], rji2) : gm_over_r3(sg[q], rji2, pair_g_ok(sg[q]));
                 px = fma(-fac, rx, px);
                 py = fma(-fac, ry, py);
                 pz = fma(-fac, rz, pz);
@@ -538,6 +544,7 @@ static WhmArgs whm_args(Ensemble &e)
     a.tlfirst = e.sy.flag_pt;  // per-system scratch flag
     a.ah0 = e.sy.ptend;        // per-system 3-vector scratch
     a.lclose = e.prm.lclose;
+    a.fast = e.tune.pair_fast;
     return a;
 }
 
@@ -678,7 +685,7 @@ int swg_kick_getacch_int_all_tri_pl(swg_ctx *ctx, int32_t npl, int32_t nplm, con
     HIPCHK(hipMemcpyAsync(dr, r, 3 * npl * sizeof(double), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dg, Gmass, npl * sizeof(double), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(da, acc, 3 * npl * sizeof(double), hipMemcpyHostToDevice, st));
-    k_leaf_tri<<<cdiv(npl, 128), 128, 0, st>>>(npl, nplm, dr, dg, dR, da);
+    k_leaf_tri<<<cdiv(npl, 128), 128, 0, st>>>(npl, nplm, dr, dg, dR, da, ctx->pair_mode);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(acc, da, 3 * npl * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -734,7 +741,7 @@ int swg_kick_getacch_int_all_flat_pl(swg_ctx *ctx, int32_t npl, int64_t nplpl, c
     HIPCHK(hipMemcpyAsync(dcs, cs.data(), (npl + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(drl, rowl.data(), rowl.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dcl, coll.data(), coll.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
-    k_leaf_flat<<<cdiv(npl, 128), 128, 0, st>>>(npl, dk, dr, dg, dR, drs, dcs, drl, dcl, da);
+    k_leaf_flat<<<cdiv(npl, 128), 128, 0, st>>>(npl, dk, dr, dg, dR, drs, dcs, drl, dcl, da, ctx->pair_mode);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(acc, da, 3 * npl * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -759,7 +766,7 @@ int swg_kick_getacch_int_all_tp(swg_ctx *ctx, int32_t ntp, int32_t npl, const do
     HIPCHK(hipMemcpyAsync(dg, GMpl, npl * sizeof(double), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dm, lmask, ntp * sizeof(int32_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(da, acc, 3 * ntp * sizeof(double), hipMemcpyHostToDevice, st));
-    k_leaf_tp<256><<<cdiv(ntp, 256), 256, 0, st>>>(ntp, npl, dt_, dp, dg, dm, da);
+    k_leaf_tp<256><<<cdiv(ntp, 256), 256, 0, st>>>(ntp, npl, dt_, dp, dg, dm, da, ctx->pair_mode);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(acc, da, 3 * ntp * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));

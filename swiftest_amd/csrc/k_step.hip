@@ -285,6 +285,193 @@ __global__ __launch_bounds__(BLK) void k_kick_pl(PlDev pl, SysDev sy, ListDev pp
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fast pair mode (SWG_PAIR_FAST): the same sums in the same j order, with the
+// pair factor from the hardware reciprocal square root (gm_over_r3_fast,
+// within a few ulp of the IEEE factor).  The partners' positions, masses and
+// radii are read at wave-uniform indices through a constant-address view:
+// the loads go through the scalar data cache into SGPRs that every lane of
+// the wavefront reads, so no LDS staging, no workgroup barrier in the loop,
+// and every VALU slot goes to the pair arithmetic (13 operations, one of
+// them quarter-rate, and 7 of the factor).
+// ---------------------------------------------------------------------------
+
+typedef const __attribute__((address_space(4))) double cdouble;
+__device__ inline const cdouble *cview(const double *p) { return (const cdouble *)p; }
+
+// Massive bodies: one tile of KPL_FAST_TILE bodies of one system per
+// workgroup of K wavefronts.  Wavefront w sums body i's row over its share of
+// the partners, j in [p0 + w L, p0 + (w+1) L) with L a multiple of 4, and
+// takes every K-th entry of the body's encounter-pair walks; the K partial
+// accelerations are added in wavefront order through LDS.  K wavefronts per
+// 64 bodies give a 32768-body system 4096 wavefronts (4 per SIMD at K = 8)
+// where one lane per body gives 512.
+template <int K>
+__global__ __launch_bounds__(64 * K) void k_kick_pl_fast(PlDev pl, SysDev sy, ListDev pp, const Tile *tiles,
+                                                         double dt, int lbeg, int use_radius, int do_pairs)
+{
+    static_assert(KPL_FAST_TILE == 64, "one lane per body in each wavefront");
+    __shared__ double part[K][3][64];
+    const Tile tl = tiles[blockIdx.x];
+    const int p0 = sy.pl_off[tl.s], p1 = sy.pl_off[tl.s + 1];
+    const int lane = (int)(threadIdx.x & 63u);
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int i = tl.b0 + lane;
+    const bool valid = i < tl.b1;
+    double xi = 0.0, yi = 0.0, zi = 0.0, Ri = 0.0;
+    if (valid) {
+        xi = pl.rh.x[i];
+        yi = pl.rh.y[i];
+        zi = pl.rh.z[i];
+        Ri = pl.R[i];
+    }
+    const cdouble *X = cview(pl.rh.x), *Y = cview(pl.rh.y), *Z = cview(pl.rh.z), *G = cview(pl.Gm),
+                  *RR = cview(pl.R);
+    const FastConst fc;
+    const int n = p1 - p0;
+    const int L = ((n + K - 1) / K + 3) & ~3;
+    const int j0 = min(p1, p0 + w * L), j1 = min(p1, j0 + L);
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    int j = j0;
+    for (; j + 4 <= j1; j += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const double rx = X[j + u] - xi, ry = Y[j + u] - yi, rz = Z[j + u] - zi;
+            const double r2 = dot3(rx, ry, rz, rx, ry, rz);
+            const double rs = Ri + RR[j + u];
+            // a self pair has r2 = 0, so the radius test leaves it out too
+            const bool take = use_radius ? (r2 > rs * rs) : (j + u != i);
+            const double f = take ? gm_over_r3_fast(G[j + u], r2, fc.c15, fc.c1875) : 0.0;
+            ax = fma(f, rx, ax);
+            ay = fma(f, ry, ay);
+            az = fma(f, rz, az);
+        }
+    }
+    for (; j < j1; j++) {
+        const double rx = X[j] - xi, ry = Y[j] - yi, rz = Z[j] - zi;
+        const double r2 = dot3(rx, ry, rz, rx, ry, rz);
+        const double rs = Ri + RR[j];
+        const bool take = use_radius ? (r2 > rs * rs) : (j != i);
+        const double f = take ? gm_over_r3_fast(G[j], r2, fc.c15, fc.c1875) : 0.0;
+        ax = fma(f, rx, ax);
+        ay = fma(f, ry, ay);
+        az = fma(f, rz, az);
+    }
+    if (valid && do_pairs) {
+        // this wavefront's share of the encounter pairs removed again
+        // (symba_kick.f90:61-70)
+        double hx = 0.0, hy = 0.0, hz = 0.0;
+        const int64_t c0 = pp.colstart[i], c1 = c0 + pp.colcnt[i];
+        for (int64_t c = c0 + w; c < c1; c += K) {  // pairs (p, i), p < i
+            const int p = pp.i1[pp.col[c]];
+            const double rx = xi - pl.rh.x[p], ry = yi - pl.rh.y[p], rz = zi - pl.rh.z[p];
+            const double r2 = dot3(rx, ry, rz, rx, ry, rz);
+            const double rs = pl.R[p] + Ri;
+            if (r2 > rs * rs) {
+                const double f = gm_over_r3_fast(pl.Gm[p], r2);
+                hx = fma(-f, rx, hx);
+                hy = fma(-f, ry, hy);
+                hz = fma(-f, rz, hz);
+            }
+        }
+        const int64_t r0 = pp.rowstart[i], r1 = r0 + pp.rowcnt[i];
+        for (int64_t k = r0 + w; k < r1; k += K) {  // pairs (i, j), j > i
+            const int q = pp.i2[k];
+            const double rx = pl.rh.x[q] - xi, ry = pl.rh.y[q] - yi, rz = pl.rh.z[q] - zi;
+            const double r2 = dot3(rx, ry, rz, rx, ry, rz);
+            const double rs = Ri + pl.R[q];
+            if (r2 > rs * rs) {
+                const double f = gm_over_r3_fast(pl.Gm[q], r2);
+                hx = fma(f, rx, hx);
+                hy = fma(f, ry, hy);
+                hz = fma(f, rz, hz);
+            }
+        }
+        ax = ax - hx;
+        ay = ay - hy;
+        az = az - hz;
+    }
+    part[w][0][lane] = ax;
+    part[w][1][lane] = ay;
+    part[w][2][lane] = az;
+    __syncthreads();
+    if (w != 0 || !valid) return;
+    ax = part[0][0][lane];
+    ay = part[0][1][lane];
+    az = part[0][2][lane];
+#pragma unroll
+    for (int q = 1; q < K; q++) {
+        ax = ax + part[q][0][lane];
+        ay = ay + part[q][1][lane];
+        az = az + part[q][2][lane];
+    }
+    Vec3p rb = lbeg ? pl.rbeg : pl.rend;
+    rb.x[i] = xi;
+    rb.y[i] = yi;
+    rb.z[i] = zi;
+    if (pl_active(pl, i)) {
+        pl.vb.x[i] = fma(ax, dt, pl.vb.x[i]);
+        pl.vb.y[i] = fma(ay, dt, pl.vb.y[i]);
+        pl.vb.z[i] = fma(az, dt, pl.vb.z[i]);
+    }
+}
+
+// Test particles: one lane per particle, the massive bodies at wave-uniform
+// indices, four per iteration.
+template <int BLK>
+__global__ __launch_bounds__(BLK) void k_kick_tp_fast(TpDev tp, PlDev pl, SysDev sy, ListDev pt, const Tile *tiles,
+                                                      double dt, int lbeg, int do_pairs)
+{
+    const Tile tl = tiles[blockIdx.x];
+    const int p0 = sy.pl_off[tl.s], p1 = sy.pl_off[tl.s + 1];
+    const int j = tl.b0 + (int)threadIdx.x;
+    const bool valid = (j < tl.b1) && tp_active(tp, j);
+    const Vec3p rp = lbeg ? pl.rbeg : pl.rend;
+    double xt = 0.0, yt = 0.0, zt = 0.0;
+    if (valid) {
+        xt = tp.rh.x[j];
+        yt = tp.rh.y[j];
+        zt = tp.rh.z[j];
+    }
+    const cdouble *X = cview(rp.x), *Y = cview(rp.y), *Z = cview(rp.z), *G = cview(pl.Gm);
+    const FastConst fc;
+    double px = 0.0, py = 0.0, pz = 0.0;
+    int q = p0;
+    for (; q + 4 <= p1; q += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const double rx = xt - X[q + u], ry = yt - Y[q + u], rz = zt - Z[q + u];
+            const double f = gm_over_r3_fast(G[q + u], dot3(rx, ry, rz, rx, ry, rz), fc.c15, fc.c1875);
+            px = fma(-f, rx, px);
+            py = fma(-f, ry, py);
+            pz = fma(-f, rz, pz);
+        }
+    }
+    for (; q < p1; q++) {
+        const double rx = xt - X[q], ry = yt - Y[q], rz = zt - Z[q];
+        const double f = gm_over_r3_fast(G[q], dot3(rx, ry, rz, rx, ry, rz), fc.c15, fc.c1875);
+        px = fma(-f, rx, px);
+        py = fma(-f, ry, py);
+        pz = fma(-f, rz, pz);
+    }
+    if (!valid) return;
+    double ax = 0.0 + px, ay = 0.0 + py, az = 0.0 + pz;
+    if (do_pairs) {
+        const int64_t c0 = pt.colstart[j], c1 = c0 + pt.colcnt[j];
+        for (int64_t c = c0; c < c1; c++) {
+            const int i = pt.i1[pt.col[c]];
+            const double dx = xt - rp.x[i], dy = yt - rp.y[i], dz = zt - rp.z[i];
+            const double f = gm_over_r3_fast(pl.Gm[i], dot3(dx, dy, dz, dx, dy, dz));
+            ax = fma(f, dx, ax);
+            ay = fma(f, dy, ay);
+            az = fma(f, dz, az);
+        }
+    }
+    tp.vb.x[j] = fma(ax, dt, tp.vb.x[j]);
+    tp.vb.y[j] = fma(ay, dt, tp.vb.y[j]);
+    tp.vb.z[j] = fma(az, dt, tp.vb.z[j]);
+}
+
 // Danby drift of the masked bodies (helio_drift_body, helio_drift.f90:14-54,
 // with the SyMBA level mask of symba_drift.f90:14-61): mu = GM_cb, (rh, vb).
 __global__ __launch_bounds__(256) void k_drift(Vec3p r, Vec3p v, int32_t *status, const int32_t *levelg, const int32_t *sys,
@@ -560,6 +747,47 @@ __global__ void k_vh2vb_apply(PlDev pl, SysDev sy, int npl)
     pl.vb.z[i] = pl.vh.z[i] + sy.cbvb[3 * s + 2];
 }
 
+// The initialize sequence of one system loaded into slot s of a running
+// ensemble (the per-system part of k_vh2vb_sum, k_vh2vb_apply and
+// k_init_tp_vb): cb%vb from vh over the slot's bodies, vb = vh + cb%vb for its
+// massive bodies and test particles.  Padding bodies have Gm = 0 and vh = 0,
+// so they add exactly nothing to the sums.  One workgroup.
+__global__ __launch_bounds__(SUM_T) void k_init_one(PlDev pl, TpDev tp, SysDev sy, int s)
+{
+    __shared__ SumStage S;
+    __shared__ double shr[4];
+    const int tid = threadIdx.x;
+    const int p0 = sy.pl_off[s], p1 = sy.pl_off[s + 1];
+    double acc = 0.0;
+    for (int c0 = p0; c0 < p1; c0 += SUM_CH) {
+        const int n = min(SUM_CH, p1 - c0);
+        sum_stage(S, pl, pl.vh, c0, n);
+        __syncthreads();
+        if (tid == 0) {
+            for (int k = 0; k < n; k++) acc = acc + S.gm[k];
+        } else if (tid <= 3) {
+            const double *v = S.v[tid - 1];
+            for (int k = 0; k < n; k++) acc = acc - S.gm[k] * v[k];
+        }
+        __syncthreads();
+    }
+    if (tid < 4) shr[tid] = acc;
+    __syncthreads();
+    if (tid < 3) sy.cbvb[3 * s + tid] = p1 > p0 ? shr[tid + 1] / (sy.GMcb[s] + shr[0]) : 0.0;
+    __syncthreads();
+    const double cx = sy.cbvb[3 * s], cy = sy.cbvb[3 * s + 1], cz = sy.cbvb[3 * s + 2];
+    for (int i = p0 + tid; i < p1; i += SUM_T) {
+        pl.vb.x[i] = pl.vh.x[i] + cx;
+        pl.vb.y[i] = pl.vh.y[i] + cy;
+        pl.vb.z[i] = pl.vh.z[i] + cz;
+    }
+    for (int j = sy.tp_off[s] + tid; j < sy.tp_off[s + 1]; j += SUM_T) {
+        tp.vb.x[j] = tp.vh.x[j] + cx;
+        tp.vb.y[j] = tp.vh.y[j] + cy;
+        tp.vb.z[j] = tp.vh.z[j] + cz;
+    }
+}
+
 // Initialization (helio_util.f90:26-28): tp vb = vh + cb%vb
 __global__ void k_init_tp_vb(TpDev tp, SysDev sy, int ntp)
 {
@@ -591,6 +819,16 @@ void launch_kick_pl(hipStream_t st, const Tuning &tu, PlDev pl, SysDev sy, ListD
                     int ntiles, double dt, int lbeg, int use_radius, int do_pairs)
 {
     if (!ntiles) return;
+    if (tu.pair_fast) {
+        switch (tu.kick_pl_split) {
+        case 1: k_kick_pl_fast<1><<<ntiles, 64, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
+        case 2: k_kick_pl_fast<2><<<ntiles, 128, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
+        case 4: k_kick_pl_fast<4><<<ntiles, 256, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
+        case 16: k_kick_pl_fast<16><<<ntiles, 1024, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
+        default: k_kick_pl_fast<8><<<ntiles, 512, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
+        }
+        return;
+    }
     switch (tu.kick_pl_block) {
     case 64: k_kick_pl<64><<<ntiles, 64, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
     case 128: k_kick_pl<128><<<ntiles, 128, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
@@ -613,6 +851,14 @@ void launch_kick_tp(hipStream_t st, const Tuning &tu, TpDev tp, PlDev pl, SysDev
                     const Tile *tiles, int ntiles, double dt, int lbeg, int do_pairs)
 {
     if (!ntiles) return;
+    if (tu.pair_fast) {
+        switch (tu.kick_tp_block) {
+        case 64: k_kick_tp_fast<64><<<ntiles, 64, 0, st>>>(tp, pl, sy, pt, tiles, dt, lbeg, do_pairs); break;
+        case 128: k_kick_tp_fast<128><<<ntiles, 128, 0, st>>>(tp, pl, sy, pt, tiles, dt, lbeg, do_pairs); break;
+        default: k_kick_tp_fast<256><<<ntiles, 256, 0, st>>>(tp, pl, sy, pt, tiles, dt, lbeg, do_pairs); break;
+        }
+        return;
+    }
     switch (tu.kick_tp_block) {
     case 64: k_kick_tp<64><<<ntiles, 64, 0, st>>>(tp, pl, sy, pt, tiles, dt, lbeg, do_pairs); break;
     case 128: k_kick_tp<128><<<ntiles, 128, 0, st>>>(tp, pl, sy, pt, tiles, dt, lbeg, do_pairs); break;
@@ -645,6 +891,10 @@ void launch_vh2vb(hipStream_t st, PlDev pl, SysDev sy, int nsys, int npl)
 {
     k_vh2vb_sum<<<nsys, SUM_T, 0, st>>>(pl, sy, nsys);
     if (npl) k_vh2vb_apply<<<cdiv(npl, 256), 256, 0, st>>>(pl, sy, npl);
+}
+void launch_init_one(hipStream_t st, PlDev pl, TpDev tp, SysDev sy, int s)
+{
+    k_init_one<<<1, SUM_T, 0, st>>>(pl, tp, sy, s);
 }
 void launch_init_tp_vb(hipStream_t st, TpDev tp, SysDev sy, int ntp)
 {

@@ -28,6 +28,9 @@ struct Ensemble {
     Tuning tune{};
     int nsys = 0, npl = 0, ntp = 0;
     std::vector<int32_t> h_pl_off, h_tp_off, h_pl_id, h_enc_pp, h_enc_pt, h_err;
+    // bodies of each system that are not slot padding (all of them unless
+    // the ensemble is a slot pool, swg_system_create_slots)
+    std::vector<int32_t> h_nreal_pl, h_nreal_tp;
     std::vector<double> h_GMcb, h_Gm;
 
     PlDev pl{};
@@ -97,6 +100,11 @@ struct Ensemble {
     int get_misc(double *rhill, double *cb_vb, double *ptbeg, double *ptend, int32_t *lfirst);
     int enc_list(int which, int64_t *n, int32_t *i1, int32_t *i2);
     int time_kernels(int reps, double *ms_tp, double *ms_pl);
+    int accel(double *pl_acc, double *tp_acc);
+    int load(int s, int npl_new, int ntp_new, double GMcb_new, const double *pl_rh, const double *pl_vh,
+             const double *Gmass, const double *radius, const double *rhill, const double *tp_rh, const double *tp_vh);
+    int get_slot(int s, double *pl_rh, double *pl_vh, double *pl_vb, double *tp_rh, double *tp_vh, double *tp_vb,
+                 int32_t *pl_status, int32_t *tp_status, int32_t *pl_id);
     int autotune(int reps, int32_t *pl_block, int32_t *tp_block);
     int set_frozen(const int32_t *frozen);
     int time_encounters(int reps, double *ms, int64_t *npairs, int64_t *ncand);
@@ -170,4 +178,5 @@ struct swg_ctx {
     int device;
     hipStream_t stream;
     swg::LeafState *leaf;
+    int pair_mode;  // SWG_PAIR_EXACT / SWG_PAIR_FAST for the leaf pair sums
 };

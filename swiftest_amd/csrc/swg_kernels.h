@@ -29,7 +29,11 @@ struct Tile {
 struct Tuning {
     int kick_pl_block = 256;   // 64, 128 or 256 threads per workgroup
     int kick_tp_block = 256;
+    int pair_fast = 0;         // 1: pair factors from the reciprocal square root (SWG_PAIR_FAST)
+    int kick_pl_split = 8;     // fast mode: wavefronts sharing one 64-body tile's rows (1, 2, 4, 8, 16)
 };
+// bodies per massive-body tile of the fast pair kernel
+constexpr int KPL_FAST_TILE = 64;
 
 // ---- step kernels (k_step.hip) ----
 void launch_reset_levels(hipStream_t st, PlDev pl, TpDev tp, int npl, int ntp);
@@ -49,6 +53,7 @@ void launch_tp_post(hipStream_t st, TpDev tp, SysDev sy, int ntp, double dth);
 void launch_lfirst_update(hipStream_t st, SysDev sy, int nsys);
 void launch_snapshot(hipStream_t st, PlDev pl, TpDev tp, SysDev sy, int nsys, int npl, int ntp);
 void launch_init_tp_vb(hipStream_t st, TpDev tp, SysDev sy, int ntp);
+void launch_init_one(hipStream_t st, PlDev pl, TpDev tp, SysDev sy, int s);
 
 // ---- energy and angular momentum (k_energy.hip) ----
 void energy_tiles(int nsys, const int32_t *pl_off, std::vector<int4> &tiles, std::vector<int32_t> &toff);

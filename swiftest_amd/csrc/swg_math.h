@@ -95,6 +95,44 @@ __device__ inline double gm_over_r3(double g, double r2, bool gok)
     return g / (r2 * sqrt(r2));
 }
 
+// g * r2^-1.5 for the fast pair mode (SWG_PAIR_FAST), not bit-identical to
+// g / (r2 * sqrt(r2)) but within a few ulp of it.  The hardware reciprocal
+// square root y = v_rsq_f64(r2) = r2^-1/2 (1 + d) has |d| < 2^-23 on gfx950
+// (measured: at most 2.5e8 ulp over 4M operands).  With e = 1 - r2 y^2 =
+// -(2d + d^2), r2^-1.5 = y^3 (1 - e)^-3/2 = y^3 (1 + 3/2 e + 15/8 e^2 +
+// O(e^3)), and e^3 < 2^-66, so the result carries only the roundings of its
+// seven operations.  The rounding of y*y enters e as well and cancels to
+// first order.  Outside 2^-1020 < r2 < 2^1020 y*y leaves the normal range
+// and the factor degrades where the IEEE one overflows or underflows anyway.
+// c15 = 1.5 and c1875 = 1.875 come from FastConst: held in VGPRs, the
+// polynomial is one v_fma_f64 (gfx9 VOP3 takes no literal, and as an
+// accumulator the constant would be re-materialized for every pair).
+__device__ inline double gm_over_r3_fast(double g, double r2, double c15, double c1875)
+{
+    const double y = __builtin_amdgcn_rsq(r2);
+    const double yy = y * y;
+    const double e = fma(-r2, yy, 1.0);
+    const double p = fma(e, c1875, c15);
+    const double g3 = (g * y) * yy;
+    return fma(g3 * e, p, g3);
+}
+
+// The two polynomial constants as loop-invariant vector registers.  The
+// empty asm statement (no instruction, no memory access) only hides their
+// values from constant folding.
+struct FastConst {
+    double c15 = 1.5, c1875 = 1.875;
+    __device__ FastConst()
+    {
+        asm volatile("" : "+v"(c15), "+v"(c1875));
+    }
+};
+
+__device__ inline double gm_over_r3_fast(double g, double r2)
+{
+    return gm_over_r3_fast(g, r2, 1.5, 1.875);
+}
+
 // ---------------------------------------------------------------------------
 // Danby / universal-variable Kepler drift (swiftest_drift.f90:111-580).
 // ---------------------------------------------------------------------------

@@ -1,18 +1,19 @@
-"""Serving loop for integration requests.
+"""Serving loop for integration requests, with continuous batching.
 
-Requests (one planetary system and a step count each) arrive over time.  A
-request is admitted once it has arrived, a lane is free and the device-memory
-budget has room for it; the requests admitted together form one batch, an
-ensemble advanced by one launch per step phase.  Each lane is a host thread
-with its own library context (its own HIP stream), so the batches of
-different lanes run on the device at the same time, and requests that arrive
-while other batches are running are admitted as lanes and memory free up.
+Requests (one planetary system and a step count each) arrive over time.  The
+device holds slot pools (swg_system_create_slots): ensembles of fixed-capacity
+slots allocated once, within the device-memory budget, one pool per size
+class.  Each pool is driven by its own host thread with its own library
+context (its own HIP stream), so the pools' steps overlap on the device.
 
-Inside a batch every system stops at its own step count: the batch advances
-to the next completion, the finished systems are frozen (left out of the
-following steps with their state kept exactly), and the rest continue.  A
-system's trajectory does not depend on the other systems of its batch, so
-each result equals that of the request run alone.
+A request goes to the smallest class its system fits.  Between steps a pool
+loads the requests that have arrived into its free slots (swg_system_load)
+while the systems in its other slots keep running; it then advances all of
+its occupied slots together, one launch per step phase, for at most `chunk`
+steps or until the next request in it completes.  A completed request's state
+is copied out (swg_system_get_slot) and its slot is frozen and free for the
+next arrival.  Padding bodies make a slot's system evolve exactly as it would
+alone, so each result equals that of the request run by itself.
 """
 from __future__ import annotations
 
@@ -34,121 +35,125 @@ class Request:
     admitted: float = -1.0
     done: float = -1.0
     lane: int = -1
+    slot: int = -1
     state: dict = field(default_factory=dict)
 
 
-def request_bytes(system) -> int:
-    """Device memory a request holds while resident: body state, the
+@dataclass
+class SlotClass:
+    """`slots` slots of room for npl_cap massive bodies and ntp_cap particles."""
+    npl_cap: int
+    ntp_cap: int
+    slots: int
+
+
+def request_bytes(npl: int, ntp: int) -> int:
+    """Device memory one slot of this capacity holds: body state, the
     encounter bit matrices (pl-pl and pl-tp, words plus prefixes) and pair
     lists sized at a few pairs per body."""
-    n, m = system.npl, system.ntp
+    n, m = npl, ntp
     words_pp = n * ((n + 63) // 64)
     words_pt = n * ((m + 63) // 64)
     return int(200 * n + 100 * m + 12 * (words_pp + words_pt) + 64 * (n + m) + (1 << 16))
 
 
+# size classes for requests of up to 4096 massive bodies and 100000 particles
+DEFAULT_CLASSES = (SlotClass(128, 100000, 16), SlotClass(1024, 100000, 8), SlotClass(4096, 100000, 4))
+
+
 class Server:
-    """Admission by memory budget, `lanes` concurrent batches of at most
-    `batch_max` requests, SyMBA (or HELIO) with one dt for all requests."""
+    """Continuous batching over slot pools, SyMBA (or HELIO) with one dt for
+    all requests."""
 
-    def __init__(self, integrator=core.SYMBA, dt=0.005, lanes=4, batch_max=128, hbm_budget=64 << 30,
-                 lclose=False, istep_out=0):
-        self.integrator, self.dt = integrator, dt
-        self.lanes, self.batch_max, self.budget = lanes, batch_max, hbm_budget
-        self.lclose, self.istep_out = lclose, istep_out
-        self.ctx = [core.new_context() for _ in range(lanes)]
+    def __init__(self, classes=DEFAULT_CLASSES, integrator=core.SYMBA, dt=0.005, hbm_budget=64 << 30,
+                 lclose=False, chunk=8, pair_mode=core.PAIR_EXACT):
+        self.classes = list(classes)
+        self.integrator, self.dt, self.lclose, self.chunk = integrator, dt, lclose, max(1, int(chunk))
+        self.pair_mode = pair_mode
+        self.pool_bytes = [c.slots * request_bytes(c.npl_cap, c.ntp_cap) for c in self.classes]
+        if sum(self.pool_bytes) > hbm_budget:
+            raise ValueError(f"slot pools need {sum(self.pool_bytes)} bytes, over the budget of {hbm_budget}")
+        self.budget = hbm_budget
         self._lock = threading.Condition()
-        self._used = 0
-        self.batches = []
+        self.loads = [0] * len(self.classes)      # requests loaded per pool
+        self.occupancy = [[] for _ in self.classes]  # occupied slots at each chunk
+        self.pools = []                             # (class index, pool) while running, for tests
 
-    # ---- admission ----
-    def _take_batch(self, pending, t0):
-        """Block until a batch can be formed; None when nothing is left."""
-        with self._lock:
-            while True:
-                if not pending:
-                    return None
-                now = time.perf_counter() - t0
-                batch, mem = [], 0
-                for r in list(pending):
-                    if r.arrival > now or len(batch) >= self.batch_max:
-                        break
-                    need = request_bytes(r.system)
-                    if self._used + mem + need > self.budget and (batch or self._used):
-                        break
-                    batch.append(r)
-                    mem += need
-                if batch:
-                    for r in batch:
-                        pending.remove(r)
-                        r.admitted = now
-                    self._used += mem
-                    return batch, mem
-                # wait for the next arrival or for memory to be released
-                wait = max(0.0, pending[0].arrival - now) if pending[0].arrival > now else 0.05
-                self._lock.wait(timeout=min(wait, 0.05) if wait > 0 else 0.05)
+    def class_of(self, system) -> int:
+        for k, c in enumerate(self.classes):
+            if system.npl <= c.npl_cap and system.ntp <= c.ntp_cap:
+                return k
+        raise ValueError(f"no slot class holds a system of {system.npl} + {system.ntp} bodies")
 
-    def _release(self, mem):
-        with self._lock:
-            self._used -= mem
-            self._lock.notify_all()
-
-    # ---- one lane ----
-    def _lane(self, lane, pending, t0, errors):
+    # ---- one pool ----
+    def _lane(self, k, pending, t0, errors):
         try:
-            while True:
-                got = self._take_batch(pending, t0)
-                if got is None:
-                    return
-                batch, mem = got
-                self._run_batch(lane, batch, t0)
-                self._release(mem)
+            c = self.classes[k]
+            ctx = core.new_context()
+            try:
+                pool = core.GpuSystem.slots(self.integrator, [c.npl_cap] * c.slots, [c.ntp_cap] * c.slots, self.dt,
+                                            lclose=self.lclose, ctx=ctx, pair_mode=self.pair_mode)
+                with self._lock:
+                    self.pools.append((k, pool))
+                self._drive(k, c, pool, pending, t0)
+                del pool
+            finally:
+                core.library().swg_finalize(ctx)
         except Exception as e:  # reported by run()
             errors.append(e)
             with self._lock:
                 self._lock.notify_all()
 
-    def _run_batch(self, lane, batch, t0):
-        g = core.GpuSystem(self.integrator, [r.system for r in batch], self.dt, lclose=self.lclose,
-                           istep_out=self.istep_out, ctx=self.ctx[lane])
-        if hasattr(g, "use_graphs"):
-            g.use_graphs(True)
-        remaining = np.array([r.nsteps for r in batch], np.int64)
-        frozen = np.zeros(len(batch), np.int32)
-        while (remaining > 0).any():
-            k = int(remaining[remaining > 0].min())
-            g.run(k)
-            remaining[remaining > 0] -= k
+    def _drive(self, k, c, pool, pending, t0):
+        held = [None] * c.slots            # the request each slot holds
+        remaining = np.zeros(c.slots, np.int64)
+        while True:
             now = time.perf_counter() - t0
-            newly = (remaining == 0) & (frozen == 0)
-            for q in np.flatnonzero(newly):
-                batch[q].done = now
-                batch[q].lane = lane
-            frozen[newly] = 1
-            if (remaining > 0).any():
-                g.set_frozen(frozen)
-        st = g.state()
-        po = to = 0
-        for r in batch:
-            n, m = r.system.npl, r.system.ntp
-            r.state = {
-                "pl_rh": st["pl_rh"][po:po + n].copy(), "pl_vh": st["pl_vh"][po:po + n].copy(),
-                "pl_id": st["pl_id"][po:po + n].copy(), "pl_status": st["pl_status"][po:po + n].copy(),
-                "tp_rh": st["tp_rh"][to:to + m].copy(), "tp_vh": st["tp_vh"][to:to + m].copy(),
-                "tp_status": st["tp_status"][to:to + m].copy()}
-            po += n
-            to += m
-        self.batches.append((lane, len(batch)))
-        del g
+            free = [s for s in range(c.slots) if held[s] is None]
+            with self._lock:
+                take = []
+                while pending and len(take) < len(free) and pending[0].arrival <= now:
+                    take.append(pending.pop(0))
+                empty = not pending
+                nxt = pending[0].arrival if pending else None
+            for s, r in zip(free, take):
+                pool.load(s, r.system)
+                r.admitted = time.perf_counter() - t0
+                r.lane, r.slot = k, s
+                held[s] = r
+                remaining[s] = r.nsteps
+                self.loads[k] += 1
+            busy = [s for s in range(c.slots) if held[s] is not None]
+            if not busy:
+                if empty:
+                    return
+                time.sleep(min(max(nxt - (time.perf_counter() - t0), 0.0), 0.05))
+                continue
+            self.occupancy[k].append(len(busy))
+            n = int(min(self.chunk, remaining[busy].min()))
+            pool.run(n)
+            remaining[busy] -= n
+            done = [s for s in busy if remaining[s] == 0]
+            if done:
+                t = time.perf_counter() - t0
+                for s in done:
+                    r = held[s]
+                    r.state = pool.get_slot(s)
+                    r.done = t
+                    held[s] = None
+                pool.set_frozen(np.array([h is None for h in held], np.int32))
 
     # ---- driver ----
     def run(self, requests):
         """Serve `requests` (arrival times relative to the start); returns the
         wall time.  Each request gets admitted / done times and its state."""
-        pending = sorted(requests, key=lambda r: r.arrival)
+        queues = [[] for _ in self.classes]
+        for r in sorted(requests, key=lambda r: r.arrival):
+            queues[self.class_of(r.system)].append(r)
         errors = []
         t0 = time.perf_counter()
-        threads = [threading.Thread(target=self._lane, args=(k, pending, t0, errors)) for k in range(self.lanes)]
+        threads = [threading.Thread(target=self._lane, args=(k, queues[k], t0, errors))
+                   for k in range(len(self.classes)) if queues[k]]
         for t in threads:
             t.start()
         for t in threads:
@@ -158,9 +163,7 @@ class Server:
         return time.perf_counter() - t0
 
     def close(self):
-        for c in self.ctx:
-            core.library().swg_finalize(c)
-        self.ctx = []
+        self.pools = []
 
 
 def poisson_requests(systems, nsteps, rate, seed):
@@ -172,6 +175,9 @@ def poisson_requests(systems, nsteps, rate, seed):
 
 def summary(requests, wall):
     lat = np.array([r.done - r.arrival for r in requests])
+    wait = np.array([r.admitted - r.arrival for r in requests])
     steps = sum(r.nsteps for r in requests)
     return {"requests": len(requests), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
-            "mean_s": float(lat.mean()), "system_steps_per_s": steps / wall, "wall_s": wall}
+            "mean_s": float(lat.mean()), "admission_wait_p50_s": float(np.percentile(wait, 50)),
+            "admission_wait_p99_s": float(np.percentile(wait, 99)), "system_steps_per_s": steps / wall,
+            "wall_s": wall}
