@@ -352,7 +352,22 @@ def main():
     ap.add_argument("--serve-seq-sample", type=int, default=8)
     ap.add_argument("--no-sc-leg", action="store_true",
                     help="skip the step with the pl-tp encounter check not short-circuited")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 2**27 doubles (1 GiB) device to device three times: the known-size copy "
+                         "that scales the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+        x = torch.arange(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        assert torch.equal(x, y)
+        print(json.dumps({"copy_calibration_bytes_read": x.numel() * 8, "bytes_written": y.numel() * 8,
+                          "copies": 3}), flush=True)
+        return 0
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

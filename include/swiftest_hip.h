@@ -73,6 +73,9 @@ typedef struct swg_params {
     int32_t encounter_check; /* ENCOUNTER_CHECK for pl-pl: 0 TRIANGULAR, 1 SORTSWEEP (same list) */
     int32_t pair_mode;     /* SWG_PAIR_EXACT or SWG_PAIR_FAST (no counterpart in the reference) */
     int32_t reserved;
+    double gmtiny;         /* GMTINY (SyMBA): massive bodies with Gmass < gmtiny interact only with the
+                              others (semi-interacting, swiftest_kick.f90:190-262, encounter_check.f90:
+                              42-109); 0 = every body fully interacting */
 } swg_params;
 
 typedef struct swg_stats {
@@ -150,6 +153,15 @@ int swg_system_stats(swg_system *sys, swg_stats *st);
 /* Level-0 encounter list of the last step: which = 0 pl-pl, 1 pl-tp;
  * 1-based body indices in storage order.  With index1 == NULL only *n is set. */
 int swg_system_enc_list(swg_system *sys, int32_t which, int64_t *n, int32_t *index1, int32_t *index2);
+/* Status of each pair of that list after the last step: 0 ACTIVE, -14
+ * COLLIDED for a pair the collision check flagged (the encounter list's
+ * status component, encounter_module.f90:30, set at collision_check.f90:131
+ * and :238).  With status == NULL only *n is set.  A SyMBA step with CHK_CLOSE
+ * in which a collision is detected is completed with the colliding pairs
+ * flagged (they drop out of the list kicks and checks for the rest of the
+ * step) and swg_system_run / swg_system_step then return SWG_ERR_COLLISION;
+ * resolving it is host work. */
+int swg_system_enc_status(swg_system *sys, int32_t which, int64_t *n, int32_t *status);
 /* Time the dominant kernels: launch the pl-tp and pl-pl acceleration kernels
  * reps times on the current state (no state change) and report the average
  * duration of one launch in ms, measured with HIP events on the system's

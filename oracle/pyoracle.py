@@ -20,7 +20,8 @@ I32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
 class Params(C.Structure):
     _fields_ = [("dt", C.c_double), ("lclose", C.c_int), ("short_circuit", C.c_int),
-                ("istep_out", C.c_int), ("rhill_present", C.c_int), ("nthreads", C.c_int)]
+                ("istep_out", C.c_int), ("rhill_present", C.c_int), ("nthreads", C.c_int),
+                ("gmtiny", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -66,6 +67,8 @@ def load(path):
     pp = C.POINTER(C.POINTER(C.c_int32))
     L.swo_encounter_check_all_plpl.argtypes = [C.c_int, D, D, D, C.c_double, pp, pp, pp]
     L.swo_encounter_check_all_plpl.restype = C.c_int64
+    L.swo_encounter_check_all_plplm.argtypes = [C.c_int, C.c_int, D, D, D, C.c_double, pp, pp, pp]
+    L.swo_encounter_check_all_plplm.restype = C.c_int64
     L.swo_encounter_check_all_pltp.argtypes = [C.c_int, C.c_int, D, D, D, D, D, C.c_double, pp, pp, pp]
     L.swo_encounter_check_all_pltp.restype = C.c_int64
     L.swo_free.argtypes = [vp]
@@ -91,6 +94,10 @@ def load(path):
     L.swo_sys_ntp.argtypes = [vp]
     L.swo_sys_list.argtypes = [vp, C.c_int, vp, vp, vp]
     L.swo_sys_list.restype = C.c_int64
+    L.swo_sys_list_status.argtypes = [vp, C.c_int, vp]
+    L.swo_sys_list_status.restype = C.c_int64
+    L.swo_sys_set_collision_mode.argtypes = [vp, C.c_int]
+    L.swo_sys_set_collision_mode.restype = None
     L.swo_norm2.argtypes = [C.c_double] * 3
     L.swo_norm2.restype = C.c_double
     L.swo_coord_vh2vb.argtypes = [C.c_int, C.c_double, D, D, D, D]
@@ -182,6 +189,13 @@ def encounters_plpl(r, v, renc, dt, omp=False):
     return _take_list(n, *p)
 
 
+def encounters_plplm(r, v, renc, nplm, dt, omp=False):
+    p = [C.POINTER(C.c_int32)() for _ in range(3)]
+    n = lib(omp).swo_encounter_check_all_plplm(len(renc), int(nplm), _c(r), _c(v), _c(renc), dt,
+                                               *[C.byref(q) for q in p])
+    return _take_list(n, *p)
+
+
 def encounters_pltp(rpl, vpl, rtp, vtp, renc, dt, omp=False):
     p = [C.POINTER(C.c_int32)() for _ in range(3)]
     n = lib(omp).swo_encounter_check_all_pltp(len(renc), _c(rtp).shape[0], _c(rpl), _c(vpl), _c(rtp), _c(vtp),
@@ -227,15 +241,19 @@ class OracleSystem:
     """A system integrated by the CPU restatement of the reference step layer."""
 
     def __init__(self, integrator, sysic, dt, lclose=True, short_circuit=True, istep_out=0,
-                 rhill=None, nthreads=1, omp=False, L=None):
+                 rhill=None, nthreads=1, omp=False, L=None, gmtiny=0.0, collision_finish_step=False):
         self.L = L if L is not None else lib(omp)
-        prm = Params(dt, int(lclose), int(short_circuit), int(istep_out), int(rhill is not None), int(nthreads))
+        prm = Params(dt, int(lclose), int(short_circuit), int(istep_out), int(rhill is not None), int(nthreads),
+                     float(gmtiny))
         self.npl, self.ntp = sysic.npl, sysic.ntp
         rh_in = None if rhill is None else _c(rhill)
         self._keep = rh_in
         self.h = self.L.swo_sys_new(integrator, self.npl, self.ntp, sysic.GMcb, sysic.Rcb, _c(sysic.pl_rh),
                                     _c(sysic.pl_vh), _c(sysic.Gmass), _c(sysic.radius), _p(rh_in),
                                     _c(sysic.tp_rh).reshape(-1, 3), _c(sysic.tp_vh).reshape(-1, 3), C.byref(prm))
+        # a detected collision: stop at once (default), or flag the pairs and
+        # finish the step (swo_sys_set_collision_mode)
+        self.L.swo_sys_set_collision_mode(self.h, int(collision_finish_step))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -289,6 +307,13 @@ class OracleSystem:
         a, b, c = (np.zeros(n, np.int32) for _ in range(3))
         self.L.swo_sys_list(self.h, which, _p(a), _p(b), _p(c))
         return a, b, c
+
+    def enc_status(self, which=0):
+        """Status of each pair of the level-0 list (0 ACTIVE, -14 COLLIDED)."""
+        n = self.L.swo_sys_list_status(self.h, which, None)
+        st = np.zeros(n, np.int32)
+        self.L.swo_sys_list_status(self.h, which, _p(st))
+        return st
 
 
 # ---- energy and angular momentum ---------------------------------------

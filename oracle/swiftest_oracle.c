@@ -590,9 +590,23 @@ static int64_t collapse(rlist *rows, int nrow, int32_t **index1, int32_t **index
 int64_t swo_encounter_check_all_plpl(int npl, const double *r, const double *v, const double *renc,
                                      double dt, int32_t **index1, int32_t **index2, int32_t **lvdotr)
 {
+    return swo_encounter_check_all_plplm(npl, npl, r, v, renc, dt, index1, index2, lvdotr);
+}
+
+/* encounter_check_all_plplm (:42-109) with both groups triangular
+ * (triangular_plpl :436-475 over the first nplm bodies, triangular_plplm
+ * :478-522 over those against the rest).  The reference concatenates the two
+ * lists and re-sorts them by index1 with its quicksort (base_module.f90:
+ * 1603-1700), which leaves pairs of equal index1 in an order set by the
+ * partitioning; here they are listed index2 ascending (the order the single
+ * triangular sweep gives, and the one the device list has).  With nplm = npl
+ * this is encounter_check_all_triangular_plpl. */
+int64_t swo_encounter_check_all_plplm(int npl, int nplm, const double *r, const double *v, const double *renc,
+                                      double dt, int32_t **index1, int32_t **index2, int32_t **lvdotr)
+{
     rlist *rows = calloc(npl > 0 ? npl : 1, sizeof(rlist));
 #pragma omp parallel for schedule(dynamic, 16)
-    for (int i = 0; i < npl; i++) {
+    for (int i = 0; i < nplm; i++) {
         for (int j = i + 1; j < npl; j++) {
             double xr = r[3 * j] - r[3 * i], yr = r[3 * j + 1] - r[3 * i + 1], zr = r[3 * j + 2] - r[3 * i + 2];
             double vxr = v[3 * j] - v[3 * i], vyr = v[3 * j + 1] - v[3 * i + 1], vzr = v[3 * j + 2] - v[3 * i + 2];
@@ -755,6 +769,7 @@ typedef struct {
 
 struct swo_sys {
     int integ, npl, ntp;
+    int nplm; /* fully interacting massive bodies (the first nplm after the mass sort) */
     double GMcb, Rcb;
     double cb_vb[3], ptbeg[3], ptend[3];
     double *rh, *vh, *vb, *ah, *rbeg, *rend, *xj, *vj;
@@ -766,6 +781,7 @@ struct swo_sys {
     int lfirst, tlfirst, lfirstkick;
     int irec;
     int err;
+    int col_finish, col_seen; /* swo_sys_set_collision_mode; a collision flagged in this step */
     /* pericentre-passage state of the discard checks */
     int32_t *pisperi, *tisperi;
     double *patp, *pperi, *tatp, *tperi;
@@ -896,7 +912,7 @@ static void lindrift_tp(swo_sys *s, double dt, int lbeg)
 static void getacch_pl_helio(swo_sys *s)
 {
     const double *rad = (s->integ == SWO_SYMBA || s->p.lclose) ? s->radius : NULL;
-    swo_kick_getacch_int_all_tri(s->npl, s->npl, s->rh, s->Gmass, rad, s->ah);
+    swo_kick_getacch_int_all_tri(s->npl, s->nplm, s->rh, s->Gmass, rad, s->ah);
     if (s->integ == SWO_SYMBA && s->plpl.n > 0) {
         double *ah_enc = dalloc(3 * (int64_t)s->npl);
         int32_t *k = ialloc(2 * s->plpl.n);
@@ -1052,7 +1068,8 @@ static int symba_check_pl(swo_sys *s, double dt, int irec)
     if (s->npl == 0) return 0;
     set_renc(s, irec);
     int32_t *i1, *i2, *lv;
-    int64_t n = swo_encounter_check_all_plpl(s->npl, s->rh, s->vb, s->renc, dt, &i1, &i2, &lv);
+    /* symba_encounter_check.f90:44-50: plplm when some bodies are below GMTINY */
+    int64_t n = swo_encounter_check_all_plplm(s->npl, s->nplm, s->rh, s->vb, s->renc, dt, &i1, &i2, &lv);
     list_resize(&s->plpl, n);
     for (int64_t k = 0; k < n; k++) {
         s->plpl.i1[k] = i1[k];
@@ -1427,7 +1444,10 @@ static int recur(swo_sys *s, int ireci)
             kick_list_plpl(s, dth, irecp, -1);
             kick_list_pltp(s, dth, irecp, -1);
         }
-        if (s->p.lclose && collision_check(s, dtl, ireci)) return -1;
+        if (s->p.lclose && collision_check(s, dtl, ireci)) {
+            if (!s->col_finish) return -1;
+            s->col_seen = 1;
+        }
         set_recur_levels(s, ireci);
     }
     return 0;
@@ -1478,6 +1498,7 @@ static int symba_step(swo_sys *s)
     int lenc = symba_check_pl(s, dt, 0);
     if (!lenc || !s->p.short_circuit) lenc = symba_check_tp(s, dt, 0) || lenc;
     int rc = 0;
+    s->col_seen = 0;
     if (lenc) {
         rc = symba_interp(s);
     } else {
@@ -1485,6 +1506,7 @@ static int symba_step(swo_sys *s)
         helio_step(s);
     }
     s->lfirstkick = s->lfirst;
+    if (!rc && s->col_seen) rc = -1;
     return rc;
 }
 
@@ -1795,6 +1817,7 @@ swo_sys *swo_sys_new(int integrator, int npl, int ntp, double GMcb, double Rcb, 
     else
         set_rhill(s);
     s->lfirst = s->tlfirst = s->lfirstkick = 1;
+    s->nplm = npl;
     if (integrator == SWO_WHM) {
         /* whm_util_setup_initialize_system (whm_util.f90:201-228): sort by
          * ir3h descending, then eta / muj (whm_util_set_mu_eta_pl :175-198) */
@@ -1821,6 +1844,13 @@ swo_sys *swo_sys_new(int integrator, int npl, int ntp, double GMcb, double Rcb, 
         memcpy(key, s->Gmass, sizeof(double) * npl);
         sort_pl(s, key);
         free(key);
+        /* symba_util_flatten_eucl_plpl (symba_util.f90:191-202), lmtiny_pl
+         * for SyMBA (swiftest_io.f90:2663): lmtiny = Gmass < GMTINY */
+        s->nplm = npl;
+        if (integrator == SWO_SYMBA && prm->gmtiny > 0.0) {
+            s->nplm = 0;
+            for (int i = 0; i < npl; i++) s->nplm += !(s->Gmass[i] < prm->gmtiny);
+        }
         vh2vb_pl(s);
         for (int i = 0; i < ntp; i++)
             for (int d = 0; d < 3; d++) s->tvb[3 * i + d] = s->tvh[3 * i + d] + s->cb_vb[d];
@@ -1936,6 +1966,15 @@ int64_t swo_sys_list(const swo_sys *s, int which, int32_t *index1, int32_t *inde
     if (lvdotr) memcpy(lvdotr, l->lvdotr, l->n * sizeof(int32_t));
     return l->n;
 }
+
+int64_t swo_sys_list_status(const swo_sys *s, int which, int32_t *status)
+{
+    const enclist *l = which == 0 ? &s->plpl : &s->pltp;
+    if (status) memcpy(status, l->status, l->n * sizeof(int32_t));
+    return l->n;
+}
+
+void swo_sys_set_collision_mode(swo_sys *s, int finish_step) { s->col_finish = finish_step != 0; }
 
 /* ======================================================================
  * Energy and angular momentum

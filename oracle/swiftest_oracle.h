@@ -59,6 +59,11 @@ void swo_drift_kepu_stumpff(double x, double *c0, double *c1, double *c2, double
 void swo_encounter_check_one(double xr, double yr, double zr, double vxr, double vyr, double vzr,
                              double renc, double dt, int32_t *lencounter, int32_t *lvdotr);
 /* Triangular pair lists.  Outputs are malloc'd (free with swo_free); indices 1-based. */
+/* encounter_check_all_plplm, TRIANGULAR (encounter_check.f90:42-109): the
+ * pairs of the first nplm (fully interacting) bodies among themselves and
+ * with the other npl - nplm, listed index1 ascending, then index2 ascending. */
+int64_t swo_encounter_check_all_plplm(int npl, int nplm, const double *r, const double *v, const double *renc,
+                                      double dt, int32_t **index1, int32_t **index2, int32_t **lvdotr);
 int64_t swo_encounter_check_all_plpl(int npl, const double *r, const double *v, const double *renc,
                                      double dt, int32_t **index1, int32_t **index2, int32_t **lvdotr);
 int64_t swo_encounter_check_all_pltp(int npl, int ntp, const double *rpl, const double *vpl,
@@ -89,6 +94,7 @@ typedef struct swo_params {
     int istep_out;       /* snapshot cadence; 0 = never */
     int rhill_present;   /* RHILL_PRESENT: keep input rhill, never refresh */
     int nthreads;        /* OpenMP threads for the pair loops (1 = serial) */
+    double gmtiny;       /* GMTINY (SyMBA): bodies with Gmass < gmtiny interact only with the others */
 } swo_params;
 
 typedef struct swo_stats {
@@ -130,6 +136,16 @@ int swo_sys_npl(const swo_sys *s);
 int swo_sys_ntp(const swo_sys *s);
 /* Encounter lists produced by the last step's level-0 checks (1-based). */
 int64_t swo_sys_list(const swo_sys *s, int which, int32_t *index1, int32_t *index2, int32_t *lvdotr);
+/* Status of each pair of those lists (0 ACTIVE, -14 COLLIDED), list order. */
+int64_t swo_sys_list_status(const swo_sys *s, int which, int32_t *status);
+/* What a detected collision does.  0 (default): the step stops at the
+ * collision check that found it and swo_sys_run returns -1 at once.  1: the
+ * colliding pairs are flagged COLLIDED, which takes them out of the list
+ * kicks and list checks for the rest of the step (symba_kick.f90:169/281,
+ * symba_encounter_check.f90:112/187), the step is finished, and -1 is
+ * returned at its end: the state a caller gets when the collision is not
+ * resolved within the step. */
+void swo_sys_set_collision_mode(swo_sys *s, int finish_step);
 
 /* ---- energy and angular momentum (output cadence) ---- */
 /* Fields of the energy record, as swiftest_nbody_system stores them. */

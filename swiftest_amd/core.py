@@ -45,7 +45,8 @@ class DiscardParams(C.Structure):
 class Params(C.Structure):
     _fields_ = [("dt", C.c_double), ("integrator", C.c_int32), ("lclose", C.c_int32),
                 ("short_circuit", C.c_int32), ("istep_out", C.c_int32), ("rhill_present", C.c_int32),
-                ("encounter_check", C.c_int32), ("pair_mode", C.c_int32), ("reserved", C.c_int32)]
+                ("encounter_check", C.c_int32), ("pair_mode", C.c_int32), ("reserved", C.c_int32),
+                ("gmtiny", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -88,6 +89,7 @@ def library():
         L.swg_system_get_misc.argtypes = [vp] * 6
         L.swg_system_stats.argtypes = [vp, C.POINTER(Stats)]
         L.swg_system_enc_list.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64), vp, vp]
+        L.swg_system_enc_status.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64), vp]
         L.swg_system_time_kernels.argtypes = [vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.swg_system_kernel_timing.argtypes = [vp, C.c_int32]
         L.swg_system_kernel_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64),
@@ -215,7 +217,7 @@ class GpuSystem:
     """
 
     def __init__(self, integrator, systems, dt, lclose=True, short_circuit=True, istep_out=0, rhill=None, ctx=None,
-                 encounter_check=0, pair_mode=PAIR_EXACT):
+                 encounter_check=0, pair_mode=PAIR_EXACT, gmtiny=0.0):
         if not isinstance(systems, (list, tuple)):
             systems = [systems]
         self.L = library()
@@ -226,7 +228,7 @@ class GpuSystem:
         cat = lambda key, w: _c(np.concatenate([getattr(s, key).reshape(-1, w) for s in systems])) \
             if systems else np.zeros((0, w))
         prm = Params(float(dt), int(integrator), int(lclose), int(short_circuit), int(istep_out),
-                     int(rhill is not None), int(encounter_check), int(pair_mode))
+                     int(rhill is not None), int(encounter_check), int(pair_mode), 0, float(gmtiny))
         rh = None if rhill is None else _c(rhill)
         h = vp()
         _check(self.L.swg_system_create(ctx if ctx is not None else context(), C.byref(prm), self.nsys,
@@ -241,7 +243,7 @@ class GpuSystem:
 
     @classmethod
     def slots(cls, integrator, npl_cap, ntp_cap, dt, lclose=True, short_circuit=True, ctx=None,
-              encounter_check=0, pair_mode=PAIR_EXACT):
+              encounter_check=0, pair_mode=PAIR_EXACT, gmtiny=0.0):
         """A slot pool (swg_system_create_slots): len(npl_cap) empty slots of
         the given capacities, all frozen until a system is loaded."""
         self = cls.__new__(cls)
@@ -251,7 +253,7 @@ class GpuSystem:
         self.ntp_s = _c(ntp_cap, np.int32)
         self.npl, self.ntp = int(self.npl_s.sum()), int(self.ntp_s.sum())
         prm = Params(float(dt), int(integrator), int(lclose), int(short_circuit), 0, 0, int(encounter_check),
-                     int(pair_mode))
+                     int(pair_mode), 0, float(gmtiny))
         h = vp()
         _check(self.L.swg_system_create_slots(ctx if ctx is not None else context(), C.byref(prm), self.nsys,
                                               self.npl_s, self.ntp_s, C.byref(h)))
@@ -318,6 +320,15 @@ class GpuSystem:
         a, b = np.zeros(max(n.value, 1), np.int32), np.zeros(max(n.value, 1), np.int32)
         _check(self.L.swg_system_enc_list(self.h, which, C.byref(n), _p(a), _p(b)))
         return a[:n.value], b[:n.value]
+
+    def enc_status(self, which=0):
+        """Status of each pair of the level-0 list of the last step (0 ACTIVE,
+        -14 COLLIDED), in list order (swg_system_enc_status)."""
+        n = C.c_int64()
+        _check(self.L.swg_system_enc_status(self.h, which, C.byref(n), None))
+        st = np.zeros(max(n.value, 1), np.int32)
+        _check(self.L.swg_system_enc_status(self.h, which, C.byref(n), _p(st)))
+        return st[:n.value]
 
     def accel(self):
         """(pl_acc, tp_acc): the direct-sum accelerations at the current

@@ -180,7 +180,14 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
     if (ns <= 0) return fail(SWG_ERR_ARG, "nsys must be positive");
     if (p.pair_mode != SWG_PAIR_EXACT && p.pair_mode != SWG_PAIR_FAST)
         return fail(SWG_ERR_ARG, "pair_mode must be SWG_PAIR_EXACT or SWG_PAIR_FAST");
+    if (!(p.gmtiny >= 0.0)) return fail(SWG_ERR_ARG, "gmtiny must be >= 0");
     tune.pair_fast = p.pair_mode == SWG_PAIR_FAST;
+    if (const char *e = getenv("SWG_KICK_PL_EVAL")) {
+        // exact-mode massive-body kernel before any autotuning: 0 one lane
+        // per body, 2 / 4 / 8 evaluating wavefronts per 64-body tile
+        const int k = atoi(e);
+        if (k == 0 || k == 2 || k == 4 || k == 8) tune.kick_pl_eval = k;
+    }
     nsys = ns;
     h_pl_off.assign(nsys + 1, 0);
     h_tp_off.assign(nsys + 1, 0);
@@ -250,7 +257,8 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
     HIPCHK(hipHostMalloc((void **)&h_small, 4096 + 16 * sizeof(int32_t) * (size_t)nsys));
     if (alloc_vec(pl.rh, npl) || alloc_vec(pl.vh, npl) || alloc_vec(pl.vb, npl) || alloc_vec(pl.rbeg, npl) ||
         alloc_vec(pl.rend, npl) || own(&pl.Gm, npl) || own(&pl.R, npl) || own(&pl.rhill, npl) || own(&pl.renc, npl) ||
-        own(&pl.status, npl) || own(&pl.levelg, npl) || own(&pl.levelm, npl) || own(&pl.sys, npl))
+        own(&pl.status, npl) || own(&pl.levelg, npl) || own(&pl.levelm, npl) || own(&pl.sys, npl) ||
+        own(&pl.tiny, npl))
         return SWG_ERR_HIP;
     if (alloc_vec(tp.rh, ntp) || alloc_vec(tp.vh, ntp) || alloc_vec(tp.vb, ntp) || own(&tp.status, ntp) ||
         own(&tp.levelg, ntp) || own(&tp.levelm, ntp) || own(&tp.sys, ntp))
@@ -258,7 +266,8 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
     if (own(&sy.pl_off, nsys + 1) || own(&sy.tp_off, nsys + 1) || own(&sy.GMcb, nsys) || own(&sy.cbvb, 3 * nsys) ||
         own(&sy.ptbeg, 3 * nsys) || own(&sy.ptend, 3 * nsys) || own(&sy.lfirst, nsys) || own(&sy.enc_pp, nsys) ||
         own(&sy.enc_pt, nsys) || own(&sy.flag_pp, nsys) || own(&sy.flag_pt, nsys) || own(&sy.err, nsys) ||
-        own(&sy.frozen, nsys) || own(&d_active, nsys) || own(&d_any, 4) || own(&d_total, 4) || own(&d_pt_run, nsys))
+        own(&sy.frozen, nsys) || own(&d_active, nsys) || own(&d_any, 4) || own(&d_total, 4) || own(&d_pt_run, nsys) ||
+        own(&sy.nplm, nsys))
         return SWG_ERR_HIP;
     hipStream_t st = stream;
     std::vector<double> tmp;
@@ -288,6 +297,27 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
     h_frozen.assign(nsys, 0);
     for (int32_t *q : {sy.enc_pp, sy.enc_pt, sy.flag_pp, sy.flag_pt, sy.err, sy.frozen})
         HIPCHK(copy_wait(stream, q, zeros.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice));
+    // semi-interacting bodies (symba_util.f90:191-202 with lmtiny_pl for
+    // SyMBA, swiftest_io.f90:2663): after the mass sort the fully interacting
+    // bodies of a system come first, nplm of them
+    {
+        std::vector<int32_t> tiny(std::max(npl, 1), 0), nplm(nsys);
+        for (int s = 0; s < nsys; s++) {
+            int a = h_pl_off[s], n = h_pl_off[s + 1] - a, m = n;
+            if (prm.integrator == SWG_SYMBA && prm.gmtiny > 0.0) {
+                m = 0;
+                for (int q = 0; q < n; q++) {
+                    tiny[a + q] = h_Gm[a + q] < prm.gmtiny;
+                    m += !tiny[a + q];
+                }
+            }
+            nplm[s] = m;
+            any_tiny |= m < n;
+        }
+        HIPCHK(copy_wait(stream, pl.tiny, tiny.data(), npl * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHK(copy_wait(stream, sy.nplm, nplm.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice));
+        tune.any_tiny = any_tiny;
+    }
 
     // ---- tiles ----
     if (int rc = build_tiles()) return rc;
@@ -375,7 +405,7 @@ int Ensemble::build_tiles()
 {
     // kick tiles: blocks of tuning.kick_*_block bodies of one system
     std::vector<Tile> tpl, ttp;
-    const int pl_tile = tune.pair_fast ? KPL_FAST_TILE : tune.kick_pl_block;
+    const int pl_tile = (tune.pair_fast || tune.kick_pl_eval) ? KPL_FAST_TILE : tune.kick_pl_block;
     for (int s = 0; s < nsys; s++) {
         if (h_frozen[s]) continue;
         // rows of the bodies a system holds; a slot's padding (inactive, at
@@ -394,12 +424,18 @@ int Ensemble::build_tiles()
     }
     // A tile array is published (its count set) only once it holds the
     // tiles: after a failed allocation or copy the count stays 0, so no
-    // kernel is launched over a missing array, and the error is returned.
+    // kernel is launched over a missing array, the error is returned, and
+    // steps refuse to run (tiles_ok) until a later rebuild succeeds.
+    // SWG_FAIL_TILE_ALLOC=1 makes the allocation fail, for tests of this path.
+    tiles_ok = false;
+    const char *inject = getenv("SWG_FAIL_TILE_ALLOC");
     auto put = [&](auto &vec, auto **dptr, int &cnt) -> int {
         using T = typename std::remove_reference<decltype(vec)>::type::value_type;
         cnt = 0;
         if (*dptr) (void)hipFree(*dptr);
         *dptr = nullptr;
+        if (inject && inject[0] == '1')
+            return fail(SWG_ERR_HIP, "hipMalloc (tile arrays): out of memory (injected by SWG_FAIL_TILE_ALLOC)");
         HIPCHK(hipMalloc((void **)dptr, sizeof(T) * (vec.size() ? vec.size() : 1)));
         if (!vec.empty()) HIPCHK(copy_wait(stream, *dptr, vec.data(), sizeof(T) * vec.size(), hipMemcpyHostToDevice));
         cnt = (int)vec.size();
@@ -413,6 +449,7 @@ int Ensemble::build_tiles()
         for (int s = 0; s < nsys; s++) total += (h_tp_off[s + 1] - h_tp_off[s] + 63) / 64;
         HIPCHK(hipMalloc((void **)&d_tiles_pt, sizeof(int4) * (total ? total : 1)));
     }
+    tiles_ok = true;
     return 0;
 }
 
@@ -900,7 +937,7 @@ uint64_t Ensemble::graph_key() const
                           pt.colstart, pt.status, d_arena,   (const void *)(intptr_t)ntiles_pl,
                           (const void *)(intptr_t)ntiles_tp, (const void *)(intptr_t)tune.kick_pl_block,
                           (const void *)(intptr_t)tune.kick_tp_block, (const void *)(intptr_t)tune.pair_fast,
-                          (const void *)(intptr_t)tune.kick_pl_split};
+                          (const void *)(intptr_t)tune.kick_pl_split, (const void *)(intptr_t)tune.kick_pl_eval};
     uint64_t h = 1469598103934665603ull;
     for (const void *p : ptrs) {
         h ^= (uint64_t)(uintptr_t)p;
@@ -1060,6 +1097,9 @@ int Ensemble::step_helio_symba()
     if (int rc = run_phase(1, second_half)) return rc;
     trace_mark(st, "second half");
     HIPCHK(hipGetLastError());
+    // a collision flagged in this step's recursion ends the run at the end of
+    // the step, with the colliding pairs flagged in the lists
+    if (any && prm.lclose) return check_errors();
     return 0;
 }
 
@@ -1085,10 +1125,11 @@ int Ensemble::snapshot()
 
 int Ensemble::step()
 {
+    if (!tiles_ok) return fail(SWG_ERR_HIP, "the tile arrays are missing after a failed allocation; rebuild them first");
     int rc = prm.integrator == SWG_WHM ? step_whm() : step_helio_symba();
-    if (rc) return rc;
-    iloop++;
-    return 0;
+    if (rc && rc != SWG_ERR_COLLISION) return rc;
+    iloop++;  // a step that found a collision was completed
+    return rc;
 }
 
 int Ensemble::run(int64_t nsteps)
@@ -1155,6 +1196,19 @@ int Ensemble::enc_list(int which, int64_t *n, int32_t *i1, int32_t *i2)
     return 0;
 }
 
+int Ensemble::enc_status(int which, int64_t *n, int32_t *status)
+{
+    ListDev &L = which == 0 ? pp : pt;
+    int64_t m = which == 0 ? npp : npt;
+    *n = m;
+    if (!status || !m) return 0;
+    std::vector<int8_t> h(m);
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(copy_wait(stream, h.data(), L.status, m, hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < m; k++) status[k] = h[k];
+    return 0;
+}
+
 int Ensemble::record_kernel_event(std::vector<hipEvent_t> &pool, size_t &n)
 {
     if (n == pool.size()) {
@@ -1191,6 +1245,7 @@ int Ensemble::kernel_times(double *ms_tp, int64_t *n_tp, double *ms_pl, int64_t 
 
 int Ensemble::time_kernels(int reps, double *ms_tp, double *ms_pl)
 {
+    if (!tiles_ok) return fail(SWG_ERR_HIP, "the tile arrays are missing after a failed allocation; rebuild them first");
     // Launch the two acceleration kernels on copies of the velocity state so
     // that the timing runs leave the system unchanged.
     hipStream_t st = stream;
@@ -1372,6 +1427,25 @@ int Ensemble::load(int s, int npl_new, int ntp_new, double GMcb_new, const doubl
         }
         HIPCHK(copy_wait(st, dis_lfirst + s, &one, sizeof one, hipMemcpyHostToDevice));
     }
+    {
+        std::vector<int32_t> tiny(std::max(cap_pl, 1), 0);
+        int32_t m = npl_new;
+        if (prm.integrator == SWG_SYMBA && prm.gmtiny > 0.0) {
+            m = 0;
+            for (int q = 0; q < npl_new; q++) {
+                tiny[q] = gm[q] < prm.gmtiny;
+                m += !tiny[q];
+            }
+        }
+        if (cap_pl) HIPCHK(copy_wait(st, pl.tiny + p0, tiny.data(), cap_pl * sizeof(int32_t), hipMemcpyHostToDevice));
+        // the padding sits after the system's own bodies: as partners of
+        // the fully interacting rows they add exact zeros
+        HIPCHK(copy_wait(st, sy.nplm + s, &m, sizeof m, hipMemcpyHostToDevice));
+        if (m < npl_new) {
+            any_tiny = true;
+            tune.any_tiny = 1;
+        }
+    }
     // device part of the initialize sequence for this system alone
     launch_init_one(st, pl, tp, sy, s);
     HIPCHK(hipGetLastError());
@@ -1412,6 +1486,7 @@ int Ensemble::get_slot(int s, double *pl_rh, double *pl_vh, double *pl_vb, doubl
 // read back and restored.
 int Ensemble::accel(double *pl_acc, double *tp_acc)
 {
+    if (!tiles_ok) return fail(SWG_ERR_HIP, "the tile arrays are missing after a failed allocation; rebuild them first");
     hipStream_t st = stream;
     if (!save_pl.x && (alloc_vec(save_pl, npl) || alloc_vec(save_tp, ntp))) return SWG_ERR_HIP;
     const Vec3p vs[2] = {pl.vb, tp.vb}, sv[2] = {save_pl, save_tp};
@@ -1521,27 +1596,38 @@ int Ensemble::autotune(int reps, int32_t *pl_block, int32_t *tp_block)
         if (tp_block) *tp_block = btp;
         return 0;
     }
+    // exact mode: the pl-tp tile size, and for the massive bodies one lane
+    // per body with 64, 128 or 256-body tiles or 64-body tiles shared by
+    // 2, 4 or 8 evaluating wavefronts (reported as -2, -4, -8)
+    const int evals[3] = {2, 4, 8};
     double best_tp = 1e300, best_pl = 1e300;
-    int btp = tune.kick_tp_block, bpl = tune.kick_pl_block;
-    for (int c : cand) {
-        tune.kick_tp_block = c;
-        tune.kick_pl_block = c;
+    int btp = tune.kick_tp_block, bpl = tune.kick_pl_block, bev = tune.kick_pl_eval;
+    for (int k = 0; k < 6; k++) {
+        if (k < 3) {
+            tune.kick_tp_block = cand[k];
+            tune.kick_pl_block = cand[k];
+            tune.kick_pl_eval = 0;
+        } else {
+            tune.kick_pl_eval = evals[k - 3];
+        }
         if (int rc = build_tiles()) return rc;
         double tms = 0.0, pms = 0.0;
-        if (int rc = time_kernels(reps, ntp ? &tms : nullptr, npl ? &pms : nullptr)) return rc;
-        if (ntp && tms < best_tp) {
+        if (int rc = time_kernels(reps, (ntp && k < 3) ? &tms : nullptr, npl ? &pms : nullptr)) return rc;
+        if (ntp && k < 3 && tms < best_tp) {
             best_tp = tms;
-            btp = c;
+            btp = cand[k];
         }
         if (npl && pms < best_pl) {
             best_pl = pms;
-            bpl = c;
+            bpl = tune.kick_pl_block;
+            bev = tune.kick_pl_eval;
         }
     }
     tune.kick_tp_block = btp;
     tune.kick_pl_block = bpl;
+    tune.kick_pl_eval = bev;
     if (int rc = build_tiles()) return rc;
-    if (pl_block) *pl_block = bpl;
+    if (pl_block) *pl_block = bev ? -bev : bpl;
     if (tp_block) *tp_block = btp;
     return 0;
 }
@@ -1656,6 +1742,12 @@ int swg_system_enc_list(swg_system *sys, int32_t which, int64_t *n, int32_t *ind
 {
     if (!sys || !n || (which != 0 && which != 1)) return fail(SWG_ERR_ARG, "bad argument");
     return sys->e->enc_list(which, n, index1, index2);
+}
+
+int swg_system_enc_status(swg_system *sys, int32_t which, int64_t *n, int32_t *status)
+{
+    if (!sys || !n || (which != 0 && which != 1)) return fail(SWG_ERR_ARG, "bad argument");
+    return sys->e->enc_status(which, n, status);
 }
 
 int swg_system_time_kernels(swg_system *sys, int32_t reps, double *ms_kick_tp, double *ms_kick_pl)

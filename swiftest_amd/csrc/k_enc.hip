@@ -29,6 +29,10 @@ __global__ __launch_bounds__(64) void k_pp_tiles(PlDev pl, SysDev sy, ListDev pp
     const int4 t = tiles[blockIdx.x];
     const int s = t.x, ib = t.y, jb = t.z;
     const int p0 = sy.pl_off[s], n = sy.pl_off[s + 1] - p0, nb = pp.nb[s];
+    // with semi-interacting bodies only pairs whose index1 is among the
+    // first nplm are checked (encounter_check_all_plplm,
+    // encounter_check.f90:42-109)
+    const int nm = sy.nplm ? sy.nplm[s] : n;
     const int lane = threadIdx.x;
     const int il0 = ib * 64 + lane;
     if (il0 < n) {
@@ -60,7 +64,7 @@ __global__ __launch_bounds__(64) void k_pp_tiles(PlDev pl, SysDev sy, ListDev pp
     for (int q = 0; q < imax; q++) {
         const int il = ib * 64 + q;
         bool f = false;
-        if (vj && jl > il) {
+        if (vj && jl > il && il < nm) {
             bool lv;
             f = encounter_one(xj - si[0][q], yj - si[1][q], zj - si[2][q], vxj - si[3][q], vyj - si[4][q],
                               vzj - si[5][q], si[6][q] + rj, dt, lv);

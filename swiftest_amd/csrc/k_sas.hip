@@ -86,7 +86,8 @@ __global__ void k_sas_narrow(PlDev pl, const int64_t *coff, const int32_t *rank,
     const int j = sidx[rank[g] + 1 + (int)(c - coff[g])];
     const int a = min(g, j), b = max(g, j);
     bool lv;
-    const bool f = encounter_one(pl.rh.x[b] - pl.rh.x[a], pl.rh.y[b] - pl.rh.y[a], pl.rh.z[b] - pl.rh.z[a],
+    // a semi-interacting index1 (GMTINY): the pair is not checked
+    const bool f = !(pl.tiny && pl.tiny[a]) && encounter_one(pl.rh.x[b] - pl.rh.x[a], pl.rh.y[b] - pl.rh.y[a], pl.rh.z[b] - pl.rh.z[a],
                                  pl.vb.x[b] - pl.vb.x[a], pl.vb.y[b] - pl.vb.y[a], pl.vb.z[b] - pl.vb.z[a],
                                  pl.renc[a] + pl.renc[b], dt, lv);
     key[c] = ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;

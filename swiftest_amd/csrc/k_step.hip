@@ -286,6 +286,287 @@ __global__ __launch_bounds__(BLK) void k_kick_pl(PlDev pl, SysDev sy, ListDev pp
 }
 
 // ---------------------------------------------------------------------------
+// The same massive-body half kick with the row's terms evaluated by KE
+// wavefronts and folded in j order by one more: a tile of 64 bodies per
+// workgroup, the row walked in chunks of KPLS_C partners.  Evaluation
+// wavefront e forms the factors of its KPLS_C / KE partners of a chunk for
+// all 64 bodies (the exact cores, grouped and branch-free as in k_kick_pl)
+// into an LDS matrix, a zero where a term is not taken; the folding
+// wavefront, one lane per body, then adds the chunk's terms in j order,
+// forming each separation again from the same operands.  fma(0, d, s) = s
+// bit for bit (d finite; a sum that starts at +0 is never -0), so every sum
+// is k_kick_pl's bit for bit, while a 32768-body system gets KE + 1
+// wavefronts per 64 bodies instead of one.  The matrix is double-buffered:
+// the fold of chunk c overlaps the evaluation of chunk c + 1.  The two
+// encounter-pair walks run on evaluation wavefronts 0 and 1 side by side.
+// Partners are read at wave-uniform indices through the scalar cache.
+// ---------------------------------------------------------------------------
+
+typedef const __attribute__((address_space(4))) double cdouble;
+__device__ inline const cdouble *cview(const double *p) { return (const cdouble *)p; }
+
+constexpr int KPLS_C = 64;  // partners per chunk
+
+template <int KE>
+__global__ __launch_bounds__(64 * (KE + 1)) void k_kick_pl_split(PlDev pl, SysDev sy, ListDev pp, const Tile *tiles,
+                                                                  double dt, int lbeg, int use_radius, int do_pairs)
+{
+    static_assert(KPLS_C % (4 * KE) == 0, "each evaluation wavefront takes whole groups of four partners");
+    __shared__ double facb[2][KPLS_C][64];
+    __shared__ double hpart[2][3][64];
+    const Tile tl = tiles[blockIdx.x];
+    const int p0 = sy.pl_off[tl.s], p1 = sy.pl_off[tl.s + 1];
+    const int lane = (int)(threadIdx.x & 63u);
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // 0..KE-1 evaluate, KE folds
+    const int i = tl.b0 + lane;
+    const bool valid = i < tl.b1;
+    double xi = 0.0, yi = 0.0, zi = 0.0, Ri = 0.0;
+    if (valid) {
+        xi = pl.rh.x[i];
+        yi = pl.rh.y[i];
+        zi = pl.rh.z[i];
+        Ri = pl.R[i];
+    }
+    const cdouble *X = cview(pl.rh.x), *Y = cview(pl.rh.y), *Z = cview(pl.rh.z), *G = cview(pl.Gm),
+                  *RR = cview(pl.R);
+    const int n = p1 - p0, nch = (n + KPLS_C - 1) / KPLS_C;
+    constexpr int PER = KPLS_C / KE;
+    // evaluation of chunk c into buffer c & 1 (wavefronts 0 .. KE-1)
+    auto evaluate = [&](int c) {
+        const int jt = p0 + c * KPLS_C, jn = min(KPLS_C, p1 - jt);
+        // every mass of the chunk inside the exact cores' range: one column per lane
+        const bool gok_lane = lane >= jn || pair_g_ok(pl.Gm[jt + lane]);
+        const bool tile_gok = __all(gok_lane);
+        double(*fb)[64] = facb[c & 1];
+        for (int q0 = w * PER; q0 < (w + 1) * PER; q0 += 4) {
+            double r2[4], fac[4], rx[4], ry[4], rz[4];
+            bool take[4], need[4];
+            bool any_need = false;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int q = q0 + u;
+                const int j = jt + min(q, jn - 1);  // in range; a column past the end is not taken
+                rx[u] = X[j] - xi;
+                ry[u] = Y[j] - yi;
+                rz[u] = Z[j] - zi;
+                r2[u] = dot3(rx[u], ry[u], rz[u], rx[u], ry[u], rz[u]);
+                const double rs = Ri + RR[j];
+                take[u] = (q < jn) & (jt + q != i) & (!use_radius | (r2[u] > rs * rs));
+                fac[u] = div_core(G[j], r2[u] * sqrt_core(r2[u]));
+                need[u] = take[u] & !(tile_gok & (r2[u] >= PAIR_R2_LO) & (r2[u] <= PAIR_R2_HI));
+                any_need = any_need | need[u];
+            }
+            if (__ballot(any_need)) {
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (need[u]) {
+                        const int j = jt + min(q0 + u, jn - 1);
+                        fac[u] = pl.Gm[j] / (r2[u] * sqrt(r2[u]));
+                    }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) fb[q0 + u][lane] = take[u] ? fac[u] : 0.0;
+        }
+    };
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    if (w < KE && nch > 0) evaluate(0);
+    __syncthreads();
+    for (int c = 0; c < nch; c++) {
+        if (w < KE) {
+            if (c + 1 < nch) evaluate(c + 1);
+        } else {
+            // fold chunk c in j order
+            const int jt = p0 + c * KPLS_C, jn = min(KPLS_C, p1 - jt);
+            const double(*fb)[64] = facb[c & 1];
+            for (int q = 0; q < jn; q++) {
+                const int j = jt + q;
+                const double f = fb[q][lane];
+                ax = fma(f, X[j] - xi, ax);
+                ay = fma(f, Y[j] - yi, ay);
+                az = fma(f, Z[j] - zi, az);
+            }
+        }
+        __syncthreads();
+    }
+    if (do_pairs && w < 2 && w < KE && valid) {
+        // w = 0: ahj over pairs (p, i), p < i; w = 1: ahi over pairs (i, j)
+        double hx = 0.0, hy = 0.0, hz = 0.0;
+        if (w == 0) {
+            const int64_t c0 = pp.colstart[i], c1 = c0 + pp.colcnt[i];
+            for (int64_t c = c0; c < c1; c++) {
+                const int p = pp.i1[pp.col[c]];
+                const double rx = xi - pl.rh.x[p], ry = yi - pl.rh.y[p], rz = zi - pl.rh.z[p];
+                const double rji2 = dot3(rx, ry, rz, rx, ry, rz);
+                const double rs = pl.R[p] + Ri;
+                if (rji2 > rs * rs) {
+                    const double fac = pl.Gm[p] * (1.0 / (rji2 * sqrt(rji2)));
+                    hx = fma(-fac, rx, hx);
+                    hy = fma(-fac, ry, hy);
+                    hz = fma(-fac, rz, hz);
+                }
+            }
+        } else {
+            const int64_t r0 = pp.rowstart[i], r1 = r0 + pp.rowcnt[i];
+            for (int64_t k = r0; k < r1; k++) {
+                const int q = pp.i2[k];
+                const double rx = pl.rh.x[q] - xi, ry = pl.rh.y[q] - yi, rz = pl.rh.z[q] - zi;
+                const double rji2 = dot3(rx, ry, rz, rx, ry, rz);
+                const double rs = Ri + pl.R[q];
+                if (rji2 > rs * rs) {
+                    const double fac = pl.Gm[q] * (1.0 / (rji2 * sqrt(rji2)));
+                    hx = fma(fac, rx, hx);
+                    hy = fma(fac, ry, hy);
+                    hz = fma(fac, rz, hz);
+                }
+            }
+        }
+        hpart[w][0][lane] = hx;
+        hpart[w][1][lane] = hy;
+        hpart[w][2][lane] = hz;
+    }
+    __syncthreads();
+    if (w != KE || !valid) return;
+    if (do_pairs) {
+        const double hjx = hpart[0][0][lane], hjy = hpart[0][1][lane], hjz = hpart[0][2][lane];
+        const double hix = KE > 1 ? hpart[1][0][lane] : 0.0, hiy = KE > 1 ? hpart[1][1][lane] : 0.0,
+                     hiz = KE > 1 ? hpart[1][2][lane] : 0.0;
+        ax = ax - ((0.0 + hix) + hjx);
+        ay = ay - ((0.0 + hiy) + hjy);
+        az = az - ((0.0 + hiz) + hjz);
+    }
+    Vec3p rb = lbeg ? pl.rbeg : pl.rend;
+    rb.x[i] = xi;
+    rb.y[i] = yi;
+    rb.z[i] = zi;
+    if (pl_active(pl, i)) {
+        pl.vb.x[i] = fma(ax, dt, pl.vb.x[i]);
+        pl.vb.y[i] = fma(ay, dt, pl.vb.y[i]);
+        pl.vb.z[i] = fma(az, dt, pl.vb.z[i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The massive-body half kick of an ensemble with semi-interacting bodies
+// (GMTINY): in system s the first nplm bodies interact with every body, the
+// others only with those nplm (swiftest_kick.f90:165-271 via
+// symba_kick.f90:14-33).  One lane per body; partners at wave-uniform
+// indices through the scalar cache.
+//  - nplt <= nplm: full rows j /= i for the fully interacting bodies
+//    (:219-240), rows over j < nplm for the others (:243-262), with the
+//    factor Gm_j / (r2 sqrt r2);
+//  - nplt > nplm: the triangular loop over pairs i < nplm, i < j (:190-216)
+//    accumulating ahi(i) += (Gm_j irij3) r and ahj(j) -= (Gm_i irij3) r with
+//    irij3 = 1 / (r2 sqrt r2) (swiftest_kick.f90:418-446), each body's ahi
+//    over j ascending and ahj over i ascending (the one-thread order of the
+//    OpenMP reduction), acc = (0 + ahi) + ahj.
+// The encounter-pair removal and the kick follow as in k_kick_pl.  fast: the
+// factors from gm_over_r3_fast.
+// ---------------------------------------------------------------------------
+__device__ inline double irij3_of(double r2, int fast)
+{
+    return fast ? gm_over_r3_fast(1.0, r2) : 1.0 / (r2 * sqrt(r2));
+}
+
+__global__ __launch_bounds__(256) void k_kick_pl_tiny(PlDev pl, SysDev sy, ListDev pp, const Tile *tiles, double dt,
+                                                      int lbeg, int use_radius, int do_pairs, int fast)
+{
+    const Tile tl = tiles[blockIdx.x];
+    const int p0 = sy.pl_off[tl.s], p1 = sy.pl_off[tl.s + 1];
+    const int nm = sy.nplm[tl.s], n = p1 - p0;
+    const int i = tl.b0 + (int)threadIdx.x;
+    const bool valid = i < tl.b1;
+    if (!valid) return;
+    const double xi = pl.rh.x[i], yi = pl.rh.y[i], zi = pl.rh.z[i], Ri = pl.R[i];
+    const cdouble *X = cview(pl.rh.x), *Y = cview(pl.rh.y), *Z = cview(pl.rh.z), *G = cview(pl.Gm),
+                  *RR = cview(pl.R);
+    const int il = i - p0;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    if (n - nm > nm) {
+        double hix = 0.0, hiy = 0.0, hiz = 0.0, hjx = 0.0, hjy = 0.0, hjz = 0.0;
+        // pairs (i, j), j > i, when i is fully interacting
+        const int jb = il < nm ? i + 1 : p1;
+        for (int j = jb; j < p1; j++) {
+            const double rx = X[j] - xi, ry = Y[j] - yi, rz = Z[j] - zi;
+            const double r2 = dot3(rx, ry, rz, rx, ry, rz);
+            const double rs = Ri + RR[j];
+            if (use_radius && !(r2 > rs * rs)) continue;
+            const double facj = G[j] * irij3_of(r2, fast);
+            hix = fma(facj, rx, hix);
+            hiy = fma(facj, ry, hiy);
+            hiz = fma(facj, rz, hiz);
+        }
+        // pairs (p, i), p < i, p fully interacting
+        const int pe = p0 + min(il, nm);
+        for (int q = p0; q < pe; q++) {
+            const double rx = xi - X[q], ry = yi - Y[q], rz = zi - Z[q];
+            const double r2 = dot3(rx, ry, rz, rx, ry, rz);
+            const double rs = RR[q] + Ri;
+            if (use_radius && !(r2 > rs * rs)) continue;
+            const double faci = G[q] * irij3_of(r2, fast);
+            hjx = fma(-faci, rx, hjx);
+            hjy = fma(-faci, ry, hjy);
+            hjz = fma(-faci, rz, hjz);
+        }
+        ax = (0.0 + hix) + hjx;
+        ay = (0.0 + hiy) + hjy;
+        az = (0.0 + hiz) + hjz;
+    } else {
+        const int je = il < nm ? p1 : p0 + nm;
+        for (int j = p0; j < je; j++) {
+            if (j == i) continue;
+            const double rx = X[j] - xi, ry = Y[j] - yi, rz = Z[j] - zi;
+            const double r2 = dot3(rx, ry, rz, rx, ry, rz);
+            const double rs = Ri + RR[j];
+            if (use_radius && !(r2 > rs * rs)) continue;
+            const double g = G[j];
+            const double fac = fast ? gm_over_r3_fast(g, r2) : gm_over_r3(g, r2, pair_g_ok(g));
+            ax = fma(fac, rx, ax);
+            ay = fma(fac, ry, ay);
+            az = fma(fac, rz, az);
+        }
+    }
+    if (do_pairs) {
+        double hix = 0.0, hiy = 0.0, hiz = 0.0, hjx = 0.0, hjy = 0.0, hjz = 0.0;
+        const int64_t c0 = pp.colstart[i], c1 = c0 + pp.colcnt[i];
+        for (int64_t c = c0; c < c1; c++) {  // pairs (p, i), p < i
+            const int p = pp.i1[pp.col[c]];
+            const double rx = xi - pl.rh.x[p], ry = yi - pl.rh.y[p], rz = zi - pl.rh.z[p];
+            const double rji2 = dot3(rx, ry, rz, rx, ry, rz);
+            const double rs = pl.R[p] + Ri;
+            if (!(rji2 > rs * rs)) continue;
+            const double fac = fast ? gm_over_r3_fast(pl.Gm[p], rji2) : pl.Gm[p] * (1.0 / (rji2 * sqrt(rji2)));
+            hjx = fma(-fac, rx, hjx);
+            hjy = fma(-fac, ry, hjy);
+            hjz = fma(-fac, rz, hjz);
+        }
+        const int64_t r0 = pp.rowstart[i], r1 = r0 + pp.rowcnt[i];
+        for (int64_t k = r0; k < r1; k++) {  // pairs (i, j), j > i
+            const int j = pp.i2[k];
+            const double rx = pl.rh.x[j] - xi, ry = pl.rh.y[j] - yi, rz = pl.rh.z[j] - zi;
+            const double rji2 = dot3(rx, ry, rz, rx, ry, rz);
+            const double rs = Ri + pl.R[j];
+            if (!(rji2 > rs * rs)) continue;
+            const double fac = fast ? gm_over_r3_fast(pl.Gm[j], rji2) : pl.Gm[j] * (1.0 / (rji2 * sqrt(rji2)));
+            hix = fma(fac, rx, hix);
+            hiy = fma(fac, ry, hiy);
+            hiz = fma(fac, rz, hiz);
+        }
+        ax = ax - ((0.0 + hix) + hjx);
+        ay = ay - ((0.0 + hiy) + hjy);
+        az = az - ((0.0 + hiz) + hjz);
+    }
+    Vec3p rb = lbeg ? pl.rbeg : pl.rend;
+    rb.x[i] = xi;
+    rb.y[i] = yi;
+    rb.z[i] = zi;
+    if (pl_active(pl, i)) {
+        pl.vb.x[i] = fma(ax, dt, pl.vb.x[i]);
+        pl.vb.y[i] = fma(ay, dt, pl.vb.y[i]);
+        pl.vb.z[i] = fma(az, dt, pl.vb.z[i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Fast pair mode (SWG_PAIR_FAST): the same sums in the same j order, with the
 // pair factor from the hardware reciprocal square root (gm_over_r3_fast,
 // within a few ulp of the IEEE factor).  The partners' positions, masses and
@@ -295,9 +576,6 @@ __global__ __launch_bounds__(BLK) void k_kick_pl(PlDev pl, SysDev sy, ListDev pp
 // and every VALU slot goes to the pair arithmetic (13 operations, one of
 // them quarter-rate, and 7 of the factor).
 // ---------------------------------------------------------------------------
-
-typedef const __attribute__((address_space(4))) double cdouble;
-__device__ inline const cdouble *cview(const double *p) { return (const cdouble *)p; }
 
 // Massive bodies: one tile of KPL_FAST_TILE bodies of one system per
 // workgroup of K wavefronts.  Wavefront w sums body i's row over its share of
@@ -819,6 +1097,10 @@ void launch_kick_pl(hipStream_t st, const Tuning &tu, PlDev pl, SysDev sy, ListD
                     int ntiles, double dt, int lbeg, int use_radius, int do_pairs)
 {
     if (!ntiles) return;
+    if (tu.any_tiny) {
+        k_kick_pl_tiny<<<ntiles, 256, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs, tu.pair_fast);
+        return;
+    }
     if (tu.pair_fast) {
         switch (tu.kick_pl_split) {
         case 1: k_kick_pl_fast<1><<<ntiles, 64, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
@@ -826,6 +1108,14 @@ void launch_kick_pl(hipStream_t st, const Tuning &tu, PlDev pl, SysDev sy, ListD
         case 4: k_kick_pl_fast<4><<<ntiles, 256, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
         case 16: k_kick_pl_fast<16><<<ntiles, 1024, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
         default: k_kick_pl_fast<8><<<ntiles, 512, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
+        }
+        return;
+    }
+    if (tu.kick_pl_eval) {
+        switch (tu.kick_pl_eval) {
+        case 2: k_kick_pl_split<2><<<ntiles, 192, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
+        case 8: k_kick_pl_split<8><<<ntiles, 576, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
+        default: k_kick_pl_split<4><<<ntiles, 320, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
         }
         return;
     }

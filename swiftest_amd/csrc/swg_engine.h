@@ -31,6 +31,7 @@ struct Ensemble {
     // bodies of each system that are not slot padding (all of them unless
     // the ensemble is a slot pool, swg_system_create_slots)
     std::vector<int32_t> h_nreal_pl, h_nreal_tp;
+    bool any_tiny = false;  // some system has semi-interacting bodies (GMTINY)
     std::vector<double> h_GMcb, h_Gm;
 
     PlDev pl{};
@@ -41,6 +42,7 @@ struct Ensemble {
     int64_t pp_words = 0, pt_words = 0, npp = 0, npt = 0, cap_pp = 0, cap_pt = 0;
 
     Tile *d_tiles_pl = nullptr, *d_tiles_tp = nullptr;
+    bool tiles_ok = false;  // every tile array built (build_tiles succeeded)
     int4 *d_tiles_pp = nullptr, *d_tiles_pt = nullptr;
     int ntiles_pl = 0, ntiles_tp = 0, ntiles_pp = 0;
     int32_t *d_any = nullptr, *d_pt_run = nullptr;
@@ -99,6 +101,7 @@ struct Ensemble {
             int32_t *pl_status, int32_t *tp_status, int32_t *pl_id);
     int get_misc(double *rhill, double *cb_vb, double *ptbeg, double *ptend, int32_t *lfirst);
     int enc_list(int which, int64_t *n, int32_t *i1, int32_t *i2);
+    int enc_status(int which, int64_t *n, int32_t *status);
     int time_kernels(int reps, double *ms_tp, double *ms_pl);
     int accel(double *pl_acc, double *tp_acc);
     int load(int s, int npl_new, int ntp_new, double GMcb_new, const double *pl_rh, const double *pl_vh,

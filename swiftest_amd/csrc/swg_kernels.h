@@ -31,6 +31,9 @@ struct Tuning {
     int kick_tp_block = 256;
     int pair_fast = 0;         // 1: pair factors from the reciprocal square root (SWG_PAIR_FAST)
     int kick_pl_split = 8;     // fast mode: wavefronts sharing one 64-body tile's rows (1, 2, 4, 8, 16)
+    int kick_pl_eval = 0;      // exact mode: 0 one lane per body (k_kick_pl), else the evaluating
+                               // wavefronts per 64-body tile of k_kick_pl_split (2, 4, 8)
+    int any_tiny = 0;          // some system has semi-interacting bodies: k_kick_pl_tiny for every tile
 };
 // bodies per massive-body tile of the fast pair kernel
 constexpr int KPL_FAST_TILE = 64;

@@ -27,6 +27,7 @@ struct PlDev {
     Vec3p rh, vh, vb, rbeg, rend;
     double *Gm, *R, *rhill, *renc;
     int32_t *status, *levelg, *levelm, *sys;
+    int32_t *tiny;  // 1: Gmass < GMTINY (SyMBA), interacts only with the fully interacting bodies; may be NULL
 };
 
 // Test particles ("tp") of all systems.
@@ -45,6 +46,7 @@ struct SysDev {
     int32_t *flag_pp, *flag_pt;   // per-substep list-check flags
     int32_t *err;                 // error bits: 1 collision, 2 pl drift failure
     int32_t *frozen;              // 1 = system left out of the steps (its state is kept as is)
+    int32_t *nplm;                // fully interacting massive bodies (the first nplm of the system); may be NULL
 };
 
 // A (pl, pl) or (pl, tp) encounter list with its bit matrix and adjacency.

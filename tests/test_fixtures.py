@@ -7,8 +7,8 @@ The reference ships inputs only for these examples (no expected outputs), so
 the oracle is pinned on them through the reference's conservation thresholds,
 and the HIP path is compared with the oracle bit for bit on them.  Run
 parameters taken from the example files: dt, RHILL_PRESENT, CHK_CLOSE.
-Changed: GR off (the GR drift is not built) and GMTINY 0 (every body fully
-interacting; the resident step has no semi-interacting bodies)."""
+GMTINY as the files set it (51 of the 108 bodies are below it).  Changed:
+GR off (the GR drift is not built)."""
 import json
 import os
 
@@ -81,8 +81,9 @@ def test_gpu_matches_oracle_on_reference_examples(name, nsteps, istep_out):
     print(f"fixture {name}", flush=True)
     s, rhill, prm = load(name)
     dt = float(prm["DT"])
-    o = O.OracleSystem(O.SYMBA, s, dt, rhill=rhill, istep_out=istep_out)
-    g = S.GpuSystem(S.SYMBA, s, dt, rhill=rhill, istep_out=istep_out)
+    gmt = float(prm.get("GMTINY", 0.0))
+    o = O.OracleSystem(O.SYMBA, s, dt, rhill=rhill, istep_out=istep_out, gmtiny=gmt)
+    g = S.GpuSystem(S.SYMBA, s, dt, rhill=rhill, istep_out=istep_out, gmtiny=gmt)
     rc = o.run(nsteps)
     try:
         g.run(nsteps)
@@ -94,3 +95,6 @@ def test_gpu_matches_oracle_on_reference_examples(name, nsteps, istep_out):
     assert np.array_equal(so["pl_id"], sg["pl_id"])
     for k in ("pl_rh", "pl_vh", "tp_rh", "tp_vh", "pl_status", "tp_status"):
         assert np.array_equal(so[k], sg[k]), k
+    a, b = g.enc_list(0)
+    i1, i2, _ = o.enc_list(0)
+    assert np.array_equal(a, i1) and np.array_equal(b, i2)

@@ -235,3 +235,64 @@ def test_leaf_operators_fast_mode():
     assert _rel(af, ae) <= 1e-13 and _rel(tf, te) <= 1e-13
     # and the exact mode is back
     assert np.array_equal(S.swiftest_kick_getacch_int_all_tri(n, n, r, gm, R, np.zeros((n, 3))), ae)
+
+
+# ------------------------------------- exact massive-body kernel variants --
+
+@pytest.mark.parametrize("kev", [2, 4, 8])
+def test_split_row_kernel_bit_exact(kev, monkeypatch):
+    """The exact massive-body kick with its terms evaluated by several
+    wavefronts and folded in order gives the one-lane-per-body sums bit for
+    bit: accelerations of the 2000-body disk and of a crowded disk with
+    encounter pairs removed, then 20 steps of a crowded disk against the
+    oracle."""
+    from helpers import compact_disk
+    monkeypatch.setenv("SWG_KICK_PL_EVAL", str(kev))
+    c2 = ics.symba_disk()
+    g = S.GpuSystem(S.SYMBA, c2, 0.005)
+    st = g.state()
+    gm, R = c2.Gmass[st["pl_id"] - 1], c2.radius[st["pl_id"] - 1]
+    pe, _ = g.accel()
+    assert np.array_equal(pe, O.accel_plpl(st["pl_rh"], gm, radius=R))
+    d = compact_disk(seed=32, npl=300, ntp=40)
+    o = O.OracleSystem(O.SYMBA, d, 0.005)
+    g = S.GpuSystem(S.SYMBA, d, 0.005)
+    for _ in range(4):
+        assert o.run(5) == 0
+        g.run(5)
+        so, sg = o.state(), g.state()
+        for key in ("pl_rh", "pl_vb", "tp_rh", "tp_vb"):
+            assert np.array_equal(so[key], sg[key]), key
+        assert g.stats()["nplpl_enc"] == o.stats()["nplpl_enc"] > 0
+
+
+# ------------------------------------------------ semi-interacting bodies --
+
+@pytest.mark.parametrize("q,mode", [(0.5, "exact"), (0.8, "exact"), (0.8, "fast")])
+def test_semi_interacting_bodies_match_the_oracle(q, mode):
+    """GMTINY below the median (rows over the fully interacting bodies) and
+    at the 80th percentile (the triangular branch): 30 SyMBA steps of a
+    crowded disk with test particles against the oracle, bit for bit in the
+    exact pair mode (1e-10 with the fast factor), the same encounter lists,
+    and no pair of two semi-interacting bodies listed."""
+    from helpers import compact_disk
+    d = compact_disk(seed=38, npl=240, ntp=30, mlo=1e-9, mhi=1e-5)
+    gmt = float(np.quantile(d.Gmass, q))
+    pm = S.PAIR_FAST if mode == "fast" else S.PAIR_EXACT
+    o = O.OracleSystem(O.SYMBA, d, 0.005, gmtiny=gmt)
+    g = S.GpuSystem(S.SYMBA, d, 0.005, gmtiny=gmt, pair_mode=pm)
+    nplm = int((d.Gmass >= gmt).sum())
+    for k in range(6):
+        assert o.run(5) == 0
+        g.run(5)
+        so, sg = o.state(), g.state()
+        if mode == "exact":
+            for key in ("pl_rh", "pl_vb", "tp_rh", "tp_vb"):
+                assert np.array_equal(so[key], sg[key]), (k, key)
+        else:
+            assert _rel(sg["pl_rh"], so["pl_rh"]) <= 1e-10
+        a, b = g.enc_list(0)
+        i1, i2, _ = o.enc_list(0)
+        assert np.array_equal(a, i1) and np.array_equal(b, i2), k
+        assert np.all(a <= nplm)
+    assert g.stats()["nplpl_enc"] > 0

@@ -190,10 +190,30 @@ def _compare_systems(o, g, label, whm=False):
         assert np.array_equal(so[k], sg[k]), f"{label} {k} ulp={ulp_diff(so[k], sg[k]) if so[k].dtype == np.float64 else ''}"
 
 
+def _compare_collision(o, g, label):
+    """Both sides stopped for a collision at the end of the same step: the
+    same level-0 lists with the same pairs flagged COLLIDED (at least one),
+    the same recursion counts and the same state, bit for bit."""
+    so, sg = o.stats(), g.stats()
+    for k in ("nplpl_enc", "npltp_enc", "max_depth", "nsubsteps"):
+        assert so[k] == sg[k], (label, k, so, sg)
+    ncol = 0
+    for which in (0, 1):
+        i1, i2, _ = o.enc_list(which)
+        a, b = g.enc_list(which)
+        assert np.array_equal(i1, a) and np.array_equal(i2, b), (label, which)
+        st_o, st_g = o.enc_status(which), g.enc_status(which)
+        assert np.array_equal(st_o, st_g), (label, which, np.flatnonzero(st_o != st_g)[:10])
+        ncol += int((st_o == -14).sum())
+    assert ncol > 0, label
+    _compare_systems(o, g, label)
+    return ncol
+
+
 @pytest.mark.parametrize("short_circuit,ntp", [(True, 200), (False, 40)])
 def test_symba_crowded_disk_bit_exact(short_circuit, ntp):
     sysic = compact_disk(npl=160, ntp=ntp)
-    o = O.OracleSystem(O.SYMBA, sysic, 0.005, short_circuit=short_circuit, istep_out=10)
+    o = O.OracleSystem(O.SYMBA, sysic, 0.005, short_circuit=short_circuit, istep_out=10, collision_finish_step=True)
     g = S.GpuSystem(S.SYMBA, sysic, 0.005, short_circuit=short_circuit, istep_out=10)
     depth = -1
     for blk in range(4):
@@ -204,9 +224,11 @@ def test_symba_crowded_disk_bit_exact(short_circuit, ntp):
             grc = 0
         except S.SwiftestHipError as e:
             grc = e.code
-        # a collision (merge is host work) must be seen by both at the same block
+        # a collision (merge is host work) ends both runs at the end of the
+        # same step, with the same pairs flagged and the same state
         assert (rc == -1) == (grc == -3), (rc, grc)
         if rc:
+            _compare_collision(o, g, f"collision in block {blk}")
             break
         so, sg = o.stats(), g.stats()
         assert so["nplpl_enc"] == sg["nplpl_enc"] and so["npltp_enc"] == sg["npltp_enc"]
@@ -332,7 +354,7 @@ def test_sortsweep_lists_equal_triangular(kind):
 
 def test_sortsweep_symba_matches_oracle():
     sysic = compact_disk(npl=160, ntp=40)
-    o = O.OracleSystem(O.SYMBA, sysic, 0.005, short_circuit=False, istep_out=10)
+    o = O.OracleSystem(O.SYMBA, sysic, 0.005, short_circuit=False, istep_out=10, collision_finish_step=True)
     g = S.GpuSystem(S.SYMBA, sysic, 0.005, short_circuit=False, istep_out=10, encounter_check=1)
     for blk in range(2):
         rc = o.run(10)
@@ -343,6 +365,7 @@ def test_sortsweep_symba_matches_oracle():
             grc = e.code
         assert (rc == -1) == (grc == -3), (rc, grc)
         if rc:
+            _compare_collision(o, g, f"sortsweep collision in block {blk}")
             break
         _compare_systems(o, g, f"sortsweep block {blk}")
 
@@ -575,3 +598,60 @@ def test_discard_symba_massive_bodies_and_ensemble():
         po += s.npl
         to += s.ntp
     assert tuple(tot) == got and got[0] == 2
+
+
+def test_failed_tile_allocation_is_an_error_not_a_launch(monkeypatch):
+    """A failed allocation of the tile arrays (forced from the host with
+    SWG_FAIL_TILE_ALLOC) comes back as SWG_ERR_HIP from set_frozen, and the
+    system then refuses to step or time its kernels (no launch over a missing
+    tile array) until a rebuild succeeds; after that it steps exactly as a
+    system that never failed."""
+    systems = [compact_disk(seed=30 + k, npl=n, ntp=t) for k, (n, t) in enumerate(((50, 20), (90, 0)))]
+    ref = S.GpuSystem(S.SYMBA, systems, 0.005, lclose=False)
+    g = S.GpuSystem(S.SYMBA, systems, 0.005, lclose=False)
+    monkeypatch.setenv("SWG_FAIL_TILE_ALLOC", "1")
+    with pytest.raises(S.SwiftestHipError) as e:
+        g.set_frozen(np.array([0, 0], np.int32))
+    assert e.value.code == -2 and "injected" in str(e.value)
+    for call in (lambda: g.run(1), g.accel):
+        with pytest.raises(S.SwiftestHipError) as e:
+            call()
+        assert e.value.code == -2 and "tile arrays are missing" in str(e.value)
+    monkeypatch.delenv("SWG_FAIL_TILE_ALLOC")
+    g.set_frozen(np.array([0, 0], np.int32))
+    g.run(3)
+    ref.run(3)
+    a, b = ref.state(), g.state()
+    for k in ("pl_rh", "pl_vb", "tp_rh", "tp_vb"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("seed,ntp", [(31, 40), (33, 0)])
+def test_collision_stop_content_matches_the_oracle(seed, ntp):
+    """Crowded disks that collide within 20 steps, stepped one at a time:
+    the step in which the collision is found, the colliding pairs (flagged
+    COLLIDED in the level-0 list), the recursion counts of that step and
+    the state at its end equal the oracle's, which flags the pairs and
+    finishes the step as the device does."""
+    d = compact_disk(seed=seed, npl=300, ntp=ntp)
+    o = O.OracleSystem(O.SYMBA, d, 0.005, collision_finish_step=True)
+    g = S.GpuSystem(S.SYMBA, d, 0.005)
+    for k in range(20):
+        print(f"collision seed {seed} step {k}", flush=True)
+        rc = o.run(1)
+        try:
+            g.run(1)
+            grc = 0
+        except S.SwiftestHipError as e:
+            grc = e.code
+        assert (rc == -1) == (grc == -3), (k, rc, grc)
+        if rc:
+            ncol = _compare_collision(o, g, f"seed {seed} step {k}")
+            print(f"collision at step {k}: {ncol} pair(s) flagged", flush=True)
+            # the run stays stopped: the next step reports it again
+            with pytest.raises(S.SwiftestHipError) as e:
+                g.run(1)
+            assert e.value.code == -3
+            return
+        _compare_systems(o, g, f"seed {seed} step {k}")
+    pytest.fail("no collision within 20 steps")

@@ -208,6 +208,63 @@ def test_encounter_list_exact_rational_brute_force():
     assert lv.all()
 
 
+@pytest.mark.parametrize("nplm", [45, 20, 0])
+def test_encounter_list_plplm_exact_rational_brute_force(nplm):
+    """encounter_check_all_plplm (encounter_check.f90:42-109): every pair
+    among the first nplm bodies and every pair of one of them with the rest;
+    none among the rest; listed index1 then index2 ascending."""
+    rng = np.random.Generator(np.random.PCG64(16))
+    n = 60
+    r = rng.uniform(-1, 1, (n, 3))
+    v = rng.normal(0, 1, (n, 3))
+    renc = rng.uniform(0.02, 0.15, n)
+    dt = 0.1
+    i1, i2, lv = O.encounters_plplm(r, v, renc, nplm, dt)
+    got = set(zip(i1.tolist(), i2.tolist()))
+    assert list(zip(i1, i2)) == sorted(got)
+    for i in range(n):
+        for j in range(i + 1, n):
+            enc, margin = _exact_encounter(r[j] - r[i], v[j] - v[i], renc[i] + renc[j], dt)
+            if margin < 1e-12:
+                continue
+            assert ((i + 1, j + 1) in got) == (enc and i < nplm), (i, j)
+    assert lv.all()
+
+
+@pytest.mark.parametrize("q", [0.5, 0.75])
+def test_semi_interacting_bodies_in_the_symba_step(q):
+    """GMTINY: the bodies below it feel the others and move the others, but
+    not each other.  Over 100 SyMBA steps of a disk whose lighter half (or
+    three quarters: the triangular branch of swiftest_kick.f90:190-216) is
+    below GMTINY, their pairs never enter the encounter list, and the
+    accelerations differ from the fully interacting ones by exactly the tiny
+    bodies' mutual attraction."""
+    d = compact_disk(seed=23, npl=60, mlo=1e-9, mhi=3e-6, alo=0.9, ahi=1.1)
+    gmt = float(np.quantile(d.Gmass, q))
+    o = O.OracleSystem(O.SYMBA, d, 0.005, gmtiny=gmt)
+    st = o.state()
+    Gm = d.Gmass[st["pl_id"] - 1]
+    nplm = int((Gm >= gmt).sum())
+    assert 0 < nplm < len(Gm)
+    for _ in range(20):
+        assert o.run(5) == 0
+        i1, i2, _ = o.enc_list(0)
+        assert np.all(i1 <= nplm)          # index1 always fully interacting
+    # the tiny bodies' mutual forces are absent: the tiny-tiny pair sum of a
+    # step equals the difference of the two kicks' sums
+    r = st["pl_rh"]
+    a_full = O.accel_plpl(r, Gm)
+    a_tiny = O.accel_plpl(r, Gm, nplm=nplm)
+    k = np.arange(len(Gm)) >= nplm
+    mutual = np.zeros_like(a_full)
+    for i in np.flatnonzero(k):
+        for j in np.flatnonzero(k):
+            if i != j:
+                dr = r[j] - r[i]
+                mutual[i] += Gm[j] * dr / np.linalg.norm(dr) ** 3
+    np.testing.assert_allclose(a_full - a_tiny, mutual, rtol=1e-9, atol=1e-14 * np.abs(a_full).max())
+
+
 # ---------------------------------------------------------- shell kicks --
 
 def test_pow_m1p5_correctly_rounded():
@@ -504,3 +561,26 @@ def test_discard_symba_massive_bodies():
     g = ics.GMSUN + 2e-4
     cb = -(1e-4 * np.array(pl_vh[0]) + 1e-4 * np.array(pl_vh[3])) / g
     np.testing.assert_allclose(st.cb_vb, cb, rtol=1e-15)
+
+
+def test_collision_modes_agree_on_the_colliding_step():
+    """A crowded disk that collides: stopping at the collision check and
+    finishing the step report the collision in the same step, from the same
+    state; finishing flags at least the pairs the stop flagged."""
+    from helpers import compact_disk
+    d = compact_disk(seed=33, npl=300, ntp=0)
+    stop = O.OracleSystem(O.SYMBA, d, 0.005)
+    fin = O.OracleSystem(O.SYMBA, d, 0.005, collision_finish_step=True)
+    for k in range(20):
+        a, b = stop.state(), fin.state()
+        assert np.array_equal(a["pl_rh"], b["pl_rh"]) and np.array_equal(a["pl_vb"], b["pl_vb"]), k
+        ra, rb = stop.run(1), fin.run(1)
+        assert ra == rb, (k, ra, rb)
+        if ra:
+            assert ra == -1
+            sa, sb = stop.enc_status(0), fin.enc_status(0)
+            assert np.array_equal(stop.enc_list(0)[0], fin.enc_list(0)[0])
+            assert (sa == -14).sum() >= 1
+            assert np.all(sb[sa == -14] == -14)
+            return
+    pytest.fail("no collision within 20 steps")
