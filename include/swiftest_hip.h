@@ -126,6 +126,27 @@ int swg_encounter_check_all_pltp(swg_ctx *ctx, int32_t npl, int32_t ntp, const d
                                  int64_t *nenc);
 int swg_encounter_list_copy(swg_ctx *ctx, int32_t *index1, int32_t *index2, int32_t *lvdotr);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): swg_encounter_check_all_plpl followed by
+ * swg_encounter_list_copy in one call that allocates nothing.  Every device
+ * buffer of the check (the staged inputs, the bit matrix, row counts, scan
+ * partials and the list) lives in one library-owned region of
+ * SWG_NOALLOC_BYTES (6.5 MiB) per GPU, held until the process ends.  Limits:
+ * npl <= SWG_NOALLOC_MAX_NPL and at most SWG_NOALLOC_MAX_PAIRS listed pairs,
+ * else SWG_ERR_ARG.  cap is the length of the caller's index1 / index2 /
+ * lvdotr arrays (any may be NULL); when the list is longer than cap, *nenc
+ * receives its length and SWG_ERR_ARG is returned with nothing copied.  The
+ * outputs are those of the allocating pair of calls.  Every call on one GPU
+ * uses that GPU's region and leaves the last call's contents there: calls
+ * that overlap on one GPU (from other threads or contexts) race. */
+#define SWG_NOALLOC_MAX_NPL 2048
+#define SWG_NOALLOC_MAX_PAIRS 524288
+#define SWG_NOALLOC_BYTES 6812160
+int swg_encounter_check_all_plpl_noalloc(swg_ctx *ctx, int32_t npl, const double *r, const double *v,
+                                         const double *renc, double dt, int64_t cap, int64_t *nenc,
+                                         int32_t *index1, int32_t *index2, int32_t *lvdotr);
+
 /* ---- resident systems ---------------------------------------------------- */
 
 /* Create nsys independent systems from initial conditions as the reference

@@ -1655,6 +1655,7 @@ int swg_init(int device, swg_ctx **ctx)
     HIPCHK(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(SWG_ERR_ARG, "no such device");
     HIPCHK(hipSetDevice(device));
+    if (int rc = swg::noalloc_setup(device)) return rc;
     swg_ctx *c = new swg_ctx();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

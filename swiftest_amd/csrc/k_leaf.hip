@@ -10,6 +10,9 @@
 #include <numeric>
 #include <vector>
 
+#include <mutex>
+#include <string>
+
 #include "swg_engine.h"
 #include "swg_math.h"
 
@@ -652,6 +655,188 @@ void *LeafState::get(int slot, size_t bytes, int *rc)
     return bufs[slot];
 }
 
+// ---------------------------------------------------------------------------
+// The encounter check that allocates nothing (swg_encounter_check_all_plpl_
+// noalloc): every device buffer of the check lives in one file-scope device
+// array, laid out for at most SWG_NOALLOC_MAX_NPL bodies and
+// SWG_NOALLOC_MAX_PAIRS listed pairs.
+// ---------------------------------------------------------------------------
+
+constexpr size_t na_align(size_t b) { return (b + 255) / 256 * 256; }
+constexpr int NA_NPL = SWG_NOALLOC_MAX_NPL;
+constexpr int64_t NA_PAIRS = SWG_NOALLOC_MAX_PAIRS;
+constexpr int NA_NB = (NA_NPL + 63) / 64;
+constexpr int NA_TILES = NA_NB * (NA_NB + 1) / 2;
+// byte offsets of the regions
+constexpr size_t NA_AOS = 0;                                             // 7 doubles per body, as given
+constexpr size_t NA_SOA = NA_AOS + na_align(sizeof(double) * 7 * NA_NPL);  // rh.x/y/z, vb.x/y/z
+constexpr size_t NA_SYS = NA_SOA + na_align(sizeof(double) * 6 * NA_NPL);
+constexpr size_t NA_SMALL = NA_SYS + na_align(sizeof(int32_t) * NA_NPL);  // offs[4], nb, woff, total
+constexpr size_t NA_TILE = NA_SMALL + 256;
+constexpr size_t NA_SCAN = NA_TILE + na_align(sizeof(int4) * NA_TILES);
+constexpr size_t NA_SCAN_BYTES = 4096;
+constexpr size_t NA_WORDS = NA_SCAN + NA_SCAN_BYTES;
+constexpr size_t NA_WPRE = NA_WORDS + na_align(sizeof(uint64_t) * NA_NPL * NA_NB);
+constexpr size_t NA_RCNT = NA_WPRE + na_align(sizeof(int32_t) * NA_NPL * NA_NB);
+constexpr size_t NA_RST = NA_RCNT + na_align(sizeof(int32_t) * NA_NPL);
+constexpr size_t NA_I1 = NA_RST + na_align(sizeof(int64_t) * NA_NPL);
+constexpr size_t NA_I2 = NA_I1 + na_align(sizeof(int32_t) * NA_PAIRS);
+constexpr size_t NA_LV = NA_I2 + na_align(sizeof(int32_t) * NA_PAIRS);
+constexpr size_t NA_ST = NA_LV + na_align(NA_PAIRS);
+constexpr size_t NA_LEVEL = NA_ST + na_align(NA_PAIRS);
+constexpr size_t NA_BYTES = NA_LEVEL + na_align(NA_PAIRS);
+static_assert(NA_BYTES == SWG_NOALLOC_BYTES, "the header states the region's size");
+
+// The region.  In host code `scratch` names a host-side stand-in, not the
+// device memory: host code uses only the address hipGetSymbolAddress gives
+// for the current device (noalloc_ptr).
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+static void *g_noalloc_ptr[NA_MAX_DEVICES] = {};
+static std::mutex g_noalloc_mu;
+
+// Once per device, with that device current (swg_init): look up the device
+// address of the region.
+int noalloc_setup(int device)
+{
+    if (device < 0 || device >= NA_MAX_DEVICES) return fail(SWG_ERR_ARG, "device index beyond the no-allocation table");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return 0;
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(SWG_ERR_HIP, "hipGetSymbolAddress returned no address for the no-allocation region");
+    g_noalloc_ptr[device] = p;
+    return 0;
+}
+
+static void *noalloc_ptr(int device)
+{
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    return (device >= 0 && device < NA_MAX_DEVICES) ? g_noalloc_ptr[device] : nullptr;
+}
+
+// the triangular 64x64 tiles (0, ib, jb), ib <= jb < nb, in the order the
+// allocating path lists them
+__global__ void k_na_tiles(int4 *tiles, int nb)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nb * (nb + 1) / 2) return;
+    int ib = 0, rest = t;
+    while (rest >= nb - ib) {
+        rest -= nb - ib;
+        ib++;
+    }
+    tiles[t] = make_int4(0, ib, ib + rest, 0);
+}
+
+// the staged AoS inputs (r, v as (3,n), then renc) into SoA
+__global__ void k_na_soa(const double *r, const double *v, double *soa, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int d = 0; d < 3; d++) {
+        soa[d * n + i] = r[3 * i + d];
+        soa[(3 + d) * n + i] = v[3 * i + d];
+    }
+}
+
+// offs = {0, npl, 0, 0}, nb, woff = 0
+__global__ void k_na_small(int32_t *offs, int32_t *nbp, int64_t *woff, int npl, int nb)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        offs[0] = 0;
+        offs[1] = npl;
+        offs[2] = 0;
+        offs[3] = 0;
+        *nbp = nb;
+        *woff = 0;
+    }
+}
+
+int noalloc_encounter_plpl(swg_ctx *ctx, int32_t npl, const double *r, const double *v, const double *renc, double dt,
+                           int64_t cap, int64_t *nenc, int32_t *index1, int32_t *index2, int32_t *lvdotr)
+{
+    *nenc = 0;
+    if (npl > NA_NPL)
+        return fail(SWG_ERR_ARG, "npl " + std::to_string(npl) + " exceeds the no-allocation limit of " +
+                                     std::to_string(NA_NPL) + " bodies");
+    if (npl == 0) return 0;
+    unsigned char *base = (unsigned char *)noalloc_ptr(ctx->device);
+    if (!base) return fail(SWG_ERR_HIP, "the no-allocation region of this device was not set up");
+    if (scan_scratch_bytes(npl) > NA_SCAN_BYTES) return fail(SWG_ERR_ARG, "scan partials exceed their region");
+    hipStream_t st = ctx->stream;
+    double *aos = (double *)(base + NA_AOS), *soa = (double *)(base + NA_SOA);
+    int32_t *psys = (int32_t *)(base + NA_SYS);
+    int32_t *offs = (int32_t *)(base + NA_SMALL), *nbp = offs + 4;
+    int64_t *woff = (int64_t *)(base + NA_SMALL + 64), *tot = woff + 1;
+    int4 *tiles = (int4 *)(base + NA_TILE);
+    const size_t nd = (size_t)npl;
+    HIPCHK(hipMemcpyAsync(aos, r, 3 * nd * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(aos + 3 * nd, v, 3 * nd * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(aos + 6 * nd, renc, nd * sizeof(double), hipMemcpyHostToDevice, st));
+    const int nb = (npl + 63) / 64, ntiles = nb * (nb + 1) / 2;
+    k_na_soa<<<cdiv(npl, 256), 256, 0, st>>>(aos, aos + 3 * nd, soa, npl);
+    k_fill_i32<<<cdiv(npl, 256), 256, 0, st>>>(psys, npl, 0);
+    k_na_small<<<1, 64, 0, st>>>(offs, nbp, woff, npl, nb);
+    k_na_tiles<<<cdiv(ntiles, 256), 256, 0, st>>>(tiles, nb);
+    PlDev pl{};
+    SysDev sy{};
+    pl.rh = Vec3p{soa, soa + nd, soa + 2 * nd};
+    pl.vb = Vec3p{soa + 3 * nd, soa + 4 * nd, soa + 5 * nd};
+    pl.renc = aos + 6 * nd;
+    pl.sys = psys;
+    sy.pl_off = offs;
+    sy.tp_off = offs + 2;
+    ListDev L{};
+    L.words = (uint64_t *)(base + NA_WORDS);
+    L.wpre = (int32_t *)(base + NA_WPRE);
+    L.rowcnt = (int32_t *)(base + NA_RCNT);
+    L.rowstart = (int64_t *)(base + NA_RST);
+    L.woff = woff;
+    L.nb = nbp;
+    L.i1 = (int32_t *)(base + NA_I1);
+    L.i2 = (int32_t *)(base + NA_I2);
+    L.lvdotr = (int8_t *)(base + NA_LV);
+    L.status = (int8_t *)(base + NA_ST);
+    L.level = (int8_t *)(base + NA_LEVEL);
+    launch_pp_tiles(st, pl, sy, L, tiles, ntiles, dt);
+    launch_rows_count(st, L, psys, 1, offs, nullptr, npl);
+    launch_scan_i32_i64(st, L.rowcnt, L.rowstart, npl, tot, base + NA_SCAN);
+    HIPCHK(hipGetLastError());
+    int64_t n = 0;
+    HIPCHK(copy_wait(st, &n, tot, sizeof n, hipMemcpyDeviceToHost));
+    *nenc = n;
+    if (n > NA_PAIRS)
+        return fail(SWG_ERR_ARG, std::to_string(n) + " encounter pairs exceed the no-allocation limit of " +
+                                     std::to_string(NA_PAIRS));
+    if (n > cap) return fail(SWG_ERR_ARG, "the caller's list arrays hold " + std::to_string(cap) + " pairs, the list has " +
+                                              std::to_string(n));
+    if (!n) return 0;
+    launch_rows_fill(st, L, psys, 1, offs, offs, npl);
+    HIPCHK(hipGetLastError());
+    // raw 0-based indices and 1-byte flags straight into the caller's arrays,
+    // then made 1-based / widened on the host in place
+    if (index1) HIPCHK(hipMemcpyAsync(index1, L.i1, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (index2) HIPCHK(hipMemcpyAsync(index2, L.i2, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (lvdotr) HIPCHK(hipMemcpyAsync(lvdotr, L.lvdotr, (size_t)n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int64_t k = 0; k < n; k++) {
+        if (index1) index1[k] += 1;
+        if (index2) index2[k] += 1;
+    }
+    if (lvdotr) {
+        // widen the n bytes at the start of the array to n int32 values, from
+        // the end: value k goes to bytes 4k..4k+3, past every byte still unread
+        const int8_t *b = (const int8_t *)lvdotr;
+        for (int64_t k = n - 1; k >= 0; k--) {
+            const int32_t x = b[k];
+            lvdotr[k] = x;
+        }
+    }
+    return 0;
+}
+
 }  // namespace swg
 
 // ===========================================================================
@@ -924,6 +1109,16 @@ int swg_encounter_check_all_pltp(swg_ctx *ctx, int32_t npl, int32_t ntp, const d
     if (!ctx || !nenc || npl < 0 || ntp < 0) return fail(SWG_ERR_ARG, "bad argument");
     if (npl && ntp && (!rpl || !vpl || !rtp || !vtp || !renc)) return fail(SWG_ERR_ARG, "NULL array");
     return leaf_encounter(ctx, 1, npl, ntp, rpl, vpl, rtp, vtp, renc, dt, nenc);
+}
+
+int swg_encounter_check_all_plpl_noalloc(swg_ctx *ctx, int32_t npl, const double *r, const double *v,
+                                         const double *renc, double dt, int64_t cap, int64_t *nenc,
+                                         int32_t *index1, int32_t *index2, int32_t *lvdotr)
+{
+    if (!ctx || !nenc || npl < 0 || cap < 0) return fail(SWG_ERR_ARG, "bad argument");
+    if (npl && (!r || !v || !renc)) return fail(SWG_ERR_ARG, "NULL array");
+    HIPCHK(hipSetDevice(ctx->device));
+    return noalloc_encounter_plpl(ctx, npl, r, v, renc, dt, cap, nenc, index1, index2, lvdotr);
 }
 
 int swg_encounter_list_copy(swg_ctx *ctx, int32_t *index1, int32_t *index2, int32_t *lvdotr)

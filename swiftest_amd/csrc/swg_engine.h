@@ -164,6 +164,10 @@ struct Ensemble {
     int whm_snapshot();
 };
 
+// The encounter check that allocates nothing (k_leaf.hip): the device
+// address of its region on `device`, looked up once with that device current.
+int noalloc_setup(int device);
+
 // Scratch of the leaf-operator entry points.
 struct LeafState {
     std::vector<void *> bufs;
@@ -183,3 +187,8 @@ struct swg_ctx {
     swg::LeafState *leaf;
     int pair_mode;  // SWG_PAIR_EXACT / SWG_PAIR_FAST for the leaf pair sums
 };
+
+namespace swg {
+int noalloc_encounter_plpl(swg_ctx *ctx, int32_t npl, const double *r, const double *v, const double *renc, double dt,
+                           int64_t cap, int64_t *nenc, int32_t *index1, int32_t *index2, int32_t *lvdotr);
+}
