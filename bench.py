@@ -273,9 +273,10 @@ def serving_leg(nreq, rate, nmax, tpmax, smin, smax, chunk, seq_sample, log):
     out = serve.summary(reqs, wall)
     occ = [float(np.mean(o)) if o else 0.0 for o in srv.occupancy]
     out.update({"metric": "served completion latency p50/p99 and system-steps/s", "arrival_rate_per_s": rate,
-                "slot_classes": [{"npl_cap": c.npl_cap, "ntp_cap": c.ntp_cap, "slots": c.slots, "loads": srv.loads[k],
-                                  "mean_occupied": occ[k], "bytes": srv.pool_bytes[k]}
+                "slot_classes": [{"npl_cap": c.npl_cap, "ntp_cap": c.ntp_cap, "slots": c.slots, "pools": c.pools,
+                                  "loads": srv.loads[k], "mean_occupied": occ[k], "bytes": srv.pool_bytes[k]}
                                  for k, c in enumerate(srv.classes)],
+                "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
                 "chunk_steps": chunk,
                 "config": (f"SyMBA dt 0.005, C3 mass/orbit laws, N log-uniform 8-{nmax}, ntp U{{0..{tpmax}}}, "
                            f"steps log-uniform {smin}-{smax} (the law's 1e3-1e5 scaled down to keep the run "
@@ -369,6 +370,10 @@ def main():
                           "copies": 3}), flush=True)
         return 0
 
+    # hardware queues for the served stream's pools, one stream each (read by
+    # the HIP runtime when this process first initializes it, below)
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
+        os.environ["GPU_MAX_HW_QUEUES"] = "16"
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py runs on one GPU only: the library has no multi-GPU path.", file=sys.stderr)

@@ -267,9 +267,22 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
         own(&sy.ptbeg, 3 * nsys) || own(&sy.ptend, 3 * nsys) || own(&sy.lfirst, nsys) || own(&sy.enc_pp, nsys) ||
         own(&sy.enc_pt, nsys) || own(&sy.flag_pp, nsys) || own(&sy.flag_pt, nsys) || own(&sy.err, nsys) ||
         own(&sy.frozen, nsys) || own(&d_active, nsys) || own(&d_any, 4) || own(&d_total, 4) || own(&d_pt_run, nsys) ||
-        own(&sy.nplm, nsys))
+        own(&sy.nplm, nsys) || own(&d_cb_pl, nsys + 1) || own(&d_cb_tp, nsys + 1))
         return SWG_ERR_HIP;
     hipStream_t st = stream;
+    {
+        // each system's first 64-column block of the pl-pl and pl-tp bit
+        // matrices (index2-side adjacency kernels)
+        std::vector<int32_t> cbp(nsys + 1, 0), cbt(nsys + 1, 0);
+        for (int s = 0; s < nsys; s++) {
+            cbp[s + 1] = cbp[s] + (h_pl_off[s + 1] - h_pl_off[s] + 63) / 64;
+            cbt[s + 1] = cbt[s] + (h_tp_off[s + 1] - h_tp_off[s] + 63) / 64;
+        }
+        nblk_pl = cbp[nsys];
+        nblk_tp = cbt[nsys];
+        HIPCHK(copy_wait(stream, d_cb_pl, cbp.data(), (nsys + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHK(copy_wait(stream, d_cb_tp, cbt.data(), (nsys + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
     std::vector<double> tmp;
     if (upload_aos(st, pl.rh, rh.data(), npl, tmp) || upload_aos(st, pl.vh, vh.data(), npl, tmp)) return SWG_ERR_HIP;
     if (upload_aos(st, tp.rh, tp_rh, ntp, tmp) || upload_aos(st, tp.vh, tp_vh, ntp, tmp)) return SWG_ERR_HIP;
@@ -562,14 +575,14 @@ int Ensemble::build_adjacency()
     if (npp && prm.encounter_check == 1) {
         if (hipError_t e = sas_adjacency(st, sas, pp, npl, npp)) return hipfail(e, "sort-and-sweep adjacency");
     } else if (npp) {
-        launch_col_count(st, pp, sy.pl_off, pl.sys, sy.pl_off, nfrozen ? d_active : nullptr, npl, 1);
+        launch_col_t(st, pp, sy.pl_off, sy.pl_off, d_cb_pl, nsys, nblk_pl, nfrozen ? d_active : nullptr, 1, 0);
         launch_scan_i32_i64(st, pp.colcnt, pp.colstart, npl, nullptr, d_scan);
-        launch_col_fill(st, pp, sy.pl_off, pl.sys, sy.pl_off, nfrozen ? d_active : nullptr, npl, 1);
+        launch_col_t(st, pp, sy.pl_off, sy.pl_off, d_cb_pl, nsys, nblk_pl, nfrozen ? d_active : nullptr, 1, 1);
     }
     if (npt) {
-        launch_col_count(st, pt, sy.pl_off, tp.sys, sy.tp_off, d_pt_run, ntp, 0);
+        launch_col_t(st, pt, sy.pl_off, sy.tp_off, d_cb_tp, nsys, nblk_tp, d_pt_run, 0, 0);
         launch_scan_i32_i64(st, pt.colcnt, pt.colstart, ntp, nullptr, d_scan);
-        launch_col_fill(st, pt, sy.pl_off, tp.sys, sy.tp_off, d_pt_run, ntp, 0);
+        launch_col_t(st, pt, sy.pl_off, sy.tp_off, d_cb_tp, nsys, nblk_tp, d_pt_run, 0, 1);
     }
     return 0;
 }
@@ -1138,10 +1151,21 @@ int Ensemble::run(int64_t nsteps)
         if (int rc = snapshot()) return rc;
         initial_snapshot_done = true;
     }
-    for (int64_t k = 0; k < nsteps; k++) {
-        if (int rc = step()) return rc;
+    for (int64_t k = 0; k < nsteps;) {
+        // WHM systems of few massive bodies: up to the next snapshot (at
+        // most 4096 steps) in one launch; otherwise one step at a time
+        int64_t m = 1;
+        if (prm.integrator == SWG_WHM && whm_fused) {
+            m = std::min<int64_t>(nsteps - k, 4096);
+            if (prm.istep_out > 0) m = std::min<int64_t>(m, prm.istep_out - iout);
+            if (int rc = run_whm_fused((int)m)) return rc;
+            iloop += m;
+        } else if (int rc = step()) {
+            return rc;
+        }
+        k += m;
         if (prm.istep_out > 0) {
-            iout++;
+            iout += m;
             if (iout == prm.istep_out) {
                 iout = 0;
                 if (int rc = snapshot()) return rc;

@@ -352,50 +352,67 @@ __global__ void k_sys_flags(ListDev L, const int32_t *pl_off, int32_t *flag, int
 }
 
 // index2-side adjacency: for column c (a pl body for pl-pl, a particle for
-// pl-tp), the pairs (p, c) in list order, i.e. rows p ascending.
-__global__ void k_col_count(ListDev L, const int32_t *row_off, const int32_t *col_sys, const int32_t *col_off,
-                            const int32_t *run, int ncols, int upper)
+// pl-tp), the pairs (p, c) in list order, i.e. rows p ascending.  One
+// wavefront per 64-column block of a system (cb_off: each system's first
+// block): lane l holds row p0 + l's word of the block, and 64 ballots
+// transpose the 64 x 64 bit tile, so that lane l gets the rows of column
+// block*64 + l with a set bit, 64 rows at a time.  Chunks whose words are all
+// zero are skipped.  With upper (pl-pl) only rows p < column count.
+template <bool FILL>
+__global__ __launch_bounds__(64) void k_col_t(ListDev L, const int32_t *row_off, const int32_t *col_off,
+                                              const int32_t *cb_off, int nsys, const int32_t *run, int upper)
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncols) return;
-    int s = col_sys[c];
+    const int blk = blockIdx.x, lane = threadIdx.x;
+    int lo = 0, hi = nsys;  // system: last s with cb_off[s] <= blk
+    while (hi - lo > 1) {
+        const int m = (lo + hi) >> 1;
+        if (cb_off[m] <= blk) lo = m;
+        else hi = m;
+    }
+    const int s = lo, cb = blk - cb_off[s];
+    const int ncol = col_off[s + 1] - col_off[s], nrow_s = row_off[s + 1] - row_off[s];
+    const int cl = cb * 64 + lane;
+    const bool mine = cl < ncol;
+    const int c = col_off[s] + cl;
     if (run && !run[s]) {
-        L.colcnt[c] = 0;
+        if (!FILL && mine) L.colcnt[c] = 0;
         return;
     }
-    int cl = c - col_off[s];
-    int nrow = upper ? cl : row_off[s + 1] - row_off[s];
-    int nb = L.nb[s];
-    const uint64_t *w = L.words + L.woff[s] + cl / 64;
-    int bit = cl & 63;
+    const int nrow = upper ? min(nrow_s, cb * 64 + 63) : nrow_s;
+    const int nb = L.nb[s];
+    const int64_t wbase = L.woff[s] + cb;
+    const int bit = lane;
+    const uint64_t lowm = (bit == 0) ? 0ull : ((~0ull) >> (64 - bit));
     int cnt = 0;
-    for (int p = 0; p < nrow; p++) cnt += (int)((w[(int64_t)p * nb] >> bit) & 1ull);
-    L.colcnt[c] = cnt;
-}
-
-__global__ void k_col_fill(ListDev L, const int32_t *row_off, const int32_t *col_sys, const int32_t *col_off,
-                           const int32_t *run, int ncols, int upper)
-{
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncols) return;
-    if (L.colcnt[c] == 0) return;
-    int s = col_sys[c];
-    if (run && !run[s]) return;
-    int cl = c - col_off[s];
-    int r0 = row_off[s];
-    int nrow = upper ? cl : row_off[s + 1] - r0;
-    int nb = L.nb[s];
-    int64_t wbase = L.woff[s] + cl / 64;
-    int bit = cl & 63;
-    uint64_t lowm = (bit == 0) ? 0ull : ((~0ull) >> (64 - bit));
-    int64_t o = L.colstart[c];
-    for (int p = 0; p < nrow; p++) {
-        uint64_t w = L.words[wbase + (int64_t)p * nb];
-        if ((w >> bit) & 1ull) {
-            int64_t k = L.rowstart[r0 + p] + L.wpre[wbase + (int64_t)p * nb] + __popcll(w & lowm);
-            L.col[o++] = (int32_t)k;
+    int64_t o = (FILL && mine) ? L.colstart[c] : 0;
+    for (int p0 = 0; p0 < nrow; p0 += 64) {
+        const int p = p0 + lane;
+        const uint64_t w = p < nrow ? L.words[wbase + (int64_t)p * nb] : 0ull;
+        if (__ballot(w != 0ull) == 0ull) continue;
+        uint64_t mycol = 0ull;
+        for (int b = 0; b < 64; b++) {
+            const uint64_t m = __ballot((w >> b) & 1ull);
+            if (b == lane) mycol = m;
+        }
+        if (!mine) continue;
+        if (upper) {
+            const int lim = cl - p0;  // rows p0 + r < cl
+            mycol &= lim >= 64 ? ~0ull : (lim <= 0 ? 0ull : ((~0ull) >> (64 - lim)));
+        }
+        if (!FILL) {
+            cnt += __popcll(mycol);
+        } else {
+            while (mycol) {
+                const int r = __builtin_ctzll(mycol);
+                mycol &= mycol - 1;
+                const int64_t widx = wbase + (int64_t)(p0 + r) * nb;
+                const uint64_t wr = L.words[widx];
+                const int64_t k = L.rowstart[row_off[s] + p0 + r] + L.wpre[widx] + __popcll(wr & lowm);
+                L.col[o++] = (int32_t)k;
+            }
         }
     }
+    if (!FILL && mine) L.colcnt[c] = cnt;
 }
 
 // ---------------------------------------------------------------------------
@@ -550,15 +567,12 @@ void launch_sys_flags(hipStream_t st, ListDev L, const int32_t *pl_off, int32_t 
 {
     k_sys_flags<<<cdiv(nsys, 256), 256, 0, st>>>(L, pl_off, flag, nsys);
 }
-void launch_col_count(hipStream_t st, ListDev L, const int32_t *row_off, const int32_t *col_sys,
-                      const int32_t *col_off, const int32_t *run, int ncols, int upper)
+void launch_col_t(hipStream_t st, ListDev L, const int32_t *row_off, const int32_t *col_off, const int32_t *cb_off,
+                  int nsys, int nblocks, const int32_t *run, int upper, int fill)
 {
-    if (ncols) k_col_count<<<cdiv(ncols, 64), 64, 0, st>>>(L, row_off, col_sys, col_off, run, ncols, upper);
-}
-void launch_col_fill(hipStream_t st, ListDev L, const int32_t *row_off, const int32_t *col_sys,
-                     const int32_t *col_off, const int32_t *run, int ncols, int upper)
-{
-    if (ncols) k_col_fill<<<cdiv(ncols, 64), 64, 0, st>>>(L, row_off, col_sys, col_off, run, ncols, upper);
+    if (!nblocks) return;
+    if (fill) k_col_t<true><<<nblocks, 64, 0, st>>>(L, row_off, col_off, cb_off, nsys, run, upper);
+    else k_col_t<false><<<nblocks, 64, 0, st>>>(L, row_off, col_off, cb_off, nsys, run, upper);
 }
 void launch_list_check_pp(hipStream_t st, ListDev pp, PlDev pl, SysDev sy, int64_t n, int irec, double dt,
                           int32_t *any)

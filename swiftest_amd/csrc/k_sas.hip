@@ -19,8 +19,10 @@
 //   3. the narrow phase runs on every candidate in parallel; the survivors are
 //      compacted and radix-sorted by (index1, index2), which is the triangular
 //      list order, and the index2-side adjacency is a stable sort by index2.
-#include <hipcub/hipcub.hpp>
-
+// The sorts and the compaction are this file's own kernels: a stable LSD
+// radix sort (8-bit digits; per pass a tile histogram, one scan, and a
+// scatter that ranks equal digits with wavefront ballots) and an ordered
+// ballot compaction.
 #include "swg_kernels.h"
 #include "swg_math.h"
 
@@ -32,13 +34,15 @@ static inline int cdiv64(int64_t a, int b) { return (int)((a + b - 1) / b); }
 // narrow phase's own rounding of r2min against R^2 many times over
 constexpr double SAS_REL = 1e-12;
 
-__global__ void k_sas_extents(PlDev pl, int npl, double dt, double *lo, double *hi, int32_t *idx)
+__device__ inline uint64_t sortable(double x);
+
+__global__ void k_sas_extents(PlDev pl, int npl, double dt, uint64_t *lokey, double *hi, int32_t *idx)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= npl) return;
     const double x = pl.rh.x[g], x1 = x + pl.vb.x[g] * dt, r = pl.renc[g];
     const double m = SAS_REL * (fabs(x) + fabs(x1) + r) + 1e-300;
-    lo[g] = fmin(x, x1) - r - m;
+    lokey[g] = sortable(fmin(x, x1) - r - m);
     hi[g] = fmax(x, x1) + r + m;
     idx[g] = g;
 }
@@ -150,6 +154,155 @@ __global__ void k_sas_cols(const int32_t *sorted_i2, int64_t n, int npl, int32_t
     colcnt[g] = (int32_t)(c1 - c0);
 }
 
+// ---- stable LSD radix sort ------------------------------------------------
+// Keys K (uint64_t or uint32_t) with optional uint32 values, digits of 8 bits
+// from begin_bit to end_bit; then, when seg is given, digits of seg[value]
+// (the system of each body), so that the result is ordered by (seg, key) and
+// stable.  A tile of RS_TILE items is handled by one workgroup of 4
+// wavefronts; each wavefront takes a contiguous quarter in chunks of 64.
+constexpr int RS_WG = 256, RS_TILE = 4096, RS_BINS = 256;
+
+template <class K>
+__device__ inline int rs_digit(K key, uint32_t val, const int32_t *seg, int shift)
+{
+    return seg ? (int)(((uint32_t)seg[val] >> shift) & 255u) : (int)((key >> shift) & (K)255);
+}
+
+template <class K>
+__global__ __launch_bounds__(RS_WG) void k_rs_hist(const K *keys, const uint32_t *vals, const int32_t *seg, int64_t n,
+                                                   int shift, int ntiles, int32_t *hist)
+{
+    __shared__ int32_t h[RS_BINS];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int k = threadIdx.x; k < RS_TILE; k += RS_WG) {
+        const int64_t i = base + k;
+        if (i < n) atomicAdd(&h[rs_digit<K>(keys[i], vals ? vals[i] : 0u, seg, shift)], 1);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+template <class K>
+__global__ __launch_bounds__(RS_WG) void k_rs_scatter(const K *kin, const uint32_t *vin, K *kout, uint32_t *vout,
+                                                      const int32_t *seg, int64_t n, int shift, int ntiles,
+                                                      const int64_t *offs)
+{
+    __shared__ int32_t wcnt[4][RS_BINS];
+    __shared__ int64_t boff[RS_BINS];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE + (int64_t)wv * (RS_TILE / 4);
+    for (int b = lane; b < RS_BINS; b += 64) wcnt[wv][b] = 0;
+    __syncthreads();
+    for (int c = 0; c < RS_TILE / 4; c += 64) {
+        const int64_t i = base + c + lane;
+        if (i < n) atomicAdd(&wcnt[wv][rs_digit<K>(kin[i], vin ? vin[i] : 0u, seg, shift)], 1);
+    }
+    __syncthreads();
+    {
+        const int b = threadIdx.x;
+        boff[b] = offs[(int64_t)b * ntiles + blockIdx.x];
+        int run = 0;
+        for (int q = 0; q < 4; q++) {
+            const int c = wcnt[q][b];
+            wcnt[q][b] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int c = 0; c < RS_TILE / 4; c += 64) {
+        const int64_t i = base + c + lane;
+        const bool valid = i < n;
+        K key = 0;
+        uint32_t val = 0;
+        int d = 0;
+        if (valid) {
+            key = kin[i];
+            val = vin ? vin[i] : 0u;
+            d = rs_digit<K>(key, val, seg, shift);
+        }
+        // lanes holding the same digit: 8 ballots
+        uint64_t m = __ballot(valid);
+        for (int bit = 0; bit < 8; bit++) {
+            const bool set = (d >> bit) & 1;
+            const uint64_t bb = __ballot(set);
+            m &= set ? bb : ~bb;
+        }
+        const int rank = __popcll(m & below), cnt = __popcll(m);
+        const int prev = valid ? wcnt[wv][d] : 0;
+        __builtin_amdgcn_wave_barrier();
+        if (valid) {
+            const int64_t pos = boff[d] + prev + rank;
+            kout[pos] = key;
+            if (vout) vout[pos] = val;
+            if (rank == 0) wcnt[wv][d] = prev + cnt;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// ---- ordered compaction of flagged keys ------------------------------------
+constexpr int CP_TILE = 4096;
+
+__global__ __launch_bounds__(RS_WG) void k_cp_count(const uint8_t *flag, int64_t n, int32_t *cnt)
+{
+    __shared__ int32_t wsum[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t base = (int64_t)blockIdx.x * CP_TILE + (int64_t)wv * (CP_TILE / 4);
+    int c = 0;
+    for (int k = 0; k < CP_TILE / 4; k += 64) {
+        const int64_t i = base + k + lane;
+        c += __popcll(__ballot(i < n && flag[i]));
+    }
+    if (lane == 0) wsum[wv] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) cnt[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+__global__ __launch_bounds__(RS_WG) void k_cp_scatter(const uint64_t *key, const uint8_t *flag, int64_t n,
+                                                      const int64_t *boff, uint64_t *out)
+{
+    __shared__ int32_t wsum[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t base = (int64_t)blockIdx.x * CP_TILE + (int64_t)wv * (CP_TILE / 4);
+    int c = 0;
+    for (int k = 0; k < CP_TILE / 4; k += 64) {
+        const int64_t i = base + k + lane;
+        c += __popcll(__ballot(i < n && flag[i]));
+    }
+    if (lane == 0) wsum[wv] = c;
+    __syncthreads();
+    int64_t pos = boff[blockIdx.x];
+    for (int q = 0; q < wv; q++) pos += wsum[q];
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int k = 0; k < CP_TILE / 4; k += 64) {
+        const int64_t i = base + k + lane;
+        const bool f = i < n && flag[i];
+        const uint64_t m = __ballot(f);
+        if (f) out[pos + __popcll(m & below)] = key[i];
+        pos += __popcll(m);
+    }
+}
+
+// transformed doubles: the unsigned order of the bits is the numeric order
+__device__ inline uint64_t sortable(double x)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(x);
+    return (u >> 63) ? ~u : (u | (1ull << 63));
+}
+__device__ inline double unsortable(uint64_t u)
+{
+    return __longlong_as_double((long long)((u >> 63) ? (u & ~(1ull << 63)) : ~u));
+}
+
+__global__ void k_sas_unkey(const uint64_t *skey, int npl, double *slo)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < npl) slo[r] = unsortable(skey[r]);
+}
+
 // ---- host side ---------------------------------------------------------------
 
 static int grow(void **p, size_t *cap, size_t need)
@@ -171,21 +324,75 @@ static int grow(void **p, size_t *cap, size_t need)
 
 SasWork::~SasWork()
 {
-    for (void *p : {(void *)lo, (void *)hi, (void *)slo, (void *)idx, (void *)sidx, (void *)rank, (void *)cnt,
-                    (void *)coff, (void *)key, (void *)flag, (void *)sel, (void *)skey, (void *)nsel, temp,
-                    (void *)pid, (void *)spid, (void *)si2})
+    for (void *p : {(void *)lokey, (void *)slokey, (void *)hi, (void *)slo, (void *)idx, (void *)sidx, (void *)rank,
+                    (void *)cnt, (void *)coff, (void *)key, (void *)flag, (void *)sel, (void *)skey, (void *)nsel,
+                    (void *)pid, (void *)spid, (void *)si2, rtk, rtv, rhist, roffs, rscan, rtotal})
         if (p) (void)hipFree(p);
+}
+
+// Stable sort of n keys (and values) by their bits [begin_bit, end_bit), then
+// by seg[value] over seg_bits bits when seg is given.  kin / vin are not
+// written; the result is in kout / vout.
+template <class K>
+static hipError_t radix_sort(hipStream_t st, SasWork &w, const K *kin, const uint32_t *vin, K *kout, uint32_t *vout,
+                             int64_t n, int begin_bit, int end_bit, const int32_t *seg, int seg_bits)
+{
+    if (n <= 0) return hipSuccess;
+    const int ntiles = (int)((n + RS_TILE - 1) / RS_TILE);
+    const int kp = (end_bit - begin_bit + 7) / 8, sp = seg ? (seg_bits + 7) / 8 : 0, np = kp + sp;
+    if (np == 0) {
+        SASCHK(hipMemcpyAsync(kout, kin, n * sizeof(K), hipMemcpyDeviceToDevice, st));
+        if (vin) SASCHK(hipMemcpyAsync(vout, vin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+        return hipSuccess;
+    }
+    const int nh = RS_BINS * ntiles;
+    if (grow(&w.rtk, &w.rtk_cap, n * sizeof(K)) || (vin && grow(&w.rtv, &w.rtv_cap, n * sizeof(uint32_t))) ||
+        grow(&w.rhist, &w.rhist_cap, nh * sizeof(int32_t)) || grow(&w.roffs, &w.roffs_cap, nh * sizeof(int64_t)) ||
+        grow(&w.rscan, &w.rscan_cap, scan_scratch_bytes(nh)) || grow(&w.rtotal, &w.rtotal_cap, sizeof(int64_t)))
+        return hipErrorOutOfMemory;
+    const K *sk = kin;
+    const uint32_t *sv = vin;
+    for (int p = 0; p < np; p++) {
+        const bool last_to_out = (np - 1 - p) % 2 == 0;
+        K *dk = last_to_out ? kout : (K *)w.rtk;
+        uint32_t *dv = vin ? (last_to_out ? vout : (uint32_t *)w.rtv) : nullptr;
+        const bool segpass = p >= kp;
+        const int shift = segpass ? 8 * (p - kp) : begin_bit + 8 * p;
+        const int32_t *sg = segpass ? seg : nullptr;
+        k_rs_hist<K><<<ntiles, RS_WG, 0, st>>>(sk, sv, sg, n, shift, ntiles, (int32_t *)w.rhist);
+        launch_scan_i32_i64(st, (const int32_t *)w.rhist, (int64_t *)w.roffs, nh, (int64_t *)w.rtotal, w.rscan);
+        k_rs_scatter<K><<<ntiles, RS_WG, 0, st>>>(sk, sv, dk, dv, sg, n, shift, ntiles, (const int64_t *)w.roffs);
+        sk = dk;
+        sv = dv;
+    }
+    return hipGetLastError();
+}
+
+// the flagged keys, in order, into out; their number into *nsel_dev
+static hipError_t compact(hipStream_t st, SasWork &w, const uint64_t *key, const uint8_t *flag, int64_t n,
+                          uint64_t *out, int64_t *nsel_dev)
+{
+    const int nt = (int)((n + CP_TILE - 1) / CP_TILE);
+    if (grow(&w.rhist, &w.rhist_cap, (size_t)(nt ? nt : 1) * sizeof(int32_t)) ||
+        grow(&w.roffs, &w.roffs_cap, (size_t)(nt ? nt : 1) * sizeof(int64_t)) ||
+        grow(&w.rscan, &w.rscan_cap, scan_scratch_bytes(nt)))
+        return hipErrorOutOfMemory;
+    if (nt) k_cp_count<<<nt, RS_WG, 0, st>>>(flag, n, (int32_t *)w.rhist);
+    launch_scan_i32_i64(st, (const int32_t *)w.rhist, (int64_t *)w.roffs, nt, nsel_dev, w.rscan);
+    if (nt) k_cp_scatter<<<nt, RS_WG, 0, st>>>(key, flag, n, (const int64_t *)w.roffs, out);
+    return hipGetLastError();
 }
 
 hipError_t sas_candidates(hipStream_t st, SasWork &w, PlDev pl, SysDev sy, int npl, int nsys,
                           const int32_t *active, double dt, int64_t *ncand_dev, void *scan_scratch)
 {
     if (w.n < npl) {
-        for (void *p : {(void *)w.lo, (void *)w.hi, (void *)w.slo, (void *)w.idx, (void *)w.sidx, (void *)w.rank,
-                        (void *)w.cnt, (void *)w.coff})
+        for (void *p : {(void *)w.lokey, (void *)w.slokey, (void *)w.hi, (void *)w.slo, (void *)w.idx,
+                        (void *)w.sidx, (void *)w.rank, (void *)w.cnt, (void *)w.coff})
             if (p) (void)hipFree(p);
         const size_t n = (size_t)npl + 1;
-        SASCHK(hipMalloc(&w.lo, n * sizeof(double)));
+        SASCHK(hipMalloc(&w.lokey, n * sizeof(uint64_t)));
+        SASCHK(hipMalloc(&w.slokey, n * sizeof(uint64_t)));
         SASCHK(hipMalloc(&w.hi, n * sizeof(double)));
         SASCHK(hipMalloc(&w.slo, n * sizeof(double)));
         SASCHK(hipMalloc(&w.idx, n * sizeof(int32_t)));
@@ -195,14 +402,14 @@ hipError_t sas_candidates(hipStream_t st, SasWork &w, PlDev pl, SysDev sy, int n
         SASCHK(hipMalloc(&w.coff, n * sizeof(int64_t)));
         w.n = npl;
     }
-    k_sas_extents<<<cdiv64(npl, 256), 256, 0, st>>>(pl, npl, dt, w.lo, w.hi, w.idx);
-    size_t bytes = 0;
-    SASCHK(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, bytes, w.lo, w.slo, w.idx, w.sidx, npl, nsys,
-                                                       sy.pl_off, sy.pl_off + 1, 0, 64, st));
-    if (grow(&w.temp, &w.temp_bytes, bytes)) return hipErrorOutOfMemory;
-    bytes = w.temp_bytes;
-    SASCHK(hipcub::DeviceSegmentedRadixSort::SortPairs(w.temp, bytes, w.lo, w.slo, w.idx, w.sidx, npl, nsys,
-                                                       sy.pl_off, sy.pl_off + 1, 0, 64, st));
+    k_sas_extents<<<cdiv64(npl, 256), 256, 0, st>>>(pl, npl, dt, w.lokey, w.hi, w.idx);
+    // each system's bodies by lower extent: the keys' 64 bits, then the system
+    int sb = 0;
+    while ((1ll << sb) < nsys) sb++;
+    if (hipError_t e = radix_sort<uint64_t>(st, w, w.lokey, (const uint32_t *)w.idx, w.slokey, (uint32_t *)w.sidx, npl,
+                                            0, 64, nsys > 1 ? pl.sys : nullptr, sb))
+        return e;
+    k_sas_unkey<<<cdiv64(npl, 256), 256, 0, st>>>(w.slokey, npl, w.slo);
     k_sas_rank<<<cdiv64(npl, 256), 256, 0, st>>>(w.sidx, npl, w.rank);
     k_sas_count<<<cdiv64(npl, 256), 256, 0, st>>>(w.slo, w.hi, w.rank, pl.sys, sy.pl_off, active, npl, w.cnt);
     launch_scan_i32_i64(st, w.cnt, w.coff, npl, ncand_dev, scan_scratch);
@@ -216,11 +423,7 @@ hipError_t sas_narrow(hipStream_t st, SasWork &w, PlDev pl, int npl, int64_t nca
         return hipErrorOutOfMemory;
     if (!w.nsel) SASCHK(hipMalloc(&w.nsel, sizeof(int64_t)));
     if (ncand) k_sas_narrow<<<cdiv64(ncand, 256), 256, 0, st>>>(pl, w.coff, w.rank, w.sidx, npl, ncand, dt, w.key, w.flag);
-    size_t bytes = 0;
-    SASCHK(hipcub::DeviceSelect::Flagged(nullptr, bytes, w.key, w.flag, w.sel, w.nsel, ncand, st));
-    if (grow(&w.temp, &w.temp_bytes, bytes)) return hipErrorOutOfMemory;
-    bytes = w.temp_bytes;
-    SASCHK(hipcub::DeviceSelect::Flagged(w.temp, bytes, w.key, w.flag, w.sel, w.nsel, ncand, st));
+    if (hipError_t e = compact(st, w, w.key, w.flag, ncand, w.sel, w.nsel)) return e;
     SASCHK(hipMemcpyAsync(nsel_host, w.nsel, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     SASCHK(hipStreamSynchronize(st));
     return hipGetLastError();
@@ -237,12 +440,8 @@ hipError_t sas_fill(hipStream_t st, SasWork &w, ListDev &pp, PlDev pl, int npl, 
     // each half vary
     int kb = 1;
     while ((1ll << kb) < npl) kb++;
-    const int endbit = 32 + kb;
-    size_t bytes = 0;
-    SASCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, w.sel, w.skey, npp, 0, endbit, st));
-    if (grow(&w.temp, &w.temp_bytes, bytes)) return hipErrorOutOfMemory;
-    bytes = w.temp_bytes;
-    SASCHK(hipcub::DeviceRadixSort::SortKeys(w.temp, bytes, w.sel, w.skey, npp, 0, endbit, st));
+    if (hipError_t e = radix_sort<uint64_t>(st, w, w.sel, nullptr, w.skey, nullptr, npp, 0, 32 + kb, nullptr, 0))
+        return e;
     k_sas_rows<<<cdiv64(npl, 256), 256, 0, st>>>(w.skey, npp, npl, pp.rowcnt, pp.rowstart);
     if (npp) k_sas_fill<<<cdiv64(npp, 256), 256, 0, st>>>(pp, w.skey, npp, pl.levelg, pl.levelm, w.pid);
     return hipGetLastError();
@@ -251,11 +450,11 @@ hipError_t sas_fill(hipStream_t st, SasWork &w, ListDev &pp, PlDev pl, int npl, 
 hipError_t sas_adjacency(hipStream_t st, SasWork &w, ListDev &pp, int npl, int64_t npp)
 {
     if (!npp) return hipSuccess;
-    size_t bytes = 0;
-    SASCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, pp.i2, w.si2, w.pid, pp.col, npp, 0, 32, st));
-    if (grow(&w.temp, &w.temp_bytes, bytes)) return hipErrorOutOfMemory;
-    bytes = w.temp_bytes;
-    SASCHK(hipcub::DeviceRadixSort::SortPairs(w.temp, bytes, pp.i2, w.si2, w.pid, pp.col, npp, 0, 32, st));
+    int kb = 1;
+    while ((1ll << kb) < npl) kb++;
+    if (hipError_t e = radix_sort<uint32_t>(st, w, (const uint32_t *)pp.i2, (const uint32_t *)w.pid,
+                                            (uint32_t *)w.si2, (uint32_t *)pp.col, npp, 0, kb, nullptr, 0))
+        return e;
     k_sas_cols<<<cdiv64(npl, 256), 256, 0, st>>>(w.si2, npp, npl, pp.colcnt, pp.colstart);
     return hipGetLastError();
 }

@@ -43,6 +43,13 @@ struct Ensemble {
 
     Tile *d_tiles_pl = nullptr, *d_tiles_tp = nullptr;
     bool tiles_ok = false;  // every tile array built (build_tiles succeeded)
+    // WHM whole-step kernel: tiles (system, first / end particle, writer)
+    int4 *d_tiles_whm = nullptr;
+    // first 64-column block of each system's pl-pl / pl-tp bit matrix
+    int32_t *d_cb_pl = nullptr, *d_cb_tp = nullptr;
+    int nblk_pl = 0, nblk_tp = 0;
+    int ntiles_whm = 0;
+    bool whm_fused = false;
     int4 *d_tiles_pp = nullptr, *d_tiles_pt = nullptr;
     int ntiles_pl = 0, ntiles_tp = 0, ntiles_pp = 0;
     int32_t *d_any = nullptr, *d_pt_run = nullptr;
@@ -161,6 +168,7 @@ struct Ensemble {
     int step_helio_symba();
     int whm_setup();
     int step_whm();
+    int run_whm_fused(int nsteps);
     int whm_snapshot();
 };
 

@@ -101,23 +101,25 @@ void launch_rows_fill(hipStream_t st, ListDev L, const int32_t *row_sys, int upp
                       const int32_t *col_off, int nrows, int32_t *lev1g = nullptr, int32_t *lev1m = nullptr,
                       int32_t *lev2g = nullptr, int32_t *lev2m = nullptr);
 void launch_sys_flags(hipStream_t st, ListDev L, const int32_t *pl_off, int32_t *flag, int nsys);
-void launch_col_count(hipStream_t st, ListDev L, const int32_t *row_off, const int32_t *col_sys,
-                      const int32_t *col_off, const int32_t *run, int ncols, int upper);
-void launch_col_fill(hipStream_t st, ListDev L, const int32_t *row_off, const int32_t *col_sys,
-                     const int32_t *col_off, const int32_t *run, int ncols, int upper);
+// index2-side adjacency (counts, then fill) of a list whose bit matrix is in
+// L.words; cb_off: each system's first 64-column block (nsys + 1 entries)
+void launch_col_t(hipStream_t st, ListDev L, const int32_t *row_off, const int32_t *col_off, const int32_t *cb_off,
+                  int nsys, int nblocks, const int32_t *run, int upper, int fill);
 
 // ---- sort-and-sweep broad phase of the level-0 pl-pl check (k_sas.hip) ----
 struct SasWork {
     int n = 0;  // bodies the per-body arrays hold
-    double *lo = nullptr, *hi = nullptr, *slo = nullptr;
+    uint64_t *lokey = nullptr, *slokey = nullptr;  // lower x-extents as sortable keys, and sorted
+    double *hi = nullptr, *slo = nullptr;
     int32_t *idx = nullptr, *sidx = nullptr, *rank = nullptr, *cnt = nullptr;
     int64_t *coff = nullptr, *nsel = nullptr;
     uint64_t *key = nullptr, *sel = nullptr, *skey = nullptr;
     uint8_t *flag = nullptr;
     int32_t *pid = nullptr, *spid = nullptr, *si2 = nullptr;
-    void *temp = nullptr;
-    size_t temp_bytes = 0, key_cap = 0, flag_cap = 0, sel_cap = 0, skey_cap = 0, pid_cap = 0, spid_cap = 0,
-           si2_cap = 0;
+    // radix sort and compaction scratch
+    void *rtk = nullptr, *rtv = nullptr, *rhist = nullptr, *roffs = nullptr, *rscan = nullptr, *rtotal = nullptr;
+    size_t key_cap = 0, flag_cap = 0, sel_cap = 0, skey_cap = 0, pid_cap = 0, spid_cap = 0, si2_cap = 0,
+           rtk_cap = 0, rtv_cap = 0, rhist_cap = 0, roffs_cap = 0, rscan_cap = 0, rtotal_cap = 0;
     ~SasWork();
 };
 // extents, per-system sort, candidate counts and their scan (total in *ncand_dev)

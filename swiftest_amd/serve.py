@@ -6,11 +6,16 @@ slots allocated once, within the device-memory budget, one pool per size
 class.  Each pool is driven by its own host thread with its own library
 context (its own HIP stream), so the pools' steps overlap on the device.
 
-A request goes to the smallest class its system fits.  Between steps a pool
-loads the requests that have arrived into its free slots (swg_system_load)
-while the systems in its other slots keep running; it then advances all of
-its occupied slots together, one launch per step phase, for at most `chunk`
-steps or until the next request in it completes.  A completed request's state
+A request goes to the smallest class its system fits.  A class may have
+several pools, each with its own thread and stream, that take requests from
+the class's queue: the SyMBA recursion of a pool's systems runs in lockstep
+(a substep at some level is taken when any of its systems needs it), so
+large crowded systems are better spread over several pools, whose steps
+overlap, than packed into one.  Between steps a pool loads the requests that
+have arrived into its free slots (swg_system_load) while the systems in its
+other slots keep running; it then advances all of its occupied slots
+together, one launch per step phase, for at most `chunk` steps or until the
+next request in it completes.  A completed request's state
 is copied out (swg_system_get_slot) and its slot is frozen and free for the
 next arrival.  Padding bodies make a slot's system evolve exactly as it would
 alone, so each result equals that of the request run by itself.
@@ -34,17 +39,19 @@ class Request:
     arrival: float = 0.0    # seconds after the server starts
     admitted: float = -1.0
     done: float = -1.0
-    lane: int = -1
+    lane: tuple = ()        # (class, pool) that served it
     slot: int = -1
     state: dict = field(default_factory=dict)
 
 
 @dataclass
 class SlotClass:
-    """`slots` slots of room for npl_cap massive bodies and ntp_cap particles."""
+    """`pools` pools of `slots` slots each, every slot with room for npl_cap
+    massive bodies and ntp_cap particles."""
     npl_cap: int
     ntp_cap: int
     slots: int
+    pools: int = 1
 
 
 def request_bytes(npl: int, ntp: int) -> int:
@@ -57,8 +64,13 @@ def request_bytes(npl: int, ntp: int) -> int:
     return int(200 * n + 100 * m + 12 * (words_pp + words_pt) + 64 * (n + m) + (1 << 16))
 
 
-# size classes for requests of up to 4096 massive bodies and 100000 particles
-DEFAULT_CLASSES = (SlotClass(128, 100000, 16), SlotClass(1024, 100000, 8), SlotClass(4096, 100000, 4))
+# size classes for requests of up to 4096 massive bodies and 100000 particles:
+# small systems packed 32 to a pool, the largest one per pool (their
+# recursions are long and run best side by side on their own streams).
+# The pools' streams overlap on the device only as far as the runtime gives
+# the process hardware queues: set GPU_MAX_HW_QUEUES to at least the number
+# of pools (15 here) before the process's first HIP call (bench.py uses 16).
+DEFAULT_CLASSES = (SlotClass(128, 100000, 32, 1), SlotClass(1024, 100000, 4, 4), SlotClass(4096, 100000, 1, 10))
 
 
 class Server:
@@ -70,7 +82,7 @@ class Server:
         self.classes = list(classes)
         self.integrator, self.dt, self.lclose, self.chunk = integrator, dt, lclose, max(1, int(chunk))
         self.pair_mode = pair_mode
-        self.pool_bytes = [c.slots * request_bytes(c.npl_cap, c.ntp_cap) for c in self.classes]
+        self.pool_bytes = [c.pools * c.slots * request_bytes(c.npl_cap, c.ntp_cap) for c in self.classes]
         if sum(self.pool_bytes) > hbm_budget:
             raise ValueError(f"slot pools need {sum(self.pool_bytes)} bytes, over the budget of {hbm_budget}")
         self.budget = hbm_budget
@@ -86,7 +98,7 @@ class Server:
         raise ValueError(f"no slot class holds a system of {system.npl} + {system.ntp} bodies")
 
     # ---- one pool ----
-    def _lane(self, k, pending, t0, errors):
+    def _lane(self, k, p, pending, t0, errors):
         try:
             c = self.classes[k]
             ctx = core.new_context()
@@ -95,7 +107,7 @@ class Server:
                                             lclose=self.lclose, ctx=ctx, pair_mode=self.pair_mode)
                 with self._lock:
                     self.pools.append((k, pool))
-                self._drive(k, c, pool, pending, t0)
+                self._drive(k, p, c, pool, pending, t0)
                 del pool
             finally:
                 core.library().swg_finalize(ctx)
@@ -104,7 +116,7 @@ class Server:
             with self._lock:
                 self._lock.notify_all()
 
-    def _drive(self, k, c, pool, pending, t0):
+    def _drive(self, k, p, c, pool, pending, t0):
         held = [None] * c.slots            # the request each slot holds
         remaining = np.zeros(c.slots, np.int64)
         while True:
@@ -119,10 +131,11 @@ class Server:
             for s, r in zip(free, take):
                 pool.load(s, r.system)
                 r.admitted = time.perf_counter() - t0
-                r.lane, r.slot = k, s
+                r.lane, r.slot = (k, p), s
                 held[s] = r
                 remaining[s] = r.nsteps
-                self.loads[k] += 1
+                with self._lock:
+                    self.loads[k] += 1
             busy = [s for s in range(c.slots) if held[s] is not None]
             if not busy:
                 if empty:
@@ -152,8 +165,8 @@ class Server:
             queues[self.class_of(r.system)].append(r)
         errors = []
         t0 = time.perf_counter()
-        threads = [threading.Thread(target=self._lane, args=(k, queues[k], t0, errors))
-                   for k in range(len(self.classes)) if queues[k]]
+        threads = [threading.Thread(target=self._lane, args=(k, p, queues[k], t0, errors))
+                   for k in range(len(self.classes)) if queues[k] for p in range(self.classes[k].pools)]
         for t in threads:
             t.start()
         for t in threads:
