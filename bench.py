@@ -353,6 +353,9 @@ def main():
     ap.add_argument("--serve-seq-sample", type=int, default=8)
     ap.add_argument("--no-sc-leg", action="store_true",
                     help="skip the step with the pl-tp encounter check not short-circuited")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the served-stream leg, printed as one JSON object (the full run starts it this way "
+                         "in a process of its own)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 2**27 doubles (1 GiB) device to device three times: the known-size copy "
                          "that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -378,6 +381,13 @@ def main():
     if args.gpus != 1 or world != 1:
         print("bench.py runs on one GPU only: the library has no multi-GPU path.", file=sys.stderr)
         return 2
+
+    serve_args = (args.serve_requests, args.serve_rate, args.serve_nmax, args.serve_tpmax, args.serve_steps[0],
+                  args.serve_steps[1], args.serve_chunk, args.serve_seq_sample)
+    if args.serve_only:
+        log = lambda *a: print(*a, file=sys.stderr, flush=True)
+        print(json.dumps(serving_leg(*serve_args, log)), flush=True)
+        return 0
 
     from swiftest_amd import ics
     import swiftest_amd as S
@@ -477,9 +487,19 @@ def main():
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(args.ens_systems, args.ens_steps, 1, args.ens_sample, log)
     if not args.no_serve:
-        out["serving"] = serving_leg(args.serve_requests, args.serve_rate, args.serve_nmax, args.serve_tpmax,
-                                     args.serve_steps[0], args.serve_steps[1], args.serve_chunk,
-                                     args.serve_seq_sample, log)
+        # in a process of its own: the served stream's pools and threads get a
+        # runtime that the legs above have not filled
+        import subprocess
+        cmd = [sys.executable, "-u", os.path.abspath(__file__), "--serve-only",
+               "--serve-requests", str(args.serve_requests), "--serve-rate", str(args.serve_rate),
+               "--serve-nmax", str(args.serve_nmax), "--serve-tpmax", str(args.serve_tpmax),
+               "--serve-steps", str(args.serve_steps[0]), str(args.serve_steps[1]),
+               "--serve-chunk", str(args.serve_chunk), "--serve-seq-sample", str(args.serve_seq_sample)]
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"served-stream leg failed with status {r.returncode}")
+        out["serving"] = json.loads([l for l in r.stdout.splitlines() if l.startswith("{")][-1])
+        out["serving"]["process"] = "own"
     if not args.no_cpu:
         log("[bench] timing the CPU baseline sample")
         out["cpu_baseline"] = cpu_baseline(sysic, args.cpu_seconds)

@@ -189,6 +189,14 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
         if (k == 0 || k == 2 || k == 4 || k == 8) tune.kick_pl_eval = k;
     }
     nsys = ns;
+    {
+        // exact-mode massive-body kick before any autotuning: with fewer
+        // than two wavefronts per SIMD in the one-lane-per-body kernel, the
+        // terms go to 8 evaluating wavefronts per 64-body tile instead
+        int64_t npl_all = 0;
+        for (int s = 0; s < ns; s++) npl_all += npl_s[s] > 0 ? npl_s[s] : 0;
+        if (!getenv("SWG_KICK_PL_EVAL") && (npl_all + 63) / 64 < 2048) tune.kick_pl_eval = 8;
+    }
     h_pl_off.assign(nsys + 1, 0);
     h_tp_off.assign(nsys + 1, 0);
     for (int s = 0; s < nsys; s++) {

@@ -537,36 +537,55 @@ __global__ void k_whm_h2j_all(WhmArgs a, int nsys)
 
 // ---------------------------------------------------------------------------
 // Whole WHM steps in one launch (systems of at most WR_N massive bodies).
-// One wavefront per tile: a system and up to 64 of its test particles.  The
-// wavefront integrates the system's massive bodies itself, one lane per
-// body (every tile of a system computes the same values from the same
-// inputs; its first tile writes them back), then its own particles, one lane
-// each, step after step, with no launch and no host round trip in between.
-// Each sum over the bodies in order (the Jacobi maps' prefixes, ah0, ah2) is
-// formed by each lane from the terms the bodies' lanes put in LDS, added in
-// the order of whm_coord.f90 / whm_kick.f90, so every value equals the one
-// the per-step kernels above (and the oracle) produce.
+// One workgroup of two wavefronts per tile: a system and up to 64 of its
+// test particles.  The first wavefront integrates the system's massive
+// bodies, one lane per body (every tile of a system computes the same values
+// from the same inputs; its first tile writes them back); the second
+// integrates the tile's particles, one lane each.  They run one step apart:
+// while the particles take step k, which needs the bodies' positions at the
+// start and at the end of step k, the bodies already take step k+1.  The
+// positions after each step pass through three LDS buffers; one workgroup
+// barrier per step.  Each sum over the bodies in order (the Jacobi maps'
+// prefixes, ah0, ah2) is formed by each lane from the terms the bodies' lanes
+// put in LDS, added in the order of whm_coord.f90 / whm_kick.f90, so every
+// value equals the one the per-step kernels above (and the oracle) produce.
 // ---------------------------------------------------------------------------
 constexpr int WR_N = 64;
 
-__global__ __launch_bounds__(64) void k_whm_run(WhmArgs a, TpDev tp, const int4 *tiles, double dt, int nsteps)
+// ordering of LDS accesses between the lanes of one wavefront
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(128) void k_whm_run(WhmArgs a, TpDev tp, const int4 *tiles, double dt, int nsteps)
 {
     __shared__ double sx[WR_N], sy[WR_N], sz[WR_N], su[WR_N], sv[WR_N], sw[WR_N];
     __shared__ double sg[WR_N], seta[WR_N], sf[WR_N], sr[WR_N];
     __shared__ int sact[WR_N];
+    // positions (and ah0 factors) of the bodies after j steps, in slot j % 3
+    __shared__ double bx[3][WR_N], by[3][WR_N], bz[3][WR_N], bf[3][WR_N];
+    __shared__ int sfail;
     const int4 tl = tiles[blockIdx.x];
-    const int s = tl.x, lane = threadIdx.x;
+    const int s = tl.x, lane = threadIdx.x & 63;
+    const bool plw = threadIdx.x < 64;  // the bodies' wavefront
     const bool writer = tl.w != 0;
     const PlDev &pl = a.pl;
     const WhmDev &w = a.w;
     const int p0 = a.sy.pl_off[s], n = a.sy.pl_off[s + 1] - p0;
     const double GMcb = a.sy.GMcb[s], dth = 0.5 * dt;
-    const bool isb = lane < n;
+    const int lfirst0 = a.sy.lfirst[s];
+    if (threadIdx.x == 0) sfail = 0;
+
+    // ---- the bodies' wavefront: state in registers, one lane per body ----
+    const bool isb = plw && lane < n;
     const int i = p0 + lane;
     double rh[3] = {0, 0, 0}, vh[3] = {0, 0, 0}, xj[3] = {0, 0, 0}, vj[3] = {0, 0, 0}, ah[3] = {0, 0, 0};
     double rb[3] = {0, 0, 0}, re[3] = {0, 0, 0};
     double gm = 0.0, muj = 0.0, ir3h = 0.0, ir3j = 0.0;
-    int st = INACTIVE;
+    int st = INACTIVE, lfirst = lfirst0;
     if (isb) {
         rh[0] = pl.rh.x[i], rh[1] = pl.rh.y[i], rh[2] = pl.rh.z[i];
         vh[0] = pl.vh.x[i], vh[1] = pl.vh.y[i], vh[2] = pl.vh.z[i];
@@ -585,68 +604,61 @@ __global__ __launch_bounds__(64) void k_whm_run(WhmArgs a, TpDev tp, const int4 
         sr[lane] = pl.R[i];
         sact[lane] = st != INACTIVE;
     }
-    const int j = tl.y + lane;
-    const bool ist = j < tl.z;
-    double trh[3] = {0, 0, 0}, tvh[3] = {0, 0, 0}, tah[3] = {0, 0, 0};
-    int tst = INACTIVE;
-    if (ist) {
-        trh[0] = tp.rh.x[j], trh[1] = tp.rh.y[j], trh[2] = tp.rh.z[j];
-        tvh[0] = tp.vh.x[j], tvh[1] = tp.vh.y[j], tvh[2] = tp.vh.z[j];
-        tah[0] = w.tah.x[j], tah[1] = w.tah.y[j], tah[2] = w.tah.z[j];
-        tst = tp.status[j];
-    }
-    int lfirst = a.sy.lfirst[s], tlfirst = a.tlfirst[s], fail = 0;
-    __syncthreads();
-
-    // ah of the massive bodies at rh / xj (whm_kick_getacch_pl, as
-    // whm_getacch_pl above), added to ah
-    auto getacch = [&]() {
+    // positions and ah0 factor of this lane's body into slot q
+    auto publish = [&](int q) __attribute__((always_inline)) {
+        if (!isb) return;
+        double r2 = dot3(rh[0], rh[1], rh[2], rh[0], rh[1], rh[2]);
+        bx[q][lane] = rh[0];
+        by[q][lane] = rh[1];
+        bz[q][lane] = rh[2];
+        bf[q][lane] = gm * (1.0 / (r2 * sqrt(r2)));
+    };
+    // ah of the massive bodies at rh / xj, added to ah (whm_kick_getacch_pl,
+    // as whm_getacch_pl above)
+    auto getacch = [&]() __attribute__((always_inline)) {
         double r2 = dot3(rh[0], rh[1], rh[2], rh[0], rh[1], rh[2]);
         double ir = 1.0 / sqrt(r2);
         ir3h = ir / r2;
         double r2j = dot3(xj[0], xj[1], xj[2], xj[0], xj[1], xj[2]);
         ir = 1.0 / sqrt(r2j);
         ir3j = ir / r2j;
-        __syncthreads();
+        wave_sync();
         if (isb) {
             sf[lane] = gm * (1.0 / (r2 * sqrt(r2)));
             sx[lane] = rh[0];
             sy[lane] = rh[1];
             sz[lane] = rh[2];
         }
-        __syncthreads();
-        if (!isb) return;
-        double h0[3] = {0, 0, 0};
-        #pragma unroll 8
-        for (int q = 1; q < n; q++) {
-            h0[0] = fma(-sf[q], sx[q], h0[0]);
-            h0[1] = fma(-sf[q], sy[q], h0[1]);
-            h0[2] = fma(-sf[q], sz[q], h0[2]);
-        }
-        for (int d = 0; d < 3; d++) ah[d] = ah[d] + h0[d];
-        const bool act = st != INACTIVE;
-        if (lane >= 1 && act)
-            for (int d = 0; d < 3; d++) {
-                double a1j = xj[d] * ir3j;
-                double a1h = rh[d] * ir3h;
-                ah[d] = ah[d] + GMcb * (a1j - a1h);
+        wave_sync();
+        double fac = 0.0;
+        if (isb) {
+            double h0[3] = {0, 0, 0};
+#pragma unroll 8
+            for (int q = 1; q < n; q++) {
+                h0[0] = fma(-sf[q], sx[q], h0[0]);
+                h0[1] = fma(-sf[q], sy[q], h0[1]);
+                h0[2] = fma(-sf[q], sz[q], h0[2]);
             }
-        // ah2: the running sums over the active bodies 1..lane
-        double etaj = GMcb;
-        #pragma unroll 8
-        for (int k = 1; k <= lane; k++)
-            if (sact[k]) etaj = etaj + sg[k - 1];
-        const double fac = ((gm * GMcb) * ir3j) / etaj;
-        su[lane] = fac * xj[0];
-        sv[lane] = fac * xj[1];
-        sw[lane] = fac * xj[2];
-    };
-    auto getacch_end = [&]() {
-        __syncthreads();
+            for (int d = 0; d < 3; d++) ah[d] = ah[d] + h0[d];
+            if (lane >= 1 && st != INACTIVE)
+                for (int d = 0; d < 3; d++) {
+                    double a1j = xj[d] * ir3j;
+                    double a1h = rh[d] * ir3h;
+                    ah[d] = ah[d] + GMcb * (a1j - a1h);
+                }
+            // ah2: the running sums over the active bodies 1..lane
+            double etaj = GMcb;
+            for (int k = 1; k <= lane; k++)
+                if (sact[k]) etaj = etaj + sg[k - 1];
+            fac = ((gm * GMcb) * ir3j) / etaj;
+            su[lane] = fac * xj[0];
+            sv[lane] = fac * xj[1];
+            sw[lane] = fac * xj[2];
+        }
+        wave_sync();
         if (!isb) return;
         if (lane >= 1 && st != INACTIVE) {
             double ah2o[3] = {0, 0, 0};
-            #pragma unroll 8
             for (int k = 1; k <= lane; k++) {
                 if (!sact[k]) continue;
                 ah2o[0] = ah2o[0] + su[k];
@@ -656,7 +668,7 @@ __global__ __launch_bounds__(64) void k_whm_run(WhmArgs a, TpDev tp, const int4 
             for (int d = 0; d < 3; d++) ah[d] = ah[d] + ah2o[d];
         }
         double ax = ah[0], ay = ah[1], az = ah[2];
-        #pragma unroll 8
+#pragma unroll 8
         for (int u = 0; u < n; u++) {
             if (u == lane) continue;
             double rx = sx[u] - rh[0], ry = sy[u] - rh[1], rz = sz[u] - rh[2];
@@ -665,45 +677,143 @@ __global__ __launch_bounds__(64) void k_whm_run(WhmArgs a, TpDev tp, const int4 
                 double rs = sr[lane] + sr[u];
                 if (!(rji2 > rs * rs)) continue;
             }
-            double fac = gm_over_r3(sg[u], rji2, pair_g_ok(sg[u]));
-            ax = fma(fac, rx, ax);
-            ay = fma(fac, ry, ay);
-            az = fma(fac, rz, az);
+            double f = gm_over_r3(sg[u], rji2, pair_g_ok(sg[u]));
+            ax = fma(f, rx, ax);
+            ay = fma(f, ry, ay);
+            az = fma(f, rz, az);
         }
         ah[0] = ax;
         ah[1] = ay;
         ah[2] = az;
     };
-    // ah0 at the massive bodies' positions p (whm_kick_getacch_ah0 over all
-    // bodies) and the particles' kick, as k_whm_tp_ah0 / k_whm_tp_kick
-    auto tp_kick = [&](const double *p, bool recompute) {
-        __syncthreads();
-        if (isb) {
-            double r2 = dot3(p[0], p[1], p[2], p[0], p[1], p[2]);
-            sf[lane] = gm * (1.0 / (r2 * sqrt(r2)));
-            sx[lane] = p[0];
-            sy[lane] = p[1];
-            sz[lane] = p[2];
+    // one step of the bodies (whm_step_pl); publishes the end positions into
+    // slot q; returns false on a failed drift (fatal: whm_drift.f90:38-53,
+    // the first failing body is marked and the step stops there)
+    auto pl_step = [&](int q) __attribute__((always_inline)) -> bool {
+        if (lfirst) {
+            // whm_coord_h2j_pl
+            wave_sync();
+            if (isb) {
+                sx[lane] = gm * rh[0], sy[lane] = gm * rh[1], sz[lane] = gm * rh[2];
+                su[lane] = gm * vh[0], sv[lane] = gm * vh[1], sw[lane] = gm * vh[2];
+            }
+            wave_sync();
+            if (isb) {
+                for (int d = 0; d < 3; d++) xj[d] = rh[d], vj[d] = vh[d];
+                if (lane > 0) {
+                    double cx[3] = {0, 0, 0}, cv[3] = {0, 0, 0};
+                    for (int k = 0; k < lane; k++) {
+                        cx[0] = cx[0] + sx[k], cx[1] = cx[1] + sy[k], cx[2] = cx[2] + sz[k];
+                        cv[0] = cv[0] + su[k], cv[1] = cv[1] + sv[k], cv[2] = cv[2] + sw[k];
+                    }
+                    for (int d = 0; d < 3; d++) {
+                        double cap = cx[d] / seta[lane - 1], capv = cv[d] / seta[lane - 1];
+                        xj[d] = rh[d] - cap;
+                        vj[d] = vh[d] - capv;
+                    }
+                }
+                ah[0] = ah[1] = ah[2] = 0.0;
+            }
+            getacch();
+            lfirst = 0;
         }
-        __syncthreads();
+        if (isb) {
+            for (int d = 0; d < 3; d++) rb[d] = rh[d];
+            if (st != INACTIVE)
+                for (int d = 0; d < 3; d++) vh[d] = fma(ah[d], dth, vh[d]);
+        }
+        // whm_coord_vh2vj_pl
+        wave_sync();
+        if (isb) su[lane] = gm * vh[0], sv[lane] = gm * vh[1], sw[lane] = gm * vh[2];
+        wave_sync();
+        int fl = 0;
+        if (isb) {
+            if (lane == 0) {
+                for (int d = 0; d < 3; d++) vj[d] = vh[d];
+            } else {
+                double cv[3] = {0, 0, 0};
+                for (int k = 0; k < lane; k++) cv[0] = cv[0] + su[k], cv[1] = cv[1] + sv[k], cv[2] = cv[2] + sw[k];
+                for (int d = 0; d < 3; d++) vj[d] = vh[d] - cv[d] / seta[lane - 1];
+            }
+            if (st != INACTIVE) fl = drift_one(muj, xj[0], xj[1], xj[2], vj[0], vj[1], vj[2], dt);
+        }
+        const unsigned long long bad = __ballot(fl != 0);
+        if (bad) {
+            if (lane == (int)__builtin_ctzll(bad)) st = DISCARDED_DRIFTERR;
+            return false;
+        }
+        // whm_coord_j2h_pl
+        wave_sync();
+        if (isb) {
+            su[lane] = (gm * xj[0]) / seta[lane];
+            sv[lane] = (gm * xj[1]) / seta[lane];
+            sw[lane] = (gm * xj[2]) / seta[lane];
+        }
+        wave_sync();
+        if (isb) {
+            double cx[3] = {0, 0, 0};
+            for (int k = 0; k < lane; k++) cx[0] = cx[0] + su[k], cx[1] = cx[1] + sv[k], cx[2] = cx[2] + sw[k];
+            if (lane == 0) {
+                for (int d = 0; d < 3; d++) rh[d] = xj[d], vh[d] = vj[d];
+            } else {
+                for (int d = 0; d < 3; d++) rh[d] = xj[d] + cx[d];
+            }
+        }
+        wave_sync();
+        if (isb) {
+            su[lane] = (gm * vj[0]) / seta[lane];
+            sv[lane] = (gm * vj[1]) / seta[lane];
+            sw[lane] = (gm * vj[2]) / seta[lane];
+        }
+        wave_sync();
+        if (isb && lane > 0) {
+            double cv[3] = {0, 0, 0};
+            for (int k = 0; k < lane; k++) cv[0] = cv[0] + su[k], cv[1] = cv[1] + sv[k], cv[2] = cv[2] + sw[k];
+            for (int d = 0; d < 3; d++) vh[d] = vj[d] + cv[d];
+        }
+        if (isb) ah[0] = ah[1] = ah[2] = 0.0;
+        getacch();
+        if (isb) {
+            for (int d = 0; d < 3; d++) re[d] = rh[d];
+            if (st != INACTIVE)
+                for (int d = 0; d < 3; d++) vh[d] = fma(ah[d], dth, vh[d]);
+        }
+        publish(q);
+        return true;
+    };
+
+    // ---- the particles' wavefront ----
+    const int j = tl.y + lane;
+    const bool ist = !plw && j < tl.z;
+    double trh[3] = {0, 0, 0}, tvh[3] = {0, 0, 0}, tah[3] = {0, 0, 0};
+    int tst = INACTIVE;
+    if (ist) {
+        trh[0] = tp.rh.x[j], trh[1] = tp.rh.y[j], trh[2] = tp.rh.z[j];
+        tvh[0] = tp.vh.x[j], tvh[1] = tp.vh.y[j], tvh[2] = tp.vh.z[j];
+        tah[0] = w.tah.x[j], tah[1] = w.tah.y[j], tah[2] = w.tah.z[j];
+        tst = tp.status[j];
+    }
+    // kick with ah0 and the direct sum at the positions in slot q (as
+    // k_whm_tp_ah0 / k_whm_tp_kick)
+    auto tp_kick = [&](int q, bool recompute) __attribute__((always_inline)) {
         if (!ist || tst != ACTIVE) return;
         if (recompute) {
             double h0[3] = {0, 0, 0};
-            #pragma unroll 8
-            for (int q = 0; q < n; q++) {
-                h0[0] = fma(-sf[q], sx[q], h0[0]);
-                h0[1] = fma(-sf[q], sy[q], h0[1]);
-                h0[2] = fma(-sf[q], sz[q], h0[2]);
+#pragma unroll 8
+            for (int u = 0; u < n; u++) {
+                h0[0] = fma(-bf[q][u], bx[q][u], h0[0]);
+                h0[1] = fma(-bf[q][u], by[q][u], h0[1]);
+                h0[2] = fma(-bf[q][u], bz[q][u], h0[2]);
             }
             double px = 0.0, py = 0.0, pz = 0.0;
-            #pragma unroll 8
-            for (int q = 0; q < n; q++) {
-                double rx = trh[0] - sx[q], ry = trh[1] - sy[q], rz = trh[2] - sz[q];
+#pragma unroll 8
+            for (int u = 0; u < n; u++) {
+                double rx = trh[0] - bx[q][u], ry = trh[1] - by[q][u], rz = trh[2] - bz[q][u];
                 double rji2 = dot3(rx, ry, rz, rx, ry, rz);
-                double fac = a.fast ? gm_over_r3_fast(sg[q], rji2) : gm_over_r3(sg[q], rji2, pair_g_ok(sg[q]));
-                px = fma(-fac, rx, px);
-                py = fma(-fac, ry, py);
-                pz = fma(-fac, rz, pz);
+                double f = a.fast ? gm_over_r3_fast(sg[u], rji2) : gm_over_r3(sg[u], rji2, pair_g_ok(sg[u]));
+                px = fma(-f, rx, px);
+                py = fma(-f, ry, py);
+                pz = fma(-f, rz, pz);
             }
             tah[0] = (0.0 + h0[0]) + px;
             tah[1] = (0.0 + h0[1]) + py;
@@ -712,140 +822,67 @@ __global__ __launch_bounds__(64) void k_whm_run(WhmArgs a, TpDev tp, const int4 
         for (int d = 0; d < 3; d++) tvh[d] = fma(tah[d], dth, tvh[d]);
     };
 
-    for (int step = 0; step < nsteps && !fail; step++) {
-        tlfirst = lfirst;
-        if (n) {
-            if (lfirst) {
-                // whm_coord_h2j_pl
-                __syncthreads();
-                if (isb) {
-                    sx[lane] = gm * rh[0], sy[lane] = gm * rh[1], sz[lane] = gm * rh[2];
-                    su[lane] = gm * vh[0], sv[lane] = gm * vh[1], sw[lane] = gm * vh[2];
-                }
-                __syncthreads();
-                if (isb) {
-                    for (int d = 0; d < 3; d++) xj[d] = rh[d], vj[d] = vh[d];
-                    if (lane > 0) {
-                        double cx[3] = {0, 0, 0}, cv[3] = {0, 0, 0};
-                        #pragma unroll 8
-                        for (int k = 0; k < lane; k++) {
-                            cx[0] = cx[0] + sx[k], cx[1] = cx[1] + sy[k], cx[2] = cx[2] + sz[k];
-                            cv[0] = cv[0] + su[k], cv[1] = cv[1] + sv[k], cv[2] = cv[2] + sw[k];
-                        }
-                        for (int d = 0; d < 3; d++) {
-                            double cap = cx[d] / seta[lane - 1], capv = cv[d] / seta[lane - 1];
-                            xj[d] = rh[d] - cap;
-                            vj[d] = vh[d] - capv;
-                        }
-                    }
-                    ah[0] = ah[1] = ah[2] = 0.0;
-                }
-                getacch();
-                getacch_end();
-                lfirst = 0;
-            }
-            if (isb) {
-                for (int d = 0; d < 3; d++) rb[d] = rh[d];
-                if (st != INACTIVE)
-                    for (int d = 0; d < 3; d++) vh[d] = fma(ah[d], dth, vh[d]);
-            }
-            // whm_coord_vh2vj_pl
+    // The two roles branch once, so that the registers of one are not held
+    // by the other; both pass the same workgroup barriers: one before the
+    // first particle step, one after each.
+    if (plw) {
+        // prologue: the bodies' positions before step 0 (slot 0) and after
+        // it (slot 1)
+        if (n && nsteps > 0) {
+            publish(0);
+            if (!pl_step(1) && lane == 0) sfail = 1;
+        }
+        __syncthreads();
+        int done = 0;  // particle steps taken
+        for (int k = 0; k < nsteps && !sfail; k++) {
+            if (n && k + 1 < nsteps && !pl_step((k + 2) % 3) && lane == 0) sfail = 2;
+            done = k + 1;
             __syncthreads();
-            if (isb) su[lane] = gm * vh[0], sv[lane] = gm * vh[1], sw[lane] = gm * vh[2];
-            __syncthreads();
-            int fl = 0;
+        }
+        const int fail = sfail;
+        if (writer) {
             if (isb) {
-                if (lane == 0) {
-                    for (int d = 0; d < 3; d++) vj[d] = vh[d];
-                } else {
-                    double cv[3] = {0, 0, 0};
-                    #pragma unroll 8
-                    for (int k = 0; k < lane; k++) cv[0] = cv[0] + su[k], cv[1] = cv[1] + sv[k], cv[2] = cv[2] + sw[k];
-                    for (int d = 0; d < 3; d++) vj[d] = vh[d] - cv[d] / seta[lane - 1];
-                }
-                if (st != INACTIVE) fl = drift_one(muj, xj[0], xj[1], xj[2], vj[0], vj[1], vj[2], dt);
+                pl.rh.x[i] = rh[0], pl.rh.y[i] = rh[1], pl.rh.z[i] = rh[2];
+                pl.vh.x[i] = vh[0], pl.vh.y[i] = vh[1], pl.vh.z[i] = vh[2];
+                w.xj.x[i] = xj[0], w.xj.y[i] = xj[1], w.xj.z[i] = xj[2];
+                w.vj.x[i] = vj[0], w.vj.y[i] = vj[1], w.vj.z[i] = vj[2];
+                w.ah.x[i] = ah[0], w.ah.y[i] = ah[1], w.ah.z[i] = ah[2];
+                pl.rbeg.x[i] = rb[0], pl.rbeg.y[i] = rb[1], pl.rbeg.z[i] = rb[2];
+                pl.rend.x[i] = re[0], pl.rend.y[i] = re[1], pl.rend.z[i] = re[2];
+                w.ir3h[i] = ir3h;
+                w.ir3j[i] = ir3j;
+                pl.status[i] = st;
             }
-            // a failed drift is fatal (whm_drift.f90:38-53): the first failing
-            // body is marked and the run stops before the Jacobi map
-            const unsigned long long bad = __ballot(fl != 0);
-            if (bad) {
-                if (lane == (int)__builtin_ctzll(bad)) st = DISCARDED_DRIFTERR;
-                fail = 1;
-                break;
-            }
-            // whm_coord_j2h_pl
-            __syncthreads();
-            if (isb) {
-                su[lane] = (gm * xj[0]) / seta[lane];
-                sv[lane] = (gm * xj[1]) / seta[lane];
-                sw[lane] = (gm * xj[2]) / seta[lane];
-            }
-            __syncthreads();
-            if (isb) {
-                double cx[3] = {0, 0, 0};
-                #pragma unroll 8
-                for (int k = 0; k < lane; k++) cx[0] = cx[0] + su[k], cx[1] = cx[1] + sv[k], cx[2] = cx[2] + sw[k];
-                if (lane == 0) {
-                    for (int d = 0; d < 3; d++) rh[d] = xj[d], vh[d] = vj[d];
-                } else {
-                    for (int d = 0; d < 3; d++) rh[d] = xj[d] + cx[d];
-                }
-            }
-            __syncthreads();
-            if (isb) {
-                su[lane] = (gm * vj[0]) / seta[lane];
-                sv[lane] = (gm * vj[1]) / seta[lane];
-                sw[lane] = (gm * vj[2]) / seta[lane];
-            }
-            __syncthreads();
-            if (isb && lane > 0) {
-                double cv[3] = {0, 0, 0};
-                #pragma unroll 8
-                for (int k = 0; k < lane; k++) cv[0] = cv[0] + su[k], cv[1] = cv[1] + sv[k], cv[2] = cv[2] + sw[k];
-                for (int d = 0; d < 3; d++) vh[d] = vj[d] + cv[d];
-            }
-            if (isb) ah[0] = ah[1] = ah[2] = 0.0;
-            getacch();
-            getacch_end();
-            if (isb) {
-                for (int d = 0; d < 3; d++) re[d] = rh[d];
-                if (st != INACTIVE)
-                    for (int d = 0; d < 3; d++) vh[d] = fma(ah[d], dth, vh[d]);
+            if (lane == 0 && nsteps > 0) {
+                a.sy.lfirst[s] = lfirst;
+                // the first-kick flag of the last step begun: lfirst0 for
+                // step 0 (and for every step when no body clears lfirst)
+                const int last = fail == 1 ? 0 : (fail == 2 ? done : done - 1);
+                a.tlfirst[s] = (n == 0 || last == 0) ? lfirst0 : 0;
+                if (fail) a.sy.err[s] |= 2;
             }
         }
-        // the particles: kick at rbeg (reusing the last step's accelerations
-        // unless this is the first kick), drift, kick at rend
-        tp_kick(rb, tlfirst != 0);
-        if (ist && tst == ACTIVE) {
-            int f = drift_one(GMcb, trh[0], trh[1], trh[2], tvh[0], tvh[1], tvh[2], dt);
-            if (f) tst = DISCARDED_DRIFTERR;
+    } else {
+        __syncthreads();
+        for (int k = 0; k < nsteps && !sfail; k++) {
+            // the particles' step k: kick at the start positions (reusing the
+            // last step's accelerations unless this is the first kick),
+            // drift, kick at the end positions
+            const bool tlf = (k == 0 || n == 0) ? lfirst0 != 0 : false;
+            tp_kick(k % 3, tlf);
+            if (ist && tst == ACTIVE) {
+                int f = drift_one(GMcb, trh[0], trh[1], trh[2], tvh[0], tvh[1], tvh[2], dt);
+                if (f) tst = DISCARDED_DRIFTERR;
+            }
+            tp_kick((k + 1) % 3, true);
+            __syncthreads();
         }
-        tp_kick(re, true);
-    }
-    if (writer) {
-        if (isb) {
-            pl.rh.x[i] = rh[0], pl.rh.y[i] = rh[1], pl.rh.z[i] = rh[2];
-            pl.vh.x[i] = vh[0], pl.vh.y[i] = vh[1], pl.vh.z[i] = vh[2];
-            w.xj.x[i] = xj[0], w.xj.y[i] = xj[1], w.xj.z[i] = xj[2];
-            w.vj.x[i] = vj[0], w.vj.y[i] = vj[1], w.vj.z[i] = vj[2];
-            w.ah.x[i] = ah[0], w.ah.y[i] = ah[1], w.ah.z[i] = ah[2];
-            pl.rbeg.x[i] = rb[0], pl.rbeg.y[i] = rb[1], pl.rbeg.z[i] = rb[2];
-            pl.rend.x[i] = re[0], pl.rend.y[i] = re[1], pl.rend.z[i] = re[2];
-            w.ir3h[i] = ir3h;
-            w.ir3j[i] = ir3j;
-            pl.status[i] = st;
+        if (ist) {
+            tp.rh.x[j] = trh[0], tp.rh.y[j] = trh[1], tp.rh.z[j] = trh[2];
+            tp.vh.x[j] = tvh[0], tp.vh.y[j] = tvh[1], tp.vh.z[j] = tvh[2];
+            w.tah.x[j] = tah[0], w.tah.y[j] = tah[1], w.tah.z[j] = tah[2];
+            tp.status[j] = tst;
         }
-        if (lane == 0) {
-            a.sy.lfirst[s] = lfirst;
-            a.tlfirst[s] = tlfirst;
-            if (fail) a.sy.err[s] |= 2;
-        }
-    }
-    if (ist) {
-        tp.rh.x[j] = trh[0], tp.rh.y[j] = trh[1], tp.rh.z[j] = trh[2];
-        tp.vh.x[j] = tvh[0], tp.vh.y[j] = tvh[1], tp.vh.z[j] = tvh[2];
-        w.tah.x[j] = tah[0], w.tah.y[j] = tah[1], w.tah.z[j] = tah[2];
-        tp.status[j] = tst;
     }
 }
 
@@ -919,7 +956,7 @@ int Ensemble::whm_setup()
 int Ensemble::run_whm_fused(int nsteps)
 {
     WhmArgs a = whm_args(*this);
-    k_whm_run<<<ntiles_whm, 64, 0, stream>>>(a, tp, d_tiles_whm, prm.dt, nsteps);
+    k_whm_run<<<ntiles_whm, 128, 0, stream>>>(a, tp, d_tiles_whm, prm.dt, nsteps);
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -40,27 +40,46 @@ __global__ void k_reset_levels(PlDev pl, TpDev tp, int npl, int ntp)
 }
 
 // Per-system sums, one workgroup per system.  Each sum is a serial chain in
-// the reference's order, run by one lane over chunks of the system's bodies
-// that the whole workgroup first stages in LDS (a chain streaming from HBM
-// would wait on every load).
+// the reference's order.  The whole workgroup stages a chunk of the system's
+// bodies in LDS as the terms of the sums (products and quotients formed in
+// parallel, each exactly as the serial expression forms it), and one lane
+// per sum adds them in order (a chain streaming from HBM would wait on every
+// load, one forming its own terms on its serial path).
 constexpr int SUM_T = 256, SUM_CH = 1024;
 
 struct SumStage {
-    double gm[SUM_CH], v[3][SUM_CH];
+    double t[4][SUM_CH];  // the terms of up to four sums
     uint8_t act[SUM_CH];
 };
 
-// stage bodies [c0, c0+n) of Gm, active flags and one 3-vector
-__device__ inline void sum_stage(SumStage &S, const PlDev &pl, const Vec3p &v, int c0, int n)
+// The sums of vh2vb (swiftest_util.f90:424-459) over bodies [p0, p1): lane 0
+// returns sum(Gm), lanes 1-3 the components of -sum(Gm vh), in body order.
+__device__ inline double sums_gm_vh(SumStage &S, const PlDev &pl, int p0, int p1)
 {
-    for (int k = threadIdx.x; k < n; k += SUM_T) {
-        const int i = c0 + k;
-        S.gm[k] = pl.Gm[i];
-        S.act[k] = pl.status[i] != INACTIVE;
-        S.v[0][k] = v.x[i];
-        S.v[1][k] = v.y[i];
-        S.v[2][k] = v.z[i];
+    const int tid = threadIdx.x;
+    double acc = 0.0;
+    for (int c0 = p0; c0 < p1; c0 += SUM_CH) {
+        const int n = min(SUM_CH, p1 - c0);
+        for (int k = tid; k < n; k += SUM_T) {
+            const int i = c0 + k;
+            const double g = pl.Gm[i];
+            S.t[0][k] = g;
+            S.t[1][k] = g * pl.vh.x[i];
+            S.t[2][k] = g * pl.vh.y[i];
+            S.t[3][k] = g * pl.vh.z[i];
+        }
+        __syncthreads();
+        if (tid == 0) {
+#pragma unroll 8
+            for (int k = 0; k < n; k++) acc = acc + S.t[0][k];
+        } else if (tid <= 3) {
+            const double *t = S.t[tid];
+#pragma unroll 8
+            for (int k = 0; k < n; k++) acc = acc - t[k];
+        }
+        __syncthreads();
     }
+    return acc;
 }
 
 // Start of a step:
@@ -71,45 +90,43 @@ __device__ inline void sum_stage(SumStage &S, const PlDev &pl, const Vec3p &v, i
 __global__ __launch_bounds__(SUM_T) void k_sums_begin(PlDev pl, SysDev sy, int nsys, int force)
 {
     __shared__ SumStage S;
-    __shared__ double shr[4];
+    __shared__ double shr[4], scb[3];
     const int s = blockIdx.x, tid = threadIdx.x;
     const int p0 = sy.pl_off[s], p1 = sy.pl_off[s + 1];
     if (p1 == p0 || sy.frozen[s]) return;
     const bool first = force || sy.lfirst[s];
-    double cb = 0.0;
     if (first) {
-        double acc = 0.0;
-        for (int c0 = p0; c0 < p1; c0 += SUM_CH) {
-            const int n = min(SUM_CH, p1 - c0);
-            sum_stage(S, pl, pl.vh, c0, n);
-            __syncthreads();
-            if (tid == 0) {
-                for (int k = 0; k < n; k++) acc = acc + S.gm[k];
-            } else if (tid <= 3) {
-                const double *v = S.v[tid - 1];
-                for (int k = 0; k < n; k++) acc = acc - S.gm[k] * v[k];
-            }
-            __syncthreads();
-        }
+        const double acc = sums_gm_vh(S, pl, p0, p1);
         if (tid < 4) shr[tid] = acc;
         __syncthreads();
         if (tid < 3) {
-            cb = shr[tid + 1] / (sy.GMcb[s] + shr[0]);
+            const double cb = shr[tid + 1] / (sy.GMcb[s] + shr[0]);
+            scb[tid] = cb;
             sy.cbvb[3 * s + tid] = cb;
         }
+        __syncthreads();
     }
+    const Vec3p v = first ? pl.vh : pl.vb;
     double pt = 0.0;
     for (int c0 = p0; c0 < p1; c0 += SUM_CH) {
         const int n = min(SUM_CH, p1 - c0);
-        sum_stage(S, pl, first ? pl.vh : pl.vb, c0, n);
+        for (int k = tid; k < n; k += SUM_T) {
+            const int i = c0 + k;
+            const double g = pl.Gm[i];
+            S.act[k] = pl.status[i] != INACTIVE;
+            const double wx = first ? v.x[i] + scb[0] : v.x[i];
+            const double wy = first ? v.y[i] + scb[1] : v.y[i];
+            const double wz = first ? v.z[i] + scb[2] : v.z[i];
+            S.t[0][k] = g * wx;
+            S.t[1][k] = g * wy;
+            S.t[2][k] = g * wz;
+        }
         __syncthreads();
         if (tid < 3) {
-            const double *v = S.v[tid];
-            for (int k = 0; k < n; k++) {
-                if (!S.act[k]) continue;
-                const double w = first ? v[k] + cb : v[k];
-                pt = pt + S.gm[k] * w;
-            }
+            const double *t = S.t[tid];
+#pragma unroll 8
+            for (int k = 0; k < n; k++)
+                if (S.act[k]) pt = pt + t[k];
         }
         __syncthreads();
     }
@@ -908,12 +925,20 @@ __global__ __launch_bounds__(SUM_T) void k_sums_end(PlDev pl, SysDev sy, int nsy
     double pt = 0.0;
     for (int c0 = p0; c0 < p1; c0 += SUM_CH) {
         const int n = min(SUM_CH, p1 - c0);
-        sum_stage(S, pl, pl.vb, c0, n);
+        for (int k = tid; k < n; k += SUM_T) {
+            const int i = c0 + k;
+            const double g = pl.Gm[i];
+            S.act[k] = pl.status[i] != INACTIVE;
+            S.t[0][k] = g * pl.vb.x[i];
+            S.t[1][k] = g * pl.vb.y[i];
+            S.t[2][k] = g * pl.vb.z[i];
+        }
         __syncthreads();
         if (tid < 3) {
-            const double *v = S.v[tid];
+            const double *t = S.t[tid];
+#pragma unroll 8
             for (int k = 0; k < n; k++)
-                if (S.act[k]) pt = pt + S.gm[k] * v[k];
+                if (S.act[k]) pt = pt + t[k];
         }
         __syncthreads();
     }
@@ -921,12 +946,20 @@ __global__ __launch_bounds__(SUM_T) void k_sums_end(PlDev pl, SysDev sy, int nsy
     double c = 0.0;
     for (int c1 = p1; c1 > p0; c1 -= SUM_CH) {
         const int c0 = max(p0, c1 - SUM_CH), n = c1 - c0;
-        sum_stage(S, pl, pl.vb, c0, n);
+        for (int k = tid; k < n; k += SUM_T) {
+            const int i = c0 + k;
+            const double g = pl.Gm[i];
+            S.act[k] = pl.status[i] != INACTIVE;
+            S.t[0][k] = (g * pl.vb.x[i]) / gm;
+            S.t[1][k] = (g * pl.vb.y[i]) / gm;
+            S.t[2][k] = (g * pl.vb.z[i]) / gm;
+        }
         __syncthreads();
         if (tid < 3) {
-            const double *v = S.v[tid];
+            const double *t = S.t[tid];
+#pragma unroll 8
             for (int k = n - 1; k >= 0; k--)
-                if (S.act[k]) c = c - (S.gm[k] * v[k]) / gm;
+                if (S.act[k]) c = c - t[k];
         }
         __syncthreads();
     }
@@ -996,19 +1029,7 @@ __global__ __launch_bounds__(SUM_T) void k_vh2vb_sum(PlDev pl, SysDev sy, int ns
     const int s = blockIdx.x, tid = threadIdx.x;
     const int p0 = sy.pl_off[s], p1 = sy.pl_off[s + 1];
     if (p1 == p0 || sy.frozen[s]) return;
-    double acc = 0.0;
-    for (int c0 = p0; c0 < p1; c0 += SUM_CH) {
-        const int n = min(SUM_CH, p1 - c0);
-        sum_stage(S, pl, pl.vh, c0, n);
-        __syncthreads();
-        if (tid == 0) {
-            for (int k = 0; k < n; k++) acc = acc + S.gm[k];
-        } else if (tid <= 3) {
-            const double *v = S.v[tid - 1];
-            for (int k = 0; k < n; k++) acc = acc - S.gm[k] * v[k];
-        }
-        __syncthreads();
-    }
+    const double acc = sums_gm_vh(S, pl, p0, p1);
     if (tid < 4) shr[tid] = acc;
     __syncthreads();
     if (tid < 3) sy.cbvb[3 * s + tid] = shr[tid + 1] / (sy.GMcb[s] + shr[0]);
@@ -1036,19 +1057,7 @@ __global__ __launch_bounds__(SUM_T) void k_init_one(PlDev pl, TpDev tp, SysDev s
     __shared__ double shr[4];
     const int tid = threadIdx.x;
     const int p0 = sy.pl_off[s], p1 = sy.pl_off[s + 1];
-    double acc = 0.0;
-    for (int c0 = p0; c0 < p1; c0 += SUM_CH) {
-        const int n = min(SUM_CH, p1 - c0);
-        sum_stage(S, pl, pl.vh, c0, n);
-        __syncthreads();
-        if (tid == 0) {
-            for (int k = 0; k < n; k++) acc = acc + S.gm[k];
-        } else if (tid <= 3) {
-            const double *v = S.v[tid - 1];
-            for (int k = 0; k < n; k++) acc = acc - S.gm[k] * v[k];
-        }
-        __syncthreads();
-    }
+    const double acc = sums_gm_vh(S, pl, p0, p1);
     if (tid < 4) shr[tid] = acc;
     __syncthreads();
     if (tid < 3) sy.cbvb[3 * s + tid] = p1 > p0 ? shr[tid + 1] / (sy.GMcb[s] + shr[0]) : 0.0;
