@@ -284,7 +284,11 @@ def serving_leg(nreq, rate, nmax, tpmax, smin, smax, chunk, seq_sample, log):
     log(f"[bench] served {nreq} requests in {wall:.2f} s: p50 {out['p50_s']:.3f} s, p99 {out['p99_s']:.3f} s, "
         f"{out['system_steps_per_s']:.0f} system-steps/s, pools {out['slot_classes']}")
     # the same requests, one after another, each alone
-    idx = np.random.Generator(np.random.PCG64(55)).choice(nreq, size=min(seq_sample, nreq), replace=False)
+    # every request (seq_sample <= 0 or >= nreq), or a seeded sample of them
+    if seq_sample <= 0 or seq_sample >= nreq:
+        idx = np.arange(nreq)
+    else:
+        idx = np.random.Generator(np.random.PCG64(55)).choice(nreq, size=seq_sample, replace=False)
     t_seq, s_seq = 0.0, 0
     for i in idx:
         r = reqs[int(i)]
@@ -294,8 +298,11 @@ def serving_leg(nreq, rate, nmax, tpmax, smin, smax, chunk, seq_sample, log):
         t_seq += time.perf_counter() - t0
         s_seq += r.nsteps
         del one
-    out["sequential"] = {"system_steps_per_s": s_seq / t_seq, "sample": int(len(idx)),
-                         "how": "a sample of the same requests, each created and run alone, one after another"}
+        log(f"[bench] alone: request {int(i)} (N {r.system.npl}, {r.system.ntp} particles, {r.nsteps} steps) "
+            f"{time.perf_counter() - t0:.2f} s")
+    out["sequential"] = {"system_steps_per_s": s_seq / t_seq, "sample": int(len(idx)), "seconds": t_seq,
+                         "how": ("the same requests" if len(idx) == nreq else "a sample of the same requests")
+                         + ", each created and run alone through the plain path, one after another"}
     out["speedup_vs_sequential"] = out["system_steps_per_s"] / out["sequential"]["system_steps_per_s"]
     log(f"[bench] sequential sample {len(idx)} requests -> {s_seq / t_seq:.1f} system-steps/s")
     return out
@@ -350,7 +357,8 @@ def main():
     ap.add_argument("--serve-tpmax", type=int, default=100000)
     ap.add_argument("--serve-steps", type=int, nargs=2, default=[20, 200])
     ap.add_argument("--serve-chunk", type=int, default=8)
-    ap.add_argument("--serve-seq-sample", type=int, default=8)
+    ap.add_argument("--serve-seq-sample", type=int, default=0,
+                    help="requests run one after another for the comparison (0: all of them)")
     ap.add_argument("--no-sc-leg", action="store_true",
                     help="skip the step with the pl-tp encounter check not short-circuited")
     ap.add_argument("--serve-only", action="store_true",
@@ -495,7 +503,10 @@ def main():
                "--serve-nmax", str(args.serve_nmax), "--serve-tpmax", str(args.serve_tpmax),
                "--serve-steps", str(args.serve_steps[0]), str(args.serve_steps[1]),
                "--serve-chunk", str(args.serve_chunk), "--serve-seq-sample", str(args.serve_seq_sample)]
-        r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True)
+        env = dict(os.environ)
+        if int(env.get("GPU_MAX_HW_QUEUES", "4")) < 32:
+            env["GPU_MAX_HW_QUEUES"] = "32"  # one stream per pool; 15 pools by default
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, env=env)
         if r.returncode != 0:
             raise RuntimeError(f"served-stream leg failed with status {r.returncode}")
         out["serving"] = json.loads([l for l in r.stdout.splitlines() if l.startswith("{")][-1])

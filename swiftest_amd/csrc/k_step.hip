@@ -329,8 +329,12 @@ __global__ __launch_bounds__(64 * (KE + 1)) void k_kick_pl_split(PlDev pl, SysDe
                                                                   double dt, int lbeg, int use_radius, int do_pairs)
 {
     static_assert(KPLS_C % (4 * KE) == 0, "each evaluation wavefront takes whole groups of four partners");
+    static_assert(KPLS_C == 64, "one partner per lane when a chunk is staged");
     __shared__ double facb[2][KPLS_C][64];
     __shared__ double hpart[2][3][64];
+    // the partners of chunk c (x, y, z, Gm, R) in slot c % 3: the chunk being
+    // folded, the one being evaluated and the next one
+    __shared__ double pj[3][5][KPLS_C];
     const Tile tl = tiles[blockIdx.x];
     const int p0 = sy.pl_off[tl.s], p1 = sy.pl_off[tl.s + 1];
     const int lane = (int)(threadIdx.x & 63u);
@@ -344,15 +348,27 @@ __global__ __launch_bounds__(64 * (KE + 1)) void k_kick_pl_split(PlDev pl, SysDe
         zi = pl.rh.z[i];
         Ri = pl.R[i];
     }
-    const cdouble *X = cview(pl.rh.x), *Y = cview(pl.rh.y), *Z = cview(pl.rh.z), *G = cview(pl.Gm),
-                  *RR = cview(pl.R);
     const int n = p1 - p0, nch = (n + KPLS_C - 1) / KPLS_C;
     constexpr int PER = KPLS_C / KE;
+    // staging of chunk c by the first wavefront, one partner per lane
+    auto stage = [&](int c) {
+        const int jt = p0 + c * KPLS_C, jn = min(KPLS_C, p1 - jt);
+        if (lane < jn) {
+            const int j = jt + lane;
+            double(*d)[KPLS_C] = pj[c % 3];
+            d[0][lane] = pl.rh.x[j];
+            d[1][lane] = pl.rh.y[j];
+            d[2][lane] = pl.rh.z[j];
+            d[3][lane] = pl.Gm[j];
+            d[4][lane] = pl.R[j];
+        }
+    };
     // evaluation of chunk c into buffer c & 1 (wavefronts 0 .. KE-1)
     auto evaluate = [&](int c) {
         const int jt = p0 + c * KPLS_C, jn = min(KPLS_C, p1 - jt);
+        const double(*d)[KPLS_C] = pj[c % 3];
         // every mass of the chunk inside the exact cores' range: one column per lane
-        const bool gok_lane = lane >= jn || pair_g_ok(pl.Gm[jt + lane]);
+        const bool gok_lane = lane >= jn || pair_g_ok(d[3][lane]);
         const bool tile_gok = __all(gok_lane);
         double(*fb)[64] = facb[c & 1];
         for (int q0 = w * PER; q0 < (w + 1) * PER; q0 += 4) {
@@ -362,45 +378,48 @@ __global__ __launch_bounds__(64 * (KE + 1)) void k_kick_pl_split(PlDev pl, SysDe
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int q = q0 + u;
-                const int j = jt + min(q, jn - 1);  // in range; a column past the end is not taken
-                rx[u] = X[j] - xi;
-                ry[u] = Y[j] - yi;
-                rz[u] = Z[j] - zi;
+                const int qq = min(q, jn - 1);  // in range; a column past the end is not taken
+                rx[u] = d[0][qq] - xi;
+                ry[u] = d[1][qq] - yi;
+                rz[u] = d[2][qq] - zi;
                 r2[u] = dot3(rx[u], ry[u], rz[u], rx[u], ry[u], rz[u]);
-                const double rs = Ri + RR[j];
+                const double rs = Ri + d[4][qq];
                 take[u] = (q < jn) & (jt + q != i) & (!use_radius | (r2[u] > rs * rs));
-                fac[u] = div_core(G[j], r2[u] * sqrt_core(r2[u]));
+                fac[u] = div_core(d[3][qq], r2[u] * sqrt_core(r2[u]));
                 need[u] = take[u] & !(tile_gok & (r2[u] >= PAIR_R2_LO) & (r2[u] <= PAIR_R2_HI));
                 any_need = any_need | need[u];
             }
             if (__ballot(any_need)) {
 #pragma unroll
                 for (int u = 0; u < 4; u++)
-                    if (need[u]) {
-                        const int j = jt + min(q0 + u, jn - 1);
-                        fac[u] = pl.Gm[j] / (r2[u] * sqrt(r2[u]));
-                    }
+                    if (need[u]) fac[u] = d[3][min(q0 + u, jn - 1)] / (r2[u] * sqrt(r2[u]));
             }
 #pragma unroll
             for (int u = 0; u < 4; u++) fb[q0 + u][lane] = take[u] ? fac[u] : 0.0;
         }
     };
     double ax = 0.0, ay = 0.0, az = 0.0;
+    if (w == 0) {
+        if (nch > 0) stage(0);
+        if (nch > 1) stage(1);
+    }
+    __syncthreads();
     if (w < KE && nch > 0) evaluate(0);
     __syncthreads();
     for (int c = 0; c < nch; c++) {
         if (w < KE) {
             if (c + 1 < nch) evaluate(c + 1);
+            if (w == 0 && c + 2 < nch) stage(c + 2);
         } else {
             // fold chunk c in j order
             const int jt = p0 + c * KPLS_C, jn = min(KPLS_C, p1 - jt);
             const double(*fb)[64] = facb[c & 1];
+            const double(*d)[KPLS_C] = pj[c % 3];
             for (int q = 0; q < jn; q++) {
-                const int j = jt + q;
                 const double f = fb[q][lane];
-                ax = fma(f, X[j] - xi, ax);
-                ay = fma(f, Y[j] - yi, ay);
-                az = fma(f, Z[j] - zi, az);
+                ax = fma(f, d[0][q] - xi, ax);
+                ay = fma(f, d[1][q] - yi, ay);
+                az = fma(f, d[2][q] - zi, az);
             }
         }
         __syncthreads();
@@ -629,14 +648,25 @@ __global__ __launch_bounds__(64 * K) void k_kick_pl_fast(PlDev pl, SysDev sy, Li
     double ax = 0.0, ay = 0.0, az = 0.0;
     int j = j0;
     for (; j + 4 <= j1; j += 4) {
+        // the four partners' data first, unconditionally, so that each
+        // array's four values come in one scalar load
+        double xs[4], ys[4], zs[4], gs[4], rr[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-            const double rx = X[j + u] - xi, ry = Y[j + u] - yi, rz = Z[j + u] - zi;
+            xs[u] = X[j + u];
+            ys[u] = Y[j + u];
+            zs[u] = Z[j + u];
+            gs[u] = G[j + u];
+            rr[u] = RR[j + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const double rx = xs[u] - xi, ry = ys[u] - yi, rz = zs[u] - zi;
             const double r2 = dot3(rx, ry, rz, rx, ry, rz);
-            const double rs = Ri + RR[j + u];
+            const double rs = Ri + rr[u];
             // a self pair has r2 = 0, so the radius test leaves it out too
             const bool take = use_radius ? (r2 > rs * rs) : (j + u != i);
-            const double f = take ? gm_over_r3_fast(G[j + u], r2, fc.c15, fc.c1875) : 0.0;
+            const double f = take ? gm_over_r3_fast(gs[u], r2, fc.c15, fc.c1875) : 0.0;
             ax = fma(f, rx, ax);
             ay = fma(f, ry, ay);
             az = fma(f, rz, az);
