@@ -69,8 +69,8 @@ def request_bytes(npl: int, ntp: int) -> int:
 # recursions are long and run best side by side on their own streams).
 # The pools' streams overlap on the device only as far as the runtime gives
 # the process hardware queues: set GPU_MAX_HW_QUEUES to at least the number
-# of pools (15 here) before the process's first HIP call (bench.py uses 16).
-DEFAULT_CLASSES = (SlotClass(128, 100000, 32, 1), SlotClass(1024, 100000, 4, 4), SlotClass(4096, 100000, 1, 10))
+# of pools (14 here) before the process's first HIP call (bench.py uses 32).
+DEFAULT_CLASSES = (SlotClass(128, 100000, 32, 1), SlotClass(1024, 100000, 4, 3), SlotClass(4096, 100000, 1, 10))
 
 
 class Server:

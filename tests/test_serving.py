@@ -130,3 +130,4 @@ def test_served_results_equal_the_oracle_alone():
         for k in ("pl_rh", "pl_vh", "pl_vb", "tp_rh", "tp_vh", "tp_vb", "pl_status", "tp_status"):
             assert np.array_equal(so[k], r.state[k]), (r.rid, k)
         assert np.array_equal(so["pl_id"], r.state["pl_id"]), r.rid
+
