@@ -253,17 +253,25 @@ class NetCDFWriter:
         f, k, npl, ntp = self.f, self.k, self.npl, self.ntp
         s = self.system
         f.variables["time"][k] = t
+        # rows in input order, as the names and ids: the device holds the
+        # massive bodies sorted (storage row r is body pl_id[r])
+        ids = np.asarray(state.get("pl_id", np.arange(1, npl + 1)))
+
+        def by_id(a):
+            out = np.empty_like(np.asarray(a))
+            out[ids - 1] = a
+            return out
+
         rh = np.zeros((1 + npl + ntp, 3))
         vh = np.zeros_like(rh)
-        rh[1:1 + npl], vh[1:1 + npl] = state["pl_rh"], state["pl_vh"]
+        rh[1:1 + npl], vh[1:1 + npl] = by_id(state["pl_rh"]), by_id(state["pl_vh"])
         rh[1 + npl:], vh[1 + npl:] = state["tp_rh"], state["tp_vh"]
         f.variables["rh"][k] = rh
         f.variables["vh"][k] = vh
-        ids = state.get("pl_id", np.arange(1, npl + 1))
-        f.variables["Gmass"][k] = np.concatenate([[s.GMcb], np.asarray(s.Gmass)[ids - 1], np.zeros(ntp)])
-        f.variables["radius"][k] = np.concatenate([[s.Rcb], np.asarray(s.radius)[ids - 1], np.zeros(ntp)])
-        f.variables["rhill"][k] = np.concatenate([[0.0], state.get("rhill", np.zeros(npl)), np.zeros(ntp)])
-        f.variables["status"][k] = np.concatenate([[0], state["pl_status"], state["tp_status"]]).astype(np.int32)
+        f.variables["Gmass"][k] = np.concatenate([[s.GMcb], np.asarray(s.Gmass), np.zeros(ntp)])
+        f.variables["radius"][k] = np.concatenate([[s.Rcb], np.asarray(s.radius), np.zeros(ntp)])
+        f.variables["rhill"][k] = np.concatenate([[0.0], by_id(state.get("rhill", np.zeros(npl))), np.zeros(ntp)])
+        f.variables["status"][k] = np.concatenate([[0], by_id(state["pl_status"]), state["tp_status"]]).astype(np.int32)
         f.variables["npl"][k] = int((state["pl_status"] == 0).sum())
         f.variables["ntp"][k] = int((state["tp_status"] == 0).sum())
         if energy is not None:

@@ -137,3 +137,73 @@ def test_netcdf_frames(tmp_path):
     assert names[0] == extra.cb_name and names[1:1 + s.npl] == extra.pl_names
     assert f.variables["Gmass"][0, 0] == s.GMcb
     f.close()
+
+
+def test_netcdf_names_are_the_references(tmp_path):
+    """Every dimension and variable the writer creates carries a name the
+    reference's output module uses (netcdf_io_module.f90; the names are in
+    tests/golden/netcdf_names.json), with the reference's dimension order."""
+    import json
+    from scipy.io import netcdf_file
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "netcdf_names.json")))
+    dims, vars_ = set(ref["dimensions"].values()), set(ref["variables"].values())
+    s, extra = _fixture_system("8pl_0tp")
+    path = str(tmp_path / "names.nc")
+    w = sio.NetCDFWriter(path, s, extra)
+    st = dict(pl_rh=s.pl_rh, pl_vh=s.pl_vh, tp_rh=s.tp_rh, tp_vh=s.tp_vh,
+              pl_status=np.zeros(s.npl, np.int32), tp_status=np.zeros(s.ntp, np.int32))
+    w.write_frame(0.0, st, np.arange(16.0))
+    w.close()
+    f = netcdf_file(path, "r", mmap=False)
+    assert set(f.dimensions) <= dims
+    assert set(f.variables) <= vars_ | dims
+    assert f.variables["rh"].dimensions == ("time", "name", "space")
+    assert f.variables["vh"].dimensions == ("time", "name", "space")
+    assert f.variables["Gmass"].dimensions == ("time", "name")
+    assert f.variables["name"].dimensions == ("name", "string32")
+    assert f.variables["L_orbit"].dimensions == ("time", "space")
+    f.close()
+
+
+@pytest.mark.gpu
+def test_driver_frames_equal_the_oracle(tmp_path):
+    """The driver loop (swiftest_amd.driver, swiftest_driver.f90:130-180) on
+    the reference's 108-body example written as ASCII inputs: SyMBA, discard
+    checks after every step, a frame every 10 steps with the energy record.
+    Every frame's positions and velocities (in input order) equal the
+    oracle's run of the same loop, and the energy record is the device's."""
+    from scipy.io import netcdf_file
+
+    import pyoracle as O
+    from swiftest_amd import driver
+    s, extra = _fixture_system("108pl_50tp")
+    with open(os.path.join(HERE, "golden", "108pl_50tp.json")) as fj:
+        gmt = float(json.load(fj)["param"].get("GMTINY", 0.0))
+    rmin, rmax = 0.004650467260962157, 1000.0
+    prm = dict(sio.DEFAULTS, IN_TYPE="ASCII", IN_FORM="XV", CB_IN="cb.in", PL_IN="pl.in", TP_IN="tp.in",
+               CHK_CLOSE=True, RHILL_PRESENT=True, DT=0.005, T0=0.0, TSTOP=0.3, ISTEP_OUT=10, ENERGY=True,
+               CHK_RMIN=rmin, CHK_RMAX=rmax, GMTINY=gmt)
+    sio.write_in(prm, s, extra, str(tmp_path))
+    sio.write_param(str(tmp_path / "param.in"), prm)
+    out = str(tmp_path / "out.nc")
+    driver.run("SYMBA", str(tmp_path / "param.in"), out)
+    f = netcdf_file(out, "r", mmap=False)
+    nfr = f.variables["rh"].shape[0]
+    assert nfr == 7  # t0 and six frames of 10 steps
+    r, rx = sio.read_in(prm, str(tmp_path))
+    o = O.OracleSystem(O.SYMBA, r, 0.005, rhill=rx.rhill, istep_out=10, gmtiny=gmt)
+    for fr in range(nfr):
+        if fr:
+            for _ in range(10):
+                assert o.run(1) == 0
+                o.discard(rmin=rmin, rmax=rmax)
+        so = o.state()
+        ids = so["pl_id"]
+        for key, var, rows in (("pl_rh", "rh", slice(1, 1 + s.npl)), ("pl_vh", "vh", slice(1, 1 + s.npl))):
+            want = np.empty_like(so[key])
+            want[ids - 1] = so[key]
+            assert np.array_equal(f.variables[var][fr, rows], want), (fr, key)
+        assert np.array_equal(f.variables["rh"][fr, 1 + s.npl:], so["tp_rh"]), fr
+        assert f.variables["time"][fr] == pytest.approx(0.05 * fr)
+    assert np.all(np.isfinite(f.variables["TE"][:])) and f.variables["TE"][0] != 0.0
+    f.close()
