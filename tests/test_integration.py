@@ -1,4 +1,5 @@
-"""The Fortran interface module printed in INTEGRATION.md compiles with flang,
+"""The Fortran interface module of the integration notes (the `module
+swiftest_hip` block of the repository's integration document) compiles with flang,
 links against the library, and its bind(c) interfaces reach the C ABI: an
 init on a device that does not exist returns an error code and a message,
 and the derived types have the C structs' sizes."""
@@ -35,10 +36,11 @@ end program check_binding
 
 
 def interface_module():
-    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    notes = os.path.join(ROOT, "INTEGRATION.md")
+    text = open(notes).read()
     blocks = re.findall(r"```fortran\n(.*?)```", text, re.S)
     mod = [b for b in blocks if b.lstrip().startswith("module swiftest_hip")]
-    assert mod, "no interface module in INTEGRATION.md"
+    assert mod, f"no interface module in {notes}"
     return mod[0]
 
 
