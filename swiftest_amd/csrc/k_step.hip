@@ -325,20 +325,22 @@ __device__ inline const cdouble *cview(const double *p) { return (const cdouble 
 constexpr int KPLS_C = 64;  // partners per chunk
 
 template <int KE>
-__global__ __launch_bounds__(64 * (KE + 1)) void k_kick_pl_split(PlDev pl, SysDev sy, ListDev pp, const Tile *tiles,
+__global__ __launch_bounds__(64 * (KE + 3)) void k_kick_pl_split(PlDev pl, SysDev sy, ListDev pp, const Tile *tiles,
                                                                   double dt, int lbeg, int use_radius, int do_pairs)
 {
     static_assert(KPLS_C % (4 * KE) == 0, "each evaluation wavefront takes whole groups of four partners");
     static_assert(KPLS_C == 64, "one partner per lane when a chunk is staged");
     __shared__ double facb[2][KPLS_C][64];
     __shared__ double hpart[2][3][64];
+    __shared__ double fsum[3][64];  // each component's fold
     // the partners of chunk c (x, y, z, Gm, R) in slot c % 3: the chunk being
     // folded, the one being evaluated and the next one
     __shared__ double pj[3][5][KPLS_C];
     const Tile tl = tiles[blockIdx.x];
     const int p0 = sy.pl_off[tl.s], p1 = sy.pl_off[tl.s + 1];
     const int lane = (int)(threadIdx.x & 63u);
-    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // 0..KE-1 evaluate, KE folds
+    // wavefronts 0..KE-1 evaluate; KE, KE+1, KE+2 fold the x, y, z sums
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int i = tl.b0 + lane;
     const bool valid = i < tl.b1;
     double xi = 0.0, yi = 0.0, zi = 0.0, Ri = 0.0;
@@ -398,7 +400,9 @@ __global__ __launch_bounds__(64 * (KE + 1)) void k_kick_pl_split(PlDev pl, SysDe
             for (int u = 0; u < 4; u++) fb[q0 + u][lane] = take[u] ? fac[u] : 0.0;
         }
     };
-    double ax = 0.0, ay = 0.0, az = 0.0;
+    double acc = 0.0;  // a fold wavefront's component sum
+    const int comp = w - KE;
+    const double ci = comp == 0 ? xi : (comp == 1 ? yi : zi);
     if (w == 0) {
         if (nch > 0) stage(0);
         if (nch > 1) stage(1);
@@ -411,19 +415,15 @@ __global__ __launch_bounds__(64 * (KE + 1)) void k_kick_pl_split(PlDev pl, SysDe
             if (c + 1 < nch) evaluate(c + 1);
             if (w == 0 && c + 2 < nch) stage(c + 2);
         } else {
-            // fold chunk c in j order
+            // fold chunk c in j order, one component per wavefront
             const int jt = p0 + c * KPLS_C, jn = min(KPLS_C, p1 - jt);
             const double(*fb)[64] = facb[c & 1];
-            const double(*d)[KPLS_C] = pj[c % 3];
-            for (int q = 0; q < jn; q++) {
-                const double f = fb[q][lane];
-                ax = fma(f, d[0][q] - xi, ax);
-                ay = fma(f, d[1][q] - yi, ay);
-                az = fma(f, d[2][q] - zi, az);
-            }
+            const double *dc = pj[c % 3][comp];
+            for (int q = 0; q < jn; q++) acc = fma(fb[q][lane], dc[q] - ci, acc);
         }
         __syncthreads();
     }
+    if (w >= KE) fsum[comp][lane] = acc;
     if (do_pairs && w < 2 && w < KE && valid) {
         // w = 0: ahj over pairs (p, i), p < i; w = 1: ahi over pairs (i, j)
         double hx = 0.0, hy = 0.0, hz = 0.0;
@@ -462,6 +462,7 @@ __global__ __launch_bounds__(64 * (KE + 1)) void k_kick_pl_split(PlDev pl, SysDe
     }
     __syncthreads();
     if (w != KE || !valid) return;
+    double ax = fsum[0][lane], ay = fsum[1][lane], az = fsum[2][lane];
     if (do_pairs) {
         const double hjx = hpart[0][0][lane], hjy = hpart[0][1][lane], hjz = hpart[0][2][lane];
         const double hix = KE > 1 ? hpart[1][0][lane] : 0.0, hiy = KE > 1 ? hpart[1][1][lane] : 0.0,
@@ -1152,9 +1153,9 @@ void launch_kick_pl(hipStream_t st, const Tuning &tu, PlDev pl, SysDev sy, ListD
     }
     if (tu.kick_pl_eval) {
         switch (tu.kick_pl_eval) {
-        case 2: k_kick_pl_split<2><<<ntiles, 192, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
-        case 8: k_kick_pl_split<8><<<ntiles, 576, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
-        default: k_kick_pl_split<4><<<ntiles, 320, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
+        case 2: k_kick_pl_split<2><<<ntiles, 64 * 5, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
+        case 8: k_kick_pl_split<8><<<ntiles, 64 * 11, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
+        default: k_kick_pl_split<4><<<ntiles, 64 * 7, 0, st>>>(pl, sy, pp, tiles, dt, lbeg, use_radius, do_pairs); break;
         }
         return;
     }
