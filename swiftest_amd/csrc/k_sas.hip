@@ -74,28 +74,60 @@ __global__ void k_sas_count(const double *slo, const double *hi, const int32_t *
     cnt[g] = lo - rank[g] - 1;
 }
 
-// narrow phase of candidate c (the pair test of k_pp_tiles with index1 < index2)
-__global__ void k_sas_narrow(PlDev pl, const int64_t *coff, const int32_t *rank, const int32_t *sidx, int npl,
-                             int64_t ncand, double dt, uint64_t *key, uint8_t *flag)
+// the bodies' data in sorted order, so that a body's candidates (a run of
+// the sorted order) are read contiguously
+__global__ void k_sas_permute(PlDev pl, const int32_t *sidx, int npl, double *srt, int8_t *stiny)
 {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncand) return;
-    int lo = 0, hi = npl;  // owner: last g with coff[g] <= c
-    while (hi - lo > 1) {
-        int m = (lo + hi) >> 1;
-        if (coff[m] <= c) lo = m;
-        else hi = m;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= npl) return;
+    const int i = sidx[r];
+    srt[r] = pl.rh.x[i];
+    srt[(size_t)npl + r] = pl.rh.y[i];
+    srt[2 * (size_t)npl + r] = pl.rh.z[i];
+    srt[3 * (size_t)npl + r] = pl.vb.x[i];
+    srt[4 * (size_t)npl + r] = pl.vb.y[i];
+    srt[5 * (size_t)npl + r] = pl.vb.z[i];
+    srt[6 * (size_t)npl + r] = pl.renc[i];
+    stiny[r] = (pl.tiny && pl.tiny[i]) ? 1 : 0;
+}
+
+// narrow phase (the pair test of k_pp_tiles with index1 < index2): one
+// wavefront per body g, its lanes over g's candidates, the bodies at sorted
+// positions rank[g] + 1 .. rank[g] + cnt[g]; results at coff[g] + k
+__global__ __launch_bounds__(256) void k_sas_narrow(PlDev pl, const int64_t *coff, const int32_t *cnt,
+                                                    const int32_t *rank, const int32_t *sidx, const double *srt,
+                                                    const int8_t *stiny, int npl, double dt, uint64_t *key,
+                                                    uint8_t *flag)
+{
+    const int g = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (g >= npl) return;
+    const int n = cnt[g];
+    if (n == 0) return;
+    const int64_t o = coff[g];
+    const int r0 = rank[g] + 1;
+    const double gx = pl.rh.x[g], gy = pl.rh.y[g], gz = pl.rh.z[g];
+    const double gvx = pl.vb.x[g], gvy = pl.vb.y[g], gvz = pl.vb.z[g], gr = pl.renc[g];
+    const bool gtiny = pl.tiny && pl.tiny[g];
+    const size_t N = (size_t)npl;
+    for (int k = lane; k < n; k += 64) {
+        const int r = r0 + k;
+        const int j = sidx[r];
+        const double jx = srt[r], jy = srt[N + r], jz = srt[2 * N + r];
+        const double jvx = srt[3 * N + r], jvy = srt[4 * N + r], jvz = srt[5 * N + r], jr = srt[6 * N + r];
+        const bool gfirst = g < j;
+        const int a = gfirst ? g : j, b = gfirst ? j : g;
+        // differences b - a, as the pair test forms them
+        const double dx = gfirst ? jx - gx : gx - jx, dy = gfirst ? jy - gy : gy - jy, dz = gfirst ? jz - gz : gz - jz;
+        const double dvx = gfirst ? jvx - gvx : gvx - jvx, dvy = gfirst ? jvy - gvy : gvy - jvy,
+                     dvz = gfirst ? jvz - gvz : gvz - jvz;
+        const double rsum = gfirst ? gr + jr : jr + gr;
+        const bool atiny = gfirst ? gtiny : stiny[r] != 0;
+        bool lv;
+        // a semi-interacting index1 (GMTINY): the pair is not checked
+        const bool f = !atiny && encounter_one(dx, dy, dz, dvx, dvy, dvz, rsum, dt, lv);
+        key[o + k] = ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
+        flag[o + k] = f;
     }
-    const int g = lo;
-    const int j = sidx[rank[g] + 1 + (int)(c - coff[g])];
-    const int a = min(g, j), b = max(g, j);
-    bool lv;
-    // a semi-interacting index1 (GMTINY): the pair is not checked
-    const bool f = !(pl.tiny && pl.tiny[a]) && encounter_one(pl.rh.x[b] - pl.rh.x[a], pl.rh.y[b] - pl.rh.y[a], pl.rh.z[b] - pl.rh.z[a],
-                                 pl.vb.x[b] - pl.vb.x[a], pl.vb.y[b] - pl.vb.y[a], pl.vb.z[b] - pl.vb.z[a],
-                                 pl.renc[a] + pl.renc[b], dt, lv);
-    key[c] = ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
-    flag[c] = f;
 }
 
 // rows of the sorted pair keys: rowstart[g] = first key >= g<<32
@@ -326,7 +358,8 @@ SasWork::~SasWork()
 {
     for (void *p : {(void *)lokey, (void *)slokey, (void *)hi, (void *)slo, (void *)idx, (void *)sidx, (void *)rank,
                     (void *)cnt, (void *)coff, (void *)key, (void *)flag, (void *)sel, (void *)skey, (void *)nsel,
-                    (void *)pid, (void *)spid, (void *)si2, rtk, rtv, rhist, roffs, rscan, rtotal})
+                    (void *)pid, (void *)spid, (void *)si2, rtk, rtv, rhist, roffs, rscan, rtotal, (void *)srt,
+                    (void *)stiny})
         if (p) (void)hipFree(p);
 }
 
@@ -388,7 +421,8 @@ hipError_t sas_candidates(hipStream_t st, SasWork &w, PlDev pl, SysDev sy, int n
 {
     if (w.n < npl) {
         for (void *p : {(void *)w.lokey, (void *)w.slokey, (void *)w.hi, (void *)w.slo, (void *)w.idx,
-                        (void *)w.sidx, (void *)w.rank, (void *)w.cnt, (void *)w.coff})
+                        (void *)w.sidx, (void *)w.rank, (void *)w.cnt, (void *)w.coff, (void *)w.srt,
+                        (void *)w.stiny})
             if (p) (void)hipFree(p);
         const size_t n = (size_t)npl + 1;
         SASCHK(hipMalloc(&w.lokey, n * sizeof(uint64_t)));
@@ -400,6 +434,8 @@ hipError_t sas_candidates(hipStream_t st, SasWork &w, PlDev pl, SysDev sy, int n
         SASCHK(hipMalloc(&w.rank, n * sizeof(int32_t)));
         SASCHK(hipMalloc(&w.cnt, n * sizeof(int32_t)));
         SASCHK(hipMalloc(&w.coff, n * sizeof(int64_t)));
+        SASCHK(hipMalloc(&w.srt, 7 * n * sizeof(double)));
+        SASCHK(hipMalloc(&w.stiny, n));
         w.n = npl;
     }
     k_sas_extents<<<cdiv64(npl, 256), 256, 0, st>>>(pl, npl, dt, w.lokey, w.hi, w.idx);
@@ -411,6 +447,7 @@ hipError_t sas_candidates(hipStream_t st, SasWork &w, PlDev pl, SysDev sy, int n
         return e;
     k_sas_unkey<<<cdiv64(npl, 256), 256, 0, st>>>(w.slokey, npl, w.slo);
     k_sas_rank<<<cdiv64(npl, 256), 256, 0, st>>>(w.sidx, npl, w.rank);
+    k_sas_permute<<<cdiv64(npl, 256), 256, 0, st>>>(pl, w.sidx, npl, w.srt, w.stiny);
     k_sas_count<<<cdiv64(npl, 256), 256, 0, st>>>(w.slo, w.hi, w.rank, pl.sys, sy.pl_off, active, npl, w.cnt);
     launch_scan_i32_i64(st, w.cnt, w.coff, npl, ncand_dev, scan_scratch);
     return hipGetLastError();
@@ -422,7 +459,9 @@ hipError_t sas_narrow(hipStream_t st, SasWork &w, PlDev pl, int npl, int64_t nca
         grow((void **)&w.sel, &w.sel_cap, (size_t)ncand * 8))
         return hipErrorOutOfMemory;
     if (!w.nsel) SASCHK(hipMalloc(&w.nsel, sizeof(int64_t)));
-    if (ncand) k_sas_narrow<<<cdiv64(ncand, 256), 256, 0, st>>>(pl, w.coff, w.rank, w.sidx, npl, ncand, dt, w.key, w.flag);
+    if (ncand && npl)
+        k_sas_narrow<<<cdiv64(npl, 4), 256, 0, st>>>(pl, w.coff, w.cnt, w.rank, w.sidx, w.srt, w.stiny, npl, dt, w.key,
+                                                     w.flag);
     if (hipError_t e = compact(st, w, w.key, w.flag, ncand, w.sel, w.nsel)) return e;
     SASCHK(hipMemcpyAsync(nsel_host, w.nsel, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     SASCHK(hipStreamSynchronize(st));

@@ -110,6 +110,8 @@ void launch_col_t(hipStream_t st, ListDev L, const int32_t *row_off, const int32
 struct SasWork {
     int n = 0;  // bodies the per-body arrays hold
     uint64_t *lokey = nullptr, *slokey = nullptr;  // lower x-extents as sortable keys, and sorted
+    double *srt = nullptr;   // positions, velocities and encounter radii in sorted order (7 x n)
+    int8_t *stiny = nullptr; // semi-interacting flags in sorted order
     double *hi = nullptr, *slo = nullptr;
     int32_t *idx = nullptr, *sidx = nullptr, *rank = nullptr, *cnt = nullptr;
     int64_t *coff = nullptr, *nsel = nullptr;
