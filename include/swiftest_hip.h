@@ -299,6 +299,36 @@ typedef struct {
 #define SWG_DISCARDED_PLR (-5)
 int swg_system_discard(swg_system *sys, const swg_discard_params *dp, const double *cb_radius,
                        int64_t *npl_discarded, int64_t *ntp_discarded);
+
+/* ---- collisions resolved by merging ---------------------------------------
+ * collision_resolve_plpl (src/collision/collision_resolve.f90:521) with the
+ * MERGE collision model: collision_generate_merge
+ * (src/collision/collision_generate.f90:158), collision_resolve_mergeaddsub
+ * (collision_resolve.f90:326) and swiftest_util_rearray_pl
+ * (src/swiftest/swiftest_util.f90:1638), called where the recursion finds the
+ * collision (src/symba/symba_step.f90:245).  After swg_system_set_merge a
+ * SyMBA step stops at the substep whose collision check flagged a pair,
+ * merges each colliding pair into one body (masses and volumes added, the
+ * pair's barycentre, its orbital angular momentum put back as a velocity
+ * torque), removes the impactors, appends the new bodies, re-sorts by mass,
+ * refreshes every rhill, re-runs the encounter check at that level over the
+ * full step (keeping the old entries' state where the same pair sits at the
+ * same list position and at most as many entries as before), checks the new
+ * list for further collisions, and continues the step on the new arrays.
+ * Needs one SyMBA system without test particles, CHK_CLOSE on and the
+ * triangular encounter check; rotation and the ENERGY torque
+ * (collision_resolve.f90:624-641) are not modelled, and a body in two
+ * colliding pairs of one check (a collision family) returns
+ * SWG_ERR_COLLISION.  GU: G in the system's units; pl_mass: masses in input
+ * order (NULL: Gmass / GU).  Body ids are 1..npl in input order; a merged
+ * body takes the next unused id.  Every level runs on the whole device in
+ * this mode. */
+int swg_system_set_merge(swg_system *sys, double GU, const double *pl_mass);
+/* Current body counts (they shrink with merges) and the merges so far. */
+int swg_system_counts(swg_system *sys, int32_t *npl, int32_t *ntp, int64_t *nmerge);
+/* Ids, masses (0 unless merging is on), Gmass and radii of the massive
+ * bodies, in storage order (npl of each; NULL pointers are skipped). */
+int swg_system_get_bodies(swg_system *sys, int32_t *id, double *mass, double *Gmass, double *radius);
 /* Autotuner: time the pl-tp and pl-pl acceleration kernels at each
  * workgroup size (64, 128, 256 bodies per tile) on the current state (no state
  * change) and keep the fastest for the following steps.  Reports the choice.

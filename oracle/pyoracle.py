@@ -98,6 +98,11 @@ def load(path):
     L.swo_sys_list_status.restype = C.c_int64
     L.swo_sys_set_collision_mode.argtypes = [vp, C.c_int]
     L.swo_sys_set_collision_mode.restype = None
+    L.swo_sys_set_merge.argtypes = [vp, C.c_double, vp]
+    L.swo_sys_get_bodies.argtypes = [vp] * 5
+    L.swo_sys_get_bodies.restype = None
+    L.swo_sys_nmerge.argtypes = [vp]
+    L.swo_sys_nmerge.restype = C.c_int64
     L.swo_norm2.argtypes = [C.c_double] * 3
     L.swo_norm2.restype = C.c_double
     L.swo_coord_vh2vb.argtypes = [C.c_int, C.c_double, D, D, D, D]
@@ -241,7 +246,8 @@ class OracleSystem:
     """A system integrated by the CPU restatement of the reference step layer."""
 
     def __init__(self, integrator, sysic, dt, lclose=True, short_circuit=True, istep_out=0,
-                 rhill=None, nthreads=1, omp=False, L=None, gmtiny=0.0, collision_finish_step=False):
+                 rhill=None, nthreads=1, omp=False, L=None, gmtiny=0.0, collision_finish_step=False,
+                 merge=None):
         self.L = L if L is not None else lib(omp)
         prm = Params(dt, int(lclose), int(short_circuit), int(istep_out), int(rhill is not None), int(nthreads),
                      float(gmtiny))
@@ -254,6 +260,13 @@ class OracleSystem:
         # a detected collision: stop at once (default), or flag the pairs and
         # finish the step (swo_sys_set_collision_mode)
         self.L.swo_sys_set_collision_mode(self.h, int(collision_finish_step))
+        # merge = (GU, pl_mass or None): collisions resolved by merging
+        # inside the step (swo_sys_set_merge)
+        if merge is not None:
+            GU, mass = merge
+            self._mass = None if mass is None else _c(mass)
+            if self.L.swo_sys_set_merge(self.h, float(GU), _p(self._mass)) != 0:
+                raise ValueError("merging needs SyMBA, no test particles and GU > 0")
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -274,6 +287,7 @@ class OracleSystem:
         return counts[0], counts[1]
 
     def state(self):
+        self.npl = self.L.swo_sys_npl(self.h)
         npl, ntp = self.npl, self.ntp
         out = dict(pl_rh=np.zeros((npl, 3)), pl_vh=np.zeros((npl, 3)), pl_vb=np.zeros((npl, 3)),
                    tp_rh=np.zeros((ntp, 3)), tp_vh=np.zeros((ntp, 3)), tp_vb=np.zeros((ntp, 3)),
@@ -289,6 +303,17 @@ class OracleSystem:
         out.update(rhill=rhill, cb_vb=cb_vb, ptbeg=ptb, ptend=pte, lfirst=int(lf[0]), pl_id=ids)
         return out
 
+    def bodies(self):
+        """Ids, masses, Gmass and radii of the massive bodies, in storage order."""
+        n = self.L.swo_sys_npl(self.h)
+        out = dict(id=np.zeros(n, np.int32), mass=np.zeros(n), Gmass=np.zeros(n), radius=np.zeros(n))
+        self.L.swo_sys_get_bodies(self.h, *[_p(out[k]) for k in ("id", "mass", "Gmass", "radius")])
+        return out
+
+    @property
+    def nmerge(self):
+        return self.L.swo_sys_nmerge(self.h)
+
     def stats(self):
         st = Stats()
         self.L.swo_sys_stats(self.h, C.byref(st))
@@ -297,6 +322,7 @@ class OracleSystem:
     def levels(self):
         """levelg, levelm of the massive bodies and of the test particles
         after the last step, in storage order."""
+        self.npl = self.L.swo_sys_npl(self.h)
         out = [np.zeros(self.npl, np.int32), np.zeros(self.npl, np.int32), np.zeros(self.ntp, np.int32),
                np.zeros(self.ntp, np.int32)]
         self.L.swo_sys_get_levels(self.h, *[_p(a) for a in out])

@@ -782,6 +782,12 @@ struct swo_sys {
     int irec;
     int err;
     int col_finish, col_seen; /* swo_sys_set_collision_mode; a collision flagged in this step */
+    /* swo_sys_set_merge: collisions resolved by merging inside the step */
+    int merge;
+    double GU;     /* G in the system's units: Gmass = GU * mass */
+    double *mass;  /* body masses (Gmass / GU unless given) */
+    int32_t maxid; /* largest body id handed out (nbody_system%maxid) */
+    int64_t nmerge;
     /* pericentre-passage state of the discard checks */
     int32_t *pisperi, *tisperi;
     double *patp, *pperi, *tatp, *tperi;
@@ -1409,6 +1415,446 @@ static void set_recur_levels(swo_sys *s, int ireci)
     s->irec = ireci;
 }
 
+/* ---- collisions resolved by merging (collision model MERGE) ----
+ *
+ * collision_resolve_plpl (collision_resolve.f90:521-662) with
+ * collision_generate_merge (collision_generate.f90:158-259),
+ * collision_resolve_mergeaddsub (collision_resolve.f90:326-520) and
+ * swiftest_util_rearray_pl (swiftest_util.f90:1638-1811), for massive bodies
+ * without rotation, without test particles and without the ENERGY
+ * bookkeeping (whose extra torque, collision_resolve.f90:624-641, is not
+ * applied).  A body in two colliding pairs of one check (a collision family,
+ * collision_resolve_make_impactors_pl) is refused with -4. */
+
+static void v3_cross(const double *a, const double *b, double *c) /* operator_cross_dp */
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+static double v3_mag(const double *a) { return swo_norm2(a[0], a[1], a[2]); } /* .mag. = norm2 */
+
+static void v3_unit(const double *a, double *u) /* operator_unit_dp */
+{
+    double m = v3_mag(a);
+    for (int d = 0; d < 3; d++) u[d] = m > DBL_MIN ? a[d] / m : 0.0;
+}
+
+/* r**3 as gfortran expands an integer power */
+static double cube(double r) { return (r * r) * r; }
+
+/* swiftest_util_coord_rh2rb_pl, swiftest_util.f90:488-521 (cb%rb only) */
+static void rh2rb_cb(const swo_sys *s, double *cb_rb)
+{
+    double Gmtot = s->GMcb, x[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < s->npl; i++) {
+        if (s->status[i] == SWO_INACTIVE) continue;
+        Gmtot = Gmtot + s->Gmass[i];
+        for (int d = 0; d < 3; d++) x[d] = x[d] + s->Gmass[i] * s->rh[3 * i + d];
+    }
+    for (int d = 0; d < 3; d++) cb_rb[d] = -x[d] / Gmtot;
+}
+
+/* collision_util_velocity_torque, collision_util.f90:1001-1029 */
+static void velocity_torque(const double *dL, double mass, const double *r, double *v)
+{
+    double dLu[3], ru[3], q[3], c[3];
+    v3_unit(dL, dLu);
+    v3_unit(r, ru);
+    double rmag = v3_mag(r), vmag = v3_mag(v);
+    for (int d = 0; d < 3; d++) q[d] = dL[d] / (rmag * mass);
+    double vapply = v3_mag(q);
+    if (vapply / vmag > DBL_EPSILON) {
+        v3_cross(dLu, ru, c);
+        for (int d = 0; d < 3; d++) v[d] = v[d] + vapply * c[d];
+    }
+}
+
+typedef struct {
+    double rh[3], vh[3], vb[3], mass, Gmass, radius;
+    int32_t id, levelg, levelm;
+} newbody;
+
+/* One merge, as a function of the two impactors alone: consolidate_impactors
+ * (collision_resolve.f90:15-177, no children), set_coordinate_impactors
+ * (collision_util.f90:609-670), the merged body of generate_merge
+ * (collision_generate.f90:199-228) and its coordinates in mergeaddsub
+ * (collision_resolve.f90:356-380).  m, Gm, R: 2 each; rh, vb: 2 x 3;
+ * cb_rb, cb_vb: the central body's barycentric position and velocity.
+ * out: rh[3] vh[3] vb[3] mass Gmass radius.  Returns the index (0 or 1) of
+ * the larger impactor, whose levels and metadata the new body takes. */
+int swo_merge_bodies(const double *m, const double *Gm, const double *R_in, const double *rh, const double *vb_in,
+                     const double *cb_rb, const double *cb_vb, double GU, double *out)
+{
+    double R[2], vol[2], rb[2][3], vb[2][3];
+    for (int k = 0; k < 2; k++) {
+        vol[k] = ((4.0 / 3.0) * PI) * cube(R_in[k]);
+        for (int d = 0; d < 3; d++) {
+            rb[k][d] = rh[3 * k + d] + cb_rb[d];
+            vb[k][d] = vb_in[3 * k + d];
+        }
+        R[k] = pow((3.0 * vol[k]) / (4.0 * PI), 1.0 / 3.0);
+    }
+    double mtot = m[0] + m[1];
+    /* shift the impactors apart when they overlap (:142-157) */
+    double rlim = R[0] + R[1], rrel[3];
+    for (int d = 0; d < 3; d++) rrel[d] = rb[1][d] - rb[0][d];
+    double rrel_mag = v3_mag(rrel);
+    if (rrel_mag < rlim) {
+        double vrel[3], vu[3], dr[3];
+        for (int d = 0; d < 3; d++) vrel[d] = vb[1][d] - vb[0][d];
+        v3_unit(vrel, vu);
+        double vdotr = (vu[0] * rrel[0] + vu[1] * rrel[1]) + vu[2] * rrel[2];
+        double root = sqrt(((rlim * rlim) - (rrel_mag * rrel_mag)) + (vdotr * vdotr));
+        double t = vdotr - copysign(1.0, vdotr) * root;
+        for (int d = 0; d < 3; d++) {
+            dr[d] = (-t) * vu[d];
+            dr[d] = (1.0 + 2.0 * DBL_EPSILON) * dr[d];
+        }
+        for (int d = 0; d < 3; d++) {
+            rb[0][d] = rb[0][d] - (dr[d] * m[1]) / mtot;
+            rb[1][d] = rb[1][d] + (dr[d] * m[0]) / mtot;
+        }
+    }
+    /* barycentre of the pair and the orbital angular momentum about it */
+    double rbcom[3], vbcom[3], L[3], Lk[2][3];
+    for (int d = 0; d < 3; d++) {
+        rbcom[d] = (m[0] * rb[0][d] + m[1] * rb[1][d]) / mtot;
+        vbcom[d] = (m[0] * vb[0][d] + m[1] * vb[1][d]) / mtot;
+    }
+    for (int k = 0; k < 2; k++) {
+        double rc[3], vc[3];
+        for (int d = 0; d < 3; d++) {
+            rc[d] = rb[k][d] - rbcom[d];
+            vc[d] = vb[k][d] - vbcom[d];
+        }
+        Lk[k][0] = m[k] * (rc[1] * vc[2] - rc[2] * vc[1]);
+        Lk[k][1] = m[k] * (rc[2] * vc[0] - rc[0] * vc[2]);
+        Lk[k][2] = m[k] * (rc[0] * vc[1] - rc[1] * vc[0]);
+    }
+    for (int d = 0; d < 3; d++) L[d] = Lk[0][d] + Lk[1][d];
+    /* the merged body: the lost orbital angular momentum (before minus the
+     * after phase's zero) goes back in as a velocity torque */
+    const double mass = m[0] + m[1];
+    const double volume = ((4.0 / 3.0) * PI) * (cube(R[0]) + cube(R[1]));
+    double v[3] = {vbcom[0], vbcom[1], vbcom[2]}, dL[3];
+    for (int d = 0; d < 3; d++) dL[d] = -(0.0 - L[d]);
+    velocity_torque(dL, mass, rbcom, v);
+    for (int d = 0; d < 3; d++) {
+        out[d] = rbcom[d] - cb_rb[d];
+        out[3 + d] = v[d] - cb_vb[d];
+        out[6 + d] = v[d];
+    }
+    out[9] = mass;
+    out[10] = GU * mass;
+    out[11] = pow((3.0 * volume) / (4.0 * PI), THIRD);
+    return Gm[1] > Gm[0] ? 1 : 0; /* maxloc: the first of equal masses */
+}
+
+/* One merge of bodies i and j (0-based) inside the step: the new body, the
+ * encounter entries of the smaller impactor renamed to the larger
+ * (collision_generate.f90:232-246), the next id (mergeaddsub :447-449). */
+static void merge_pair(swo_sys *s, int i, int j, const double *cb_rb, int32_t *nid1, int32_t *nid2, newbody *nb)
+{
+    const int idx[2] = {i, j};
+    double m[2], Gm[2], R[2], rh[6], vb[6], out[12];
+    for (int k = 0; k < 2; k++) {
+        m[k] = s->mass[idx[k]];
+        Gm[k] = s->Gmass[idx[k]];
+        R[k] = s->radius[idx[k]];
+        for (int d = 0; d < 3; d++) {
+            rh[3 * k + d] = s->rh[3 * idx[k] + d];
+            vb[3 * k + d] = s->vb[3 * idx[k] + d];
+        }
+    }
+    const int ib = idx[swo_merge_bodies(m, Gm, R, rh, vb, cb_rb, s->cb_vb, s->GU, out)];
+    enclist *l = &s->plpl;
+    for (int64_t k = 0; k < l->n; k++)
+        for (int q = 0; q < 2; q++) {
+            const int b = idx[q];
+            if (b == ib) continue;
+            if (nid1[k] == s->id[b]) {
+                nid1[k] = s->id[ib];
+                l->i1[k] = b + 1;
+            }
+            if (nid2[k] == s->id[b]) {
+                nid2[k] = s->id[ib];
+                l->i2[k] = b + 1;
+            }
+            if (nid1[k] == nid2[k]) l->status[k] = SWO_INACTIVE;
+        }
+    s->maxid = s->maxid + 1;
+    nb->id = s->maxid;
+    for (int d = 0; d < 3; d++) {
+        nb->rh[d] = out[d];
+        nb->vh[d] = out[3 + d];
+        nb->vb[d] = out[6 + d];
+    }
+    nb->mass = out[9];
+    nb->Gmass = out[10];
+    nb->radius = out[11];
+    nb->levelg = s->levelg[ib];
+    nb->levelm = s->levelm[ib];
+}
+
+/* swiftest_util_rearray_pl (swiftest_util.f90:1638-1811) after the merges:
+ * drop the discards (keep[] = 0), append the new bodies, mass sort, rhill,
+ * the level-irec encounter check over the full step, and the old entries'
+ * state carried over where the same pair sits at the same list position. */
+static void rearray(swo_sys *s, const char *keep, const newbody *nb, int nnew, const int32_t *oid1,
+                    const int32_t *oid2, int irec)
+{
+    const int nold = s->npl;
+    int n = 0;
+    int *src = malloc(sizeof(int) * (nold + nnew + 1));
+    for (int i = 0; i < nold; i++)
+        if (keep[i]) src[n++] = i;
+    for (int q = 0; q < nnew; q++) src[n++] = -1 - q;
+    /* mass sort, descending (swiftest_util_sort_pl); ties keep their order */
+    double *key = dalloc(n);
+    for (int q = 0; q < n; q++) key[q] = src[q] >= 0 ? s->mass[src[q]] : nb[-1 - src[q]].mass;
+    int *ord = malloc(sizeof(int) * (n + 1));
+    for (int q = 0; q < n; q++) ord[q] = q;
+    for (int q = 1; q < n; q++) {
+        int v = ord[q], p = q - 1;
+        while (p >= 0 && key[ord[p]] < key[v]) {
+            ord[p + 1] = ord[p];
+            p--;
+        }
+        ord[p + 1] = v;
+    }
+    double *rh = dalloc(3 * n), *vh = dalloc(3 * n), *vb = dalloc(3 * n), *Gm = dalloc(n), *ms = dalloc(n),
+           *R = dalloc(n);
+    int32_t *id = ialloc(n), *lg = ialloc(n), *lm = ialloc(n), *isp = ialloc(n);
+    double *atp = dalloc(n), *peri = dalloc(n);
+    for (int q = 0; q < n; q++) {
+        const int a = src[ord[q]];
+        if (a >= 0) {
+            memcpy(&rh[3 * q], &s->rh[3 * a], 3 * sizeof(double));
+            memcpy(&vh[3 * q], &s->vh[3 * a], 3 * sizeof(double));
+            memcpy(&vb[3 * q], &s->vb[3 * a], 3 * sizeof(double));
+            Gm[q] = s->Gmass[a];
+            ms[q] = s->mass[a];
+            R[q] = s->radius[a];
+            id[q] = s->id[a];
+            lg[q] = s->levelg[a];
+            lm[q] = s->levelm[a];
+            if (s->dis_ready) {
+                isp[q] = s->pisperi[a];
+                atp[q] = s->patp[a];
+                peri[q] = s->pperi[a];
+            }
+        } else {
+            const newbody *b = &nb[-1 - a];
+            memcpy(&rh[3 * q], b->rh, 3 * sizeof(double));
+            memcpy(&vh[3 * q], b->vh, 3 * sizeof(double));
+            memcpy(&vb[3 * q], b->vb, 3 * sizeof(double));
+            Gm[q] = b->Gmass;
+            ms[q] = b->mass;
+            R[q] = b->radius;
+            id[q] = b->id;
+            lg[q] = b->levelg;
+            lm[q] = b->levelm;
+            isp[q] = 1; /* swiftest_util_setup_body: isperi = 1 */
+            atp[q] = 0.0;
+            peri[q] = 0.0;
+        }
+    }
+    s->npl = n;
+    memcpy(s->rh, rh, 3 * sizeof(double) * n);
+    memcpy(s->vh, vh, 3 * sizeof(double) * n);
+    memcpy(s->vb, vb, 3 * sizeof(double) * n);
+    memcpy(s->Gmass, Gm, sizeof(double) * n);
+    memcpy(s->mass, ms, sizeof(double) * n);
+    memcpy(s->radius, R, sizeof(double) * n);
+    memcpy(s->id, id, sizeof(int32_t) * n);
+    memcpy(s->levelg, lg, sizeof(int32_t) * n);
+    memcpy(s->levelm, lm, sizeof(int32_t) * n);
+    if (s->dis_ready) {
+        memcpy(s->pisperi, isp, sizeof(int32_t) * n);
+        memcpy(s->patp, atp, sizeof(double) * n);
+        memcpy(s->pperi, peri, sizeof(double) * n);
+    }
+    for (int i = 0; i < n; i++) {
+        s->status[i] = SWO_ACTIVE;
+        s->lmask[i] = 1;
+        s->mu[i] = s->GMcb + s->Gmass[i];
+    }
+    free(rh);
+    free(vh);
+    free(vb);
+    free(Gm);
+    free(ms);
+    free(R);
+    free(id);
+    free(lg);
+    free(lm);
+    free(isp);
+    free(atp);
+    free(peri);
+    free(key);
+    free(ord);
+    free(src);
+    if (s->integ == SWO_SYMBA && s->p.gmtiny > 0.0) {
+        s->nplm = 0;
+        for (int i = 0; i < n; i++) s->nplm += !(s->Gmass[i] < s->p.gmtiny);
+    } else {
+        s->nplm = n;
+    }
+    set_rhill(s);
+    /* the old list, as the merges left it */
+    enclist *l = &s->plpl;
+    const int64_t nenc_old = l->n;
+    int32_t *olv = ialloc(nenc_old), *ost = ialloc(nenc_old), *olev = ialloc(nenc_old);
+    memcpy(olv, l->lvdotr, sizeof(int32_t) * nenc_old);
+    memcpy(ost, l->status, sizeof(int32_t) * nenc_old);
+    memcpy(olev, l->level, sizeof(int32_t) * nenc_old);
+    /* symba_encounter_check_pl at level irec over the full step: the list is
+     * replaced only when the check finds an encounter */
+    int32_t *nid1, *nid2;
+    int64_t nnew_enc = 0;
+    {
+        set_renc(s, irec);
+        int32_t *i1, *i2, *lv;
+        nnew_enc = swo_encounter_check_all_plplm(s->npl, s->nplm, s->rh, s->vb, s->renc, s->p.dt, &i1, &i2, &lv);
+        if (nnew_enc > 0) {
+            list_resize(l, nnew_enc);
+            nid1 = ialloc(nnew_enc);
+            nid2 = ialloc(nnew_enc);
+            for (int64_t k = 0; k < nnew_enc; k++) {
+                l->i1[k] = i1[k];
+                l->i2[k] = i2[k];
+                l->lvdotr[k] = lv[k];
+                l->status[k] = SWO_ACTIVE;
+                l->level[k] = irec;
+                int a = i1[k] - 1, b = i2[k] - 1;
+                nid1[k] = s->id[a];
+                nid2[k] = s->id[b];
+                s->levelg[a] = irec;
+                s->levelm[a] = irec;
+                s->levelg[b] = irec;
+                s->levelm[b] = irec;
+            }
+        } else {
+            nid1 = ialloc(nenc_old);
+            nid2 = ialloc(nenc_old);
+            memcpy(nid1, oid1, sizeof(int32_t) * nenc_old);
+            memcpy(nid2, oid2, sizeof(int32_t) * nenc_old);
+        }
+        free(i1);
+        free(i2);
+        free(lv);
+    }
+    if (nenc_old > 0) {
+        int64_t nmin = l->n < nenc_old ? l->n : nenc_old;
+        l->n = nmin;
+        int bad = 0;
+        for (int64_t k = 0; k < nmin; k++) {
+            if ((nid1[k] == oid1[k] && nid2[k] == oid2[k]) || (nid1[k] == oid2[k] && nid2[k] == oid1[k])) {
+                l->lvdotr[k] = olv[k];
+                l->status[k] = ost[k];
+                l->level[k] = olev[k];
+            }
+            int a = 0, b = 0;
+            for (int q = 0; q < s->npl && !(a && b); q++) {
+                if (!a && s->id[q] == nid1[k]) a = q + 1;
+                if (!b && s->id[q] == nid2[k]) b = q + 1;
+            }
+            l->i1[k] = a;
+            l->i2[k] = b;
+            bad |= (a == 0 || b == 0);
+        }
+        if (bad) { /* pack the entries whose bodies are gone */
+            int64_t o = 0;
+            for (int64_t k = 0; k < nmin; k++) {
+                if (l->i1[k] == 0 || l->i2[k] == 0) continue;
+                l->i1[o] = l->i1[k];
+                l->i2[o] = l->i2[k];
+                l->lvdotr[o] = l->lvdotr[k];
+                l->status[o] = l->status[k];
+                l->level[o] = l->level[k];
+                o++;
+            }
+            l->n = o;
+        }
+    }
+    free(nid1);
+    free(nid2);
+    free(olv);
+    free(ost);
+    free(olev);
+}
+
+/* collision_resolve_plpl (collision_resolve.f90:521-662) */
+static int resolve_plpl(swo_sys *s, double dt, int irec)
+{
+    vb2vh_pl(s); /* :557-558: vh and cb%vb, then cb%rb, from the current state */
+    double cb_rb[3];
+    rh2rb_cb(s, cb_rb);
+    for (int loop = 1; loop <= 1000; loop++) {
+        enclist *l = &s->plpl;
+        /* collision_resolve_extract_plpl (:180-250): the colliding entries
+         * leave the list, in list order */
+        int64_t nc = 0;
+        for (int64_t k = 0; k < l->n; k++) nc += l->status[k] == SWO_COLLIDED;
+        if (nc == 0) return 0;
+        int32_t *ci = ialloc(nc), *cj = ialloc(nc);
+        char *seen = calloc(s->npl > 0 ? s->npl : 1, 1);
+        int64_t o = 0, c = 0;
+        int family = 0;
+        for (int64_t k = 0; k < l->n; k++) {
+            if (l->status[k] == SWO_COLLIDED) {
+                ci[c] = l->i1[k] - 1;
+                cj[c] = l->i2[k] - 1;
+                family |= seen[ci[c]] | seen[cj[c]];
+                seen[ci[c]] = seen[cj[c]] = 1;
+                c++;
+                continue;
+            }
+            l->i1[o] = l->i1[k];
+            l->i2[o] = l->i2[k];
+            l->lvdotr[o] = l->lvdotr[k];
+            l->status[o] = l->status[k];
+            l->level[o] = l->level[k];
+            o++;
+        }
+        l->n = o;
+        if (family) {
+            free(ci);
+            free(cj);
+            free(seen);
+            return -4;
+        }
+        /* the list's ids (id1/id2 of the entries), and a copy of them as
+         * they stand before the merges rename the impactors */
+        int32_t *nid1 = ialloc(l->n), *nid2 = ialloc(l->n);
+        for (int64_t k = 0; k < l->n; k++) {
+            nid1[k] = s->id[l->i1[k] - 1];
+            nid2[k] = s->id[l->i2[k] - 1];
+        }
+        newbody *nb = calloc(nc, sizeof(newbody));
+        char *keep = malloc(s->npl > 0 ? s->npl : 1);
+        memset(keep, 1, s->npl > 0 ? s->npl : 1);
+        for (int64_t q = 0; q < nc; q++) {
+            merge_pair(s, ci[q], cj[q], cb_rb, nid1, nid2, &nb[q]);
+            keep[ci[q]] = keep[cj[q]] = 0;
+            s->nmerge++;
+        }
+        rearray(s, keep, nb, (int)nc, nid1, nid2, irec);
+        free(nid1);
+        free(nid2);
+        free(nb);
+        free(keep);
+        free(ci);
+        free(cj);
+        free(seen);
+        /* a new body may itself be colliding (:646-648) */
+        if (!collision_check(s, dt, irec)) return 0;
+    }
+    return -5; /* a runaway collisional cascade (:649-654) */
+}
+
 /* symba_step_recur_system, symba_step.f90:163-262 */
 static int recur(swo_sys *s, int ireci)
 {
@@ -1445,8 +1891,13 @@ static int recur(swo_sys *s, int ireci)
             kick_list_pltp(s, dth, irecp, -1);
         }
         if (s->p.lclose && collision_check(s, dtl, ireci)) {
-            if (!s->col_finish) return -1;
-            s->col_seen = 1;
+            if (s->merge) {
+                int rc = resolve_plpl(s, dtl, ireci);
+                if (rc) return rc;
+            } else {
+                if (!s->col_finish) return -1;
+                s->col_seen = 1;
+            }
         }
         set_recur_levels(s, ireci);
     }
@@ -1917,7 +2368,7 @@ void swo_sys_free(swo_sys *s)
                     s->id, s->iflag, s->trh, s->tvh, s->tvb, s->tah, s->tstatus, s->tlevelg, s->tlevelm,
                     s->tlmask, s->tid, s->tiflag, s->plpl.i1, s->plpl.i2, s->plpl.lvdotr, s->plpl.status,
                     s->plpl.level, s->pltp.i1, s->pltp.i2, s->pltp.lvdotr, s->pltp.status, s->pltp.level,
-                    s->pisperi, s->tisperi, s->patp, s->pperi, s->tatp, s->tperi};
+                    s->pisperi, s->tisperi, s->patp, s->pperi, s->tatp, s->tperi, s->mass};
     for (size_t q = 0; q < sizeof ptrs / sizeof ptrs[0]; q++) free(ptrs[q]);
     free(s);
 }
@@ -1975,6 +2426,33 @@ int64_t swo_sys_list_status(const swo_sys *s, int which, int32_t *status)
 }
 
 void swo_sys_set_collision_mode(swo_sys *s, int finish_step) { s->col_finish = finish_step != 0; }
+
+/* Collisions resolved by merging inside the step (SyMBA without test
+ * particles).  pl_mass in input order, or NULL for Gmass / GU.  Body ids are
+ * 1..npl in input order; merged bodies take the next ids. */
+int swo_sys_set_merge(swo_sys *s, double GU, const double *pl_mass)
+{
+    if (s->integ != SWO_SYMBA || s->ntp > 0 || !(GU > 0.0)) return -1;
+    free(s->mass);
+    s->mass = dalloc(s->npl);
+    for (int i = 0; i < s->npl; i++) s->mass[i] = pl_mass ? pl_mass[s->id[i] - 1] : s->Gmass[i] / GU;
+    s->GU = GU;
+    s->merge = 1;
+    s->maxid = s->npl + s->ntp;
+    return 0;
+}
+
+void swo_sys_get_bodies(const swo_sys *s, int32_t *id, double *mass, double *Gmass, double *radius)
+{
+    for (int i = 0; i < s->npl; i++) {
+        if (id) id[i] = s->id[i];
+        if (mass) mass[i] = s->mass ? s->mass[i] : 0.0;
+        if (Gmass) Gmass[i] = s->Gmass[i];
+        if (radius) radius[i] = s->radius[i];
+    }
+}
+
+int64_t swo_sys_nmerge(const swo_sys *s) { return s->nmerge; }
 
 /* ======================================================================
  * Energy and angular momentum

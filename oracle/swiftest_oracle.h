@@ -146,6 +146,11 @@ int64_t swo_sys_list_status(const swo_sys *s, int which, int32_t *status);
  * returned at its end: the state a caller gets when the collision is not
  * resolved within the step. */
 void swo_sys_set_collision_mode(swo_sys *s, int finish_step);
+int swo_sys_set_merge(swo_sys *s, double GU, const double *pl_mass);
+void swo_sys_get_bodies(const swo_sys *s, int32_t *id, double *mass, double *Gmass, double *radius);
+int64_t swo_sys_nmerge(const swo_sys *s);
+int swo_merge_bodies(const double *m, const double *Gm, const double *R, const double *rh, const double *vb,
+                     const double *cb_rb, const double *cb_vb, double GU, double *out);
 
 /* ---- energy and angular momentum (output cadence) ---- */
 /* Fields of the energy record, as swiftest_nbody_system stores them. */

@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libswiftest_hip.so")
 SOURCES = ["engine.hip", "k_step.hip", "k_enc.hip", "k_leaf.hip", "k_level.hip", "k_subtree.hip", "k_sas.hip", "k_energy.hip",
-           "k_discard.hip"]
+           "k_discard.hip", "merge.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

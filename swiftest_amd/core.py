@@ -103,6 +103,9 @@ def library():
         L.swg_system_set_pair_mode.argtypes = [vp, C.c_int32]
         L.swg_system_energy.argtypes = [vp, D, D, D, D]
         L.swg_system_discard.argtypes = [vp, C.POINTER(DiscardParams), D, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.swg_system_set_merge.argtypes = [vp, C.c_double, vp]
+        L.swg_system_counts.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
+        L.swg_system_get_bodies.argtypes = [vp] * 5
         _lib = L
     return _lib
 
@@ -217,7 +220,7 @@ class GpuSystem:
     """
 
     def __init__(self, integrator, systems, dt, lclose=True, short_circuit=True, istep_out=0, rhill=None, ctx=None,
-                 encounter_check=0, pair_mode=PAIR_EXACT, gmtiny=0.0):
+                 encounter_check=0, pair_mode=PAIR_EXACT, gmtiny=0.0, merge=None):
         if not isinstance(systems, (list, tuple)):
             systems = [systems]
         self.L = library()
@@ -240,6 +243,14 @@ class GpuSystem:
         self.h = h
         self._cb = (np.array([s.GMcb for s in systems], np.float64), np.array([s.Rcb for s in systems], np.float64))
         self._gm_in = [np.asarray(s.Gmass, np.float64) for s in systems]
+        # merge = (GU, pl_mass or None): collisions resolved by merging inside
+        # the step (swg_system_set_merge); the body count then shrinks
+        self._merge_gu = None
+        if merge is not None:
+            GU, mass = merge
+            self._mass_in = None if mass is None else _c(mass)
+            _check(self.L.swg_system_set_merge(self.h, float(GU), _p(self._mass_in)))
+            self._merge_gu = float(GU)
 
     @classmethod
     def slots(cls, integrator, npl_cap, ntp_cap, dt, lclose=True, short_circuit=True, ctx=None,
@@ -294,7 +305,29 @@ class GpuSystem:
     def step(self):
         _check(self.L.swg_system_step(self.h))
 
+    def _sync_counts(self):
+        n, m, k = C.c_int32(), C.c_int32(), C.c_int64()
+        _check(self.L.swg_system_counts(self.h, C.byref(n), C.byref(m), C.byref(k)))
+        if self.nsys == 1:
+            self.npl_s[0] = n.value
+        self.npl, self.ntp = n.value, m.value
+        return k.value
+
+    @property
+    def nmerge(self):
+        """Merges resolved so far (swg_system_counts)."""
+        return self._sync_counts()
+
+    def bodies(self):
+        """Ids, masses, Gmass and radii of the massive bodies, in storage order."""
+        self._sync_counts()
+        n = self.npl
+        out = dict(id=np.zeros(n, np.int32), mass=np.zeros(n), Gmass=np.zeros(n), radius=np.zeros(n))
+        _check(self.L.swg_system_get_bodies(self.h, *[_p(out[k]) for k in ("id", "mass", "Gmass", "radius")]))
+        return out
+
     def state(self):
+        self._sync_counts()
         npl, ntp = self.npl, self.ntp
         out = dict(pl_rh=np.zeros((npl, 3)), pl_vh=np.zeros((npl, 3)), pl_vb=np.zeros((npl, 3)),
                    tp_rh=np.zeros((ntp, 3)), tp_vh=np.zeros((ntp, 3)), tp_vb=np.zeros((ntp, 3)),
@@ -333,6 +366,7 @@ class GpuSystem:
     def accel(self):
         """(pl_acc, tp_acc): the direct-sum accelerations at the current
         positions from the step's own kernels (swg_system_accel)."""
+        self._sync_counts()
         a, b = np.zeros((self.npl, 3)), np.zeros((self.ntp, 3))
         _check(self.L.swg_system_accel(self.h, _p(a), _p(b)))
         return a, b
@@ -374,6 +408,9 @@ class GpuSystem:
             cb_mass = self._cb[0] / gu
         if cb_radius is None:
             cb_radius = self._cb[1]
+        self._sync_counts()
+        if pl_mass is None and self._merge_gu is not None:
+            pl_mass = self.bodies()["mass"]
         if pl_mass is None:
             ids = np.zeros(self.npl, np.int32)
             _check(self.L.swg_system_get(self.h, *([None] * 8), _p(ids)))

@@ -394,6 +394,8 @@ int Ensemble::create(const swg_params &p, int ns, const int32_t *npl_s, const in
     bw.maxsz = (int32_t *)(bw.tot + 6);
     lv_buf.assign(MAXLEV + 2, nullptr);
     lv_cap.assign(MAXLEV + 2, 0);
+    lvv.assign(MAXLEV + 2, LevelView{});
+    lvc.assign(MAXLEV + 2, LevelCounts{});
     if (const char *e = getenv("SWG_SUBTREE_MAX")) {
         subtree_max = strtoll(e, nullptr, 10);
         subtree_forced = true;
@@ -648,19 +650,21 @@ int Ensemble::recur0()
     int32_t lenc = 0;
     if (int rc = build_wait(lenc)) return rc;
     if (lenc) {
-        LevelView v;
-        LevelCounts c;
         bool handoff = false;
-        if (int rc = build_end(1, nullptr, v, c, handoff)) return rc;
+        if (int rc = build_end(1, nullptr, lvv[1], lvc[1], handoff)) return rc;
         tr_hand[1] += handoff;
         if (handoff) {
-            if (int rc = run_subtree(1, v)) return rc;
+            if (int rc = run_subtree(1, lvv[1])) return rc;
         } else {
-            if (int rc = recur_level(1, v, c)) return rc;
+            if (int rc = recur_level(1)) return rc;
         }
     }
-    launch_level_kicks(st, la, nullptr, npl, ntp, 0, dth);
+    // a merge inside the recursion changes the bodies and the list
+    const LevelArgs lb = level_args();
+    launch_level_kicks(st, lb, nullptr, npl, ntp, 0, dth);
     if (prm.lclose) launch_collision(st, pp, pt, pl, tp, sy, npp, npt, 0, dt, d_any);
+    if (merge_on)
+        if (int rc = merge_point(0, dt)) return rc;
     launch_set_levels(st, pl, tp, pp, pt, npl, ntp, npp, npt, 0);
     return 0;
 }
@@ -744,7 +748,7 @@ int Ensemble::build_end(int dnew, const LevelView *src, LevelView &v, LevelCount
 // symba_step_recur_system (symba_step.f90:163-262) at level d >= 1 on the
 // whole device, all systems in lockstep: a system with no pair at level d has
 // an empty share of the working set, and every launch is a no-op for it.
-int Ensemble::recur_level(int d, const LevelView &v, const LevelCounts &c)
+int Ensemble::recur_level(int d)
 {
     hipStream_t st = stream;
     const double dt = prm.dt;
@@ -753,9 +757,13 @@ int Ensemble::recur_level(int d, const LevelView &v, const LevelCounts &c)
     const double dtl = dt / p3, dth = 0.5 * dtl;
     if (dtl / dt < DBL_EPSILON) return fail(SWG_ERR_DEPTH, "recursion step below double precision");
     if (d > st_.max_depth) st_.max_depth = d;
-    const LevelArgs la = level_args();
-    const int nB = (int)c.nB, nBt = (int)c.nBt;
+    // the level's working set; a merge rebuilds it in place, so the sizes
+    // and the state arguments are read again after anything that can merge
+    const LevelView &v = lvv[d];
+    const LevelCounts &c = lvc[d];
     for (int j = 1; j <= NTENC; j++) {
+        LevelArgs la = level_args();
+        int nB = (int)c.nB, nBt = (int)c.nBt;
         st_.nsubsteps++;
         tr_sub[d]++;
         tr_B[d] += c.nB + c.nBt;
@@ -773,19 +781,24 @@ int Ensemble::recur_level(int d, const LevelView &v, const LevelCounts &c)
         if (int rc = room ? build_wait(lenc) : wait_any(lenc)) return rc;
         if (lenc) {
             if (!room) return fail(SWG_ERR_DEPTH, "recursion deeper than the level stack");
-            LevelView nv;
-            LevelCounts nc;
             bool handoff = false;
-            if (int rc = build_end(d + 1, &v, nv, nc, handoff)) return rc;
+            if (int rc = build_end(d + 1, &v, lvv[d + 1], lvc[d + 1], handoff)) return rc;
             tr_hand[d + 1] += handoff;
             if (handoff) {
-                if (int rc = run_subtree(d + 1, nv)) return rc;
+                if (int rc = run_subtree(d + 1, lvv[d + 1])) return rc;
             } else {
-                if (int rc = recur_level(d + 1, nv, nc)) return rc;
+                if (int rc = recur_level(d + 1)) return rc;
             }
+            la = level_args();
+            nB = (int)c.nB;
+            nBt = (int)c.nBt;
         }
         launch_level_kicks(st, la, &v, nB, nBt, d, dth, (j < NTENC && !prm.lclose) ? 1 : 0, d_kc);
         if (prm.lclose) launch_level_collision(st, la, v, nB, nBt, dtl);
+        if (merge_on) {
+            if (int rc = merge_point(d, dtl)) return rc;
+            la = level_args();
+        }
         launch_level_set_levels(st, la, v, c, d, j < NTENC ? d_any : nullptr);
     }
     return 0;
@@ -1909,6 +1922,28 @@ int swg_system_use_graphs(swg_system *sys, int32_t enable)
     sys->e->use_graphs = enable != 0;
     if (!enable) sys->e->drop_graphs();
     return 0;
+}
+
+int swg_system_set_merge(swg_system *sys, double GU, const double *pl_mass)
+{
+    if (!sys) return fail(SWG_ERR_ARG, "NULL system");
+    HIPCHK(hipStreamSynchronize(sys->e->stream));
+    return sys->e->set_merge(GU, pl_mass);
+}
+
+int swg_system_counts(swg_system *sys, int32_t *npl, int32_t *ntp, int64_t *nmerge)
+{
+    if (!sys) return fail(SWG_ERR_ARG, "NULL system");
+    if (npl) *npl = sys->e->npl;
+    if (ntp) *ntp = sys->e->ntp;
+    if (nmerge) *nmerge = sys->e->nmerge;
+    return 0;
+}
+
+int swg_system_get_bodies(swg_system *sys, int32_t *id, double *mass, double *Gmass, double *radius)
+{
+    if (!sys) return fail(SWG_ERR_ARG, "NULL system");
+    return sys->e->get_bodies(id, mass, Gmass, radius);
 }
 
 int swg_system_set_frozen(swg_system *sys, const int32_t *frozen)

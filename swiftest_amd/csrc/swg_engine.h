@@ -75,7 +75,11 @@ struct Ensemble {
     int build_begin(int dnew, const LevelView *src);
     int build_wait(int32_t &lenc);
     int build_end(int dnew, const LevelView *src, LevelView &v, LevelCounts &c, bool &handoff);
-    int recur_level(int d, const LevelView &v, const LevelCounts &c);
+    int recur_level(int d);
+    // the working set of each level the recursion is in (rebuilt in place
+    // after a merge changes the bodies and the list)
+    std::vector<LevelView> lvv;
+    std::vector<LevelCounts> lvc;
     int run_subtree(int d0, const LevelView &v);
     // host-driven recursion counters per level (substeps, bodies, entries,
     // hand-offs), reported with SWG_TRACE=1
@@ -150,6 +154,18 @@ struct Ensemble {
     Vec3p save_pl{}, save_tp{};  // velocity copies kept across kernel timings
     int check_errors();
     int err_code();
+
+    // collisions resolved by merging inside the step (merge.hip)
+    bool merge_on = false;
+    double GU = 0.0;
+    std::vector<double> h_mass;
+    int32_t maxid = 0, nplm_cur = 0;
+    int64_t nmerge = 0;
+    int set_merge(double GU, const double *pl_mass);
+    int merge_point(int irec, double dt);
+    int resolve_merges(int irec, double dt);
+    int upload_merged(const void *host_sys, const double *cbvb, int irec);
+    int get_bodies(int32_t *id, double *mass, double *Gmass, double *radius);
 
     // internals
     template <class T> int own(T **p, size_t n);

@@ -514,7 +514,7 @@ __host__ __device__ inline bool encounter_one(double xr, double yr, double zr, d
 
 // swiftest_orbel_xv2aeq (swiftest_orbel.f90:700-764): semi-major axis and
 // pericentre
-__device__ inline void xv2aq(double mu, double rx, double ry, double rz, double vx, double vy, double vz, double &a,
+__host__ __device__ inline void xv2aq(double mu, double rx, double ry, double rz, double vx, double vy, double vz, double &a,
                              double &q)
 {
     a = 0.0;
@@ -552,7 +552,7 @@ __device__ inline void xv2aq(double mu, double rx, double ry, double rz, double 
 }
 
 // the pericentre alone
-__device__ inline double xv2q(double mu, double rx, double ry, double rz, double vx, double vy, double vz)
+__host__ __device__ inline double xv2q(double mu, double rx, double ry, double rz, double vx, double vy, double vz)
 {
     double a, q;
     xv2aq(mu, rx, ry, rz, vx, vy, vz, a, q);
@@ -560,7 +560,7 @@ __device__ inline double xv2q(double mu, double rx, double ry, double rz, double
 }
 
 // collision_check_one (collision_check.f90:16-50)
-__device__ inline bool collision_one(double xr, double yr, double zr, double vxr, double vyr, double vzr,
+__host__ __device__ inline bool collision_one(double xr, double yr, double zr, double vxr, double vyr, double vzr,
                                      double Gmtot, double rlim, double dt, bool lvdotr)
 {
     double r2 = dot3(xr, yr, zr, xr, yr, zr);

@@ -1,0 +1,59 @@
+"""GPU parity of SyMBA steps whose collisions are resolved by merging inside
+the step (swg_system_set_merge): the device stops at the substep whose
+collision check flagged a pair, the host merges and rearranges the bodies,
+and the step continues.  The oracle does the same in its recursion
+(collision_resolve_plpl, collision_resolve.f90:521-662, with the MERGE model
+and swiftest_util_rearray_pl, swiftest_util.f90:1638-1811).
+
+Bar: bit-identical states, body ids, masses, radii, recursion counts and
+level-0 lists after every block of steps.  The crowded disks below merge at
+the deepest levels (where the rearray's encounter check over the full step
+finds nothing and the old list is kept) and, with radii inflated 30x and
+100x, at levels 2-6 (where it finds pairs, and the old entries' state is
+carried over position by position and the list is cut to its old length).
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+import swiftest_amd as S
+from helpers import compact_disk, ulp_diff
+from swiftest_amd import ics
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(o, g, label):
+    so, sg = o.state(), g.state()
+    bo, bg = o.bodies(), g.bodies()
+    assert so["pl_rh"].shape == sg["pl_rh"].shape, (label, so["pl_rh"].shape, sg["pl_rh"].shape)
+    for k in ("id", "mass", "Gmass", "radius"):
+        assert np.array_equal(bo[k], bg[k]), (label, k)
+    assert np.array_equal(so["pl_id"], sg["pl_id"]), label
+    for k in ("pl_rh", "pl_vh", "pl_vb", "pl_status", "rhill"):
+        assert np.array_equal(so[k], sg[k]), f"{label} {k} ulp={ulp_diff(so[k], sg[k]) if so[k].dtype == np.float64 else ''}"
+    st_o, st_g = o.stats(), g.stats()
+    for k in ("nplpl_enc", "max_depth", "nsubsteps"):
+        assert st_o[k] == st_g[k], (label, k, st_o, st_g)
+    i1, i2, _ = o.enc_list(0)
+    a, b = g.enc_list(0)
+    assert np.array_equal(i1, a) and np.array_equal(i2, b), label
+    assert o.nmerge == g.nmerge, (label, o.nmerge, g.nmerge)
+
+
+@pytest.mark.parametrize("seed,scale,min_merges", [(11, 1.0, 1), (11, 30.0, 3), (13, 100.0, 9)])
+def test_merges_bit_exact(seed, scale, min_merges):
+    sysic = compact_disk(seed=seed, npl=160, ntp=0)
+    sysic.radius[:] = sysic.radius * scale
+    o = O.OracleSystem(O.SYMBA, sysic, 0.005, istep_out=10, merge=(ics.GU, None))
+    g = S.GpuSystem(S.SYMBA, sysic, 0.005, istep_out=10, merge=(ics.GU, None))
+    nblk = 6 if scale == 1.0 else 4
+    for blk in range(nblk):
+        print(f"merge seed={seed} scale={scale} block {blk}", flush=True)
+        assert o.run(10) == 0
+        g.run(10)
+        _compare(o, g, f"seed {seed} scale {scale} block {blk}")
+    assert g.nmerge >= min_merges
+    b = g.bodies()
+    assert len(b["id"]) == 160 - g.nmerge
+    assert set(b["id"][b["id"] > 160]) <= set(range(161, 161 + g.nmerge))
