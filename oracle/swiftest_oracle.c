@@ -9,7 +9,12 @@
  * in tests/test_oracle.py: closed forms for the pair kernels and the encounter
  * test, analytic two-body propagation and invariants for the Danby drift,
  * Stumpff closed forms, shell-polynomial continuity, and the reference's own
- * conservation thresholds (tests/test_swiftest.py:118-121) on whole runs.
+ * conservation thresholds (tests/test_swiftest.py:118-121) on whole runs;
+ * the collision merges in tests/test_oracle_merge.py (conservation laws, the
+ * closed form of the velocity torque, the overlap shift).  Parity unpinned:
+ * the rearray's carry-over of old entries by list position and the cut of
+ * the list to its old length after a merge (swiftest_util.f90:1744-1805),
+ * compared only with the device's restatement.
  */
 #include "swiftest_oracle.h"
 
