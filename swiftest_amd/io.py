@@ -205,16 +205,20 @@ class NetCDFWriter:
     """Frames of one system in the reference's output variables.
 
     Dimensions: time (record), name (central body, then massive bodies, then
-    test particles), space (x, y, z) and string32.  Per frame: rh, vh
-    (time, name, space), Gmass, radius, rhill (time, name), status, npl, ntp,
-    and when given the energy record KE_orb, KE_spin, PE, BE, TE, L_orbit,
-    L_spin (the fields of swg_system_energy)."""
+    test particles, then room for `room` bodies made by merges), space
+    (x, y, z) and string32.  Row q holds the body of id q (the central body
+    is id 0; new bodies take ids after the last test particle).  Per frame:
+    rh, vh (time, name, space), Gmass, radius, rhill (time, name), status,
+    npl, ntp, and when given the energy record KE_orb, KE_spin, PE, BE, TE,
+    L_orbit, L_spin (the fields of swg_system_energy).  Rows of bodies not
+    in the frame (merged away, or not made yet) hold NaN and status
+    INACTIVE."""
 
-    def __init__(self, path: str, system: ics.System, extra: Bodies | None = None):
+    def __init__(self, path: str, system: ics.System, extra: Bodies | None = None, room: int = 0):
         from scipy.io import netcdf_file
         extra = extra or Bodies()
-        self.npl, self.ntp = system.npl, system.ntp
-        n = 1 + self.npl + self.ntp
+        self.npl, self.ntp, self.room = system.npl, system.ntp, int(room)
+        n = 1 + self.npl + self.ntp + self.room
         self.f = netcdf_file(path, "w", version=2)
         f = self.f
         f.createDimension("time", None)
@@ -224,14 +228,18 @@ class NetCDFWriter:
         f.createVariable("time", "d", ("time",))
         sp = f.createVariable("space", "c", ("space",))
         sp[:] = np.array(list("xyz"), "S1")
+        # a merged body is named for its id (the reference appends _MERGE and
+        # the id to the larger impactor's name, collision_resolve.f90:449-457)
         names = [extra.cb_name] + (extra.pl_names or [f"pl{i + 1}" for i in range(self.npl)]) + \
-            (extra.tp_names or [f"tp{i + 1}" for i in range(self.ntp)])
+            (extra.tp_names or [f"tp{i + 1}" for i in range(self.ntp)]) + \
+            [f"MERGE{1 + self.npl + self.ntp + q:07d}" for q in range(self.room)]
         nm = f.createVariable("name", "c", ("name", "string32"))
         nm[:] = np.array([list(s[:32].ljust(32)) for s in names], "S1")
         idv = f.createVariable("id", "i", ("name",))
         idv[:] = np.arange(n, dtype=np.int32)
         pt = f.createVariable("particle_type", "c", ("name", "string32"))
-        kinds = ["Central Body"] + ["Massive Body"] * self.npl + ["Test Particle"] * self.ntp
+        kinds = ["Central Body"] + ["Massive Body"] * self.npl + ["Test Particle"] * self.ntp + \
+            ["Massive Body"] * self.room
         pt[:] = np.array([list(s.ljust(32)) for s in kinds], "S1")
         for v in ("rh", "vh"):
             f.createVariable(v, "d", ("time", "name", "space"))
@@ -247,40 +255,58 @@ class NetCDFWriter:
         self.k = 0
         self.system = system
 
-    def write_frame(self, t: float, state: dict, energy=None):
+    def write_frame(self, t: float, state: dict, energy=None, bodies: dict | None = None):
         """One frame from GpuSystem.state() (one system) and, optionally, its
-        energy row (GpuSystem.energy()[s])."""
+        energy row (GpuSystem.energy()[s]) and its bodies (GpuSystem.bodies():
+        the Gmass and radius of each id, needed once merges change them)."""
         f, k, npl, ntp = self.f, self.k, self.npl, self.ntp
         s = self.system
+        n = 1 + npl + ntp + self.room
         f.variables["time"][k] = t
-        # rows in input order, as the names and ids: the device holds the
-        # massive bodies sorted (storage row r is body pl_id[r])
+        # rows by id: the device holds the massive bodies sorted (storage row
+        # r is body pl_id[r])
         ids = np.asarray(state.get("pl_id", np.arange(1, npl + 1)))
+        nan = np.nan
 
-        def by_id(a):
-            out = np.empty_like(np.asarray(a))
-            out[ids - 1] = a
+        def rows(a, fill, width=None):
+            a = np.asarray(a)
+            out = np.full((n,) + a.shape[1:], fill, dtype=a.dtype if a.dtype != np.float64 else np.float64)
+            out[ids] = a
             return out
 
-        rh = np.zeros((1 + npl + ntp, 3))
-        vh = np.zeros_like(rh)
-        rh[1:1 + npl], vh[1:1 + npl] = by_id(state["pl_rh"]), by_id(state["pl_vh"])
-        rh[1 + npl:], vh[1 + npl:] = state["tp_rh"], state["tp_vh"]
+        rh, vh = rows(state["pl_rh"], nan), rows(state["pl_vh"], nan)
+        rh[0] = vh[0] = 0.0
+        rh[1 + npl:1 + npl + ntp], vh[1 + npl:1 + npl + ntp] = state["tp_rh"], state["tp_vh"]
         f.variables["rh"][k] = rh
         f.variables["vh"][k] = vh
-        f.variables["Gmass"][k] = np.concatenate([[s.GMcb], np.asarray(s.Gmass), np.zeros(ntp)])
-        f.variables["radius"][k] = np.concatenate([[s.Rcb], np.asarray(s.radius), np.zeros(ntp)])
-        f.variables["rhill"][k] = np.concatenate([[0.0], by_id(state.get("rhill", np.zeros(npl))), np.zeros(ntp)])
-        f.variables["status"][k] = np.concatenate([[0], by_id(state["pl_status"]), state["tp_status"]]).astype(np.int32)
+        if bodies is None:
+            gm = np.full(n, nan)
+            rad = np.full(n, nan)
+            gm[1:1 + npl], rad[1:1 + npl] = s.Gmass, s.radius
+        else:
+            gm, rad = rows(bodies["Gmass"], nan), rows(bodies["radius"], nan)
+        gm[0], rad[0] = s.GMcb, s.Rcb
+        gm[1 + npl:1 + npl + ntp] = rad[1 + npl:1 + npl + ntp] = 0.0
+        f.variables["Gmass"][k] = gm
+        f.variables["radius"][k] = rad
+        rhill = rows(state.get("rhill", np.zeros(len(ids))), nan)
+        rhill[0] = 0.0
+        rhill[1 + npl:1 + npl + ntp] = 0.0
+        f.variables["rhill"][k] = rhill
+        status = rows(np.asarray(state["pl_status"], np.int32), np.int32(1))
+        status[0] = 0
+        status[1 + npl:1 + npl + ntp] = state["tp_status"]
+        f.variables["status"][k] = status.astype(np.int32)
         f.variables["npl"][k] = int((state["pl_status"] == 0).sum())
         f.variables["ntp"][k] = int((state["tp_status"] == 0).sum())
-        if energy is not None:
-            e = np.asarray(energy)
-            for name, j in (("KE_orb", 0), ("KE_spin", 1), ("PE", 2), ("BE", 3), ("TE", 5)):
-                f.variables[name][k] = e[j]
-            f.variables["L_orbit"][k] = e[6:9]
-            f.variables["L_spin"][k] = e[9:12]
-            f.variables["GMescape"][k] = 0.0
+        # every record variable gets a value in every frame (the record
+        # layout needs them all); without an energy record those are NaN
+        e = np.asarray(energy) if energy is not None else np.full(16, np.nan)
+        for name, j in (("KE_orb", 0), ("KE_spin", 1), ("PE", 2), ("BE", 3), ("TE", 5)):
+            f.variables[name][k] = e[j]
+        f.variables["L_orbit"][k] = e[6:9]
+        f.variables["L_spin"][k] = e[9:12]
+        f.variables["GMescape"][k] = 0.0 if energy is not None else np.nan
         self.k += 1
 
     def close(self):

@@ -116,6 +116,23 @@ def test_c2_symba_disk_1e3_steps_bit_exact(c2_oracle):
     assert nsub > 0  # the recursion ran at the checkpoints
 
 
+def test_c2_merge_mode_200_steps_bit_exact(c2_oracle):
+    """C2 as specified, merge-only collisions: the same run with collisions
+    merged inside the step (every level on the whole device, the host told
+    of each substep's collision check).  The disk has no collision in its
+    first 1000 steps, so the states equal the plain run's checkpoints."""
+    c2, ref = c2_oracle
+    g = S.GpuSystem(S.SYMBA, c2, 0.005, istep_out=100, merge=(ics.GU, None))
+    for k, (so, sto, (i1, i2)) in enumerate(ref[:2]):
+        g.run(100)
+        sg, stg = g.state(), g.stats()
+        for key in ("pl_rh", "pl_vh", "pl_vb"):
+            assert np.array_equal(so[key], sg[key]), (k, key)
+        for key in ("nplpl_enc", "max_depth", "nsubsteps"):
+            assert stg[key] == sto[key], (k, key, stg[key], sto[key])
+    assert g.nmerge == 0
+
+
 def test_c2_fast_pairs_1e3_steps_within_1e_10(c2_oracle):
     c2, ref = c2_oracle
     g = S.GpuSystem(S.SYMBA, c2, 0.005, istep_out=100, pair_mode=S.PAIR_FAST)

@@ -207,3 +207,75 @@ def test_driver_frames_equal_the_oracle(tmp_path):
         assert f.variables["time"][fr] == pytest.approx(0.05 * fr)
     assert np.all(np.isfinite(f.variables["TE"][:])) and f.variables["TE"][0] != 0.0
     f.close()
+
+
+def test_netcdf_frames_follow_a_changing_body_set(tmp_path):
+    """Frames without an energy record read back, and rows follow ids: a body
+    made by a merge lands in the room after the input bodies, bodies gone
+    read NaN / INACTIVE."""
+    from scipy.io import netcdf_file
+    s = ics.symba_disk(npl=20)
+    w = sio.NetCDFWriter(str(tmp_path / "o.nc"), s, None, room=3)
+    frames = [np.arange(1, 21), np.r_[np.arange(3, 21), [21]], np.r_[np.arange(5, 21), [21, 22]]]
+    rng = np.random.Generator(np.random.PCG64(1))
+    data = []
+    for fr, ids in enumerate(frames):
+        n = len(ids)
+        st = dict(pl_rh=rng.normal(size=(n, 3)), pl_vh=rng.normal(size=(n, 3)), tp_rh=np.zeros((0, 3)),
+                  tp_vh=np.zeros((0, 3)), pl_status=np.zeros(n, np.int32), tp_status=np.zeros(0, np.int32),
+                  pl_id=ids.astype(np.int32), rhill=rng.uniform(size=n))
+        b = dict(Gmass=rng.uniform(size=n), radius=rng.uniform(size=n))
+        w.write_frame(0.5 * fr, st, None, b)
+        data.append((ids, st, b))
+    w.close()
+    f = netcdf_file(str(tmp_path / "o.nc"), "r", mmap=False)
+    assert f.variables["rh"].shape == (3, 24, 3)
+    for fr, (ids, st, b) in enumerate(data):
+        assert np.array_equal(f.variables["rh"][fr, ids], st["pl_rh"])
+        assert np.array_equal(f.variables["Gmass"][fr, ids], b["Gmass"])
+        gone = np.setdiff1d(np.arange(1, 24), ids)
+        assert np.all(np.isnan(f.variables["rh"][fr, gone])) and np.all(f.variables["status"][fr, gone] == 1)
+        assert np.isnan(f.variables["TE"][fr])
+    assert bytes(f.variables["name"][21]).decode().strip() == "MERGE0000021"
+    f.close()
+
+
+@pytest.mark.gpu
+def test_driver_merges_follow_the_oracle(tmp_path):
+    """The driver loop with CHK_CLOSE on a crowded SyMBA disk whose radii are
+    inflated 30x, so that bodies merge (COLLISION_MODEL MERGE, merged inside
+    the step): every frame's rows, by id, equal the oracle's merge-mode run,
+    merged bodies appear in the rows after the last input body with their
+    new Gmass and radius, and merged-away bodies turn to NaN and INACTIVE."""
+    from scipy.io import netcdf_file
+
+    import pyoracle as O
+    from helpers import compact_disk
+    from swiftest_amd import driver, ics
+    s = compact_disk(seed=11, npl=160, ntp=0)
+    s.radius[:] = s.radius * 30.0
+    extra = sio.Bodies(cb_name="Sun", pl_names=[f"p{i + 1}" for i in range(s.npl)])
+    prm = dict(sio.DEFAULTS, IN_TYPE="ASCII", IN_FORM="XV", CB_IN="cb.in", PL_IN="pl.in", TP_IN="tp.in",
+               CHK_CLOSE=True, DT=0.005, T0=0.0, TSTOP=0.2, ISTEP_OUT=10, COLLISION_MODEL="MERGE")
+    sio.write_in(prm, s, extra, str(tmp_path))
+    sio.write_param(str(tmp_path / "param.in"), prm)
+    out = str(tmp_path / "out.nc")
+    g = driver.run("SYMBA", str(tmp_path / "param.in"), out)
+    assert g.nmerge >= 1
+    f = netcdf_file(out, "r", mmap=False)
+    assert f.variables["rh"].shape[0] == 5
+    r, _ = sio.read_in(prm, str(tmp_path))
+    o = O.OracleSystem(O.SYMBA, r, 0.005, istep_out=10, merge=(ics.GU, None))
+    for fr in range(5):
+        if fr:
+            assert o.run(10) == 0
+        so, b = o.state(), o.bodies()
+        ids = b["id"]
+        assert np.array_equal(f.variables["rh"][fr, ids], so["pl_rh"]), fr
+        assert np.array_equal(f.variables["vh"][fr, ids], so["pl_vh"]), fr
+        assert np.array_equal(f.variables["Gmass"][fr, ids], b["Gmass"]), fr
+        assert np.array_equal(f.variables["radius"][fr, ids], b["radius"]), fr
+        gone = np.setdiff1d(np.arange(1, f.variables["rh"].shape[1]), ids)
+        assert np.all(np.isnan(f.variables["rh"][fr, gone])) and np.all(f.variables["status"][fr, gone] == 1), fr
+        assert f.variables["npl"][fr] == len(ids)
+    f.close()
