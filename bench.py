@@ -125,6 +125,18 @@ def small_workloads_leg(log, cpu=True, c2_cpu_steps_1thread=100):
                        "how": "one system, exact pair mode, steps 2..N timed"}}
         del g
         log(f"[bench] {name}: GPU {res['gpu']['steps_per_s']:.1f} steps/s")
+        if integ == S.SYMBA and lclose and sysic.ntp == 0:
+            # the workload's merge-only collisions resolved inside the step
+            # (swg_system_set_merge): a step that collides runs again from
+            # its start with a stop at each substep's collision check
+            g = S.GpuSystem(integ, sysic, dt, istep_out=100, lclose=lclose, merge=(ics.GU, None))
+            g.run(1)
+            t0 = time.perf_counter()
+            g.run(nsteps - 1)
+            t_m = time.perf_counter() - t0
+            res["gpu_merge_mode"] = {"steps_per_s": (nsteps - 1) / t_m, "seconds": t_m, "merges": g.nmerge}
+            del g
+            log(f"[bench] {name}: GPU merge mode {res['gpu_merge_mode']['steps_per_s']:.1f} steps/s")
         if cpu:
             sys.path.insert(0, os.path.join(ROOT, "oracle"))
             import pyoracle as O

@@ -321,8 +321,10 @@ int swg_system_discard(swg_system *sys, const swg_discard_params *dp, const doub
  * colliding pairs of one check (a collision family) returns
  * SWG_ERR_COLLISION.  GU: G in the system's units; pl_mass: masses in input
  * order (NULL: Gmass / GU).  Body ids are 1..npl in input order; a merged
- * body takes the next unused id.  Every level runs on the whole device in
- * this mode. */
+ * body takes the next unused id.  A step that flags a collision runs twice:
+ * as usual, then again from a copy of its starting state with every level
+ * on the whole device and a stop at each substep's collision check; other
+ * steps run as usual after a device-to-device copy of the bodies' state. */
 int swg_system_set_merge(swg_system *sys, double GU, const double *pl_mass);
 /* Current body counts (they shrink with merges) and the merges so far. */
 int swg_system_counts(swg_system *sys, int32_t *npl, int32_t *ntp, int64_t *nmerge);

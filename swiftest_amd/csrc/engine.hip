@@ -663,7 +663,7 @@ int Ensemble::recur0()
     const LevelArgs lb = level_args();
     launch_level_kicks(st, lb, nullptr, npl, ntp, 0, dth);
     if (prm.lclose) launch_collision(st, pp, pt, pl, tp, sy, npp, npt, 0, dt, d_any);
-    if (merge_on)
+    if (merge_replay)
         if (int rc = merge_point(0, dt)) return rc;
     launch_set_levels(st, pl, tp, pp, pt, npl, ntp, npp, npt, 0);
     return 0;
@@ -741,7 +741,8 @@ int Ensemble::build_end(int dnew, const LevelView *src, LevelView &v, LevelCount
     // alone (hand-off modes of the tests).
     const bool fits = subtree_lds_fits(mx[1], mx[2], mx[3], mx[4]);
     level_fits = fits;
-    handoff = subtree_max >= 0 && maxsz <= subtree_max && (subtree_forced || fits || mx[5] >= SUBTREE_MIN_SYSTEMS);
+    handoff = !merge_replay && subtree_max >= 0 && maxsz <= subtree_max &&
+              (subtree_forced || fits || mx[5] >= SUBTREE_MIN_SYSTEMS);
     return 0;
 }
 
@@ -795,7 +796,7 @@ int Ensemble::recur_level(int d)
         }
         launch_level_kicks(st, la, &v, nB, nBt, d, dth, (j < NTENC && !prm.lclose) ? 1 : 0, d_kc);
         if (prm.lclose) launch_level_collision(st, la, v, nB, nBt, dtl);
-        if (merge_on) {
+        if (merge_replay) {
             if (int rc = merge_point(d, dtl)) return rc;
             la = level_args();
         }
@@ -1160,7 +1161,7 @@ int Ensemble::snapshot()
 int Ensemble::step()
 {
     if (!tiles_ok) return fail(SWG_ERR_HIP, "the tile arrays are missing after a failed allocation; rebuild them first");
-    int rc = prm.integrator == SWG_WHM ? step_whm() : step_helio_symba();
+    int rc = prm.integrator == SWG_WHM ? step_whm() : merge_on ? step_merge() : step_helio_symba();
     if (rc && rc != SWG_ERR_COLLISION) return rc;
     iloop++;  // a step that found a collision was completed
     return rc;

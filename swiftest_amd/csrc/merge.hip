@@ -198,6 +198,11 @@ int Ensemble::set_merge(double GU_, const double *pl_mass)
     GU = GU_;
     h_mass.resize(npl);
     for (int i = 0; i < npl; i++) h_mass[i] = pl_mass ? pl_mass[h_pl_id[i] - 1] : h_Gm[i] / GU;
+    if (!mb_cb) {
+        if (alloc_vec(mb_rh, npl) || alloc_vec(mb_vh, npl) || alloc_vec(mb_vb, npl) || own(&mb_status, npl) ||
+            own(&mb_sys, 2 * (size_t)nsys) || own(&mb_cb, 9 * (size_t)nsys))
+            return SWG_ERR_HIP;
+    }
     maxid = npl + ntp;
     nplm_cur = npl;
     if (prm.gmtiny > 0.0) {
@@ -205,12 +210,58 @@ int Ensemble::set_merge(double GU_, const double *pl_mass)
         for (int i = 0; i < npl; i++) nplm_cur += !(h_Gm[i] < prm.gmtiny);
     }
     merge_on = true;
-    // the merges need every level on the whole device, where the host sees
-    // each substep's collision check, and no captured launch sequences
-    subtree_max = -1;
+    // the replay of a step that merges changes the body count mid-step: no
+    // captured launch sequences
     use_graphs = false;
     drop_graphs();
     return 0;
+}
+
+// The state a step changes (bodies' positions, velocities and status, the
+// central body's sums, lfirst, the error bits), copied aside at the start of
+// the step (restore = false) or put back (restore = true).  Everything else a
+// step uses is rebuilt at its start (levels, renc, lists, flags).
+int Ensemble::merge_save(bool restore)
+{
+    hipStream_t st = stream;
+    auto cp = [&](void *a, void *b, size_t bytes) -> int {
+        if (bytes) MCHK(hipMemcpyAsync(restore ? a : b, restore ? b : a, bytes, hipMemcpyDeviceToDevice, st));
+        return 0;
+    };
+    const size_t n = (size_t)npl * sizeof(double);
+    for (auto pr : {std::make_pair(pl.rh, mb_rh), std::make_pair(pl.vh, mb_vh), std::make_pair(pl.vb, mb_vb)}) {
+        if (int rc = cp(pr.first.x, pr.second.x, n)) return rc;
+        if (int rc = cp(pr.first.y, pr.second.y, n)) return rc;
+        if (int rc = cp(pr.first.z, pr.second.z, n)) return rc;
+    }
+    if (int rc = cp(pl.status, mb_status, (size_t)npl * sizeof(int32_t))) return rc;
+    if (int rc = cp(sy.lfirst, mb_sys, nsys * sizeof(int32_t))) return rc;
+    if (int rc = cp(sy.err, mb_sys + nsys, nsys * sizeof(int32_t))) return rc;
+    if (int rc = cp(sy.cbvb, mb_cb, 3 * nsys * sizeof(double))) return rc;
+    if (int rc = cp(sy.ptbeg, mb_cb + 3 * nsys, 3 * nsys * sizeof(double))) return rc;
+    if (int rc = cp(sy.ptend, mb_cb + 6 * nsys, 3 * nsys * sizeof(double))) return rc;
+    return 0;
+}
+
+// One step with merges: the step as usual (the persistent recursion kernels
+// included); when it flags a collision, the step again from its start with
+// every level on the whole device, resolving each collision where the
+// recursion finds it.  The recursion is deterministic and gives the same
+// bits in every scheduling mode, so the replay retraces the first pass up to
+// the collision.
+int Ensemble::step_merge()
+{
+    if (int rc = merge_save(false)) return rc;
+    merge_replay = false;
+    const int64_t ncol = st_.ncollision;
+    int rc = step_helio_symba();
+    if (rc != SWG_ERR_COLLISION) return rc;
+    if (int r2 = merge_save(true)) return r2;
+    st_.ncollision = ncol;
+    merge_replay = true;
+    rc = step_helio_symba();
+    merge_replay = false;
+    return rc;
 }
 
 // After the collision launch of a substep at level irec: resolve what it
@@ -221,6 +272,7 @@ int Ensemble::merge_point(int irec, double dt)
     MCHK(hipMemcpyAsync(h, sy.err, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
     MCHK(hipStreamSynchronize(stream));
     if (!(h[0] & 1)) return 0;
+    st_.ncollision++;
     if (int rc = resolve_merges(irec, dt)) return rc;
     h[0] &= ~1;
     MCHK(hipMemcpyAsync(sy.err, h, sizeof(int32_t), hipMemcpyHostToDevice, stream));

@@ -161,6 +161,15 @@ struct Ensemble {
     std::vector<double> h_mass;
     int32_t maxid = 0, nplm_cur = 0;
     int64_t nmerge = 0;
+    // a step runs as usual; one that flags a collision is run again from a
+    // copy of the state taken at its start, with every level on the whole
+    // device and a stop at each substep's collision check (merge_replay)
+    bool merge_replay = false;
+    Vec3p mb_rh{}, mb_vh{}, mb_vb{};
+    int32_t *mb_status = nullptr, *mb_sys = nullptr;  // pl status; per system lfirst, err
+    double *mb_cb = nullptr;                           // per system cbvb, ptbeg, ptend
+    int step_merge();
+    int merge_save(bool restore);
     int set_merge(double GU, const double *pl_mass);
     int merge_point(int irec, double dt);
     int resolve_merges(int irec, double dt);

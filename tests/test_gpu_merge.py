@@ -33,7 +33,7 @@ def _compare(o, g, label):
     for k in ("pl_rh", "pl_vh", "pl_vb", "pl_status", "rhill"):
         assert np.array_equal(so[k], sg[k]), f"{label} {k} ulp={ulp_diff(so[k], sg[k]) if so[k].dtype == np.float64 else ''}"
     st_o, st_g = o.stats(), g.stats()
-    for k in ("nplpl_enc", "max_depth", "nsubsteps"):
+    for k in ("nplpl_enc", "max_depth", "nsubsteps", "ncollision"):
         assert st_o[k] == st_g[k], (label, k, st_o, st_g)
     i1, i2, _ = o.enc_list(0)
     a, b = g.enc_list(0)
