@@ -57,3 +57,32 @@ def test_merges_bit_exact(seed, scale, min_merges):
     b = g.bodies()
     assert len(b["id"]) == 160 - g.nmerge
     assert set(b["id"][b["id"] > 160]) <= set(range(161, 161 + g.nmerge))
+
+
+def test_merges_with_discard_checks_and_energy():
+    """Merges between discard checks: the pericentre state of the checks
+    (isperi, atp, peri per body) follows the bodies through the rearray, so
+    the checks after each step give the oracle's counts and state; and the
+    energy record after the merges, with the merged bodies' masses, equals
+    the oracle's energy of the same state to 1e-12."""
+    sysic = compact_disk(seed=13, npl=160, ntp=0)
+    sysic.radius[:] = sysic.radius * 100.0
+    kw = dict(rmin=0.3, rmax=5.0, qmin=0.5, qmin_alo=0.1, qmin_ahi=10.0, qmin_coord=0)
+    o = O.OracleSystem(O.SYMBA, sysic, 0.005, merge=(ics.GU, None))
+    g = S.GpuSystem(S.SYMBA, sysic, 0.005, merge=(ics.GU, None))
+    for step in range(30):
+        print(f"merge+discard step {step}", flush=True)
+        assert o.step() == 0
+        g.step()
+        assert g.discard(**kw) == o.discard(**kw) == (0, 0), step
+        _compare(o, g, f"step {step}")
+    assert g.nmerge >= 5
+    st, b = o.state(), o.bodies()
+    cbm = sysic.GMcb / ics.GU
+    got = g.energy(cbm, np.array([sysic.Rcb]))[0]
+    ref, _ = O.energy(sysic.GMcb, cbm, sysic.Rcb, st["cb_vb"], st["pl_rh"], st["pl_vb"], b["Gmass"], b["mass"],
+                      b["radius"], st["pl_status"], lclose=True)
+    for f in (0, 2, 3, 4, 5, 15):
+        assert got[f] == pytest.approx(ref[f], rel=1e-12, abs=0), f
+    for a, c in ((6, 9), (12, 15)):
+        assert np.linalg.norm(got[a:c] - ref[a:c]) <= 1e-12 * np.linalg.norm(ref[a:c]), a
