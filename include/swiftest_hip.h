@@ -319,8 +319,9 @@ int swg_system_discard(swg_system *sys, const swg_discard_params *dp, const doub
  * triangular encounter check; rotation and the ENERGY torque
  * (collision_resolve.f90:624-641) are not modelled, and a body in two
  * colliding pairs of one check (a collision family) returns
- * SWG_ERR_COLLISION.  GU: G in the system's units; pl_mass: masses in input
- * order (NULL: Gmass / GU).  Body ids are 1..npl in input order; a merged
+ * SWG_ERR_COLLISION at that substep: the system is then left as it stood
+ * there, inside the step, and is not to be stepped further.  GU: G in the
+ * system's units; pl_mass: masses in input order (NULL: Gmass / GU).  Body ids are 1..npl in input order; a merged
  * body takes the next unused id.  A step that flags a collision runs twice:
  * as usual, then again from a copy of its starting state with every level
  * on the whole device and a stop at each substep's collision check; other
