@@ -19,6 +19,10 @@ next request in it completes.  A completed request's state
 is copied out (swg_system_get_slot) and its slot is frozen and free for the
 next arrival.  Padding bodies make a slot's system evolve exactly as it would
 alone, so each result equals that of the request run by itself.
+
+The pools are allocated when the server starts, before the clock of the
+arrivals: arrival, admission and completion times count from the moment
+every pool is ready.
 """
 from __future__ import annotations
 
@@ -98,7 +102,7 @@ class Server:
         raise ValueError(f"no slot class holds a system of {system.npl} + {system.ntp} bodies")
 
     # ---- one pool ----
-    def _lane(self, k, p, pending, t0, errors):
+    def _lane(self, k, p, pending, ready, errors):
         try:
             c = self.classes[k]
             ctx = core.new_context()
@@ -107,12 +111,14 @@ class Server:
                                             lclose=self.lclose, ctx=ctx, pair_mode=self.pair_mode)
                 with self._lock:
                     self.pools.append((k, pool))
-                self._drive(k, p, c, pool, pending, t0)
+                ready.wait()  # every pool allocated: the clock starts
+                self._drive(k, p, c, pool, pending, self._t0)
                 del pool
             finally:
                 core.library().swg_finalize(ctx)
         except Exception as e:  # reported by run()
             errors.append(e)
+            ready.abort()
             with self._lock:
                 self._lock.notify_all()
 
@@ -164,16 +170,20 @@ class Server:
         for r in sorted(requests, key=lambda r: r.arrival):
             queues[self.class_of(r.system)].append(r)
         errors = []
-        t0 = time.perf_counter()
-        threads = [threading.Thread(target=self._lane, args=(k, p, queues[k], t0, errors))
-                   for k in range(len(self.classes)) if queues[k] for p in range(self.classes[k].pools)]
+        lanes = [(k, p) for k in range(len(self.classes)) if queues[k] for p in range(self.classes[k].pools)]
+        ready = threading.Barrier(len(lanes) + 1, action=lambda: setattr(self, "_t0", time.perf_counter()))
+        threads = [threading.Thread(target=self._lane, args=(k, p, queues[k], ready, errors)) for k, p in lanes]
         for t in threads:
             t.start()
+        try:
+            ready.wait()
+        except threading.BrokenBarrierError:
+            pass  # a pool failed to start; its error is raised below
         for t in threads:
             t.join()
         if errors:
             raise errors[0]
-        return time.perf_counter() - t0
+        return time.perf_counter() - self._t0
 
     def close(self):
         self.pools = []
