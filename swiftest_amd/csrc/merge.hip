@@ -191,6 +191,7 @@ int Ensemble::set_merge(double GU_, const double *pl_mass)
 {
     if (prm.integrator != SWG_SYMBA) return fail(SWG_ERR_ARG, "merging collisions needs SyMBA");
     if (nsys != 1) return fail(SWG_ERR_ARG, "merging collisions needs a single system");
+    if (h_nreal_pl[0] != npl) return fail(SWG_ERR_ARG, "merging collisions needs a system, not a slot pool");
     if (ntp != 0) return fail(SWG_ERR_ARG, "merging collisions needs a system without test particles");
     if (!prm.lclose) return fail(SWG_ERR_ARG, "merging collisions needs the collision checks (lclose)");
     if (prm.encounter_check != 0) return fail(SWG_ERR_ARG, "merging collisions needs the triangular encounter check");

@@ -86,3 +86,21 @@ def test_merges_with_discard_checks_and_energy():
         assert got[f] == pytest.approx(ref[f], rel=1e-12, abs=0), f
     for a, c in ((6, 9), (12, 15)):
         assert np.linalg.norm(got[a:c] - ref[a:c]) <= 1e-12 * np.linalg.norm(ref[a:c]), a
+
+
+def test_merge_mode_refusals():
+    """Outside its scope the merge mode is refused at set-up, with an error
+    and no change to the system."""
+    disk = compact_disk(seed=11, npl=40, ntp=0)
+    with_tp = compact_disk(seed=11, npl=40, ntp=8)
+    cases = [lambda: S.GpuSystem(S.HELIO, disk, 0.005, merge=(ics.GU, None)),
+             lambda: S.GpuSystem(S.SYMBA, with_tp, 0.005, merge=(ics.GU, None)),
+             lambda: S.GpuSystem(S.SYMBA, disk, 0.005, lclose=False, merge=(ics.GU, None)),
+             lambda: S.GpuSystem(S.SYMBA, [disk, disk], 0.005, merge=(ics.GU, None)),
+             lambda: S.GpuSystem(S.SYMBA, disk, 0.005, encounter_check=1, merge=(ics.GU, None)),
+             lambda: S.GpuSystem(S.SYMBA, disk, 0.005, merge=(0.0, None))]
+    for make in cases:
+        with pytest.raises(S.SwiftestHipError):
+            make()
+    pool = S.GpuSystem.slots(S.SYMBA, [64], [0], 0.005)  # one slot: a slot pool all the same
+    assert pool.L.swg_system_set_merge(pool.h, ics.GU, None) != 0
