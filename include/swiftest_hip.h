@@ -315,8 +315,9 @@ int swg_system_discard(swg_system *sys, const swg_discard_params *dp, const doub
  * full step (keeping the old entries' state where the same pair sits at the
  * same list position and at most as many entries as before), checks the new
  * list for further collisions, and continues the step on the new arrays.
- * Needs one SyMBA system without test particles, CHK_CLOSE on and the
- * triangular encounter check; rotation and the ENERGY torque
+ * Needs SyMBA systems without test particles (one, or an ensemble: each
+ * system's merges are its own), CHK_CLOSE on and the triangular encounter
+ * check; rotation and the ENERGY torque
  * (collision_resolve.f90:624-641) are not modelled, and a body in two
  * colliding pairs of one check (a collision family) returns
  * SWG_ERR_COLLISION at that substep: the system is then left as it stood
@@ -329,6 +330,9 @@ int swg_system_discard(swg_system *sys, const swg_discard_params *dp, const doub
 int swg_system_set_merge(swg_system *sys, double GU, const double *pl_mass);
 /* Current body counts (they shrink with merges) and the merges so far. */
 int swg_system_counts(swg_system *sys, int32_t *npl, int32_t *ntp, int64_t *nmerge);
+/* Where each system's massive bodies start in the concatenated arrays
+ * (nsys + 1 entries; they move when merges shrink earlier systems). */
+int swg_system_pl_offsets(swg_system *sys, int32_t *pl_off);
 /* Ids, masses (0 unless merging is on), Gmass and radii of the massive
  * bodies, in storage order (npl of each; NULL pointers are skipped). */
 int swg_system_get_bodies(swg_system *sys, int32_t *id, double *mass, double *Gmass, double *radius);

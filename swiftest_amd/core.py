@@ -106,6 +106,7 @@ def library():
         L.swg_system_set_merge.argtypes = [vp, C.c_double, vp]
         L.swg_system_counts.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
         L.swg_system_get_bodies.argtypes = [vp] * 5
+        L.swg_system_pl_offsets.argtypes = [vp, vp]
         _lib = L
     return _lib
 
@@ -308,8 +309,9 @@ class GpuSystem:
     def _sync_counts(self):
         n, m, k = C.c_int32(), C.c_int32(), C.c_int64()
         _check(self.L.swg_system_counts(self.h, C.byref(n), C.byref(m), C.byref(k)))
-        if self.nsys == 1:
-            self.npl_s[0] = n.value
+        off = np.zeros(self.nsys + 1, np.int32)
+        _check(self.L.swg_system_pl_offsets(self.h, _p(off)))
+        self.npl_s = np.diff(off).astype(np.int32)
         self.npl, self.ntp = n.value, m.value
         return k.value
 
@@ -409,7 +411,7 @@ class GpuSystem:
         if cb_radius is None:
             cb_radius = self._cb[1]
         self._sync_counts()
-        if pl_mass is None and self._merge_gu is not None:
+        if pl_mass is None and getattr(self, "_merge_gu", None) is not None:
             pl_mass = self.bodies()["mass"]
         if pl_mass is None:
             ids = np.zeros(self.npl, np.int32)

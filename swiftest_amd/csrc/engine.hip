@@ -1941,6 +1941,13 @@ int swg_system_counts(swg_system *sys, int32_t *npl, int32_t *ntp, int64_t *nmer
     return 0;
 }
 
+int swg_system_pl_offsets(swg_system *sys, int32_t *pl_off)
+{
+    if (!sys || !pl_off) return fail(SWG_ERR_ARG, "NULL argument");
+    std::copy(sys->e->h_pl_off.begin(), sys->e->h_pl_off.end(), pl_off);
+    return 0;
+}
+
 int swg_system_get_bodies(swg_system *sys, int32_t *id, double *mass, double *Gmass, double *radius)
 {
     if (!sys) return fail(SWG_ERR_ARG, "NULL system");

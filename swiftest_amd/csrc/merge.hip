@@ -13,9 +13,9 @@
 // host code does the bookkeeping of the reference on a host copy of the
 // system, uploads the new arrays and list, and rebuilds the working sets of
 // the levels the recursion is in, so that the step continues where it
-// stopped.  Scope: the MERGE collision model without rotation, one system
-// without test particles, and no ENERGY torque (collision_resolve.f90:
-// 624-641).  A body in two colliding pairs of one check (a collision family,
+// stopped.  Scope: the MERGE collision model without rotation, systems
+// (one, or an ensemble) without test particles, and no ENERGY torque
+// (collision_resolve.f90:624-641).  A body in two colliding pairs of one check (a collision family,
 // collision_resolve_make_impactors_pl) is refused with SWG_ERR_COLLISION.
 #include <algorithm>
 #include <cmath>
@@ -190,25 +190,33 @@ struct HostSys {
 int Ensemble::set_merge(double GU_, const double *pl_mass)
 {
     if (prm.integrator != SWG_SYMBA) return fail(SWG_ERR_ARG, "merging collisions needs SyMBA");
-    if (nsys != 1) return fail(SWG_ERR_ARG, "merging collisions needs a single system");
-    if (h_nreal_pl[0] != npl) return fail(SWG_ERR_ARG, "merging collisions needs a system, not a slot pool");
+    for (int s = 0; s < nsys; s++)
+        if (h_nreal_pl[s] != h_pl_off[s + 1] - h_pl_off[s])
+            return fail(SWG_ERR_ARG, "merging collisions needs systems, not a slot pool");
     if (ntp != 0) return fail(SWG_ERR_ARG, "merging collisions needs a system without test particles");
     if (!prm.lclose) return fail(SWG_ERR_ARG, "merging collisions needs the collision checks (lclose)");
     if (prm.encounter_check != 0) return fail(SWG_ERR_ARG, "merging collisions needs the triangular encounter check");
     if (!(GU_ > 0.0)) return fail(SWG_ERR_ARG, "GU must be positive");
     GU = GU_;
     h_mass.resize(npl);
-    for (int i = 0; i < npl; i++) h_mass[i] = pl_mass ? pl_mass[h_pl_id[i] - 1] : h_Gm[i] / GU;
+    for (int s = 0; s < nsys; s++)
+        for (int i = h_pl_off[s]; i < h_pl_off[s + 1]; i++)
+            h_mass[i] = pl_mass ? pl_mass[h_pl_off[s] + h_pl_id[i] - 1] : h_Gm[i] / GU;
     if (!mb_cb) {
         if (alloc_vec(mb_rh, npl) || alloc_vec(mb_vh, npl) || alloc_vec(mb_vb, npl) || own(&mb_status, npl) ||
             own(&mb_sys, 2 * (size_t)nsys) || own(&mb_cb, 9 * (size_t)nsys))
             return SWG_ERR_HIP;
     }
-    maxid = npl + ntp;
-    nplm_cur = npl;
-    if (prm.gmtiny > 0.0) {
-        nplm_cur = 0;
-        for (int i = 0; i < npl; i++) nplm_cur += !(h_Gm[i] < prm.gmtiny);
+    h_maxid.assign(nsys, 0);
+    h_nplm.assign(nsys, 0);
+    for (int s = 0; s < nsys; s++) {
+        const int a = h_pl_off[s], n = h_pl_off[s + 1] - a;
+        h_maxid[s] = n + (h_tp_off[s + 1] - h_tp_off[s]);  // ids 1..n in input order, per system
+        h_nplm[s] = n;
+        if (prm.gmtiny > 0.0) {
+            h_nplm[s] = 0;
+            for (int i = a; i < a + n; i++) h_nplm[s] += !(h_Gm[i] < prm.gmtiny);
+        }
     }
     merge_on = true;
     // the replay of a step that merges changes the body count mid-step: no
@@ -269,89 +277,36 @@ int Ensemble::step_merge()
 // flagged.  sy.err bit 1 is the device's collision flag.
 int Ensemble::merge_point(int irec, double dt)
 {
-    int32_t *h = (int32_t *)(h_small + 48);
-    MCHK(hipMemcpyAsync(h, sy.err, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    std::vector<int32_t> err(nsys);
+    MCHK(hipMemcpyAsync(err.data(), sy.err, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
     MCHK(hipStreamSynchronize(stream));
-    if (!(h[0] & 1)) return 0;
-    st_.ncollision++;
+    bool any = false;
+    for (int s = 0; s < nsys; s++)
+        if (err[s] & 1) {
+            any = true;
+            st_.ncollision++;  // one count per system whose check found a collision
+            err[s] &= ~1;
+        }
+    if (!any) return 0;
     if (int rc = resolve_merges(irec, dt)) return rc;
-    h[0] &= ~1;
-    MCHK(hipMemcpyAsync(sy.err, h, sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    MCHK(hipMemcpyAsync(sy.err, err.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice, stream));
     MCHK(hipStreamSynchronize(stream));
     return 0;
 }
 
-int Ensemble::resolve_merges(int irec, double dt)
+// The merges of one system (local indices) on its host copy S: the
+// central body's barycentric state, then rounds of extract, merge, rearray
+// and collision check until no pair collides.  status: the bodies' status
+// before the merges; cbvb receives the central body's barycentric velocity.
+int Ensemble::resolve_system(void *hs, const std::vector<int32_t> &status, double GMcb, int irec, double dt,
+                             int32_t &maxid_s, int32_t &nplm_s, int64_t &ncol, double *cbvb)
 {
-    hipStream_t st = stream;
-    MCHK(hipStreamSynchronize(st));
-    const double GMcb = h_GMcb[0];
-    HostSys S;
-    S.n = npl;
-    const int n0 = npl;
-    S.rh.resize(3 * (size_t)n0);
-    S.vh.resize(3 * (size_t)n0);
-    S.vb.resize(3 * (size_t)n0);
-    S.R.resize(n0);
-    S.levelg.resize(n0);
-    S.levelm.resize(n0);
-    std::vector<int32_t> status(n0);
-    {
-        // device SoA -> host AoS
-        std::vector<double> x(n0), y(n0), z(n0);
-        auto down3 = [&](Vec3p v, std::vector<double> &out) -> int {
-            MCHK(hipMemcpyAsync(x.data(), v.x, n0 * sizeof(double), hipMemcpyDeviceToHost, st));
-            MCHK(hipMemcpyAsync(y.data(), v.y, n0 * sizeof(double), hipMemcpyDeviceToHost, st));
-            MCHK(hipMemcpyAsync(z.data(), v.z, n0 * sizeof(double), hipMemcpyDeviceToHost, st));
-            MCHK(hipStreamSynchronize(st));
-            for (int i = 0; i < n0; i++) {
-                out[3 * (size_t)i] = x[i];
-                out[3 * (size_t)i + 1] = y[i];
-                out[3 * (size_t)i + 2] = z[i];
-            }
-            return 0;
-        };
-        if (n0) {
-            if (int rc = down3(pl.rh, S.rh)) return rc;
-            if (int rc = down3(pl.vb, S.vb)) return rc;
-            MCHK(hipMemcpyAsync(S.R.data(), pl.R, n0 * sizeof(double), hipMemcpyDeviceToHost, st));
-            MCHK(hipMemcpyAsync(status.data(), pl.status, n0 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-            MCHK(hipMemcpyAsync(S.levelg.data(), pl.levelg, n0 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-            MCHK(hipMemcpyAsync(S.levelm.data(), pl.levelm, n0 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        }
-    }
-    S.Gm = h_Gm;
-    S.mass = h_mass;
-    S.id = h_pl_id;
-    if (dis_ready && n0) {
-        S.isperi.resize(n0);
-        S.atp.resize(n0);
-        S.peri.resize(n0);
-        MCHK(hipMemcpyAsync(S.isperi.data(), dis_pl_isperi, n0 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        MCHK(hipMemcpyAsync(S.atp.data(), dis_pl_atp, n0 * sizeof(double), hipMemcpyDeviceToHost, st));
-        MCHK(hipMemcpyAsync(S.peri.data(), dis_pl_peri, n0 * sizeof(double), hipMemcpyDeviceToHost, st));
-    }
-    {
-        const int64_t m = npp;
-        S.i1.resize(m);
-        S.i2.resize(m);
-        std::vector<int8_t> a(m), b(m), c(m);
-        if (m) {
-            MCHK(hipMemcpyAsync(S.i1.data(), pp.i1, m * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-            MCHK(hipMemcpyAsync(S.i2.data(), pp.i2, m * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-            MCHK(hipMemcpyAsync(a.data(), pp.lvdotr, m, hipMemcpyDeviceToHost, st));
-            MCHK(hipMemcpyAsync(b.data(), pp.status, m, hipMemcpyDeviceToHost, st));
-            MCHK(hipMemcpyAsync(c.data(), pp.level, m, hipMemcpyDeviceToHost, st));
-        }
-        MCHK(hipStreamSynchronize(st));
-        S.lvdotr.assign(a.begin(), a.end());
-        S.status.assign(b.begin(), b.end());
-        S.level.assign(c.begin(), c.end());
-    }
-
+    HostSys &S = *(HostSys *)hs;
+    const int n0 = S.n;
     // pl%vb2vh and pl%rh2rb (collision_resolve.f90:557-558): the central
     // body's barycentric velocity (bodies summed last to first) and position
-    double cbvb[3] = {0.0, 0.0, 0.0}, cbrb[3];
+    double cbrb[3];
+    cbvb[0] = cbvb[1] = cbvb[2] = 0.0;
     for (int i = n0 - 1; i >= 0; i--) {
         if (status[i] == INACTIVE) continue;
         for (int d = 0; d < 3; d++) cbvb[d] = cbvb[d] - (S.Gm[i] * S.vb[3 * (size_t)i + d]) / GMcb;
@@ -441,8 +396,8 @@ int Ensemble::resolve_merges(int irec, double dt)
                     }
                     if (S.id1[k] == S.id2[k]) S.status[k] = INACTIVE;
                 }
-            maxid = maxid + 1;
-            nb.id = maxid;
+            maxid_s = maxid_s + 1;
+            nb.id = maxid_s;
             nb.levelg = S.levelg[ib];
             nb.levelm = S.levelm[ib];
             add.push_back(nb);
@@ -615,7 +570,7 @@ int Ensemble::resolve_merges(int irec, double dt)
         T.id1.clear();
         T.id2.clear();
         S = std::move(T);
-        nplm_cur = nplm_new;
+        nplm_s = nplm_new;
         // a new body may itself be colliding (collision_resolve.f90:646-648)
         bool any = false;
         for (size_t k = 0; k < S.i1.size(); k++) {
@@ -629,28 +584,181 @@ int Ensemble::resolve_merges(int irec, double dt)
                 any = true;
             }
         }
-        if (any) st_.ncollision++;
+        if (any) ncol++;
         if (!any) break;
     }
-    return upload_merged(&S, cbvb, irec);
+    return 0;
 }
 
-// Put the merged system on the device: bodies, list and its adjacency, the
-// per-system sizes, tiles; then the working sets of levels 1..irec.
-int Ensemble::upload_merged(const void *hs, const double *cbvb, int irec)
+// Download the ensemble, resolve the merges of every system with a
+// colliding pair, and put the result back on the device.
+int Ensemble::resolve_merges(int irec, double dt)
 {
-    const HostSys &S = *(const HostSys *)hs;
     hipStream_t st = stream;
-    const int n = S.n;
+    MCHK(hipStreamSynchronize(st));
+    const int n0 = npl;
+    std::vector<double> rh(3 * (size_t)n0), vh(3 * (size_t)n0), vb(3 * (size_t)n0), R(n0), rhill(n0);
+    std::vector<int32_t> status(n0), lg(n0), lm(n0), isperi;
+    std::vector<double> atp, peri;
+    {
+        // device SoA -> host AoS
+        std::vector<double> x(n0), y(n0), z(n0);
+        auto down3 = [&](Vec3p v, std::vector<double> &out) -> int {
+            MCHK(hipMemcpyAsync(x.data(), v.x, n0 * sizeof(double), hipMemcpyDeviceToHost, st));
+            MCHK(hipMemcpyAsync(y.data(), v.y, n0 * sizeof(double), hipMemcpyDeviceToHost, st));
+            MCHK(hipMemcpyAsync(z.data(), v.z, n0 * sizeof(double), hipMemcpyDeviceToHost, st));
+            MCHK(hipStreamSynchronize(st));
+            for (int i = 0; i < n0; i++) {
+                out[3 * (size_t)i] = x[i];
+                out[3 * (size_t)i + 1] = y[i];
+                out[3 * (size_t)i + 2] = z[i];
+            }
+            return 0;
+        };
+        if (n0) {
+            if (int rc = down3(pl.rh, rh)) return rc;
+            if (int rc = down3(pl.vh, vh)) return rc;
+            if (int rc = down3(pl.vb, vb)) return rc;
+            MCHK(hipMemcpyAsync(R.data(), pl.R, n0 * sizeof(double), hipMemcpyDeviceToHost, st));
+            MCHK(hipMemcpyAsync(rhill.data(), pl.rhill, n0 * sizeof(double), hipMemcpyDeviceToHost, st));
+            MCHK(hipMemcpyAsync(status.data(), pl.status, n0 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            MCHK(hipMemcpyAsync(lg.data(), pl.levelg, n0 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            MCHK(hipMemcpyAsync(lm.data(), pl.levelm, n0 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        }
+        if (dis_ready && n0) {
+            isperi.resize(n0);
+            atp.resize(n0);
+            peri.resize(n0);
+            MCHK(hipMemcpyAsync(isperi.data(), dis_pl_isperi, n0 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            MCHK(hipMemcpyAsync(atp.data(), dis_pl_atp, n0 * sizeof(double), hipMemcpyDeviceToHost, st));
+            MCHK(hipMemcpyAsync(peri.data(), dis_pl_peri, n0 * sizeof(double), hipMemcpyDeviceToHost, st));
+        }
+    }
+    const int64_t m0 = npp;
+    std::vector<int32_t> gi1(m0), gi2(m0);
+    std::vector<int8_t> glv(m0), gst(m0), glev(m0);
+    if (m0) {
+        MCHK(hipMemcpyAsync(gi1.data(), pp.i1, m0 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        MCHK(hipMemcpyAsync(gi2.data(), pp.i2, m0 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        MCHK(hipMemcpyAsync(glv.data(), pp.lvdotr, m0, hipMemcpyDeviceToHost, st));
+        MCHK(hipMemcpyAsync(gst.data(), pp.status, m0, hipMemcpyDeviceToHost, st));
+        MCHK(hipMemcpyAsync(glev.data(), pp.level, m0, hipMemcpyDeviceToHost, st));
+    }
+    std::vector<double> cbvb(3 * (size_t)nsys);
+    MCHK(hipMemcpyAsync(cbvb.data(), sy.cbvb, 3 * nsys * sizeof(double), hipMemcpyDeviceToHost, st));
+    MCHK(hipStreamSynchronize(st));
+
+    // one host system per system; the list is ordered by index1, so each
+    // system's entries are one run of it
+    std::vector<HostSys> sys(nsys);
+    std::vector<char> hit(nsys, 0);
+    {
+        int64_t k = 0;
+        for (int s = 0; s < nsys; s++) {
+            const int a = h_pl_off[s], n = h_pl_off[s + 1] - a;
+            HostSys &S = sys[s];
+            S.n = n;
+            S.rh.assign(rh.begin() + 3 * (size_t)a, rh.begin() + 3 * (size_t)(a + n));
+            S.vh.assign(vh.begin() + 3 * (size_t)a, vh.begin() + 3 * (size_t)(a + n));
+            S.vb.assign(vb.begin() + 3 * (size_t)a, vb.begin() + 3 * (size_t)(a + n));
+            S.R.assign(R.begin() + a, R.begin() + a + n);
+            S.rhill.assign(rhill.begin() + a, rhill.begin() + a + n);
+            S.Gm.assign(h_Gm.begin() + a, h_Gm.begin() + a + n);
+            S.mass.assign(h_mass.begin() + a, h_mass.begin() + a + n);
+            S.id.assign(h_pl_id.begin() + a, h_pl_id.begin() + a + n);
+            S.levelg.assign(lg.begin() + a, lg.begin() + a + n);
+            S.levelm.assign(lm.begin() + a, lm.begin() + a + n);
+            if (!isperi.empty()) {
+                S.isperi.assign(isperi.begin() + a, isperi.begin() + a + n);
+                S.atp.assign(atp.begin() + a, atp.begin() + a + n);
+                S.peri.assign(peri.begin() + a, peri.begin() + a + n);
+            }
+            for (; k < m0 && gi1[k] < a + n; k++) {
+                S.i1.push_back(gi1[k] - a);
+                S.i2.push_back(gi2[k] - a);
+                S.lvdotr.push_back(glv[k]);
+                S.status.push_back(gst[k]);
+                S.level.push_back(glev[k]);
+                hit[s] |= gst[k] == COLLIDED;
+            }
+        }
+        if (k != m0) return fail(SWG_ERR_COLLISION, "encounter list out of system order at a merge");
+    }
+    for (int s = 0; s < nsys; s++) {
+        if (!hit[s]) continue;
+        const int a = h_pl_off[s];
+        std::vector<int32_t> st_s(status.begin() + a, status.begin() + a + sys[s].n);
+        if (int rc = resolve_system(&sys[s], st_s, h_GMcb[s], irec, dt, h_maxid[s], h_nplm[s], st_.ncollision,
+                                    &cbvb[3 * (size_t)s]))
+            return rc;
+    }
+    return upload_merged(sys.data(), hit, cbvb.data(), irec);
+}
+
+
+// Put the merged ensemble on the device: bodies, list and its adjacency, the
+// per-system sizes, tiles; then the working sets of levels 1..irec.  The
+// systems without merges keep their device state; the arrays are written
+// whole because the later systems' bodies move down.
+int Ensemble::upload_merged(const void *hs, const std::vector<char> &hit, const double *cbvb, int irec)
+{
+    const HostSys *sys = (const HostSys *)hs;
+    hipStream_t st = stream;
+    std::vector<int32_t> off(nsys + 1, 0);
+    for (int s = 0; s < nsys; s++) off[s + 1] = off[s] + sys[s].n;
+    const int n = off[nsys];
     // the body and list structures only shrink in a merge; the list may grow
-    // when every old entry was a collision
-    if (n > npl) return fail(SWG_ERR_COLLISION, "merged system larger than before");
+    // when every old entry of a system was a collision
+    if (n > npl) return fail(SWG_ERR_COLLISION, "merged ensemble larger than before");
+    std::vector<double> rh(3 * (size_t)n), vh(3 * (size_t)n), vb(3 * (size_t)n), Gm(n), mass(n), R(n), rhill(n);
+    std::vector<int32_t> id(n), lg(n), lm(n), sysid(n), tiny(std::max(n, 1), 0), isperi;
+    std::vector<double> atp, peri;
+    const bool dis = dis_ready && n;
+    if (dis) {
+        isperi.resize(n);
+        atp.resize(n);
+        peri.resize(n);
+    }
+    std::vector<int32_t> i1, i2;
+    std::vector<int8_t> lv, ls, ll;
+    for (int s = 0; s < nsys; s++) {
+        const HostSys &S = sys[s];
+        const int a = off[s];
+        std::copy(S.rh.begin(), S.rh.end(), rh.begin() + 3 * (size_t)a);
+        std::copy(S.vh.begin(), S.vh.end(), vh.begin() + 3 * (size_t)a);
+        std::copy(S.vb.begin(), S.vb.end(), vb.begin() + 3 * (size_t)a);
+        std::copy(S.Gm.begin(), S.Gm.end(), Gm.begin() + a);
+        std::copy(S.mass.begin(), S.mass.end(), mass.begin() + a);
+        std::copy(S.R.begin(), S.R.end(), R.begin() + a);
+        std::copy(S.rhill.begin(), S.rhill.end(), rhill.begin() + a);
+        std::copy(S.id.begin(), S.id.end(), id.begin() + a);
+        std::copy(S.levelg.begin(), S.levelg.end(), lg.begin() + a);
+        std::copy(S.levelm.begin(), S.levelm.end(), lm.begin() + a);
+        for (int q = 0; q < S.n; q++) {
+            sysid[a + q] = s;
+            tiny[a + q] = prm.gmtiny > 0.0 && S.Gm[q] < prm.gmtiny;
+        }
+        if (dis) {
+            std::copy(S.isperi.begin(), S.isperi.end(), isperi.begin() + a);
+            std::copy(S.atp.begin(), S.atp.end(), atp.begin() + a);
+            std::copy(S.peri.begin(), S.peri.end(), peri.begin() + a);
+        }
+        for (size_t k = 0; k < S.i1.size(); k++) {
+            if (k && S.i1[k] < S.i1[k - 1]) return fail(SWG_ERR_COLLISION, "encounter list out of row order after a merge");
+            if (S.i1[k] >= S.i2[k]) return fail(SWG_ERR_COLLISION, "encounter pair out of order after a merge");
+            i1.push_back(S.i1[k] + a);
+            i2.push_back(S.i2[k] + a);
+            lv.push_back((int8_t)S.lvdotr[k]);
+            ls.push_back((int8_t)S.status[k]);
+            ll.push_back((int8_t)S.level[k]);
+        }
+    }
     npl = n;
-    h_pl_off[1] = n;
-    h_nreal_pl[0] = n;
-    h_Gm = S.Gm;
-    h_mass = S.mass;
-    h_pl_id = S.id;
+    h_pl_off = off;
+    for (int s = 0; s < nsys; s++) h_nreal_pl[s] = sys[s].n;
+    h_Gm = Gm;
+    h_mass = mass;
+    h_pl_id = id;
     {
         std::vector<double> x(n), y(n), z(n);
         auto up3 = [&](Vec3p v, const std::vector<double> &in) -> int {
@@ -667,53 +775,55 @@ int Ensemble::upload_merged(const void *hs, const double *cbvb, int irec)
             MCHK(hipStreamSynchronize(st));
             return 0;
         };
-        if (int rc = up3(pl.rh, S.rh)) return rc;
-        if (int rc = up3(pl.vh, S.vh)) return rc;
-        if (int rc = up3(pl.vb, S.vb)) return rc;
+        if (int rc = up3(pl.rh, rh)) return rc;
+        if (int rc = up3(pl.vh, vh)) return rc;
+        if (int rc = up3(pl.vb, vb)) return rc;
     }
-    std::vector<int32_t> zeros(std::max(n, 1), 0), tiny(std::max(n, 1), 0);
-    for (int i = 0; i < n; i++) tiny[i] = prm.gmtiny > 0.0 && S.Gm[i] < prm.gmtiny;
-    const std::vector<double> &rhill = S.rhill;
+    std::vector<int32_t> zeros(std::max(n, 1), 0);
     if (n) {
-        MCHK(hipMemcpyAsync(pl.Gm, S.Gm.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
-        MCHK(hipMemcpyAsync(pl.R, S.R.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+        MCHK(hipMemcpyAsync(pl.Gm, Gm.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+        MCHK(hipMemcpyAsync(pl.R, R.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
         MCHK(hipMemcpyAsync(pl.rhill, rhill.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
         MCHK(hipMemcpyAsync(pl.status, zeros.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
-        MCHK(hipMemcpyAsync(pl.sys, zeros.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
-        MCHK(hipMemcpyAsync(pl.levelg, S.levelg.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
-        MCHK(hipMemcpyAsync(pl.levelm, S.levelm.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        MCHK(hipMemcpyAsync(pl.sys, sysid.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        MCHK(hipMemcpyAsync(pl.levelg, lg.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        MCHK(hipMemcpyAsync(pl.levelm, lm.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
         MCHK(hipMemcpyAsync(pl.tiny, tiny.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
     }
-    MCHK(hipMemcpyAsync(sy.cbvb, cbvb, 3 * sizeof(double), hipMemcpyHostToDevice, st));
-    MCHK(hipMemcpyAsync(sy.pl_off, h_pl_off.data(), 2 * sizeof(int32_t), hipMemcpyHostToDevice, st));
-    MCHK(hipMemcpyAsync(sy.nplm, &nplm_cur, sizeof(int32_t), hipMemcpyHostToDevice, st));
-    if (!S.isperi.empty() && n) {
-        MCHK(hipMemcpyAsync(dis_pl_isperi, S.isperi.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
-        MCHK(hipMemcpyAsync(dis_pl_atp, S.atp.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
-        MCHK(hipMemcpyAsync(dis_pl_peri, S.peri.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+    if (dis) {
+        MCHK(hipMemcpyAsync(dis_pl_isperi, isperi.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        MCHK(hipMemcpyAsync(dis_pl_atp, atp.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+        MCHK(hipMemcpyAsync(dis_pl_peri, peri.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
     }
+    MCHK(hipMemcpyAsync(sy.cbvb, cbvb, 3 * nsys * sizeof(double), hipMemcpyHostToDevice, st));
+    MCHK(hipMemcpyAsync(sy.pl_off, h_pl_off.data(), (nsys + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    MCHK(hipMemcpyAsync(sy.nplm, h_nplm.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice, st));
     // bit-matrix layout of the next step's detection and its column blocks
+    std::vector<int64_t> woff(nsys);
+    std::vector<int32_t> nb(nsys), cb(nsys + 1, 0);
     {
-        const int32_t nb = (n + 63) / 64, cb[2] = {0, nb};
-        const int64_t wo = 0;
-        MCHK(hipMemcpyAsync(pp.woff, &wo, sizeof(int64_t), hipMemcpyHostToDevice, st));
-        MCHK(hipMemcpyAsync(pp.nb, &nb, sizeof(int32_t), hipMemcpyHostToDevice, st));
-        MCHK(hipMemcpyAsync(d_cb_pl, cb, 2 * sizeof(int32_t), hipMemcpyHostToDevice, st));
-        nblk_pl = nb;
+        int64_t w = 0;
+        for (int s = 0; s < nsys; s++) {
+            nb[s] = (sys[s].n + 63) / 64;
+            woff[s] = w;
+            w += (int64_t)sys[s].n * nb[s];
+            cb[s + 1] = cb[s] + nb[s];
+        }
+        nblk_pl = cb[nsys];
+        MCHK(hipMemcpyAsync(pp.woff, woff.data(), nsys * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        MCHK(hipMemcpyAsync(pp.nb, nb.data(), nsys * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        MCHK(hipMemcpyAsync(d_cb_pl, cb.data(), (nsys + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
         MCHK(hipStreamSynchronize(st));
     }
     // the list, its rows (index1 side, grouped in list order) and columns
-    const int64_t m = (int64_t)S.i1.size();
-    for (int64_t k = 1; k < m; k++)
-        if (S.i1[k] < S.i1[k - 1] || S.i1[k] >= S.i2[k])
-            return fail(SWG_ERR_COLLISION, "encounter list out of row order after a merge");
+    const int64_t m = (int64_t)i1.size();
     if (int rc = ensure_list(pp, m, cap_pp)) return rc;
     npp = m;
     std::vector<int32_t> rowcnt(n, 0), colcnt(n, 0), col(std::max<int64_t>(m, 1));
     std::vector<int64_t> rowstart(n, 0), colstart(n, 0);
     for (int64_t k = 0; k < m; k++) {
-        rowcnt[S.i1[k]]++;
-        colcnt[S.i2[k]]++;
+        rowcnt[i1[k]]++;
+        colcnt[i2[k]]++;
     }
     for (int i = 1; i < n; i++) {
         rowstart[i] = rowstart[i - 1] + rowcnt[i - 1];
@@ -721,17 +831,11 @@ int Ensemble::upload_merged(const void *hs, const double *cbvb, int irec)
     }
     {
         std::vector<int64_t> fill(colstart);
-        for (int64_t k = 0; k < m; k++) col[fill[S.i2[k]]++] = (int32_t)k;
-    }
-    std::vector<int8_t> lv(m), ls(m), ll(m);
-    for (int64_t k = 0; k < m; k++) {
-        lv[k] = (int8_t)S.lvdotr[k];
-        ls[k] = (int8_t)S.status[k];
-        ll[k] = (int8_t)S.level[k];
+        for (int64_t k = 0; k < m; k++) col[fill[i2[k]]++] = (int32_t)k;
     }
     if (m) {
-        MCHK(hipMemcpyAsync(pp.i1, S.i1.data(), m * sizeof(int32_t), hipMemcpyHostToDevice, st));
-        MCHK(hipMemcpyAsync(pp.i2, S.i2.data(), m * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        MCHK(hipMemcpyAsync(pp.i1, i1.data(), m * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        MCHK(hipMemcpyAsync(pp.i2, i2.data(), m * sizeof(int32_t), hipMemcpyHostToDevice, st));
         MCHK(hipMemcpyAsync(pp.lvdotr, lv.data(), m, hipMemcpyHostToDevice, st));
         MCHK(hipMemcpyAsync(pp.status, ls.data(), m, hipMemcpyHostToDevice, st));
         MCHK(hipMemcpyAsync(pp.level, ll.data(), m, hipMemcpyHostToDevice, st));
@@ -743,14 +847,15 @@ int Ensemble::upload_merged(const void *hs, const double *cbvb, int irec)
         MCHK(hipMemcpyAsync(pp.colcnt, colcnt.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
         MCHK(hipMemcpyAsync(pp.colstart, colstart.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, st));
     }
-    // sy.enc_pp keeps the step's level-0 flag: the step took the encounter
-    // path whatever the merges leave in the list (it also decides lfirst)
+    // sy.enc_pp keeps the step's level-0 flags: a system took the encounter
+    // path whatever the merges leave in its list (it also decides lfirst)
     MCHK(hipStreamSynchronize(st));
-    any_tiny = nplm_cur < n;
+    any_tiny = false;
+    for (int s = 0; s < nsys; s++) any_tiny |= h_nplm[s] < sys[s].n;
     tune.any_tiny = any_tiny;
     if (int rc = build_tiles()) return rc;
     if (en_ready) {
-        // the energy tiles follow the body count
+        // the energy tiles follow the body counts
         std::vector<int4> tiles;
         std::vector<int32_t> toff;
         energy_tiles(nsys, h_pl_off.data(), tiles, toff);
@@ -762,6 +867,7 @@ int Ensemble::upload_merged(const void *hs, const double *cbvb, int irec)
         MCHK(hipStreamSynchronize(st));
     }
     drop_graphs();
+    (void)hit;
     // the working sets of the levels the recursion is in, from the new list
     for (int k = 1; k <= irec; k++) {
         const LevelView *src = k == 1 ? nullptr : &lvv[k - 1];

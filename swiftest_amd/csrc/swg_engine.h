@@ -159,7 +159,7 @@ struct Ensemble {
     bool merge_on = false;
     double GU = 0.0;
     std::vector<double> h_mass;
-    int32_t maxid = 0, nplm_cur = 0;
+    std::vector<int32_t> h_maxid, h_nplm;  // per system: largest id handed out, fully interacting bodies
     int64_t nmerge = 0;
     // a step runs as usual; one that flags a collision is run again from a
     // copy of the state taken at its start, with every level on the whole
@@ -173,7 +173,9 @@ struct Ensemble {
     int set_merge(double GU, const double *pl_mass);
     int merge_point(int irec, double dt);
     int resolve_merges(int irec, double dt);
-    int upload_merged(const void *host_sys, const double *cbvb, int irec);
+    int resolve_system(void *host_sys, const std::vector<int32_t> &status, double GMcb, int irec, double dt,
+                       int32_t &maxid_s, int32_t &nplm_s, int64_t &ncol, double *cbvb);
+    int upload_merged(const void *host_systems, const std::vector<char> &hit, const double *cbvb, int irec);
     int get_bodies(int32_t *id, double *mass, double *Gmass, double *radius);
 
     // internals

@@ -88,6 +88,36 @@ def test_merges_with_discard_checks_and_energy():
         assert np.linalg.norm(got[a:c] - ref[a:c]) <= 1e-12 * np.linalg.norm(ref[a:c]), a
 
 
+def test_merges_in_an_ensemble():
+    """Three crowded disks in one ensemble, each system's merges its own: a
+    merge shrinks its system and moves the later systems' bodies down.
+    Every system equals its own oracle merge-mode run, bit for bit."""
+    systems = []
+    for seed, scale in ((11, 30.0), (12, 1.0), (13, 100.0)):
+        s = compact_disk(seed=seed, npl=160 if seed != 12 else 120, ntp=0)
+        s.radius[:] = s.radius * scale
+        systems.append(s)
+    g = S.GpuSystem(S.SYMBA, systems, 0.005, istep_out=10, merge=(ics.GU, None))
+    orc = [O.OracleSystem(O.SYMBA, s, 0.005, istep_out=10, merge=(ics.GU, None)) for s in systems]
+    for blk in range(4):
+        print(f"ensemble merges block {blk}", flush=True)
+        g.run(10)
+        for o in orc:
+            assert o.run(10) == 0
+        sg, bg = g.state(), g.bodies()
+        off = np.concatenate([[0], np.cumsum(g.npl_s)])
+        assert g.nmerge == sum(o.nmerge for o in orc)
+        for k, o in enumerate(orc):
+            sl = slice(off[k], off[k + 1])
+            so, bo = o.state(), o.bodies()
+            for key in ("id", "mass", "Gmass", "radius"):
+                assert np.array_equal(bg[key][sl], bo[key]), (blk, k, key)
+            for key in ("pl_rh", "pl_vh", "pl_vb", "rhill"):
+                assert np.array_equal(sg[key][sl], so[key]), (blk, k, key, ulp_diff(sg[key][sl], so[key]))
+            assert np.array_equal(sg["cb_vb"][k], so["cb_vb"]), (blk, k)
+    assert g.nmerge >= 10
+
+
 def test_merge_mode_refusals():
     """Outside its scope the merge mode is refused at set-up, with an error
     and no change to the system."""
@@ -96,7 +126,6 @@ def test_merge_mode_refusals():
     cases = [lambda: S.GpuSystem(S.HELIO, disk, 0.005, merge=(ics.GU, None)),
              lambda: S.GpuSystem(S.SYMBA, with_tp, 0.005, merge=(ics.GU, None)),
              lambda: S.GpuSystem(S.SYMBA, disk, 0.005, lclose=False, merge=(ics.GU, None)),
-             lambda: S.GpuSystem(S.SYMBA, [disk, disk], 0.005, merge=(ics.GU, None)),
              lambda: S.GpuSystem(S.SYMBA, disk, 0.005, encounter_check=1, merge=(ics.GU, None)),
              lambda: S.GpuSystem(S.SYMBA, disk, 0.005, merge=(0.0, None))]
     for make in cases:
