@@ -763,14 +763,43 @@ __global__ __launch_bounds__(BLK) void k_kick_tp_fast(TpDev tp, PlDev pl, SysDev
     const FastConst fc;
     double px = 0.0, py = 0.0, pz = 0.0;
     int q = p0;
-    for (; q + 4 <= p1; q += 4) {
+    if (q + 4 <= p1) {
+        // the next four partners are loaded while the current four are
+        // applied, so the scalar loads' latency hides behind the arithmetic;
+        // the last batch reloads itself (index clamped to p1 - 4 >= q)
+        double cx[4], cy[4], cz[4], cg[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-            const double rx = xt - X[q + u], ry = yt - Y[q + u], rz = zt - Z[q + u];
-            const double f = gm_over_r3_fast(G[q + u], dot3(rx, ry, rz, rx, ry, rz), fc.c15, fc.c1875);
-            px = fma(-f, rx, px);
-            py = fma(-f, ry, py);
-            pz = fma(-f, rz, pz);
+            cx[u] = X[q + u];
+            cy[u] = Y[q + u];
+            cz[u] = Z[q + u];
+            cg[u] = G[q + u];
+        }
+        for (; q + 4 <= p1; q += 4) {
+            const int qn = min(q + 4, p1 - 4);
+            double nx[4], ny[4], nz[4], ng[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                nx[u] = X[qn + u];
+                ny[u] = Y[qn + u];
+                nz[u] = Z[qn + u];
+                ng[u] = G[qn + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const double rx = xt - cx[u], ry = yt - cy[u], rz = zt - cz[u];
+                const double f = gm_over_r3_fast(cg[u], dot3(rx, ry, rz, rx, ry, rz), fc.c15, fc.c1875);
+                px = fma(-f, rx, px);
+                py = fma(-f, ry, py);
+                pz = fma(-f, rz, pz);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                cx[u] = nx[u];
+                cy[u] = ny[u];
+                cz[u] = nz[u];
+                cg[u] = ng[u];
+            }
         }
     }
     for (; q < p1; q++) {
