@@ -210,6 +210,10 @@ int swg_system_load(swg_system *sys, int32_t slot, int32_t npl, int32_t ntp, dou
                     const double *tp_rh, const double *tp_vh);
 int swg_system_get_slot(swg_system *sys, int32_t slot, double *pl_rh, double *pl_vh, double *pl_vb, double *tp_rh,
                         double *tp_vh, double *tp_vb, int32_t *pl_status, int32_t *tp_status, int32_t *pl_id);
+/* Bodies the system in a slot has now (a merge, swg_system_set_merge,
+ * lowers npl): the sizes of swg_system_get_slot's arrays.  Part of the slot
+ * interface above, which has no counterpart in the reference. */
+int swg_system_slot_counts(swg_system *sys, int32_t slot, int32_t *npl, int32_t *ntp);
 /* Accelerations of the two direct sums at the current positions, computed by
  * the kernels and tiles the steps use (with the system's pair_mode), into
  * host arrays pl_acc[npl][3] and tp_acc[ntp][3] (either may be NULL): each
@@ -315,14 +319,17 @@ int swg_system_discard(swg_system *sys, const swg_discard_params *dp, const doub
  * full step (keeping the old entries' state where the same pair sits at the
  * same list position and at most as many entries as before), checks the new
  * list for further collisions, and continues the step on the new arrays.
- * Needs SyMBA systems without test particles (one, or an ensemble: each
- * system's merges are its own), CHK_CLOSE on and the triangular encounter
+ * Needs SyMBA systems without test particles (one, an ensemble or a slot
+ * pool: each system's merges are its own; a shrunk slot keeps its range
+ * and pads the rest, and a system loaded into a slot later starts its own
+ * ids and masses, Gmass / GU), CHK_CLOSE on and the triangular encounter
  * check; rotation and the ENERGY torque
  * (collision_resolve.f90:624-641) are not modelled, and a body in two
  * colliding pairs of one check (a collision family) returns
  * SWG_ERR_COLLISION at that substep: the system is then left as it stood
  * there, inside the step, and is not to be stepped further.  GU: G in the
- * system's units; pl_mass: masses in input order (NULL: Gmass / GU).  Body ids are 1..npl in input order; a merged
+ * system's units; pl_mass: masses in input order (NULL: Gmass / GU; NULL
+ * for a slot pool).  Body ids are 1..npl in input order; a merged
  * body takes the next unused id.  A step that flags a collision runs twice:
  * as usual, then again from a copy of its starting state with every level
  * on the whole device and a stop at each substep's collision check; other
@@ -334,7 +341,8 @@ int swg_system_counts(swg_system *sys, int32_t *npl, int32_t *ntp, int64_t *nmer
  * (nsys + 1 entries; they move when merges shrink earlier systems). */
 int swg_system_pl_offsets(swg_system *sys, int32_t *pl_off);
 /* Ids, masses (0 unless merging is on), Gmass and radii of the massive
- * bodies, in storage order (npl of each; NULL pointers are skipped). */
+ * bodies, in storage order (npl of each; NULL pointers are skipped; a slot
+ * pool's padding has id 0 and zero mass). */
 int swg_system_get_bodies(swg_system *sys, int32_t *id, double *mass, double *Gmass, double *radius);
 /* Autotuner: time the pl-tp and pl-pl acceleration kernels at each
  * workgroup size (64, 128, 256 bodies per tile) on the current state (no state

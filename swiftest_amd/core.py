@@ -99,6 +99,7 @@ def library():
         L.swg_system_create_slots.argtypes = [vp, C.POINTER(Params), C.c_int32, I32, I32, C.POINTER(vp)]
         L.swg_system_load.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_double] + [vp] * 7
         L.swg_system_get_slot.argtypes = [vp, C.c_int32] + [vp] * 9
+        L.swg_system_slot_counts.argtypes = [vp, C.c_int32, vp, vp]
         L.swg_set_pair_mode.argtypes = [vp, C.c_int32]
         L.swg_system_set_pair_mode.argtypes = [vp, C.c_int32]
         L.swg_system_energy.argtypes = [vp, D, D, D, D]
@@ -255,9 +256,11 @@ class GpuSystem:
 
     @classmethod
     def slots(cls, integrator, npl_cap, ntp_cap, dt, lclose=True, short_circuit=True, ctx=None,
-              encounter_check=0, pair_mode=PAIR_EXACT, gmtiny=0.0):
+              encounter_check=0, pair_mode=PAIR_EXACT, gmtiny=0.0, merge_gu=None):
         """A slot pool (swg_system_create_slots): len(npl_cap) empty slots of
-        the given capacities, all frozen until a system is loaded."""
+        the given capacities, all frozen until a system is loaded.  With
+        merge_gu (G in the systems' units) collisions merge inside the step
+        (swg_system_set_merge; masses Gmass / GU)."""
         self = cls.__new__(cls)
         self.L = library()
         self.nsys = len(npl_cap)
@@ -272,7 +275,10 @@ class GpuSystem:
         self.h = h
         self._cb = (np.ones(self.nsys), np.ones(self.nsys))
         self._gm_in = [np.zeros(n) for n in self.npl_s]
-        self._held = [(0, 0)] * self.nsys
+        self._merge_gu = None
+        if merge_gu is not None:
+            _check(self.L.swg_system_set_merge(self.h, float(merge_gu), None))
+            self._merge_gu = float(merge_gu)
         return self
 
     def load(self, slot, system, rhill=None):
@@ -282,12 +288,26 @@ class GpuSystem:
                                       _p(_c(system.pl_rh).reshape(-1, 3)), _p(_c(system.pl_vh).reshape(-1, 3)),
                                       _p(_c(system.Gmass)), _p(_c(system.radius)), _p(rh),
                                       _p(_c(system.tp_rh).reshape(-1, 3)), _p(_c(system.tp_vh).reshape(-1, 3))))
-        self._held[slot] = (system.npl, system.ntp)
         self._cb[0][slot], self._cb[1][slot] = system.GMcb, system.Rcb
+
+    def slot_counts(self, slot):
+        """Bodies the system in `slot` has now (swg_system_slot_counts)."""
+        n, m = C.c_int32(), C.c_int32()
+        _check(self.L.swg_system_slot_counts(self.h, int(slot), C.byref(n), C.byref(m)))
+        return n.value, m.value
+
+    def slot_bodies(self, slot):
+        """Ids, masses, Gmass and radii of the massive bodies of the system in
+        `slot`, in storage order."""
+        b = self.bodies()
+        off = np.zeros(self.nsys + 1, np.int32)
+        _check(self.L.swg_system_pl_offsets(self.h, _p(off)))
+        n, _ = self.slot_counts(slot)
+        return {k: v[off[slot]:off[slot] + n] for k, v in b.items()}
 
     def get_slot(self, slot):
         """State of the system `slot` holds (swg_system_get_slot)."""
-        n, m = self._held[slot]
+        n, m = self.slot_counts(slot)
         out = dict(pl_rh=np.zeros((n, 3)), pl_vh=np.zeros((n, 3)), pl_vb=np.zeros((n, 3)), tp_rh=np.zeros((m, 3)),
                    tp_vh=np.zeros((m, 3)), tp_vb=np.zeros((m, 3)), pl_status=np.zeros(n, np.int32),
                    tp_status=np.zeros(m, np.int32), pl_id=np.zeros(n, np.int32))

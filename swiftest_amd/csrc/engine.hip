@@ -1343,19 +1343,6 @@ int Ensemble::time_kernels(int reps, double *ms_tp, double *ms_pl)
 // systems while the others keep running (continuous batching).
 // ---------------------------------------------------------------------------
 
-// Padding of a slot beyond the system it holds: inactive massive bodies
-// with zero mass, radius and Hill radius, and inactive test particles, far
-// from everything (10^30 AU, each at its own place) and at rest.  A padding
-// body adds exactly nothing to any sum (every term it enters has a zero
-// mass factor, fma(0, d, s) = s), is never drifted, kicked or listed in an
-// encounter, and sorts after the real bodies.
-static void pad_position(int k, double *r)
-{
-    r[0] = 1e30;
-    r[1] = 1e27 * (double)(k + 1);
-    r[2] = 0.0;
-}
-
 // Replace system s by a new one of npl_new <= capacity massive bodies and
 // ntp_new <= capacity test particles (initial conditions as for create), run
 // the reference's initialize sequence for it alone, and put it back into the
@@ -1498,6 +1485,11 @@ int Ensemble::load(int s, int npl_new, int ntp_new, double GMcb_new, const doubl
     HIPCHK(hipStreamSynchronize(st));
     h_nreal_pl[s] = npl_new;
     h_nreal_tp[s] = ntp_new;
+    if (merge_on) {
+        // the new system's masses (Gmass / GU) and ids
+        for (int q = 0; q < cap_pl; q++) h_mass[p0 + q] = q < npl_new ? gm[q] / GU : 0.0;
+        merge_reset_system(s);
+    }
     if (h_frozen[s]) nfrozen--;
     h_frozen[s] = 0;
     return build_tiles();
@@ -1833,6 +1825,7 @@ int swg_system_create_slots(swg_ctx *ctx, const swg_params *prm, int32_t nslots,
                                z.data(), trh.data(), tvh.data(), sys);
     if (rc) return rc;
     swg::Ensemble *e = (*sys)->e;
+    e->slot_pool = true;
     e->prm.rhill_present = prm->rhill_present;
     std::vector<int32_t> inact(std::max<int64_t>(std::max(np, nt), 1), INACTIVE), fr(nslots, 1);
     for (int s = 0; s < nslots; s++) {
@@ -1865,6 +1858,16 @@ int swg_system_get_slot(swg_system *sys, int32_t slot, double *pl_rh, double *pl
 {
     if (!sys) return fail(SWG_ERR_ARG, "NULL system");
     return sys->e->get_slot(slot, pl_rh, pl_vh, pl_vb, tp_rh, tp_vh, tp_vb, pl_status, tp_status, pl_id);
+}
+
+int swg_system_slot_counts(swg_system *sys, int32_t slot, int32_t *npl, int32_t *ntp)
+{
+    if (!sys || !npl || !ntp) return fail(SWG_ERR_ARG, "bad argument");
+    swg::Ensemble *e = sys->e;
+    if (slot < 0 || slot >= e->nsys) return fail(SWG_ERR_ARG, "no such slot");
+    *npl = e->h_nreal_pl[slot];
+    *ntp = e->h_nreal_tp[slot];
+    return 0;
 }
 
 int swg_system_accel(swg_system *sys, double *pl_acc, double *tp_acc)

@@ -190,17 +190,16 @@ struct HostSys {
 int Ensemble::set_merge(double GU_, const double *pl_mass)
 {
     if (prm.integrator != SWG_SYMBA) return fail(SWG_ERR_ARG, "merging collisions needs SyMBA");
-    for (int s = 0; s < nsys; s++)
-        if (h_nreal_pl[s] != h_pl_off[s + 1] - h_pl_off[s])
-            return fail(SWG_ERR_ARG, "merging collisions needs systems, not a slot pool");
+    if (slot_pool && pl_mass) return fail(SWG_ERR_ARG, "slot pools take the masses as Gmass / GU");
     if (ntp != 0) return fail(SWG_ERR_ARG, "merging collisions needs a system without test particles");
     if (!prm.lclose) return fail(SWG_ERR_ARG, "merging collisions needs the collision checks (lclose)");
     if (prm.encounter_check != 0) return fail(SWG_ERR_ARG, "merging collisions needs the triangular encounter check");
     if (!(GU_ > 0.0)) return fail(SWG_ERR_ARG, "GU must be positive");
     GU = GU_;
-    h_mass.resize(npl);
+    // a slot's padding (after its real bodies) has mass 0
+    h_mass.assign(npl, 0.0);
     for (int s = 0; s < nsys; s++)
-        for (int i = h_pl_off[s]; i < h_pl_off[s + 1]; i++)
+        for (int i = h_pl_off[s]; i < h_pl_off[s] + h_nreal_pl[s]; i++)
             h_mass[i] = pl_mass ? pl_mass[h_pl_off[s] + h_pl_id[i] - 1] : h_Gm[i] / GU;
     if (!mb_cb) {
         if (alloc_vec(mb_rh, npl) || alloc_vec(mb_vh, npl) || alloc_vec(mb_vb, npl) || own(&mb_status, npl) ||
@@ -209,21 +208,27 @@ int Ensemble::set_merge(double GU_, const double *pl_mass)
     }
     h_maxid.assign(nsys, 0);
     h_nplm.assign(nsys, 0);
-    for (int s = 0; s < nsys; s++) {
-        const int a = h_pl_off[s], n = h_pl_off[s + 1] - a;
-        h_maxid[s] = n + (h_tp_off[s + 1] - h_tp_off[s]);  // ids 1..n in input order, per system
-        h_nplm[s] = n;
-        if (prm.gmtiny > 0.0) {
-            h_nplm[s] = 0;
-            for (int i = a; i < a + n; i++) h_nplm[s] += !(h_Gm[i] < prm.gmtiny);
-        }
-    }
+    for (int s = 0; s < nsys; s++) merge_reset_system(s);
     merge_on = true;
     // the replay of a step that merges changes the body count mid-step: no
     // captured launch sequences
     use_graphs = false;
     drop_graphs();
     return 0;
+}
+
+// Id and fully-interacting count bookkeeping of system s as it stands: ids
+// 1..n in input order, the next merged body n + 1 (swg_system_load calls this
+// for a slot's new system).
+void Ensemble::merge_reset_system(int s)
+{
+    const int a = h_pl_off[s], n = h_nreal_pl[s];
+    h_maxid[s] = n + h_nreal_tp[s];
+    h_nplm[s] = n;
+    if (prm.gmtiny > 0.0) {
+        h_nplm[s] = 0;
+        for (int i = a; i < a + n; i++) h_nplm[s] += !(h_Gm[i] < prm.gmtiny);
+    }
 }
 
 // The state a step changes (bodies' positions, velocities and status, the
@@ -655,7 +660,7 @@ int Ensemble::resolve_merges(int irec, double dt)
     {
         int64_t k = 0;
         for (int s = 0; s < nsys; s++) {
-            const int a = h_pl_off[s], n = h_pl_off[s + 1] - a;
+            const int a = h_pl_off[s], n = h_nreal_pl[s];
             HostSys &S = sys[s];
             S.n = n;
             S.rh.assign(rh.begin() + 3 * (size_t)a, rh.begin() + 3 * (size_t)(a + n));
@@ -704,20 +709,26 @@ int Ensemble::upload_merged(const void *hs, const std::vector<char> &hit, const 
 {
     const HostSys *sys = (const HostSys *)hs;
     hipStream_t st = stream;
+    // a slot pool keeps every slot's range and pads the rest of a shrunk slot
+    // as swg_system_load does; other ensembles close up
     std::vector<int32_t> off(nsys + 1, 0);
-    for (int s = 0; s < nsys; s++) off[s + 1] = off[s] + sys[s].n;
+    if (slot_pool)
+        off = h_pl_off;
+    else
+        for (int s = 0; s < nsys; s++) off[s + 1] = off[s] + sys[s].n;
     const int n = off[nsys];
     // the body and list structures only shrink in a merge; the list may grow
     // when every old entry of a system was a collision
     if (n > npl) return fail(SWG_ERR_COLLISION, "merged ensemble larger than before");
     std::vector<double> rh(3 * (size_t)n), vh(3 * (size_t)n), vb(3 * (size_t)n), Gm(n), mass(n), R(n), rhill(n);
-    std::vector<int32_t> id(n), lg(n), lm(n), sysid(n), tiny(std::max(n, 1), 0), isperi;
+    std::vector<int32_t> id(n, 0), lg(n, -1), lm(n, -1), sysid(n), tiny(std::max(n, 1), 0), stat(std::max(n, 1), INACTIVE),
+        isperi;
     std::vector<double> atp, peri;
     const bool dis = dis_ready && n;
     if (dis) {
-        isperi.resize(n);
-        atp.resize(n);
-        peri.resize(n);
+        isperi.assign(n, 1);
+        atp.assign(n, 0.0);
+        peri.assign(n, 0.0);
     }
     std::vector<int32_t> i1, i2;
     std::vector<int8_t> lv, ls, ll;
@@ -734,9 +745,14 @@ int Ensemble::upload_merged(const void *hs, const std::vector<char> &hit, const 
         std::copy(S.id.begin(), S.id.end(), id.begin() + a);
         std::copy(S.levelg.begin(), S.levelg.end(), lg.begin() + a);
         std::copy(S.levelm.begin(), S.levelm.end(), lm.begin() + a);
-        for (int q = 0; q < S.n; q++) {
+        for (int q = 0; q < off[s + 1] - a; q++) {
             sysid[a + q] = s;
-            tiny[a + q] = prm.gmtiny > 0.0 && S.Gm[q] < prm.gmtiny;
+            if (q < S.n) {
+                tiny[a + q] = prm.gmtiny > 0.0 && S.Gm[q] < prm.gmtiny;
+                stat[a + q] = ACTIVE;
+            } else {
+                pad_position(q - S.n, &rh[3 * (size_t)(a + q)]);  // at rest, massless, inactive
+            }
         }
         if (dis) {
             std::copy(S.isperi.begin(), S.isperi.end(), isperi.begin() + a);
@@ -778,13 +794,29 @@ int Ensemble::upload_merged(const void *hs, const std::vector<char> &hit, const 
         if (int rc = up3(pl.rh, rh)) return rc;
         if (int rc = up3(pl.vh, vh)) return rc;
         if (int rc = up3(pl.vb, vb)) return rc;
+        // the padding's rbeg / rend are read as partner positions (zero
+        // mass, but a distinct place for each, as swg_system_load leaves them)
+        for (Vec3p v : {pl.rbeg, pl.rend})
+            for (int s = 0; s < nsys && slot_pool; s++) {
+                const int a = off[s] + sys[s].n, np = off[s + 1] - a;
+                for (int i = 0; i < np; i++) {
+                    x[i] = rh[3 * (size_t)(a + i)];
+                    y[i] = rh[3 * (size_t)(a + i) + 1];
+                    z[i] = rh[3 * (size_t)(a + i) + 2];
+                }
+                if (np) {
+                    MCHK(hipMemcpyAsync(v.x + a, x.data(), np * sizeof(double), hipMemcpyHostToDevice, st));
+                    MCHK(hipMemcpyAsync(v.y + a, y.data(), np * sizeof(double), hipMemcpyHostToDevice, st));
+                    MCHK(hipMemcpyAsync(v.z + a, z.data(), np * sizeof(double), hipMemcpyHostToDevice, st));
+                    MCHK(hipStreamSynchronize(st));
+                }
+            }
     }
-    std::vector<int32_t> zeros(std::max(n, 1), 0);
     if (n) {
         MCHK(hipMemcpyAsync(pl.Gm, Gm.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
         MCHK(hipMemcpyAsync(pl.R, R.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
         MCHK(hipMemcpyAsync(pl.rhill, rhill.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
-        MCHK(hipMemcpyAsync(pl.status, zeros.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        MCHK(hipMemcpyAsync(pl.status, stat.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
         MCHK(hipMemcpyAsync(pl.sys, sysid.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
         MCHK(hipMemcpyAsync(pl.levelg, lg.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
         MCHK(hipMemcpyAsync(pl.levelm, lm.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
@@ -804,9 +836,10 @@ int Ensemble::upload_merged(const void *hs, const std::vector<char> &hit, const 
     {
         int64_t w = 0;
         for (int s = 0; s < nsys; s++) {
-            nb[s] = (sys[s].n + 63) / 64;
+            const int ns = off[s + 1] - off[s];
+            nb[s] = (ns + 63) / 64;
             woff[s] = w;
-            w += (int64_t)sys[s].n * nb[s];
+            w += (int64_t)ns * nb[s];
             cb[s + 1] = cb[s] + nb[s];
         }
         nblk_pl = cb[nsys];

@@ -21,6 +21,19 @@ struct WhmDev {
     int32_t *iflag;
 };
 
+// Padding of a slot beyond the system it holds: inactive massive bodies
+// with zero mass, radius and Hill radius, and inactive test particles, far
+// from everything (10^30 AU, each at its own place) and at rest.  A padding
+// body adds exactly nothing to any sum (every term it enters has a zero
+// mass factor, fma(0, d, s) = s), is never drifted, kicked or listed in an
+// encounter, and sorts after the real bodies.
+inline void pad_position(int k, double *r)
+{
+    r[0] = 1e30;
+    r[1] = 1e27 * (double)(k + 1);
+    r[2] = 0.0;
+}
+
 // A resident ensemble of independent systems.
 struct Ensemble {
     hipStream_t stream = nullptr;
@@ -31,6 +44,7 @@ struct Ensemble {
     // bodies of each system that are not slot padding (all of them unless
     // the ensemble is a slot pool, swg_system_create_slots)
     std::vector<int32_t> h_nreal_pl, h_nreal_tp;
+    bool slot_pool = false;  // made by swg_system_create_slots: slots keep their ranges
     bool any_tiny = false;  // some system has semi-interacting bodies (GMTINY)
     std::vector<double> h_GMcb, h_Gm;
 
@@ -171,6 +185,7 @@ struct Ensemble {
     int step_merge();
     int merge_save(bool restore);
     int set_merge(double GU, const double *pl_mass);
+    void merge_reset_system(int s);
     int merge_point(int irec, double dt);
     int resolve_merges(int irec, double dt);
     int resolve_system(void *host_sys, const std::vector<int32_t> &status, double GMcb, int irec, double dt,

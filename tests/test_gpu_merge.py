@@ -131,5 +131,52 @@ def test_merge_mode_refusals():
     for make in cases:
         with pytest.raises(S.SwiftestHipError):
             make()
-    pool = S.GpuSystem.slots(S.SYMBA, [64], [0], 0.005)  # one slot: a slot pool all the same
+    pool = S.GpuSystem.slots(S.SYMBA, [64], [8], 0.005)  # room for test particles
     assert pool.L.swg_system_set_merge(pool.h, ics.GU, None) != 0
+    pool = S.GpuSystem.slots(S.SYMBA, [64], [0], 0.005)  # a slot pool takes no masses
+    mass = np.ones(64)
+    assert pool.L.swg_system_set_merge(pool.h, ics.GU, mass.ctypes.data) != 0
+
+
+def test_merges_in_a_slot_pool():
+    """Merges in a slot pool (continuous batching): a slot that merges keeps
+    its range and pads what its system no longer fills, a slot loaded later
+    runs its own system from that point, and a slot reloaded after merges
+    gives its new system fresh ids and masses.  Each system equals the
+    oracle's merge-mode run of it alone, bit for bit."""
+    def disk(seed, npl, scale):
+        s = compact_disk(seed=seed, npl=npl, ntp=0)
+        s.radius[:] = s.radius * scale
+        return s
+
+    pool = S.GpuSystem.slots(S.SYMBA, [170, 130, 170], [0, 0, 0], 0.005, merge_gu=ics.GU)
+    orc = {}
+    done_merges = 0
+
+    def load(k, s):
+        nonlocal done_merges
+        if k in orc:
+            done_merges += orc[k].nmerge
+        pool.load(k, s)
+        orc[k] = O.OracleSystem(O.SYMBA, s, 0.005, merge=(ics.GU, None))
+
+    load(0, disk(11, 160, 30.0))
+    load(2, disk(13, 160, 100.0))
+    for blk in range(4):
+        if blk == 2:
+            load(1, disk(12, 120, 1.0))     # a slot that was empty
+            load(2, disk(14, 150, 100.0))   # a slot whose system merged
+        print(f"slot-pool merges block {blk}", flush=True)
+        pool.run(10)
+        for o in orc.values():
+            assert o.run(10) == 0
+        for k, o in orc.items():
+            sg, bg = pool.get_slot(k), pool.slot_bodies(k)
+            so, bo = o.state(), o.bodies()
+            assert pool.slot_counts(k)[0] == len(bo["id"]), (blk, k)
+            for key in ("id", "mass", "Gmass", "radius"):
+                assert np.array_equal(bg[key], bo[key]), (blk, k, key)
+            for key in ("pl_rh", "pl_vh", "pl_vb", "pl_status", "pl_id"):
+                assert np.array_equal(sg[key], so[key]), (blk, k, key)
+        assert pool.nmerge == done_merges + sum(o.nmerge for o in orc.values())
+    assert orc[2].nmerge >= 1 and done_merges >= 3
