@@ -49,8 +49,40 @@ constexpr int SUM_T = 256, SUM_CH = 1024;
 
 struct SumStage {
     double t[4][SUM_CH];  // the terms of up to four sums
-    uint8_t act[SUM_CH];
 };
+
+// The serial chains over a staged chunk, in order: acc + t[0] + ... + t[n-1]
+// (SUB false) or acc - t[0] - ... - t[n-1] (SUB true), walking t[n-1] down to
+// t[0] when BACK.  The reads of the next eight terms are issued before the
+// adds of the current eight.  A body a masked sum skips stages +0.0
+// instead: the chains start at +0.0, a sum that starts at +0 is never -0
+// (x + (-x) = +0 in round-to-nearest), and s + (+0) = s, s - (+0) = s bit
+// for bit for every other s, so the sums are the masked ones exactly.
+template <bool SUB, bool BACK>
+__device__ inline double chain(double acc, const double *t, int n)
+{
+    auto at = [&](int k) { return BACK ? t[n - 1 - k] : t[k]; };
+    auto step = [](double s, double x) { return SUB ? s - x : s + x; };
+    int k = 0;
+    if (n >= 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = at(u);
+        for (k = 8; k + 8 <= n; k += 8) {
+            double w[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) w[u] = at(k + u);
+#pragma unroll
+            for (int u = 0; u < 8; u++) acc = step(acc, v[u]);
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = w[u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc = step(acc, v[u]);
+    }
+    for (; k < n; k++) acc = step(acc, at(k));
+    return acc;
+}
 
 // The sums of vh2vb (swiftest_util.f90:424-459) over bodies [p0, p1): lane 0
 // returns sum(Gm), lanes 1-3 the components of -sum(Gm vh), in body order.
@@ -69,14 +101,10 @@ __device__ inline double sums_gm_vh(SumStage &S, const PlDev &pl, int p0, int p1
             S.t[3][k] = g * pl.vh.z[i];
         }
         __syncthreads();
-        if (tid == 0) {
-#pragma unroll 8
-            for (int k = 0; k < n; k++) acc = acc + S.t[0][k];
-        } else if (tid <= 3) {
-            const double *t = S.t[tid];
-#pragma unroll 8
-            for (int k = 0; k < n; k++) acc = acc - t[k];
-        }
+        if (tid == 0)
+            acc = chain<false, false>(acc, S.t[0], n);
+        else if (tid <= 3)
+            acc = chain<true, false>(acc, S.t[tid], n);
         __syncthreads();
     }
     return acc;
@@ -113,21 +141,16 @@ __global__ __launch_bounds__(SUM_T) void k_sums_begin(PlDev pl, SysDev sy, int n
         for (int k = tid; k < n; k += SUM_T) {
             const int i = c0 + k;
             const double g = pl.Gm[i];
-            S.act[k] = pl.status[i] != INACTIVE;
+            const bool act = pl.status[i] != INACTIVE;
             const double wx = first ? v.x[i] + scb[0] : v.x[i];
             const double wy = first ? v.y[i] + scb[1] : v.y[i];
             const double wz = first ? v.z[i] + scb[2] : v.z[i];
-            S.t[0][k] = g * wx;
-            S.t[1][k] = g * wy;
-            S.t[2][k] = g * wz;
+            S.t[0][k] = act ? g * wx : 0.0;
+            S.t[1][k] = act ? g * wy : 0.0;
+            S.t[2][k] = act ? g * wz : 0.0;
         }
         __syncthreads();
-        if (tid < 3) {
-            const double *t = S.t[tid];
-#pragma unroll 8
-            for (int k = 0; k < n; k++)
-                if (S.act[k]) pt = pt + t[k];
-        }
+        if (tid < 3) pt = chain<false, false>(pt, S.t[tid], n);
         __syncthreads();
     }
     if (tid < 3) sy.ptbeg[3 * s + tid] = pt / sy.GMcb[s];
@@ -988,18 +1011,13 @@ __global__ __launch_bounds__(SUM_T) void k_sums_end(PlDev pl, SysDev sy, int nsy
         for (int k = tid; k < n; k += SUM_T) {
             const int i = c0 + k;
             const double g = pl.Gm[i];
-            S.act[k] = pl.status[i] != INACTIVE;
-            S.t[0][k] = g * pl.vb.x[i];
-            S.t[1][k] = g * pl.vb.y[i];
-            S.t[2][k] = g * pl.vb.z[i];
+            const bool act = pl.status[i] != INACTIVE;
+            S.t[0][k] = act ? g * pl.vb.x[i] : 0.0;
+            S.t[1][k] = act ? g * pl.vb.y[i] : 0.0;
+            S.t[2][k] = act ? g * pl.vb.z[i] : 0.0;
         }
         __syncthreads();
-        if (tid < 3) {
-            const double *t = S.t[tid];
-#pragma unroll 8
-            for (int k = 0; k < n; k++)
-                if (S.act[k]) pt = pt + t[k];
-        }
+        if (tid < 3) pt = chain<false, false>(pt, S.t[tid], n);
         __syncthreads();
     }
     if (tid < 3) sy.ptend[3 * s + tid] = pt / gm;
@@ -1009,18 +1027,13 @@ __global__ __launch_bounds__(SUM_T) void k_sums_end(PlDev pl, SysDev sy, int nsy
         for (int k = tid; k < n; k += SUM_T) {
             const int i = c0 + k;
             const double g = pl.Gm[i];
-            S.act[k] = pl.status[i] != INACTIVE;
-            S.t[0][k] = (g * pl.vb.x[i]) / gm;
-            S.t[1][k] = (g * pl.vb.y[i]) / gm;
-            S.t[2][k] = (g * pl.vb.z[i]) / gm;
+            const bool act = pl.status[i] != INACTIVE;
+            S.t[0][k] = act ? (g * pl.vb.x[i]) / gm : 0.0;
+            S.t[1][k] = act ? (g * pl.vb.y[i]) / gm : 0.0;
+            S.t[2][k] = act ? (g * pl.vb.z[i]) / gm : 0.0;
         }
         __syncthreads();
-        if (tid < 3) {
-            const double *t = S.t[tid];
-#pragma unroll 8
-            for (int k = n - 1; k >= 0; k--)
-                if (S.act[k]) c = c - t[k];
-        }
+        if (tid < 3) c = chain<true, true>(c, S.t[tid], n);
         __syncthreads();
     }
     if (tid < 3) sy.cbvb[3 * s + tid] = c;
