@@ -59,6 +59,27 @@ def test_merges_bit_exact(seed, scale, min_merges):
     assert set(b["id"][b["id"] > 160]) <= set(range(161, 161 + g.nmerge))
 
 
+@pytest.mark.parametrize("seed,scale", [(11, 30.0), (13, 100.0)])
+def test_merges_with_semi_interacting_bodies(seed, scale):
+    """Merges in a disk whose lighter half is semi-interacting (GMTINY at the
+    median Gmass): a merged body is fully interacting or not by its own
+    Gmass, the count of fully interacting bodies follows the rearray, and
+    every block equals the oracle's bit for bit."""
+    sysic = compact_disk(seed=seed, npl=160, ntp=0)
+    sysic.radius[:] = sysic.radius * scale
+    gt = float(np.median(sysic.Gmass))
+    o = O.OracleSystem(O.SYMBA, sysic, 0.005, istep_out=10, merge=(ics.GU, None), gmtiny=gt)
+    g = S.GpuSystem(S.SYMBA, sysic, 0.005, istep_out=10, merge=(ics.GU, None), gmtiny=gt)
+    for blk in range(4):
+        print(f"merge gmtiny seed={seed} scale={scale} block {blk}", flush=True)
+        assert o.run(10) == 0
+        g.run(10)
+        _compare(o, g, f"gmtiny seed {seed} scale {scale} block {blk}")
+    assert g.nmerge >= 3
+    b = g.bodies()
+    assert 0 < (b["Gmass"] < gt).sum() < len(b["id"])
+
+
 def test_merges_with_discard_checks_and_energy():
     """Merges between discard checks: the pericentre state of the checks
     (isperi, atp, peri per body) follows the bodies through the rearray, so
