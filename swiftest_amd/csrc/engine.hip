@@ -584,6 +584,8 @@ int Ensemble::build_adjacency()
     hipStream_t st = stream;
     if (npp && prm.encounter_check == 1) {
         if (hipError_t e = sas_adjacency(st, sas, pp, npl, npp)) return hipfail(e, "sort-and-sweep adjacency");
+    } else if (npp && npp <= COLS_MAX && npl <= COLS_NPL && !nfrozen) {
+        launch_col_small(st, pp, npl, (int)npp);
     } else if (npp) {
         launch_col_t(st, pp, sy.pl_off, sy.pl_off, d_cb_pl, nsys, nblk_pl, nfrozen ? d_active : nullptr, 1, 0);
         launch_scan_i32_i64(st, pp.colcnt, pp.colstart, npl, nullptr, d_scan);

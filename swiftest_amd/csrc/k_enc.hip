@@ -574,6 +574,61 @@ void launch_col_t(hipStream_t st, ListDev L, const int32_t *row_off, const int32
     if (fill) k_col_t<true><<<nblocks, 64, 0, st>>>(L, row_off, col_off, cb_off, nsys, run, upper);
     else k_col_t<false><<<nblocks, 64, 0, st>>>(L, row_off, col_off, cb_off, nsys, run, upper);
 }
+// The same index2-side adjacency of a short pl-pl list in one workgroup
+// (npp <= COLS_MAX, npl <= COLS_NPL): the counts per column, their exclusive
+// scan, and each column's entries in list order, placed by their rank among
+// the earlier entries of the column.  Two launches over every 64-column block
+// and a device-wide scan cost more than this when a few hundred pairs are
+// listed among a few thousand bodies.
+constexpr int COLS_T = 1024;
+__global__ __launch_bounds__(COLS_T) void k_col_small(ListDev L, int npl, int npp)
+{
+    __shared__ int32_t s2[COLS_MAX];
+    __shared__ int32_t cnt[COLS_NPL];  // counts, then each column's first slot
+    __shared__ int32_t part[COLS_T];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < npl; i += COLS_T) cnt[i] = 0;
+    for (int k = tid; k < npp; k += COLS_T) s2[k] = L.i2[k];
+    __syncthreads();
+    for (int k = tid; k < npp; k += COLS_T) atomicAdd(&cnt[s2[k]], 1);
+    __syncthreads();
+    // exclusive scan: each thread a contiguous run of columns
+    const int per = (npl + COLS_T - 1) / COLS_T, a = min(npl, tid * per), b = min(npl, a + per);
+    int32_t s = 0;
+    for (int i = a; i < b; i++) s += cnt[i];
+    part[tid] = s;
+    __syncthreads();
+    if (tid == 0) {
+        int32_t r = 0;
+        for (int q = 0; q < COLS_T; q++) {
+            const int32_t v = part[q];
+            part[q] = r;
+            r += v;
+        }
+    }
+    __syncthreads();
+    int32_t r = part[tid];
+    for (int i = a; i < b; i++) {
+        const int32_t c = cnt[i];
+        L.colcnt[i] = c;
+        L.colstart[i] = r;
+        cnt[i] = r;
+        r += c;
+    }
+    __syncthreads();
+    for (int k = tid; k < npp; k += COLS_T) {
+        const int c = s2[k];
+        int rank = 0;
+        for (int q = 0; q < k; q++) rank += s2[q] == c;
+        L.col[cnt[c] + rank] = k;
+    }
+}
+
+void launch_col_small(hipStream_t st, ListDev L, int npl, int npp)
+{
+    if (npl > 0) k_col_small<<<1, COLS_T, 0, st>>>(L, npl, npp);
+}
+
 void launch_list_check_pp(hipStream_t st, ListDev pp, PlDev pl, SysDev sy, int64_t n, int irec, double dt,
                           int32_t *any)
 {

@@ -105,6 +105,10 @@ void launch_sys_flags(hipStream_t st, ListDev L, const int32_t *pl_off, int32_t 
 // L.words; cb_off: each system's first 64-column block (nsys + 1 entries)
 void launch_col_t(hipStream_t st, ListDev L, const int32_t *row_off, const int32_t *col_off, const int32_t *cb_off,
                   int nsys, int nblocks, const int32_t *run, int upper, int fill);
+// the same arrays for a short pl-pl list, in one workgroup (counts, scan and
+// fill over all npl columns); only for npp <= COLS_MAX and npl <= COLS_NPL
+constexpr int COLS_MAX = 1024, COLS_NPL = 8192;
+void launch_col_small(hipStream_t st, ListDev L, int npl, int npp);
 
 // ---- sort-and-sweep broad phase of the level-0 pl-pl check (k_sas.hip) ----
 struct SasWork {
